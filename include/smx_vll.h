@@ -1,0 +1,140 @@
+/*
+ * smx_vll.h — C ABI of the MI355X VerticalLineLocus engine (libsmx.so).
+ *
+ * The reference has no plugin layer: its hot path is three C++ free functions
+ * declared in setsm_code.hpp and called once per (tile, level, iteration) from
+ * Matching_SETSM (setsm_code.cpp:3827, 3829, 3844):
+ *
+ *   void InitializeVoxel(const ProInfo*, VOXEL**, LevelInfo&, UGRID*, NCCresult*,
+ *                        const int iteration, const double *minmaxHeight);      setsm_code.hpp:75
+ *   int  VerticalLineLocus(VOXEL**, const ProInfo*, NCCresult*, LevelInfo&,
+ *                          const UGRID*, const uint8 iteration,
+ *                          const double *minmaxHeight);                         setsm_code.hpp:81
+ *   void AWNCC(ProInfo*, VOXEL**, CSize Size_Grid2D, UGRID*, NCCresult*,
+ *              double step_height, uint8 Pyramid_step, uint8 iteration,
+ *              int MaxNumberofHeightVoxel);                                     setsm_code.hpp:91
+ *
+ * This ABI keeps their meaning but moves the state they share — the pyramid,
+ * the per-cell NCCresult array and the VOXEL cube — into HBM, owned by a job
+ * (one tile). The cube is stored in CSR order (cell rows back to back, offsets
+ * = prefix sum of NCCresult.NumOfHeight) with one 32-bit word per hypothesis
+ * laid out like VOXEL (flag_cal in bits 0..7, INCC in bits 16..31).
+ *
+ * All functions return 0 (or a non-negative count) on success and a negative
+ * SMX_E* code on failure; smx_last_error() describes the last failure of the
+ * calling thread. Calls on one job are not thread safe; different jobs may be
+ * driven from different threads.
+ */
+#ifndef SMX_VLL_H
+#define SMX_VLL_H
+
+#include "smx_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMX_OK            0
+#define SMX_E_ARG        -1   /* bad argument or unsupported configuration   */
+#define SMX_E_HIP        -2   /* HIP runtime error                          */
+#define SMX_E_NOMEM      -3   /* device allocation failed / budget exceeded */
+#define SMX_E_UNSUPPORTED -4  /* input needs a term this library lacks       */
+#define SMX_E_STATE      -5   /* call out of order (e.g. no level set)       */
+
+/* Arithmetic of the VerticalLineLocus kernel. */
+#define SMX_PREC_FP64     0   /* plain FP64, same operation order as SETSM:
+                                 INCC cube bit-identical to the CPU path      */
+#define SMX_PREC_FAST     1   /* FP16/uint16 texels staged in LDS, FP64 RPC,
+                                 FP32 pivot-shifted moments with compensated
+                                 summation; |rho - rho_fp64| <= 1e-5          */
+
+typedef struct smx_job smx_job;
+
+const char *smx_last_error(void);
+int  smx_version(void);
+/* Select the HIP device used by jobs created afterwards on this thread. */
+int  smx_set_device(int device);
+
+/* ------------------------------------------------------------------ jobs */
+/* One tile. Holds its pyramid, RPCs, per-level grid, search space,
+ * NCCresult array and voxel cube in HBM until destroyed. */
+int  smx_job_create(smx_job **out, int num_levels);
+int  smx_job_destroy(smx_job *job);
+/* Bytes of HBM the job currently holds. */
+long long smx_job_device_bytes(const smx_job *job);
+/* Cap the HBM the job may allocate (0 = no cap); allocations beyond it fail
+ * with SMX_E_NOMEM. */
+int  smx_job_set_budget(smx_job *job, long long bytes);
+
+/* Upload one pyramid level of image `img` (0 = reference, 1 = target):
+ * intensity, Sobel magnitude and orientation, as SetPyramidImages
+ * (SubFunctions.cpp:3383-3406) leaves them. start_x/start_y are py_Startpos. */
+int  smx_job_set_view(smx_job *job, int img, int level,
+                      const uint16_t *intensity, const uint16_t *magnitude,
+                      const uint8_t *orientation, uint32_t width, uint32_t height,
+                      double start_x, double start_y);
+
+/* Build levels 0..num_levels-1 of image `img` on the GPU from its level-0
+ * intensity (9x9 sigma-1.5 Gaussian decimation, Sobel magnitude, 18-bin
+ * orientation: Template.hpp:82-132, SubFunctions.cpp:2418-2571).
+ * start_x0/start_y0 are the level-0 subset start (Startpos_ori). */
+int  smx_job_build_pyramid(smx_job *job, int img, const uint16_t *level0,
+                           uint32_t width, uint32_t height, double start_x0, double start_y0);
+
+/* Download a pyramid level (any pointer may be NULL). */
+int  smx_job_get_view(smx_job *job, int img, int level, uint16_t *intensity,
+                      uint16_t *magnitude, uint8_t *orientation);
+
+/* Make `params->level` current: copies the RPCs, scalars and the cell grid
+ * (grid_pts / grid_wgs, 2 doubles per cell). The view pointers inside params
+ * are ignored: the job's uploaded or GPU-built pyramid is used. Resets the
+ * NCCresult array and voxel cube (SETSM allocates both per level,
+ * setsm_code.cpp:3745, 3769). */
+int  smx_job_set_level(smx_job *job, const smx_level *params);
+
+/* Upload the search space GridPT3 for the current level (one smx_ugrid per
+ * cell) and the level's minmaxHeight. */
+int  smx_job_set_searchspace(smx_job *job, const smx_ugrid *gridpt3, const double minmax[2]);
+/* Read back GridPT3 (InitializeVoxel may rewrite a band to -100). */
+int  smx_job_get_searchspace(smx_job *job, smx_ugrid *gridpt3);
+/* Read / overwrite the NCCresult array (one per cell). */
+int  smx_job_get_nccresult(smx_job *job, smx_nccresult *out);
+int  smx_job_set_nccresult(smx_job *job, const smx_nccresult *in);
+
+/* ------------------------------------------------------- reference trio */
+/* InitializeVoxel (setsm_code.cpp:5759-6012) on the resident state: decides
+ * per cell whether its height list is (re)built, sets NumOfHeight and
+ * re-lays the voxel cube (rows of unchanged cells are kept). */
+int  smx_initialize_voxel(smx_job *job, int iteration);
+
+/* VerticalLineLocus (setsm_code.cpp:6049-6548) on the resident state.
+ * precision = SMX_PREC_FP64 or SMX_PREC_FAST. Returns Accessable_grid (>= 0)
+ * or an error. keep_rho != 0 also stores each evaluated hypothesis' mean NCC
+ * before quantisation (float) for precision studies. */
+long long smx_vertical_line_locus(smx_job *job, int iteration, int precision, int keep_rho);
+
+/* AWNCC peak scan (setsm_code.cpp:7511-7702) without SGM aggregation, i.e.
+ * the path taken when Pyramid_step > SGM_py. */
+int  smx_awncc(smx_job *job, int iteration);
+
+/* Number of hypotheses in the cube (sum of NumOfHeight). */
+long long smx_job_num_hypotheses(const smx_job *job);
+/* Download the cube in CSR order: words[total], rho[total] (rho may be NULL,
+ * NaN where not evaluated), offsets[ncells + 1] (may be NULL). */
+int  smx_job_get_voxels(smx_job *job, uint32_t *words, float *rho, int64_t *offsets);
+/* Replace the cube (CSR order, lengths from the current NumOfHeight). */
+int  smx_job_set_voxels(smx_job *job, const uint32_t *words);
+
+/* Milliseconds the last smx_vertical_line_locus spent in its VLL kernel
+ * (HIP events on the job's stream) and the number of kernel launches. */
+double smx_job_last_kernel_ms(const smx_job *job);
+/* Hypotheses the last smx_vertical_line_locus evaluated (passed every gate). */
+long long smx_job_last_evaluated(const smx_job *job);
+
+/* Wait for all work queued on the job's stream. */
+int  smx_job_sync(smx_job *job);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SMX_VLL_H */

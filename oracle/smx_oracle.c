@@ -1,0 +1,1026 @@
+/*
+ * smx_oracle.c — CPU FP64 restatement of SETSM 4.3.16's VerticalLineLocus
+ * hot path, used as the parity checker and the CPU baseline.
+ *
+ * TEST INFRASTRUCTURE ONLY: never linked into or called by the product.
+ *
+ * Floating point follows the reference expression by expression (same
+ * operand order, same casts, no FMA contraction: build with
+ * -ffp-contract=off), so the FP64 GPU kernel can be compared bit for bit.
+ *
+ * Pins (checks that do not just retype these formulas) live in
+ * tests/test_oracle.py: closed-form RPCs, window counts, numpy corrcoef,
+ * brute-force bilinear, hand peak sequences, and the real reference
+ * functions built into oracle/_ref where they compile from their own sources.
+ *
+ * Scope limits (parity unpinned / unsupported here and in the product):
+ *  - two images only (reference_id 0, ti 1) and the SB (RPC) sensor;
+ *  - the ortho GNCC term (setsm_code.cpp:6240-6305) is not computed: a call
+ *    where it would be active returns -2 (see orc_gncc_active()).
+ */
+#include "smx_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_PI 3.141592653589793          /* Typedefine.hpp:27 */
+#define ORC_PWRTWO(x) (1 << (x))          /* Typedefine.hpp:43 */
+
+static int g_threads = 0;
+void orc_set_threads(int n) { g_threads = n; }
+int  orc_get_threads(void)
+{
+#ifdef _OPENMP
+    return g_threads > 0 ? g_threads : omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ---------------------------------------------------------------- voxels */
+
+orc_voxels *orc_voxels_create(long ncells, int keep_rho)
+{
+    orc_voxels *v = (orc_voxels *)calloc(1, sizeof(orc_voxels));
+    v->ncells = ncells;
+    v->rows = (uint32_t **)calloc((size_t)ncells, sizeof(uint32_t *));
+    v->len = (uint16_t *)calloc((size_t)ncells, sizeof(uint16_t));
+    if (keep_rho)
+        v->rho = (double **)calloc((size_t)ncells, sizeof(double *));
+    return v;
+}
+
+static void vox_free_row(orc_voxels *v, long c)
+{
+    free(v->rows[c]);
+    v->rows[c] = NULL;
+    v->len[c] = 0;
+    if (v->rho) {
+        free(v->rho[c]);
+        v->rho[c] = NULL;
+    }
+}
+
+void orc_voxels_destroy(orc_voxels *v)
+{
+    if (!v) return;
+    for (long c = 0; c < v->ncells; c++)
+        vox_free_row(v, c);
+    free(v->rows);
+    free(v->len);
+    free(v->rho);
+    free(v);
+}
+
+long orc_voxels_export(const orc_voxels *v, const smx_nccresult *ncc, uint32_t *words_out, double *rho_out)
+{
+    long k = 0;
+    for (long c = 0; c < v->ncells; c++) {
+        int n = ncc[c].NumOfHeight;
+        for (int h = 0; h < n; h++, k++) {
+            words_out[k] = (v->rows[c] && h < v->len[c]) ? v->rows[c][h] : 0u;
+            if (rho_out)
+                rho_out[k] = (v->rho && v->rho[c] && h < v->len[c]) ? v->rho[c][h] : NAN;
+        }
+    }
+    return k;
+}
+
+void orc_voxels_import(orc_voxels *v, const smx_nccresult *ncc, const uint32_t *words)
+{
+    long k = 0;
+    for (long c = 0; c < v->ncells; c++) {
+        vox_free_row(v, c);
+        int n = ncc[c].NumOfHeight;
+        if (n > 0) {
+            v->rows[c] = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)n);
+            memcpy(v->rows[c], words + k, sizeof(uint32_t) * (size_t)n);
+            v->len[c] = (uint16_t)n;
+            if (v->rho) {
+                v->rho[c] = (double *)malloc(sizeof(double) * (size_t)n);
+                for (int h = 0; h < n; h++) v->rho[c][h] = NAN;
+            }
+        }
+        k += n;
+    }
+}
+
+/* InitializeVoxel's row allocation: flag_cal = true, INCC =
+ * DoubleToSignedChar_voxel(-1) (setsm_code.cpp:5940-5948). */
+static void vox_alloc_row(orc_voxels *v, long c, int n)
+{
+    vox_free_row(v, c);
+    v->rows[c] = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)n);
+    for (int h = 0; h < n; h++)
+        v->rows[c][h] = SMX_VOXEL_WORD(1, -1000);
+    v->len[c] = (uint16_t)n;
+    if (v->rho) {
+        v->rho[c] = (double *)malloc(sizeof(double) * (size_t)n);
+        for (int h = 0; h < n; h++) v->rho[c][h] = NAN;
+    }
+}
+
+/* ------------------------------------------------------- quantisation */
+
+/* Template.hpp:964-967 DoubleToSignedChar_result: (short)round(v*1000).
+ * The double->short conversion goes through a 32-bit integer and keeps the low
+ * 16 bits, which is what GCC on x86-64 does for out-of-range values such as
+ * max_WNCC's -100 sentinel (-100000 -> 31072). */
+static int16_t q_result(double v) { return (int16_t)(int32_t)round(v * 1000.0); }
+/* Template.hpp:984-987 DoubleToSignedChar_voxel: (short)(v*1000), truncation */
+static int16_t q_voxel(double v) { return (int16_t)(int32_t)(v * 1000.0); }
+/* Template.hpp:969-972 / 989-992 */
+static double  dq(int16_t v) { return (double)v / 1000.0; }
+
+/* ------------------------------------------------------------- geometry */
+
+void orc_rpc_ground_to_image(const smx_rpc *rpc, const double adjust[2], double gx, double gy, double gz,
+                             double *img_x, double *img_y)
+{
+    /* CoordConversion.hpp:26-87 */
+    double L = (gx - rpc->off[2]) / rpc->scale[2];
+    double P = (gy - rpc->off[3]) / rpc->scale[3];
+    double H = (gz - rpc->off[4]) / rpc->scale[4];
+
+    if (L < -10.0 || L > 10.0) {
+        if (gx > 0) gx = gx - 360;
+        else        gx = gx + 360;
+        L = (gx - rpc->off[2]) / rpc->scale[2];
+    }
+    if (P < -10.0 || P > 10.0) {
+        if (gy > 0) gy = gy - 360;
+        else        gy = gy + 360;
+        P = (gy - rpc->off[3]) / rpc->scale[3];
+    }
+
+    double Coeff[4];
+    for (int j = 0; j < 4; j++) {
+        const double *c = rpc->coef[j];
+        Coeff[j] = c[0] * 1.0 + c[1] * L + c[2] * P
+                 + c[3] * H + c[4] * L * P + c[5] * L * H
+                 + c[6] * P * H + c[7] * L * L + c[8] * P * P
+                 + c[9] * H * H + c[10] * (P * L) * H + c[11] * (L * L) * L
+                 + c[12] * (L * P) * P + c[13] * (L * H) * H + c[14] * (L * L) * P
+                 + c[15] * (P * P) * P + c[16] * (P * H) * H + c[17] * (L * L) * H
+                 + c[18] * (P * P) * H + c[19] * (H * H) * H;
+    }
+    double Line = ((Coeff[0] / Coeff[1]) * rpc->scale[0] + rpc->off[0]);
+    double Samp = ((Coeff[2] / Coeff[3]) * rpc->scale[1] + rpc->off[1]);
+
+    double y = adjust[0] + Line;
+    double x = adjust[1] + Samp;
+    if (y < 0) y = 0;
+    if (y > rpc->off[0] + rpc->scale[0] * 1.2) y = rpc->off[0] + rpc->scale[0] * 1.2;
+    if (x < 0) x = 0;
+    if (x > rpc->off[1] + rpc->scale[1] * 1.2) x = rpc->off[1] + rpc->scale[1] * 1.2;
+    *img_x = x;
+    *img_y = y;
+}
+
+/* CoordConversion.hpp:96-105 OriginalToPyramid_single */
+static void to_pyramid(double x, double y, double sx, double sy, int level, double *px, double *py)
+{
+    *px = (x / ORC_PWRTWO(level)) - sx;
+    *py = (y / ORC_PWRTWO(level)) - sy;
+}
+
+double orc_height_step(int level, double im_resolution)
+{
+    /* setsm_code.cpp:5741-5757 */
+    const double h_divide = 2;
+    im_resolution = im_resolution * ORC_PWRTWO(level);
+    double HS = (double)(im_resolution / h_divide);
+    double tt1 = HS * 1000.0;
+    double tt2 = floor(tt1 + 0.1);
+    HS = tt2 / 1000.0;
+    if (HS > 3) HS = 3;
+    return HS;
+}
+
+int orc_vll_half_template(int level, int template_size, double resolution)
+{
+    /* setsm_code.cpp:6057-6077 */
+    int ts = template_size;
+    if (level >= 1) {
+        double template_area = 5.0;
+        int t = (int)((template_area / (resolution * ORC_PWRTWO(level))) / 2.0) * 2 + 1;
+        if (template_size < t) ts = t;
+    }
+    int half = ts / 2;
+    if (level == 3) half = half - 1;
+    else if (level < 3) half = half - 2;
+    return half;
+}
+
+int orc_window_bits(int half, int row, int col)
+{
+    /* setsm_code.cpp:6434 (outer disc) and SubFunctions.cpp:3282-3312 */
+    int r2 = row * row + col * col;
+    if (row < -half || row > half || col < -half || col > half) return 0;
+    if (r2 > (half + 1) * (half + 1)) return 0;
+    int bits = 1;
+    const int s1 = (int)(half / 2);
+    if (r2 <= (half - s1 + 1) * (half - s1 + 1) &&
+        row >= -half + s1 && row <= half - s1 && col >= -half + s1 && col <= half - s1)
+        bits |= 2;
+    const int s2 = s1 + (int)((s1 / 2.0) + 0.5);
+    if (r2 <= (half - s2 + 1) * (half - s2 + 1) &&
+        row >= -half + s2 && row <= half - s2 && col >= -half + s2 && col <= half - s2)
+        bits |= 4;
+    return bits;
+}
+
+int orc_check_image_boundary(const smx_level *L, long cell, double minH, double maxH, int half)
+{
+    /* setsm_code.cpp:11002-11083, SB sensor, images 0 and 1 both selected */
+    const int buff_pixel = 1;
+    int selected = 0;
+    int check_reference = 1;
+    const double lon = L->grid_wgs[2 * cell], lat = L->grid_wgs[2 * cell + 1];
+    for (int ti = 0; ti < 2; ti++) {
+        const smx_view *v = &L->view[ti];
+        int ok[2];
+        for (int e = 0; e < 2; e++) {
+            double x, y, px, py;
+            orc_rpc_ground_to_image(&L->rpc[ti], L->adjust[ti], lon, lat, e == 0 ? minH : maxH, &x, &y);
+            to_pyramid(x, y, v->start_x, v->start_y, L->level, &px, &py);
+            ok[e] = (px > half + buff_pixel && px < v->width - half - buff_pixel &&
+                     py > half + buff_pixel && py < v->height - half - buff_pixel);
+        }
+        if (ok[0] && ok[1]) selected++;
+        else if (ti == 0) check_reference = 0;
+    }
+    if (!check_reference) return 0;
+    return selected >= 2;
+}
+
+double orc_ncc_alpha(int Pyramid_step, int iteration, int IsRA)
+{
+    /* setsm_code.cpp:6014-6047 */
+    double a;
+    if (Pyramid_step >= 4) {
+        a = 1.0 - ((4 - Pyramid_step) * 0.2 + (iteration - 1) * 0.05);
+        if (a < 0.8) a = 0.8;
+    } else if (Pyramid_step >= 3) {
+        a = 1.0 - ((4 - Pyramid_step) * 0.2 + (iteration - 1) * 0.05);
+        if (a < 0.6) a = 0.6;
+    } else if (Pyramid_step == 2) {
+        a = 1.0 - (0.4 + (iteration - 1) * 0.05);
+        if (a < 0.4) a = 0.4;
+    } else
+        a = 1.0 - ((4 - Pyramid_step) * 0.2 + (iteration - 1) * 0.05);
+    if (a < 0.1) a = 0.1;
+    if (IsRA == 1) a = 1.0;
+    return a;
+}
+
+/* --------------------------------------------------------------- NCC */
+
+double orc_correlate(const double *L, const double *R, const int N)
+{
+    /* SubFunctions.cpp:974-1018, two-pass Pearson with the -99 sentinel */
+    double Lmean = 0, Rmean = 0, rho;
+    if (N > 0) {
+        for (int i = 0; i < N; i++) {
+            Lmean += L[i];
+            Rmean += R[i];
+        }
+        Lmean = Lmean / N;
+        Rmean = Rmean / N;
+        double SumLR = 0, SumL2 = 0, SumR2 = 0;
+        for (int i = 0; i < N; i++) {
+            SumLR += (L[i] - Lmean) * (R[i] - Rmean);
+            SumL2 += (L[i] - Lmean) * (L[i] - Lmean);
+            SumR2 += (R[i] - Rmean) * (R[i] - Rmean);
+        }
+        if (SumL2 > 1e-8 && SumR2 > 1e-8)
+            rho = SumLR / (sqrt(SumL2 * SumR2));
+        else
+            rho = (double)-99;
+    } else
+        rho = (double)-99;
+    return rho;
+}
+
+/* Per-thread patch vectors, SetKernel (Typedefine.hpp:727-754). */
+typedef struct patchvec {
+    int cap;
+    double *l[3], *lm[3], *r[3], *rm[3];
+} patchvec;
+
+static void pv_init(patchvec *p, int half)
+{
+    p->cap = (2 * half + 1) * (2 * half + 1);
+    for (int k = 0; k < 3; k++) {
+        p->l[k] = (double *)malloc(sizeof(double) * (size_t)p->cap);
+        p->lm[k] = (double *)malloc(sizeof(double) * (size_t)p->cap);
+        p->r[k] = (double *)malloc(sizeof(double) * (size_t)p->cap);
+        p->rm[k] = (double *)malloc(sizeof(double) * (size_t)p->cap);
+    }
+}
+
+static void pv_free(patchvec *p)
+{
+    for (int k = 0; k < 3; k++) {
+        free(p->l[k]); free(p->lm[k]); free(p->r[k]); free(p->rm[k]);
+    }
+}
+
+/* SubFunctions.cpp:3317-3326 InterpolatePatch */
+static double interp(const uint16_t *I, long pos, uint32_t w, double dx, double dy)
+{
+    return (double)(I[pos]) * (1 - dx) * (1 - dy) +
+           (double)(I[pos + 1]) * dx * (1 - dy) +
+           (double)(I[pos + w]) * (1 - dx) * dy +
+           (double)(I[pos + 1 + w]) * dx * dy;
+}
+
+/* SubFunctions.cpp:3249-3315 SetVecKernelValue */
+static void set_vec_kernel_value(patchvec *pv, int half, const smx_view *Lv, const smx_view *Rv,
+                                 int row, int col, double lx, double ly, double rx, double ry,
+                                 int radius2, int *Count_N)
+{
+    if (ry >= 0 && ry + 1 < Rv->height && rx >= 0 && rx + 1 < Rv->width &&
+        ly >= 0 && ly + 1 < Lv->height && lx >= 0 && lx + 1 < Lv->width) {
+        const long position = (long)lx + (long)ly * (long)Lv->width;
+        const long position_right = (long)rx + (long)ry * (long)Rv->width;
+        double dx = lx - floor(lx);
+        double dy = ly - floor(ly);
+        double dx_r = rx - floor(rx);
+        double dy_r = ry - floor(ry);
+        double lp = interp(Lv->img, position, Lv->width, dx, dy);
+        double lmp = interp(Lv->mag, position, Lv->width, dx, dy);
+        double rp = interp(Rv->img, position_right, Rv->width, dx_r, dy_r);
+        double rmp = interp(Rv->mag, position_right, Rv->width, dx_r, dy_r);
+        if (lp > 0 && rp > 0) {
+            pv->l[0][Count_N[0]] = lp; pv->lm[0][Count_N[0]] = lmp;
+            pv->r[0][Count_N[0]] = rp; pv->rm[0][Count_N[0]] = rmp;
+            Count_N[0]++;
+            const int size_1 = (int)(half / 2);
+            if (radius2 <= (half - size_1 + 1) * (half - size_1 + 1) &&
+                row >= -half + size_1 && row <= half - size_1 &&
+                col >= -half + size_1 && col <= half - size_1) {
+                pv->l[1][Count_N[1]] = lp; pv->lm[1][Count_N[1]] = lmp;
+                pv->r[1][Count_N[1]] = rp; pv->rm[1][Count_N[1]] = rmp;
+                Count_N[1]++;
+            }
+            const int size_2 = size_1 + (int)((size_1 / 2.0) + 0.5);
+            if (radius2 <= (half - size_2 + 1) * (half - size_2 + 1) &&
+                row >= -half + size_2 && row <= half - size_2 &&
+                col >= -half + size_2 && col <= half - size_2) {
+                pv->l[2][Count_N[2]] = lp; pv->lm[2][Count_N[2]] = lmp;
+                pv->r[2][Count_N[2]] = rp; pv->rm[2][Count_N[2]] = rmp;
+                Count_N[2]++;
+            }
+        }
+    }
+}
+
+/* SubFunctions.cpp:3328-3359 ComputeMultiNCC with Th_rho = 0 */
+static void compute_multi_ncc(const patchvec *pv, const int *Count_N, double *count_NCC, double *sum_NCC_multi)
+{
+    if (Count_N[0] > 0 && Count_N[1] > 0 && Count_N[2] > 0) {
+        double temp_roh = 0, count_roh = 0;
+        for (int k = 0; k < 3; k++) {
+            const double ncc = orc_correlate(pv->l[k], pv->r[k], Count_N[k]);
+            if (ncc != -99) { count_roh++; temp_roh += ncc; }
+            const double ncc_mag = orc_correlate(pv->lm[k], pv->rm[k], Count_N[k]);
+            if (ncc_mag != -99) { count_roh++; temp_roh += ncc_mag; }
+        }
+        if (count_roh > 0) {
+            *sum_NCC_multi += temp_roh / count_roh;
+            *count_NCC += 1;
+        }
+    }
+}
+
+/* ------------------------------------------------------ InitializeVoxel */
+
+void orc_initialize_voxel(const smx_level *Lv, orc_voxels *vox, smx_ugrid *GridPT3,
+                          smx_nccresult *nccresult, int iteration, const double *minmaxHeight)
+{
+    /* setsm_code.cpp:5759-6012 (check_image_boundary with H_template_size 7) */
+    const double height_step = Lv->height_step;
+    const int pyramid_step = Lv->level;
+    const long n = (long)Lv->grid_w * (long)Lv->grid_h;
+#pragma omp parallel for schedule(guided) num_threads(orc_get_threads())
+    for (long t_i = 0; t_i < n; t_i++) {
+        smx_nccresult *nr = &nccresult[t_i];
+        smx_ugrid *g = &GridPT3[t_i];
+        if (orc_check_image_boundary(Lv, t_i, g->minHeight, g->maxHeight, 7)) {
+            int change_step_min = 0, change_step_max = 0;
+            int check_blunder_cell = 1;
+            double th_height = 1000;
+            if (Lv->dem_resolution <= 4) th_height = 500;
+
+            if (Lv->check_matchtag || iteration == 1) {
+                nr->minHeight = g->minHeight;
+                nr->maxHeight = g->maxHeight;
+                nr->GNCC = q_result(-1.0);
+                nr->check_height_change = 1;
+                check_blunder_cell = 0;
+            } else {
+                if (pyramid_step >= 2)
+                    check_blunder_cell = 0;
+                else if (g->Matched_flag != 0) {
+                    if (g->maxHeight - g->minHeight > 0) {
+                        if (pyramid_step <= 1) {
+                            if (iteration > 1) {
+                                if (g->maxHeight <= minmaxHeight[1] && g->minHeight >= minmaxHeight[0] &&
+                                    nr->maxHeight <= minmaxHeight[1] && nr->minHeight >= minmaxHeight[0])
+                                    check_blunder_cell = 0;
+                                else
+                                    check_blunder_cell = 1;
+                            } else {
+                                if (g->maxHeight <= minmaxHeight[1] && g->minHeight >= minmaxHeight[0])
+                                    check_blunder_cell = 0;
+                                else
+                                    check_blunder_cell = 1;
+                            }
+                        }
+                        if (pyramid_step == 1) {
+                            if (abs(g->maxHeight - g->minHeight) < 1000) check_blunder_cell = 0;
+                            else check_blunder_cell = 1;
+                        }
+                        if (pyramid_step == 0) {
+                            if (iteration > 1) {
+                                if ((abs(nr->maxHeight - nr->minHeight) < th_height) &&
+                                    (abs(g->maxHeight - g->minHeight) < th_height))
+                                    check_blunder_cell = 0;
+                                else
+                                    check_blunder_cell = 1;
+                            } else {
+                                if (abs(g->maxHeight - g->minHeight) < th_height) check_blunder_cell = 0;
+                                else nr->NumOfHeight = 0;
+                            }
+                        }
+                    } else {
+                        if (nr->NumOfHeight > 0) vox_free_row(vox, t_i);
+                        nr->NumOfHeight = 0;
+                        check_blunder_cell = 1;
+                    }
+                } else {
+                    if (nr->NumOfHeight > 0) vox_free_row(vox, t_i);
+                    nr->NumOfHeight = 0;
+                    check_blunder_cell = 1;
+                }
+            }
+
+            if (!check_blunder_cell) {
+                if (iteration == 1 || (pyramid_step == 0 && Lv->dem_resolution < 2)) {
+                    nr->minHeight = g->minHeight;
+                    nr->maxHeight = g->maxHeight;
+                    nr->GNCC = q_result(-1.0);
+                    nr->check_height_change = 1;
+                    if (pyramid_step == 1) {
+                        if (abs((int)g->maxHeight - (int)g->minHeight) > 1000) {
+                            g->maxHeight = -100;
+                            g->minHeight = -100;
+                            nr->check_height_change = 0;
+                        }
+                    } else if (pyramid_step == 0) {
+                        if (abs((int)g->maxHeight - (int)g->minHeight) > th_height) {
+                            g->maxHeight = -100;
+                            g->minHeight = -100;
+                            nr->check_height_change = 0;
+                        }
+                    }
+                } else {
+                    const int NumberofHeightVoxel = (int)((g->maxHeight - g->minHeight) / height_step);
+                    if (nr->NumOfHeight == 0 && NumberofHeightVoxel > 0) {
+                        change_step_min = 0;
+                        change_step_max = 0;
+                        nr->check_height_change = 1;
+                    } else {
+                        if (nr->minHeight > g->minHeight || nr->maxHeight < g->maxHeight) {
+                            if (nr->minHeight > g->minHeight)
+                                change_step_min = (int)((nr->minHeight - g->minHeight) / height_step + 0.5);
+                            if (nr->maxHeight < g->maxHeight)
+                                change_step_max = (int)((g->maxHeight - nr->maxHeight) / height_step + 0.5);
+                            nr->minHeight = (int16_t)floor(nr->minHeight - change_step_min * height_step);
+                            nr->maxHeight = (int16_t)ceil(nr->maxHeight + change_step_max * height_step);
+                            nr->check_height_change = 1;
+                            if (pyramid_step == 0 && abs(nr->maxHeight - nr->minHeight) > th_height)
+                                nr->check_height_change = 0;
+                        } else
+                            nr->check_height_change = 0;
+                    }
+                }
+
+                if (nr->check_height_change) {
+                    nr->minHeight = (int16_t)floor(nr->minHeight - change_step_min * height_step);
+                    nr->maxHeight = (int16_t)ceil(nr->maxHeight + change_step_max * height_step);
+                    const int NumberofHeightVoxel = (int)((float)(nr->maxHeight - nr->minHeight) / height_step);
+                    if (NumberofHeightVoxel > 0) {
+                        nr->NumOfHeight = (uint16_t)NumberofHeightVoxel;
+                        vox_alloc_row(vox, t_i, NumberofHeightVoxel);
+                    } else {
+                        if (nr->NumOfHeight > 0) vox_free_row(vox, t_i);
+                        nr->NumOfHeight = 0;
+                        nr->check_height_change = 0;
+                    }
+                }
+            } else {
+                if (nr->NumOfHeight > 0) vox_free_row(vox, t_i);
+                nr->NumOfHeight = 0;
+                nr->check_height_change = 0;
+            }
+        } else {
+            if (nr->NumOfHeight > 0) vox_free_row(vox, t_i);
+            nr->NumOfHeight = 0;
+            nr->check_height_change = 0;
+        }
+
+        if (nr->minHeight == 0 || nr->maxHeight == 0) {
+            if (nr->NumOfHeight > 0) vox_free_row(vox, t_i);
+            nr->NumOfHeight = 0;
+            nr->check_height_change = 0;
+        }
+        if (pyramid_step == 0 && nr->NumOfHeight > 1000) {
+            if (nr->NumOfHeight > 0) vox_free_row(vox, t_i);
+            nr->NumOfHeight = 0;
+            nr->check_height_change = 0;
+        }
+    }
+}
+
+/* --------------------------------------------------- VerticalLineLocus */
+
+long orc_vertical_line_locus(const smx_level *Lv, orc_voxels *vox, smx_nccresult *nccresult,
+                             const smx_ugrid *GridPT3, int iteration, const double *minmaxHeight)
+{
+    /* setsm_code.cpp:6049-6548 */
+    const int check_matchtag = Lv->check_matchtag;
+    const int IsRA = Lv->is_ra;
+    long Accessable_grid = 0;
+    const int Pyramid_step = Lv->level;
+    const int Half_template_size = orc_vll_half_template(Pyramid_step, Lv->template_size, Lv->resolution);
+    const long numofpts = (long)Lv->grid_w * (long)Lv->grid_h;
+
+    int check_combined_WNCC = 0, check_combined_WNCC_INCC = 0;
+    if (!IsRA && (Pyramid_step > Lv->py_combined_level)) check_combined_WNCC = 1;
+    if (check_combined_WNCC && (Pyramid_step >= 1 && iteration % 2 == 0)) check_combined_WNCC_INCC = 1;
+
+    int check_ortho;
+    if ((Pyramid_step == 4 && iteration == 1) || IsRA) check_ortho = 0;
+    else check_ortho = 1;
+    if (Lv->pre_dem && !(Pyramid_step == 4 && iteration == 1)) check_ortho = 1;
+    if (check_matchtag) check_ortho = 1;
+    if (Pyramid_step <= Lv->sgm_py) check_ortho = 0;
+
+    const double ortho_th = 0.7 - (4 - Pyramid_step) * 0.10;
+    /* GNCC (setsm_code.cpp:6240-6305) is outside this restatement: refuse
+     * inputs where it would be computed. */
+    if (check_ortho) {
+        for (long c = 0; c < numofpts; c++)
+            if (dq(GridPT3[c].ortho_ncc[1]) > ortho_th) return -2;
+    }
+    if (check_combined_WNCC_INCC && (!Lv->next[0].img || !Lv->next[1].img)) return -3;
+
+    const double height_step = Lv->height_step;
+    const int check_matching_rate = Lv->check_matching_rate;
+
+#pragma omp parallel num_threads(orc_get_threads())
+    {
+        patchvec pv, pvn;
+        pv_init(&pv, Half_template_size);
+        pv_init(&pvn, Half_template_size);
+
+#pragma omp for schedule(dynamic, 1) reduction(+ : Accessable_grid)
+        for (long pt_index = 0; pt_index < numofpts; pt_index++) {
+            const int start_H = GridPT3[pt_index].minHeight;
+            const int end_H = GridPT3[pt_index].maxHeight;
+            smx_nccresult *nr = &nccresult[pt_index];
+
+            if (!orc_check_image_boundary(Lv, pt_index, start_H, end_H, Half_template_size))
+                continue;
+
+            int check_blunder_cell = 0;
+            double th_height = 1000;
+            if (Lv->dem_resolution <= 4) th_height = 500;
+            if (Pyramid_step >= 2)
+                check_blunder_cell = 0;
+            else {
+                if (GridPT3[pt_index].Matched_flag == 0)
+                    check_blunder_cell = 1;
+                else {
+                    if (abs(GridPT3[pt_index].maxHeight - GridPT3[pt_index].minHeight) < th_height)
+                        check_blunder_cell = 0;
+                    else
+                        check_blunder_cell = 1;
+                }
+            }
+            if (Pyramid_step <= 1) {
+                if (GridPT3[pt_index].maxHeight > minmaxHeight[1] || GridPT3[pt_index].minHeight < minmaxHeight[0])
+                    check_blunder_cell = 1;
+            }
+            if (check_matchtag && iteration <= 2) {
+                check_blunder_cell = 0;
+                if (GridPT3[pt_index].maxHeight > minmaxHeight[1] || GridPT3[pt_index].minHeight < minmaxHeight[0])
+                    check_blunder_cell = 1;
+                else if (abs(GridPT3[pt_index].maxHeight - GridPT3[pt_index].minHeight) > th_height)
+                    check_blunder_cell = 1;
+            }
+            if (check_blunder_cell)
+                continue;
+
+            /* ti = 1 only (stereo pair) */
+            Accessable_grid++;
+            nr->GNCC = q_result(-1.0);   /* GNCC term inactive (checked above) */
+
+            const int NumOfHeights = (int)((end_H - start_H) / height_step);
+            if (IsRA || check_matching_rate) {
+                nr->check_height_change = 1;
+                nr->NumOfHeight = (uint16_t)NumOfHeights;
+            }
+
+            if (IsRA || check_matching_rate || nr->check_height_change || (Pyramid_step == 0 && Lv->dem_resolution < 2)) {
+                double max_WNCC = -100;
+                nr->result0 = q_result(-1.0);
+                nr->result2 = -1000;
+                nr->result1 = q_result(-1.0);
+                nr->result3 = -1000;
+                nr->result4 = 0;
+
+                /* FindPeakNcc state (check_matching_rate mode only) */
+                double pre_rho = -1.0;
+                float pre_height = 0.0f;
+                int direction = 0;
+                int check_rho = 0;
+
+                for (int hidx = 0; hidx < nr->NumOfHeight; hidx++) {
+                    float iter_height;
+                    if (IsRA || check_matching_rate)
+                        iter_height = (float)(start_H + hidx * height_step);
+                    else
+                        iter_height = (float)(nr->minHeight + hidx * height_step);
+                    if (!(iter_height >= start_H && iter_height <= end_H))
+                        continue;
+
+                    const smx_view *Lw = &Lv->view[0], *Rw = &Lv->view[1];
+                    const double lon = Lv->grid_wgs[2 * pt_index], lat = Lv->grid_wgs[2 * pt_index + 1];
+                    double rix, riy, tix, tiy, rpx, rpy, tpx, tpy;
+                    orc_rpc_ground_to_image(&Lv->rpc[0], Lv->adjust[0], lon, lat, (double)iter_height, &rix, &riy);
+                    orc_rpc_ground_to_image(&Lv->rpc[1], Lv->adjust[1], lon, lat, (double)iter_height, &tix, &tiy);
+                    to_pyramid(rix, riy, Lw->start_x, Lw->start_y, Pyramid_step, &rpx, &rpy);
+                    to_pyramid(tix, tiy, Rw->start_x, Rw->start_y, Pyramid_step, &tpx, &tpy);
+
+                    const int check_py_image_pt =
+                        (int)rpy >= 0 && (int)rpy + 1 < (long)Lw->height && (int)rpx >= 0 && (int)rpx + 1 < (long)Lw->width &&
+                        (int)tpy >= 0 && (int)tpy + 1 < (long)Rw->height && (int)tpx >= 0 && (int)tpx + 1 < (long)Rw->width;
+
+                    int check_py_image_pt_next = 1;
+                    double rnx = 0, rny = 0, tnx = 0, tny = 0;
+                    const smx_view *Ln = &Lv->next[0], *Rn = &Lv->next[1];
+                    if (check_combined_WNCC_INCC) {
+                        to_pyramid(rix, riy, Ln->start_x, Ln->start_y, Pyramid_step - 1, &rnx, &rny);
+                        to_pyramid(tix, tiy, Rn->start_x, Rn->start_y, Pyramid_step - 1, &tnx, &tny);
+                        check_py_image_pt_next =
+                            (int)rny >= 0 && (int)rny + 1 < (long)Ln->height && (int)rnx >= 0 && (int)rnx + 1 < (long)Ln->width &&
+                            (int)tny >= 0 && (int)tny + 1 < (long)Rn->height && (int)tnx >= 0 && (int)tnx + 1 < (long)Rn->width;
+                    }
+                    if (!(check_py_image_pt && check_py_image_pt_next))
+                        continue;
+
+                    const int ori_diff = (int)Lw->ori[(int)rpy * Lw->width + (int)rpx] - (int)Rw->ori[(int)tpy * Rw->width + (int)tpx];
+                    int ori_diff_next = 0;
+                    if (check_combined_WNCC_INCC)
+                        ori_diff_next = (int)Ln->ori[(int)rny * Ln->width + (int)rnx] - (int)Rn->ori[(int)tny * Rn->width + (int)tnx];
+                    double diff_theta = (double)ori_diff;
+                    if (check_combined_WNCC_INCC)
+                        diff_theta = (diff_theta + (double)ori_diff_next) / 2.0;
+
+                    int check_orientation = 0;
+                    if (IsRA != 1) {
+                        if (Pyramid_step >= 3 && diff_theta * Lv->bin_angle < 90 && diff_theta * Lv->bin_angle > -90)
+                            check_orientation = 1;
+                        else if (Pyramid_step <= 2)
+                            check_orientation = 1;
+                    } else
+                        check_orientation = 1;
+
+                    if (!check_orientation) {
+                        if (!check_matching_rate && !IsRA)
+                            vox->rows[pt_index][hidx] &= ~0xffu;   /* flag_cal = false */
+                        continue;
+                    }
+
+                    double sum_INCC_multi = 0, count_INCC = 0;
+                    int Count_N[3] = {0, 0, 0}, Count_N_next[3] = {0, 0, 0};
+                    const double rot_theta = (double)(diff_theta * Lv->bin_angle * ORC_PI / 180.0);
+                    const double cos0 = cos(-rot_theta);
+                    const double sin0 = sin(-rot_theta);
+                    const int H = Half_template_size;
+                    for (int row = -H; row <= H; row++) {
+                        for (int col = -H; col <= H; col++) {
+                            int radius2 = row * row + col * col;
+                            if (radius2 <= (H + 1) * (H + 1)) {
+                                double lx = rpx + col, ly = rpy + row;
+                                double ox = cos0 * col - sin0 * row, oy = sin0 * col + cos0 * row;
+                                double rx = tpx + ox, ry = tpy + oy;
+                                set_vec_kernel_value(&pv, H, Lw, Rw, row, col, lx, ly, rx, ry, radius2, Count_N);
+                                if (check_combined_WNCC_INCC) {
+                                    double lnx = rnx + col, lny = rny + row;
+                                    double onx = cos0 * col - sin0 * row, ony = sin0 * col + cos0 * row;
+                                    double rnx2 = tnx + onx, rny2 = tny + ony;
+                                    set_vec_kernel_value(&pvn, H, Ln, Rn, row, col, lnx, lny, rnx2, rny2, radius2, Count_N_next);
+                                }
+                            }
+                        }
+                    }
+
+                    compute_multi_ncc(&pv, Count_N, &count_INCC, &sum_INCC_multi);
+                    if (Count_N[0] > 0 && Count_N[1] > 0 && Count_N[2] > 0) {
+                        if (!check_matching_rate && !IsRA)
+                            vox->rows[pt_index][hidx] |= 1u;
+                    }
+                    if (check_combined_WNCC_INCC)
+                        compute_multi_ncc(&pvn, Count_N_next, &count_INCC, &sum_INCC_multi);
+
+                    if (!(IsRA || check_matching_rate)) {
+                        int16_t incc = (count_INCC > 0) ? q_voxel(sum_INCC_multi / count_INCC) : q_voxel(-1.0);
+                        vox->rows[pt_index][hidx] = (vox->rows[pt_index][hidx] & 0xffffu) | ((uint32_t)(uint16_t)incc << 16);
+                        if (vox->rho)
+                            vox->rho[pt_index][hidx] = (count_INCC > 0) ? sum_INCC_multi / count_INCC : -1.0;
+                    }
+
+                    double temp_rho;
+                    if (count_INCC > 0) temp_rho = sum_INCC_multi / count_INCC;
+                    else temp_rho = -1.0;
+                    if (max_WNCC < temp_rho) max_WNCC = temp_rho;
+
+                    if (check_matching_rate) {
+                        /* setsm_code.cpp:6672-6752 FindPeakNcc */
+                        double diff_rho;
+                        int t_direction;
+                        if (temp_rho >= 0 && pre_rho >= 0) diff_rho = temp_rho - pre_rho;
+                        else if (temp_rho < 0 && pre_rho < 0) diff_rho = -(fabs(temp_rho) - fabs(pre_rho));
+                        else if (temp_rho >= 0 && pre_rho < 0) diff_rho = 1;
+                        else diff_rho = -1;
+                        if (diff_rho > 0) t_direction = 1;
+                        else if (diff_rho < 0) t_direction = -1;
+                        else t_direction = 0;
+                        if (!check_rho) {
+                            check_rho = 1;
+                            t_direction = -1;
+                        } else if (pre_rho != -1) {
+                            if (direction > 0 && t_direction < 0) {
+                                if (nr->result4 < 10) nr->result4 += 1;
+                                if (nr->result0 < q_result(pre_rho)) {
+                                    double temp_1 = dq(nr->result0);
+                                    nr->result0 = q_result(pre_rho);
+                                    float temp_2 = nr->result2;
+                                    nr->result2 = pre_height;
+                                    if (nr->result1 < q_result(temp_1)) {
+                                        nr->result1 = q_result(temp_1);
+                                        nr->result3 = temp_2;
+                                    }
+                                } else {
+                                    if (nr->result1 < q_result(pre_rho)) {
+                                        nr->result1 = q_result(pre_rho);
+                                        nr->result3 = pre_height;
+                                    }
+                                }
+                            }
+                        }
+                        pre_rho = temp_rho;
+                        pre_height = iter_height;
+                        direction = t_direction;
+                        if (max_WNCC < temp_rho) max_WNCC = temp_rho;
+                    }
+                }
+                if (nr->check_height_change)
+                    nr->max_WNCC = q_result(max_WNCC);
+            }
+        }
+        pv_free(&pv);
+        pv_free(&pvn);
+    }
+    return Accessable_grid;
+}
+
+/* -------------------------------------------------------- AWNCC peak */
+
+void orc_awncc_peak(const smx_level *Lv, const orc_voxels *vox, const smx_ugrid *GridPT3,
+                    smx_nccresult *nccresult, int iteration)
+{
+    /* setsm_code.cpp:7511-7702 with check_SGM_peak false */
+    const int Pyramid_step = Lv->level;
+    const double step_height = Lv->height_step;
+    int check_ortho;
+    if (Pyramid_step == 4 && iteration == 1) check_ortho = 0;
+    else check_ortho = 1;
+    if (Lv->pre_dem && !(Pyramid_step == 4 && iteration == 1)) check_ortho = 1;
+    const double ncc_alpha = orc_ncc_alpha(Pyramid_step, iteration, Lv->is_ra);
+    const double ncc_beta = 1.0 - ncc_alpha;
+    const long n = (long)Lv->grid_w * (long)Lv->grid_h;
+
+#pragma omp parallel for schedule(guided) num_threads(orc_get_threads())
+    for (long pt_index = 0; pt_index < n; pt_index++) {
+        smx_nccresult *nr = &nccresult[pt_index];
+        double pre_rho = -1.0, pre_rho_WNCC = -1.0;
+        float pre_height = 0.0f;
+        int direction = 0;
+        int check_rho = 0;
+        double max_roh = 0;
+        double temp_nccresult = -100.0, temp_nccresult_sec = -100.0;
+        (void)max_roh;
+
+        nr->result0 = q_result(-1.0);
+        nr->result2 = -1000;
+        nr->result1 = q_result(-1.0);
+        nr->result3 = -1000;
+        nr->result4 = 0;
+
+        for (long hs = 0; hs < nr->NumOfHeight; hs++) {
+            float iter_height = (float)(nr->minHeight + hs * step_height);
+            if (!(iter_height >= GridPT3[pt_index].minHeight && iter_height <= GridPT3[pt_index].maxHeight))
+                continue;
+            double WNCC_sum;
+            const double incc = dq(SMX_VOXEL_INCC(vox->rows[pt_index][hs]));
+            if (check_ortho && dq(nr->GNCC) > -1.0)
+                WNCC_sum = incc * ncc_alpha + dq(nr->GNCC) * ncc_beta;
+            else
+                WNCC_sum = incc;
+            double WNCC_temp_rho = WNCC_sum;
+            double temp_rho = WNCC_sum;
+
+            double diff_rho;
+            int t_direction;
+            if (temp_rho >= 0 && pre_rho >= 0) diff_rho = temp_rho - pre_rho;
+            else if (temp_rho < 0 && pre_rho < 0) diff_rho = -(fabs(temp_rho) - fabs(pre_rho));
+            else if (temp_rho >= 0 && pre_rho < 0) diff_rho = 1;
+            else diff_rho = -1;
+            if (diff_rho > 0) t_direction = 1;
+            else if (diff_rho < 0) t_direction = -1;
+            else t_direction = 0;
+
+            if (!check_rho) {
+                check_rho = 1;
+                t_direction = -1;
+            } else if (pre_rho != -1) {
+                int check_peak = (direction > 0 && t_direction < 0);
+                if ((iteration <= 3 && Pyramid_step == 4) || (iteration <= 2 && Pyramid_step == 3) ||
+                    (iteration <= 1 && Pyramid_step == 2)) {
+                    if (temp_nccresult < temp_rho) {
+                        if (nr->result4 < 10) nr->result4 += 1;
+                        temp_nccresult = temp_rho;
+                        nr->result2 = iter_height;
+                        temp_nccresult_sec = -1.0;
+                        nr->result3 = -9999;
+                        max_roh = WNCC_temp_rho;
+                    }
+                } else if (check_peak) {
+                    if (nr->result4 < 10) nr->result4 += 1;
+                    if (temp_nccresult < pre_rho) {
+                        double temp_1 = temp_nccresult;
+                        temp_nccresult = pre_rho;
+                        float temp_2 = nr->result2;
+                        nr->result2 = pre_height;
+                        max_roh = pre_rho_WNCC;
+                        if (temp_nccresult_sec < temp_1) {
+                            temp_nccresult_sec = temp_1;
+                            nr->result3 = temp_2;
+                        }
+                    } else {
+                        if (temp_nccresult_sec < pre_rho) {
+                            temp_nccresult_sec = pre_rho;
+                            nr->result3 = pre_height;
+                        }
+                    }
+                }
+            }
+            pre_rho = temp_rho;
+            pre_rho_WNCC = WNCC_temp_rho;
+            pre_height = iter_height;
+            direction = t_direction;
+        }
+        nr->result0 = q_result(temp_nccresult);
+        nr->result1 = q_result(temp_nccresult_sec);
+    }
+}
+
+/* ------------------------------------------------------------ pyramid */
+
+void orc_gauss_pyramid(const uint16_t *in, uint32_t w, uint32_t h, uint16_t *out)
+{
+    /* Template.hpp:45-78 CreateGaussianFilter(9, 1.5) + Template.hpp:82-132 */
+    const int fs = 9, half = 4;
+    const double sigma = 1.5;
+    double G[9][9];
+    double sum = 0;
+    double tmp = -1 / (2 * sigma * sigma);
+    double scale = 1 / (sqrt(2 * ORC_PI) * sigma);
+    for (int i = -half; i <= half; i++)
+        for (int j = -half; j <= half; j++) {
+            double temp = (i * i + j * j) * tmp;
+            double val = exp(temp) * scale;
+            G[i + half][j + half] = val;
+            sum += val;
+        }
+    for (int i = 0; i < fs; i++)
+        for (int j = 0; j < fs; j++)
+            G[i][j] /= sum;
+
+    const long ow = w / 2, oh = h / 2;
+#pragma omp parallel for schedule(guided) num_threads(orc_get_threads())
+    for (long r = 0; r < oh; r++) {
+        for (long c = 0; c < ow; c++) {
+            double temp_v = 0;
+            int count = 0;
+            for (int l = -half; l <= half; l++)
+                for (int k = -half; k <= half; k++) {
+                    if ((2 * r + l) >= 0 && (2 * c + k) >= 0 && (2 * r + l) < (long)h && (2 * c + k) < (long)w) {
+                        temp_v += G[l + half][k + half] * in[(2 * r + l) * (long)w + (2 * c + k)];
+                        count++;
+                    }
+                }
+            if (count == fs * fs)
+                out[r * ow + c] = (uint16_t)temp_v;
+            else
+                out[r * ow + c] = in[(2 * r) * (long)w + (2 * c)];
+        }
+    }
+}
+
+void orc_sobel(const uint16_t *src, uint32_t w, uint32_t h, uint16_t *mag, int16_t *dir)
+{
+    /* SubFunctions.cpp:2418-2470 */
+#pragma omp parallel for num_threads(orc_get_threads())
+    for (long i = 0; i < (long)h; i++) {
+        for (long j = 0; j < (long)w; j++) {
+            const long W = (long)w;
+            if (j == 0 || j == W - 1 || i == 0 || i == (long)h - 1) {
+                mag[i * W + j] = 0;
+                dir[i * W + j] = 0;
+            } else {
+                const int a00 = src[(i - 1) * W + (j - 1)], a01 = src[(i - 1) * W + j], a02 = src[(i - 1) * W + (j + 1)];
+                const int a10 = src[i * W + (j - 1)], a11 = src[i * W + j], a12 = src[i * W + (j + 1)];
+                const int a20 = src[(i + 1) * W + (j - 1)], a21 = src[(i + 1) * W + j], a22 = src[(i + 1) * W + (j + 1)];
+                (void)a11;
+                /* integer sums of the 3x3 masks, exactly as the int products in the reference */
+                double gx = (double)((-1 * a00 + 0 * a01 + 1 * a02) + (-2 * a10 + 0 * a11 + 2 * a12) + (-1 * a20 + 0 * a21 + 1 * a22));
+                double gy = (double)((1 * a00 + 2 * a01 + 1 * a02) + (0 * a10 + 0 * a11 + 0 * a12) + (-1 * a20 - 2 * a21 - 1 * a22));
+                double t = sqrt(gx * gx + gy * gy);
+                dir[i * W + j] = (int16_t)(floor(atan2(gy, gx) * 180 / ORC_PI));
+                mag[i * W + j] = (uint16_t)(t + 0.5);
+            }
+        }
+    }
+}
+
+static const int orc_numcols[7] = {6, 5, 5, 5, 4, 3, 0};   /* SubFunctions.cpp:8 */
+
+void orc_orientation(const uint16_t *Gmag, const int16_t *Gdir, uint32_t w, uint32_t h, int template_size, uint8_t *plhs)
+{
+    /* SubFunctions.cpp:2473-2571. The column range starts at
+     * max(-mask_row, -dcol) exactly as written in the reference; a linear
+     * index that falls before the first pixel reads as magnitude 0. */
+    const int half = template_size / 2;
+    const long main_col = w, main_row = h;
+    double gw[13][13];
+    for (int row = -half + 1; row <= half - 1; row++)
+        for (int col = -half + 1; col <= half - 1; col++)
+            gw[half - 1 + row][half - 1 + col] = exp(-(double)(row * row + col * col) / 4.5);
+    const double SUB_RATIO = 4.6;   /* SubFunctions.hpp:33 */
+
+#pragma omp parallel for collapse(2) num_threads(orc_get_threads())
+    for (long mask_row = 0; mask_row < main_row; mask_row++) {
+        for (long mask_col = 0; mask_col < main_col; mask_col++) {
+            double bin[18] = {0.0};
+            double sub_bin[4] = {0.0};
+            long rlo = -mask_row > -half + 1 ? -mask_row : -half + 1;
+            long rhi = main_row - mask_row - 1 < half - 1 ? main_row - mask_row - 1 : half - 1;
+            for (long row = rlo; row <= rhi; row++) {
+                int dcol = orc_numcols[labs(row)];
+                long clo = -mask_row > -dcol ? -mask_row : -dcol;
+                long chi = dcol < main_col - mask_col - 1 ? dcol : main_col - mask_col - 1;
+                for (long col = clo; col <= chi; col++) {
+                    double gu_weight = gw[half - 1 + row][half - 1 + col];
+                    long idx = (mask_row + row) * main_col + (mask_col + col);
+                    double mag = idx >= 0 ? (double)Gmag[idx] : 0.0;
+                    double theta = idx >= 0 ? (double)Gdir[idx] : 0.0;
+                    if (theta < 0) theta += 360;
+                    else if (theta > 360) theta -= 360;
+                    long index = (long)(theta / 20.0);
+                    if (index >= 0 && index < 18) {
+                        bin[index] += mag * gu_weight;
+                        sub_bin[(int)(index / SUB_RATIO)] += mag * gu_weight;
+                    }
+                }
+            }
+            double max_th = -100;
+            int sub_th = 0, th = 0;
+            for (int i = 0; i < 4; i++)
+                if (sub_bin[i] > max_th) { max_th = sub_bin[i]; sub_th = i; }
+            max_th = -100;
+            for (int i = (int)(sub_th * SUB_RATIO); i < (int)(sub_th * SUB_RATIO + SUB_RATIO); i++)
+                if (bin[i] > max_th) { max_th = bin[i]; th = i; }
+            plhs[mask_row * main_col + mask_col] = (uint8_t)th;
+        }
+    }
+}

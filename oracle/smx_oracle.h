@@ -1,0 +1,84 @@
+/*
+ * smx_oracle.h — CPU FP64 restatement of SETSM's VerticalLineLocus hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker that the MI355X
+ * kernels are compared against and the CPU baseline that bench.py times. It is
+ * never linked into, loaded by or called from the product library; only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+ *
+ * Every function names the reference file:line it restates.
+ */
+#ifndef SMX_ORACLE_H
+#define SMX_ORACLE_H
+
+#include "smx_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Per-cell voxel rows, the oracle's model of SETSM's VOXEL **grid_voxel
+ * (setsm_code.cpp:3745, rows calloc'd in InitializeVoxel). */
+typedef struct orc_voxels {
+    long      ncells;
+    uint32_t **rows;     /* rows[c] has nccresult[c].NumOfHeight words (SMX_VOXEL_WORD) */
+    uint16_t  *len;      /* allocated length of rows[c] */
+    double   **rho;      /* optional per-hypothesis mean NCC before quantisation        */
+} orc_voxels;
+
+orc_voxels *orc_voxels_create(long ncells, int keep_rho);
+void        orc_voxels_destroy(orc_voxels *v);
+/* Pack rows into CSR order (offsets = prefix sum of NumOfHeight). Returns the
+ * number of words written. rho_out may be NULL. */
+long        orc_voxels_export(const orc_voxels *v, const smx_nccresult *ncc,
+                              uint32_t *words_out, double *rho_out);
+/* Replace all rows from CSR (used to seed a state). */
+void        orc_voxels_import(orc_voxels *v, const smx_nccresult *ncc, const uint32_t *words);
+
+/* CoordConversion.hpp:26-87 GetObjectToImageRPC_single */
+void   orc_rpc_ground_to_image(const smx_rpc *rpc, const double adjust[2],
+                               double lon, double lat, double h,
+                               double *img_x, double *img_y);
+/* setsm_code.cpp:5741-5757 GetHeightStep */
+double orc_height_step(int level, double resolution);
+/* setsm_code.cpp:6057-6077 — the Half_template_size VerticalLineLocus uses */
+int    orc_vll_half_template(int level, int template_size, double resolution);
+/* SubFunctions.cpp:3274-3312 — nested-window membership bits of an offset */
+int    orc_window_bits(int half, int row, int col);
+/* SubFunctions.cpp:974-1018 Correlate */
+double orc_correlate(const double *L, const double *R, int N);
+/* setsm_code.cpp:11002-11083 check_image_boundary (SB sensor, 2 images) */
+int    orc_check_image_boundary(const smx_level *L, long cell, double minH, double maxH, int half);
+/* setsm_code.cpp:6014-6047 SetNCC_alpha */
+double orc_ncc_alpha(int level, int iteration, int is_ra);
+
+/* setsm_code.cpp:5759-6012 InitializeVoxel */
+void orc_initialize_voxel(const smx_level *L, orc_voxels *vox, smx_ugrid *gridpt3,
+                          smx_nccresult *ncc, int iteration, const double minmax[2]);
+
+/* setsm_code.cpp:6049-6548 VerticalLineLocus (INCC term; the ortho GNCC term is
+ * computed when gncc_coords is non-NULL, see orc_set_ortho_coords).
+ * Returns Accessable_grid. OpenMP dynamic,1 over cells like the reference. */
+long orc_vertical_line_locus(const smx_level *L, orc_voxels *vox, smx_nccresult *ncc,
+                             const smx_ugrid *gridpt3, int iteration, const double minmax[2]);
+
+/* setsm_code.cpp:7511-7702 AWNCC peak scan without SGM aggregation
+ * (check_SGM false, i.e. Pyramid_step > SGM_py). */
+void orc_awncc_peak(const smx_level *L, const orc_voxels *vox, const smx_ugrid *gridpt3,
+                    smx_nccresult *ncc, int iteration);
+
+/* Pyramid inputs. Template.hpp:45-132 (9x9, sigma 1.5 decimation),
+ * SubFunctions.cpp:2418-2470 (Sobel), SubFunctions.cpp:2473-2571 (Orientation). */
+void orc_gauss_pyramid(const uint16_t *in, uint32_t w, uint32_t h, uint16_t *out);
+void orc_sobel(const uint16_t *in, uint32_t w, uint32_t h, uint16_t *mag, int16_t *dir);
+void orc_orientation(const uint16_t *mag, const int16_t *dir, uint32_t w, uint32_t h,
+                     int template_size, uint8_t *ori);
+
+/* Threads used by the OpenMP loops (0 = OpenMP default). */
+void orc_set_threads(int n);
+int  orc_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

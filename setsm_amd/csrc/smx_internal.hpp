@@ -1,0 +1,96 @@
+// smx_internal.hpp — host-side job state and the launchers each kernel file
+// exports to smx_api.cpp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <utility>
+#include <vector>
+
+#include "smx_kernels.hpp"
+#include "smx_vll.h"
+
+namespace smx {
+
+struct LevelView {
+    uint32_t *texel = nullptr;   // intensity | magnitude << 16
+    uint8_t *ori = nullptr;
+    uint32_t w = 0, h = 0;
+    double sx = 0, sy = 0;
+};
+
+}  // namespace smx
+
+struct smx_job {
+    int device = 0;
+    int num_levels = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int64_t *h_small = nullptr;                  // pinned readback slots
+    std::vector<smx::LevelView> views[2];
+    long long device_bytes = 0;
+    long long budget = 0;                        // 0 = unlimited
+    std::vector<std::pair<void *, size_t>> sizes;
+
+    // current level
+    bool level_set = false;
+    bool searchspace_set = false;
+    smx_level lv{};
+    smx::VllParams hp{};
+    smx::VllParams *d_params = nullptr;
+    long ncells = 0;
+    double minmax[2] = {0, 0};
+    int16_t max_ortho_ncc = 0;
+    double *d_grid_wgs = nullptr;
+    smx_ugrid *d_gridpt3 = nullptr;
+    smx_nccresult *d_ncc = nullptr;
+    int64_t *d_vox_off = nullptr;
+    uint32_t *d_vox = nullptr;
+    float *d_rho = nullptr;
+    long long vox_total = 0;
+
+    // scratch of the launch sequences
+    int64_t *d_run_count = nullptr, *d_run_off = nullptr;
+    int2 *d_runs = nullptr;
+    long runs_cap = 0;
+    unsigned long long *d_max_bits = nullptr;
+    unsigned long long *d_counters = nullptr;
+    uint8_t *d_action = nullptr;
+    int64_t *d_new_len = nullptr, *d_new_off = nullptr;
+    void *scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+
+    // fast-kernel launch shape (autotuned)
+    int fast_cfg = -1;
+
+    double last_ms = 0;
+    long long last_evaluated = 0;
+};
+
+namespace smx {
+
+int fail(int code, const char *fmt, ...);
+int job_alloc(smx_job *j, void **p, size_t bytes);
+void job_free(smx_job *j, void *p);
+double ncc_alpha(int Pyramid_step, int iteration, int IsRA);
+int vll_half_template(int level, int template_size, double resolution);
+
+hipError_t scan_i64(const int64_t *in, int64_t *out, long n, void *tmp, size_t &tmp_bytes, hipStream_t s);
+hipError_t launch_gauss_decimate(const uint16_t *in, uint32_t w, uint32_t h, uint16_t *out, hipStream_t s);
+hipError_t launch_sobel(const uint16_t *in, uint32_t w, uint32_t h, uint16_t *mag, int16_t *dir, hipStream_t s);
+hipError_t launch_orientation(const uint16_t *mag, const int16_t *dir, uint32_t w, uint32_t h, uint8_t *ori,
+                              hipStream_t s);
+hipError_t launch_pack(const uint16_t *img, const uint16_t *mag, uint32_t *texel, long n, hipStream_t s);
+hipError_t launch_unpack(const uint32_t *texel, uint16_t *img, uint16_t *mag, long n, hipStream_t s);
+hipError_t launch_dir_probe(const int *gx, const int *gy, int16_t *out, long n, hipStream_t s);
+hipError_t launch_vll_gate(const VllParams *dP, const CellArgs &A, hipStream_t s);
+hipError_t launch_vll_fill_runs(const CellArgs &A, hipStream_t s);
+hipError_t launch_vll_fp64(int half, const VllParams *dP, const CellArgs &A, hipStream_t s);
+hipError_t launch_vll_finalize(const CellArgs &A, hipStream_t s);
+hipError_t launch_init_voxel(const VllParams *dP, const InitArgs &A, hipStream_t s);
+hipError_t launch_relayout(const InitArgs &A, hipStream_t s);
+hipError_t launch_awncc(const VllParams *dP, const PeakArgs &A, hipStream_t s);
+int launch_vll_fast(smx_job *j, int half, const CellArgs &A);
+
+}  // namespace smx

@@ -1,0 +1,151 @@
+// smx_kernels.hpp — argument blocks and per-hypothesis helpers shared by the
+// VerticalLineLocus kernels and their host launchers.
+#pragma once
+
+#include "smx_device.hpp"
+
+namespace smx {
+
+constexpr int kRun = 64;   // heights per run = one wavefront of hypotheses
+
+struct CellArgs {
+    long ncells;
+    const smx_ugrid *gridpt3;
+    smx_nccresult *ncc;
+    const double *grid_wgs;
+    const int64_t *vox_off;        // CSR row offsets (ncells + 1)
+    uint32_t *vox;                 // VOXEL words
+    float *rho;                    // optional mean NCC per hypothesis
+    int64_t *run_count;            // per cell (ncells + 1)
+    const int64_t *run_offset;     // exclusive prefix sum of run_count
+    int2 *runs;                    // (cell, first height)
+    long nruns;
+    unsigned long long *max_bits;  // per cell, order_key of max_WNCC
+    unsigned long long *accessible;
+    unsigned long long *evaluated;
+};
+
+struct InitArgs {
+    long ncells;
+    smx_nccresult *ncc;
+    smx_ugrid *gridpt3;
+    const double *grid_wgs;
+    uint8_t *action;
+    int64_t *new_len;              // ncells + 1
+    const int64_t *old_off;
+    const int64_t *new_off;
+    const uint32_t *old_vox;
+    uint32_t *new_vox;
+    const float *old_rho;
+    float *new_rho;
+};
+
+struct PeakArgs {
+    long ncells;
+    smx_nccresult *ncc;
+    const smx_ugrid *gridpt3;
+    const int64_t *vox_off;
+    const uint32_t *vox;
+};
+
+// SubFunctions.cpp:3274-3312 plus the outer disc test of setsm_code.cpp:6434:
+// bit 0 = 11x11 disc, bit 1 = middle window, bit 2 = inner window.
+__host__ __device__ constexpr int window_bits_ce(int half, int row, int col)
+{
+    const int r2 = row * row + col * col;
+    const int s1 = half / 2;
+    const int s2 = s1 + (int)((s1 / 2.0) + 0.5);
+    return (row < -half || row > half || col < -half || col > half || r2 > (half + 1) * (half + 1))
+               ? 0
+               : (1 | ((r2 <= (half - s1 + 1) * (half - s1 + 1) && row >= -half + s1 && row <= half - s1 &&
+                        col >= -half + s1 && col <= half - s1)
+                           ? 2
+                           : 0) |
+                  ((r2 <= (half - s2 + 1) * (half - s2 + 1) && row >= -half + s2 && row <= half - s2 &&
+                    col >= -half + s2 && col <= half - s2)
+                       ? 4
+                       : 0));
+}
+
+__device__ __forceinline__ unsigned long long order_key(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double order_value(unsigned long long k)
+{
+    const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double((long long)b);
+}
+
+// Everything about one (cell, height) hypothesis before the window loop.
+struct Hyp {
+    double rpx, rpy, tpx, tpy;     // pyramid coordinates, reference / target
+    double rnx, rny, tnx, tny;     // same at level-1 (combined INCC)
+    double cos0, sin0;
+    float height;
+    bool orient_ok;
+};
+
+// setsm_code.cpp:6329-6428 for one lane: float height, RPC pair, pyramid
+// coordinates, image test, orientation difference and rotation. Returns false
+// when the hypothesis is skipped without touching the cube.
+__device__ __forceinline__ bool project_hypothesis(const VllParams &P, const CellArgs &A, long cell, int hidx,
+                                                   const smx_nccresult &nr, int start_H, int end_H, Hyp &hy)
+{
+#pragma clang fp contract(off)
+    if (hidx >= (int)nr.NumOfHeight) return false;
+    const float iter_height = (float)(nr.minHeight + hidx * P.height_step);
+    if (!(iter_height >= start_H && iter_height <= end_H)) return false;
+    hy.height = iter_height;
+    const double lon = A.grid_wgs[2 * cell], lat = A.grid_wgs[2 * cell + 1];
+    double rix, riy, tix, tiy;
+    rpc_ground_to_image(P.rpc[0], P.adjust[0], lon, lat, (double)iter_height, rix, riy);
+    rpc_ground_to_image(P.rpc[1], P.adjust[1], lon, lat, (double)iter_height, tix, tiy);
+    to_pyramid(rix, riy, P.view[0], P.level, hy.rpx, hy.rpy);
+    to_pyramid(tix, tiy, P.view[1], P.level, hy.tpx, hy.tpy);
+    if (!(py_point_inside(hy.rpx, hy.rpy, P.view[0]) && py_point_inside(hy.tpx, hy.tpy, P.view[1]))) return false;
+    int diff2;
+    const int od = (int)P.view[0].ori[(size_t)(int)hy.rpy * P.view[0].width + (int)hy.rpx] -
+                   (int)P.view[1].ori[(size_t)(int)hy.tpy * P.view[1].width + (int)hy.tpx];
+    if (P.combined_incc) {
+        to_pyramid(rix, riy, P.next[0], P.level - 1, hy.rnx, hy.rny);
+        to_pyramid(tix, tiy, P.next[1], P.level - 1, hy.tnx, hy.tny);
+        if (!(py_point_inside(hy.rnx, hy.rny, P.next[0]) && py_point_inside(hy.tnx, hy.tny, P.next[1])))
+            return false;
+        const int odn = (int)P.next[0].ori[(size_t)(int)hy.rny * P.next[0].width + (int)hy.rnx] -
+                        (int)P.next[1].ori[(size_t)(int)hy.tny * P.next[1].width + (int)hy.tnx];
+        diff2 = od + odn;
+    } else {
+        hy.rnx = hy.rny = hy.tnx = hy.tny = 0.0;
+        diff2 = 2 * od;
+    }
+    const double diff_theta = P.combined_incc ? ((double)od + (double)(diff2 - od)) / 2.0 : (double)od;
+    if (P.is_ra != 1)
+        hy.orient_ok = P.level <= 2 || (diff_theta * P.bin_angle < 90 && diff_theta * P.bin_angle > -90);
+    else
+        hy.orient_ok = true;
+    hy.cos0 = P.rot_cos[diff2 + 34];
+    hy.sin0 = P.rot_sin[diff2 + 34];
+    return true;
+}
+
+// Wave-wide max of temp_rho into the cell's max_WNCC key and a count of the
+// hypotheses that computed an NCC. Every lane of the wave must call it.
+__device__ __forceinline__ void wave_max_commit(double v, unsigned evaluated, unsigned long long *cell_key,
+                                                unsigned long long *counter)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const double o = __shfl_xor(v, m, 64);
+        v = o > v ? o : v;
+    }
+    const unsigned long long any = __ballot(evaluated != 0);
+    if ((threadIdx.x & 63) == 0 && any) {
+        atomicMax(cell_key, order_key(v));
+        atomicAdd(counter, (unsigned long long)__popcll(any));
+    }
+}
+
+}  // namespace smx

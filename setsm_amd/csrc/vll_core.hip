@@ -1,0 +1,560 @@
+// vll_core.hip — per-cell gating, the plain FP64 VerticalLineLocus kernel,
+// InitializeVoxel with the voxel-cube relayout, and the AWNCC peak scan.
+//
+// Built with -ffp-contract=off: every expression is evaluated in the operand
+// order of the reference C++ so the FP64 kernel's INCC cube is bit-identical
+// to SETSM's CPU path on the same inputs.
+//
+// Work decomposition: the cube is CSR (rows of NumOfHeight words per cell).
+// The gate kernel marks cells whose height list must be evaluated; a prefix
+// sum turns them into a flat list of (cell, first height) runs of 64 heights;
+// one wavefront evaluates one run, one lane per height hypothesis.
+
+#include "smx_kernels.hpp"
+
+namespace smx {
+
+// ---------------------------------------------------------------- gating
+
+__device__ __forceinline__ bool gate_cell(const VllParams *__restrict__ P, CellArgs &A, long c);
+
+// setsm_code.cpp:6157-6204 + 6259-6279: which cells enter the INCC loop.
+__global__ void __launch_bounds__(256) k_vll_gate(const VllParams *__restrict__ P, CellArgs A)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool accessible = false;
+    if (c < A.ncells) accessible = gate_cell(P, A, c);
+    const unsigned long long any = __ballot(accessible);
+    if ((threadIdx.x & 63) == 0 && any) atomicAdd(A.accessible, (unsigned long long)__popcll(any));
+}
+
+__device__ __forceinline__ bool gate_cell(const VllParams *__restrict__ P, CellArgs &A, long c)
+{
+    bool accessible = false;
+    const smx_ugrid g = A.gridpt3[c];
+    smx_nccresult *nr = &A.ncc[c];
+    int runs = 0;
+    const double lon = A.grid_wgs[2 * c], lat = A.grid_wgs[2 * c + 1];
+    const int start_H = g.minHeight, end_H = g.maxHeight;
+    if (check_image_boundary(P->rpc, P->adjust, P->view, P->level, lon, lat, (double)start_H, (double)end_H, P->half)) {
+        bool blunder = false;
+        if (P->level >= 2)
+            blunder = false;
+        else if (g.Matched_flag == 0)
+            blunder = true;
+        else
+            blunder = !(abs(g.maxHeight - g.minHeight) < P->th_height);
+        if (P->level <= 1 && (g.maxHeight > P->minmax[1] || g.minHeight < P->minmax[0]))
+            blunder = true;
+        if (P->check_matchtag && P->iteration <= 2) {
+            blunder = false;
+            if (g.maxHeight > P->minmax[1] || g.minHeight < P->minmax[0])
+                blunder = true;
+            else if (abs(g.maxHeight - g.minHeight) > P->th_height)
+                blunder = true;
+        }
+        if (!blunder) {
+            accessible = true;
+            nr->GNCC = q_result(-1.0);   // ortho term inactive (checked on the host)
+            if (nr->check_height_change || P->dem_lt2) {
+                nr->result0 = q_result(-1.0);
+                nr->result2 = -1000;
+                nr->result1 = q_result(-1.0);
+                nr->result3 = -1000;
+                nr->result4 = 0;
+                A.max_bits[c] = order_key(-100.0);
+                runs = (nr->NumOfHeight + kRun - 1) / kRun;
+            }
+        }
+    }
+    A.run_count[c] = runs;
+    return accessible;
+}
+
+__global__ void __launch_bounds__(256) k_vll_fill_runs(CellArgs A)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= A.ncells) return;
+    const int n = A.run_count[c];
+    const long base = A.run_offset[c];
+    for (int i = 0; i < n; i++)
+        A.runs[base + i] = make_int2((int)c, i * kRun);
+}
+
+// setsm_code.cpp:6509-6510: max_WNCC of cells whose heights were evaluated.
+__global__ void __launch_bounds__(256) k_vll_finalize(CellArgs A)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= A.ncells) return;
+    if (A.run_count[c] > 0 && A.ncc[c].check_height_change)
+        A.ncc[c].max_WNCC = q_result(order_value(A.max_bits[c]));
+}
+
+// ------------------------------------------------------ FP64 plain kernel
+
+// SubFunctions.cpp:3317-3326 InterpolatePatch on both channels of a packed
+// texel quad.
+__device__ __forceinline__ void interp2(const uint32_t *T, long pos, uint32_t w, double dx, double dy, double &v,
+                                        double &m)
+{
+    const uint32_t a = T[pos], b = T[pos + 1], c = T[pos + w], d = T[pos + 1 + w];
+    v = (double)(a & 0xffffu) * (1 - dx) * (1 - dy) + (double)(b & 0xffffu) * dx * (1 - dy) +
+        (double)(c & 0xffffu) * (1 - dx) * dy + (double)(d & 0xffffu) * dx * dy;
+    m = (double)(a >> 16) * (1 - dx) * (1 - dy) + (double)(b >> 16) * dx * (1 - dy) + (double)(c >> 16) * (1 - dx) * dy +
+        (double)(d >> 16) * dx * dy;
+}
+
+// SubFunctions.cpp:3249-3273: bounds test, then the four bilinear samples;
+// the sample is kept only if both intensities are > 0.
+__device__ __forceinline__ bool sample_fp64(const DevView &Lv, const DevView &Rv, double lx, double ly, double rx,
+                                            double ry, double &l, double &lm, double &r, double &rm)
+{
+    if (ry >= 0 && ry + 1 < Rv.height && rx >= 0 && rx + 1 < Rv.width && ly >= 0 && ly + 1 < Lv.height && lx >= 0 &&
+        lx + 1 < Lv.width) {
+        const long pl = (long)lx + (long)ly * (long)Lv.width;
+        const long pr = (long)rx + (long)ry * (long)Rv.width;
+        interp2(Lv.texel, pl, Lv.width, lx - floor(lx), ly - floor(ly), l, lm);
+        interp2(Rv.texel, pr, Rv.width, rx - floor(rx), ry - floor(ry), r, rm);
+        return l > 0 && r > 0;
+    }
+    return false;
+}
+
+// SubFunctions.cpp:974-1018 Correlate from the two passes' sums.
+__device__ __forceinline__ double corr_from(double sLR, double sL2, double sR2)
+{
+    if (sL2 > 1e-8 && sR2 > 1e-8) return sLR / (sqrt(sL2 * sR2));
+    return -99.0;
+}
+
+// One level's multi-window NCC for one hypothesis: SetVecKernelValue over the
+// window (setsm_code.cpp:6429-6453) and ComputeMultiNCC (SubFunctions.cpp:
+// 3328-3359), two passes like Correlate. Returns true if all 3 counts > 0.
+template <int H>
+__device__ __forceinline__ bool multi_ncc_fp64(const DevView &Lv, const DevView &Rv, double rpx, double rpy, double tpx,
+                                               double tpy, double cos0, double sin0, double &sum_multi,
+                                               double &count_multi)
+{
+    double sL[3] = {0, 0, 0}, sLm[3] = {0, 0, 0}, sR[3] = {0, 0, 0}, sRm[3] = {0, 0, 0};
+    int cnt[3] = {0, 0, 0};
+#pragma unroll
+    for (int row = -H; row <= H; row++) {
+#pragma unroll
+        for (int col = -H; col <= H; col++) {
+            const int bits = window_bits_ce(H, row, col);
+            if (bits) {
+                const double lx = rpx + col, ly = rpy + row;
+                const double ox = cos0 * col - sin0 * row, oy = sin0 * col + cos0 * row;
+                double l, lm, r, rm;
+                if (sample_fp64(Lv, Rv, lx, ly, tpx + ox, tpy + oy, l, lm, r, rm)) {
+#pragma unroll
+                    for (int k = 0; k < 3; k++)
+                        if (bits & (1 << k)) {
+                            sL[k] += l; sLm[k] += lm; sR[k] += r; sRm[k] += rm; cnt[k]++;
+                        }
+                }
+            }
+        }
+    }
+    if (!(cnt[0] > 0 && cnt[1] > 0 && cnt[2] > 0)) return false;
+    double mL[3], mLm[3], mR[3], mRm[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        mL[k] = sL[k] / cnt[k]; mR[k] = sR[k] / cnt[k];
+        mLm[k] = sLm[k] / cnt[k]; mRm[k] = sRm[k] / cnt[k];
+    }
+    double LR[3] = {0, 0, 0}, L2[3] = {0, 0, 0}, R2[3] = {0, 0, 0};
+    double mLR[3] = {0, 0, 0}, mL2[3] = {0, 0, 0}, mR2[3] = {0, 0, 0};
+#pragma unroll
+    for (int row = -H; row <= H; row++) {
+#pragma unroll
+        for (int col = -H; col <= H; col++) {
+            const int bits = window_bits_ce(H, row, col);
+            if (bits) {
+                const double lx = rpx + col, ly = rpy + row;
+                const double ox = cos0 * col - sin0 * row, oy = sin0 * col + cos0 * row;
+                double l, lm, r, rm;
+                if (sample_fp64(Lv, Rv, lx, ly, tpx + ox, tpy + oy, l, lm, r, rm)) {
+#pragma unroll
+                    for (int k = 0; k < 3; k++)
+                        if (bits & (1 << k)) {
+                            LR[k] += (l - mL[k]) * (r - mR[k]);
+                            L2[k] += (l - mL[k]) * (l - mL[k]);
+                            R2[k] += (r - mR[k]) * (r - mR[k]);
+                            mLR[k] += (lm - mLm[k]) * (rm - mRm[k]);
+                            mL2[k] += (lm - mLm[k]) * (lm - mLm[k]);
+                            mR2[k] += (rm - mRm[k]) * (rm - mRm[k]);
+                        }
+                }
+            }
+        }
+    }
+    double temp_roh = 0, count_roh = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double ncc = corr_from(LR[k], L2[k], R2[k]);
+        if (ncc != -99) { count_roh++; temp_roh += ncc; }
+        const double ncm = corr_from(mLR[k], mL2[k], mR2[k]);
+        if (ncm != -99) { count_roh++; temp_roh += ncm; }
+    }
+    if (count_roh > 0) {
+        sum_multi += temp_roh / count_roh;
+        count_multi++;
+    }
+    return true;
+}
+
+template <int H>
+__global__ void __launch_bounds__(256) k_vll_fp64(const VllParams *__restrict__ P, CellArgs A)
+{
+    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= A.nruns) return;
+    const int2 run = A.runs[wave];
+    const long cell = run.x;
+    const int hidx = run.y + lane;
+    const smx_nccresult nr = A.ncc[cell];
+    const smx_ugrid g = A.gridpt3[cell];
+    const int start_H = g.minHeight, end_H = g.maxHeight;
+
+    Hyp hy;
+    bool act = project_hypothesis(*P, A, cell, hidx, nr, start_H, end_H, hy);
+    double temp_rho = -100.0;
+    unsigned evaluated = 0;
+    if (act) {
+        const long wi = A.vox_off[cell] + hidx;
+        uint32_t word = A.vox[wi];
+        if (!hy.orient_ok) {
+            A.vox[wi] = word & ~0xffu;   // flag_cal = false (setsm_code.cpp:6503-6506)
+        } else {
+            double sum_multi = 0, count_multi = 0;
+            const bool all3 = multi_ncc_fp64<H>(P->view[0], P->view[1], hy.rpx, hy.rpy, hy.tpx, hy.tpy, hy.cos0,
+                                                hy.sin0, sum_multi, count_multi);
+            if (P->combined_incc)
+                multi_ncc_fp64<H>(P->next[0], P->next[1], hy.rnx, hy.rny, hy.tnx, hy.tny, hy.cos0, hy.sin0,
+                                  sum_multi, count_multi);
+            const double rho = count_multi > 0 ? sum_multi / count_multi : -1.0;
+            word = (word & 0xffu) | (all3 ? 1u : 0u);
+            word |= (uint32_t)(uint16_t)q_voxel(rho) << 16;
+            A.vox[wi] = word;
+            if (A.rho) A.rho[wi] = (float)rho;
+            temp_rho = rho;
+            evaluated = 1;
+        }
+    }
+    wave_max_commit(temp_rho, evaluated, &A.max_bits[cell], A.evaluated);
+}
+
+template __global__ void k_vll_fp64<5>(const VllParams *, CellArgs);
+template __global__ void k_vll_fp64<6>(const VllParams *, CellArgs);
+template __global__ void k_vll_fp64<7>(const VllParams *, CellArgs);
+
+// ------------------------------------------------------ InitializeVoxel
+
+// setsm_code.cpp:5759-6012, one thread per cell. Writes the new NCCresult
+// fields, possibly rewrites GridPT3's band, and leaves an action per cell:
+// 0 = no row, 1 = keep the old row, 2 = fresh row (flag 1, INCC -1000).
+__global__ void __launch_bounds__(256) k_init_voxel(const VllParams *__restrict__ P, InitArgs A)
+{
+    const long t_i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t_i >= A.ncells) return;
+    smx_nccresult nr = A.ncc[t_i];
+    smx_ugrid g = A.gridpt3[t_i];
+    const int iteration = P->iteration;
+    const int pyramid_step = P->level;
+    const double height_step = P->height_step;
+    const double *minmaxHeight = P->minmax;
+    int action = nr.NumOfHeight > 0 ? 1 : 0;
+    const double lon = A.grid_wgs[2 * t_i], lat = A.grid_wgs[2 * t_i + 1];
+
+    auto drop = [&]() {
+        nr.NumOfHeight = 0;
+        action = 0;
+    };
+
+    if (check_image_boundary(P->rpc, P->adjust, P->view, pyramid_step, lon, lat, (double)g.minHeight,
+                             (double)g.maxHeight, 7)) {
+        int change_step_min = 0, change_step_max = 0;
+        bool check_blunder_cell = true;
+        const double th_height = P->th_height;
+        if (P->check_matchtag || iteration == 1) {
+            nr.minHeight = g.minHeight;
+            nr.maxHeight = g.maxHeight;
+            nr.GNCC = q_result(-1.0);
+            nr.check_height_change = 1;
+            check_blunder_cell = false;
+        } else {
+            if (pyramid_step >= 2)
+                check_blunder_cell = false;
+            else if (g.Matched_flag != 0) {
+                if (g.maxHeight - g.minHeight > 0) {
+                    if (pyramid_step <= 1) {
+                        if (iteration > 1)
+                            check_blunder_cell = !(g.maxHeight <= minmaxHeight[1] && g.minHeight >= minmaxHeight[0] &&
+                                                   nr.maxHeight <= minmaxHeight[1] && nr.minHeight >= minmaxHeight[0]);
+                        else
+                            check_blunder_cell = !(g.maxHeight <= minmaxHeight[1] && g.minHeight >= minmaxHeight[0]);
+                    }
+                    if (pyramid_step == 1) check_blunder_cell = !(abs(g.maxHeight - g.minHeight) < 1000);
+                    if (pyramid_step == 0) {
+                        if (iteration > 1)
+                            check_blunder_cell = !((abs(nr.maxHeight - nr.minHeight) < th_height) &&
+                                                   (abs(g.maxHeight - g.minHeight) < th_height));
+                        else {
+                            if (abs(g.maxHeight - g.minHeight) < th_height) check_blunder_cell = false;
+                            else nr.NumOfHeight = 0;
+                        }
+                    }
+                } else {
+                    drop();
+                    check_blunder_cell = true;
+                }
+            } else {
+                drop();
+                check_blunder_cell = true;
+            }
+        }
+
+        if (!check_blunder_cell) {
+            if (iteration == 1 || (pyramid_step == 0 && P->dem_lt2)) {
+                nr.minHeight = g.minHeight;
+                nr.maxHeight = g.maxHeight;
+                nr.GNCC = q_result(-1.0);
+                nr.check_height_change = 1;
+                if (pyramid_step == 1) {
+                    if (abs((int)g.maxHeight - (int)g.minHeight) > 1000) {
+                        g.maxHeight = -100; g.minHeight = -100;
+                        nr.check_height_change = 0;
+                    }
+                } else if (pyramid_step == 0) {
+                    if (abs((int)g.maxHeight - (int)g.minHeight) > th_height) {
+                        g.maxHeight = -100; g.minHeight = -100;
+                        nr.check_height_change = 0;
+                    }
+                }
+            } else {
+                const int NumberofHeightVoxel = (int)((g.maxHeight - g.minHeight) / height_step);
+                if (nr.NumOfHeight == 0 && NumberofHeightVoxel > 0) {
+                    change_step_min = 0;
+                    change_step_max = 0;
+                    nr.check_height_change = 1;
+                } else {
+                    if (nr.minHeight > g.minHeight || nr.maxHeight < g.maxHeight) {
+                        if (nr.minHeight > g.minHeight)
+                            change_step_min = (int)((nr.minHeight - g.minHeight) / height_step + 0.5);
+                        if (nr.maxHeight < g.maxHeight)
+                            change_step_max = (int)((g.maxHeight - nr.maxHeight) / height_step + 0.5);
+                        nr.minHeight = (int16_t)floor(nr.minHeight - change_step_min * height_step);
+                        nr.maxHeight = (int16_t)ceil(nr.maxHeight + change_step_max * height_step);
+                        nr.check_height_change = 1;
+                        if (pyramid_step == 0 && abs(nr.maxHeight - nr.minHeight) > th_height)
+                            nr.check_height_change = 0;
+                    } else
+                        nr.check_height_change = 0;
+                }
+            }
+            if (nr.check_height_change) {
+                nr.minHeight = (int16_t)floor(nr.minHeight - change_step_min * height_step);
+                nr.maxHeight = (int16_t)ceil(nr.maxHeight + change_step_max * height_step);
+                const int NumberofHeightVoxel = (int)((float)(nr.maxHeight - nr.minHeight) / height_step);
+                if (NumberofHeightVoxel > 0) {
+                    nr.NumOfHeight = (uint16_t)NumberofHeightVoxel;
+                    action = 2;
+                } else {
+                    drop();
+                    nr.check_height_change = 0;
+                }
+            }
+        } else {
+            drop();
+            nr.check_height_change = 0;
+        }
+    } else {
+        drop();
+        nr.check_height_change = 0;
+    }
+    if (nr.minHeight == 0 || nr.maxHeight == 0) {
+        drop();
+        nr.check_height_change = 0;
+    }
+    if (pyramid_step == 0 && nr.NumOfHeight > 1000) {
+        drop();
+        nr.check_height_change = 0;
+    }
+    if (nr.NumOfHeight == 0) action = 0;
+    A.ncc[t_i] = nr;
+    A.gridpt3[t_i] = g;
+    A.action[t_i] = (uint8_t)action;
+    A.new_len[t_i] = nr.NumOfHeight;
+}
+
+// Re-lays the cube after InitializeVoxel: one wavefront per cell copies or
+// initialises its row in 64-word strides.
+__global__ void __launch_bounds__(256) k_relayout(InitArgs A)
+{
+    const long c = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (c >= A.ncells) return;
+    const int act = A.action[c];
+    const int n = A.new_len[c];
+    if (act == 0 || n == 0) return;
+    const long dst = A.new_off[c];
+    const long src = A.old_off[c];
+    for (int h = lane; h < n; h += 64) {
+        uint32_t w = SMX_VOXEL_WORD(1, -1000);
+        float r = __builtin_nanf("");
+        if (act == 1) {
+            w = A.old_vox[src + h];
+            if (A.old_rho) r = A.old_rho[src + h];
+        }
+        A.new_vox[dst + h] = w;
+        if (A.new_rho) A.new_rho[dst + h] = r;
+    }
+}
+
+// ------------------------------------------------------------ AWNCC peak
+
+// setsm_code.cpp:7527-7702 with check_SGM_peak false; one thread per cell
+// scanning its heights in order.
+__global__ void __launch_bounds__(256) k_awncc(const VllParams *__restrict__ P, PeakArgs A)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= A.ncells) return;
+    smx_nccresult nr = A.ncc[c];
+    const smx_ugrid g = A.gridpt3[c];
+    const int Pyramid_step = P->level;
+    const int iteration = P->iteration;
+    const double ncc_alpha = P->ncc_alpha;
+    const double ncc_beta = 1.0 - ncc_alpha;
+    const bool check_ortho = P->awncc_check_ortho;
+
+    double pre_rho = -1.0, pre_rho_WNCC = -1.0;
+    float pre_height = 0.0f;
+    int direction = 0;
+    bool check_rho = false;
+    double temp_nccresult = -100.0, temp_nccresult_sec = -100.0;
+    nr.result0 = q_result(-1.0);
+    nr.result2 = -1000;
+    nr.result1 = q_result(-1.0);
+    nr.result3 = -1000;
+    nr.result4 = 0;
+    const bool max_mode = (iteration <= 3 && Pyramid_step == 4) || (iteration <= 2 && Pyramid_step == 3) ||
+                          (iteration <= 1 && Pyramid_step == 2);
+    const long base = A.vox_off[c];
+    const double gncc = dq(nr.GNCC);
+    for (long hs = 0; hs < nr.NumOfHeight; hs++) {
+        const float iter_height = (float)(nr.minHeight + hs * P->height_step);
+        if (!(iter_height >= g.minHeight && iter_height <= g.maxHeight)) continue;
+        const double incc = dq(SMX_VOXEL_INCC(A.vox[base + hs]));
+        double WNCC_sum;
+        if (check_ortho && gncc > -1.0) WNCC_sum = incc * ncc_alpha + gncc * ncc_beta;
+        else WNCC_sum = incc;
+        const double WNCC_temp_rho = WNCC_sum;
+        const double temp_rho = WNCC_sum;
+        double diff_rho;
+        int t_direction;
+        if (temp_rho >= 0 && pre_rho >= 0) diff_rho = temp_rho - pre_rho;
+        else if (temp_rho < 0 && pre_rho < 0) diff_rho = -(fabs(temp_rho) - fabs(pre_rho));
+        else if (temp_rho >= 0 && pre_rho < 0) diff_rho = 1;
+        else diff_rho = -1;
+        if (diff_rho > 0) t_direction = 1;
+        else if (diff_rho < 0) t_direction = -1;
+        else t_direction = 0;
+        if (!check_rho) {
+            check_rho = true;
+            t_direction = -1;
+        } else if (pre_rho != -1) {
+            const bool check_peak = direction > 0 && t_direction < 0;
+            if (max_mode) {
+                if (temp_nccresult < temp_rho) {
+                    if (nr.result4 < 10) nr.result4 += 1;
+                    temp_nccresult = temp_rho;
+                    nr.result2 = iter_height;
+                    temp_nccresult_sec = -1.0;
+                    nr.result3 = -9999;
+                }
+            } else if (check_peak) {
+                if (nr.result4 < 10) nr.result4 += 1;
+                if (temp_nccresult < pre_rho) {
+                    const double temp_1 = temp_nccresult;
+                    temp_nccresult = pre_rho;
+                    const float temp_2 = nr.result2;
+                    nr.result2 = pre_height;
+                    if (temp_nccresult_sec < temp_1) {
+                        temp_nccresult_sec = temp_1;
+                        nr.result3 = temp_2;
+                    }
+                } else if (temp_nccresult_sec < pre_rho) {
+                    temp_nccresult_sec = pre_rho;
+                    nr.result3 = pre_height;
+                }
+            }
+        }
+        pre_rho = temp_rho;
+        pre_rho_WNCC = WNCC_temp_rho;
+        pre_height = iter_height;
+        direction = t_direction;
+    }
+    (void)pre_rho_WNCC;
+    nr.result0 = q_result(temp_nccresult);
+    nr.result1 = q_result(temp_nccresult_sec);
+    A.ncc[c] = nr;
+}
+
+}  // namespace smx
+
+// Host launchers ------------------------------------------------------------
+namespace smx {
+
+static inline unsigned nblk(long n, int per = 256) { return (unsigned)((n + per - 1) / per); }
+
+hipError_t launch_vll_gate(const VllParams *dP, const CellArgs &A, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_vll_gate, dim3(nblk(A.ncells)), dim3(256), 0, s, dP, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_vll_fill_runs(const CellArgs &A, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_vll_fill_runs, dim3(nblk(A.ncells)), dim3(256), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_vll_fp64(int half, const VllParams *dP, const CellArgs &A, hipStream_t s)
+{
+    if (A.nruns == 0) return hipSuccess;
+    const dim3 grid(nblk(A.nruns, 4)), block(256);   // 4 waves of 64 lanes per block
+    switch (half) {
+    case 5: hipLaunchKernelGGL(k_vll_fp64<5>, grid, block, 0, s, dP, A); break;
+    case 6: hipLaunchKernelGGL(k_vll_fp64<6>, grid, block, 0, s, dP, A); break;
+    case 7: hipLaunchKernelGGL(k_vll_fp64<7>, grid, block, 0, s, dP, A); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_vll_finalize(const CellArgs &A, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_vll_finalize, dim3(nblk(A.ncells)), dim3(256), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_voxel(const VllParams *dP, const InitArgs &A, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_init_voxel, dim3(nblk(A.ncells)), dim3(256), 0, s, dP, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_relayout(const InitArgs &A, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_relayout, dim3(nblk(A.ncells, 4)), dim3(256), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_awncc(const VllParams *dP, const PeakArgs &A, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_awncc, dim3(nblk(A.ncells)), dim3(256), 0, s, dP, A);
+    return hipGetLastError();
+}
+
+}  // namespace smx

@@ -1,0 +1,219 @@
+"""Host binding of libsmx.so (include/smx_vll.h) over ctypes.
+
+`Job` owns one tile resident in HBM; its methods keep the names and meaning of
+the reference entry points they stand in for:
+
+    InitializeVoxel    setsm_code.cpp:5759-6012   -> Job.initialize_voxel
+    VerticalLineLocus  setsm_code.cpp:6049-6548   -> Job.vertical_line_locus
+    AWNCC (no SGM)     setsm_code.cpp:7511-7702   -> Job.awncc
+
+Every failure raises SmxError with the library's message. There is no CPU
+fallback: without the built library (setsm_amd/lib/libsmx.so) or a GPU the
+calls fail.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .records import NCCRESULT_DTYPE, UGRID_DTYPE, SmxLevel
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libsmx.so")
+
+PREC_FP64 = 0
+PREC_FAST = 1
+
+EXPORTS = [
+    "smx_last_error", "smx_version", "smx_set_device", "smx_job_create", "smx_job_destroy",
+    "smx_job_device_bytes", "smx_job_set_budget", "smx_job_set_view", "smx_job_build_pyramid",
+    "smx_job_get_view", "smx_job_set_level", "smx_job_set_searchspace", "smx_job_get_searchspace",
+    "smx_job_get_nccresult", "smx_job_set_nccresult", "smx_initialize_voxel", "smx_vertical_line_locus",
+    "smx_awncc", "smx_job_num_hypotheses", "smx_job_get_voxels", "smx_job_set_voxels",
+    "smx_job_last_kernel_ms", "smx_job_last_evaluated", "smx_job_sync",
+]
+
+
+class SmxError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE, "-j4"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SmxError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp, i, ll = C.c_void_p, C.c_int, C.c_longlong
+        sig = {
+            "smx_last_error": (C.c_char_p, []),
+            "smx_version": (i, []),
+            "smx_set_device": (i, [i]),
+            "smx_job_create": (i, [C.POINTER(vp), i]),
+            "smx_job_destroy": (i, [vp]),
+            "smx_job_device_bytes": (ll, [vp]),
+            "smx_job_set_budget": (i, [vp, ll]),
+            "smx_job_set_view": (i, [vp, i, i, vp, vp, vp, C.c_uint32, C.c_uint32, C.c_double, C.c_double]),
+            "smx_job_build_pyramid": (i, [vp, i, vp, C.c_uint32, C.c_uint32, C.c_double, C.c_double]),
+            "smx_job_get_view": (i, [vp, i, i, vp, vp, vp]),
+            "smx_job_set_level": (i, [vp, vp]),
+            "smx_job_set_searchspace": (i, [vp, vp, vp]),
+            "smx_job_get_searchspace": (i, [vp, vp]),
+            "smx_job_get_nccresult": (i, [vp, vp]),
+            "smx_job_set_nccresult": (i, [vp, vp]),
+            "smx_initialize_voxel": (i, [vp, i]),
+            "smx_vertical_line_locus": (ll, [vp, i, i, i]),
+            "smx_awncc": (i, [vp, i]),
+            "smx_job_num_hypotheses": (ll, [vp]),
+            "smx_job_get_voxels": (i, [vp, vp, vp, vp]),
+            "smx_job_set_voxels": (i, [vp, vp]),
+            "smx_job_last_kernel_ms": (C.c_double, [vp]),
+            "smx_job_last_evaluated": (ll, [vp]),
+            "smx_job_sync": (i, [vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        msg = lib().smx_last_error().decode(errors="replace")
+        raise SmxError(f"libsmx error {rc}: {msg}")
+    return rc
+
+
+def _p(a):
+    return a.ctypes.data if a is not None else None
+
+
+class Job:
+    """One tile resident in HBM (smx_job)."""
+
+    def __init__(self, num_levels: int, budget_bytes: int = 0):
+        h = C.c_void_p()
+        _check(lib().smx_job_create(C.byref(h), int(num_levels)))
+        self.h = h
+        self.num_levels = num_levels
+        self.ncells = 0
+        if budget_bytes:
+            _check(lib().smx_job_set_budget(self.h, int(budget_bytes)))
+
+    def close(self):
+        if self.h:
+            lib().smx_job_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---------------------------------------------------------- pyramid
+    def set_view(self, img, level, intensity, magnitude, orientation, start=(0.0, 0.0)):
+        a = np.ascontiguousarray(intensity, np.uint16)
+        m = np.ascontiguousarray(magnitude, np.uint16)
+        o = np.ascontiguousarray(orientation, np.uint8)
+        h, w = a.shape
+        _check(lib().smx_job_set_view(self.h, img, level, _p(a), _p(m), _p(o), w, h, float(start[0]), float(start[1])))
+
+    def build_pyramid(self, img, level0, start0=(0.0, 0.0)):
+        a = np.ascontiguousarray(level0, np.uint16)
+        h, w = a.shape
+        _check(lib().smx_job_build_pyramid(self.h, img, _p(a), w, h, float(start0[0]), float(start0[1])))
+
+    def get_view(self, img, level, shape):
+        h, w = shape
+        a = np.zeros((h, w), np.uint16)
+        m = np.zeros((h, w), np.uint16)
+        o = np.zeros((h, w), np.uint8)
+        _check(lib().smx_job_get_view(self.h, img, level, _p(a), _p(m), _p(o)))
+        return a, m, o
+
+    # ------------------------------------------------------------ level
+    def set_level(self, level: SmxLevel):
+        _check(lib().smx_job_set_level(self.h, C.addressof(level)))
+        self.ncells = int(level.grid_w) * int(level.grid_h)
+
+    def set_searchspace(self, ugrid, minmax):
+        ug = np.ascontiguousarray(ugrid, UGRID_DTYPE)
+        assert ug.shape[0] == self.ncells
+        mm = np.ascontiguousarray(minmax, np.float64)
+        _check(lib().smx_job_set_searchspace(self.h, _p(ug), _p(mm)))
+
+    def get_searchspace(self):
+        out = np.zeros(self.ncells, UGRID_DTYPE)
+        _check(lib().smx_job_get_searchspace(self.h, _p(out)))
+        return out
+
+    def get_nccresult(self):
+        out = np.zeros(self.ncells, NCCRESULT_DTYPE)
+        _check(lib().smx_job_get_nccresult(self.h, _p(out)))
+        return out
+
+    def set_nccresult(self, ncc):
+        a = np.ascontiguousarray(ncc, NCCRESULT_DTYPE)
+        _check(lib().smx_job_set_nccresult(self.h, _p(a)))
+
+    # -------------------------------------------------- reference trio
+    def initialize_voxel(self, iteration: int):
+        _check(lib().smx_initialize_voxel(self.h, int(iteration)))
+
+    def vertical_line_locus(self, iteration: int, precision: int = PREC_FP64, keep_rho: bool = False) -> int:
+        return _check(lib().smx_vertical_line_locus(self.h, int(iteration), int(precision), 1 if keep_rho else 0))
+
+    def awncc(self, iteration: int):
+        _check(lib().smx_awncc(self.h, int(iteration)))
+
+    # ------------------------------------------------------------- cube
+    @property
+    def num_hypotheses(self) -> int:
+        return int(lib().smx_job_num_hypotheses(self.h))
+
+    def get_voxels(self, with_rho=False):
+        n = self.num_hypotheses
+        words = np.zeros(max(n, 1), np.uint32)
+        rho = np.zeros(max(n, 1), np.float32) if with_rho else None
+        off = np.zeros(self.ncells + 1, np.int64)
+        _check(lib().smx_job_get_voxels(self.h, _p(words), _p(rho), _p(off)))
+        return words[:n], (rho[:n] if with_rho else None), off
+
+    def set_voxels(self, words):
+        w = np.ascontiguousarray(words, np.uint32)
+        assert w.shape[0] == self.num_hypotheses
+        _check(lib().smx_job_set_voxels(self.h, _p(w)))
+
+    @property
+    def last_kernel_ms(self) -> float:
+        return float(lib().smx_job_last_kernel_ms(self.h))
+
+    @property
+    def last_evaluated(self) -> int:
+        return int(lib().smx_job_last_evaluated(self.h))
+
+    @property
+    def device_bytes(self) -> int:
+        return int(lib().smx_job_device_bytes(self.h))
+
+    def sync(self):
+        _check(lib().smx_job_sync(self.h))
