@@ -122,8 +122,18 @@ long long smx_job_num_hypotheses(const smx_job *job);
 /* Download the cube in CSR order: words[total], rho[total] (rho may be NULL,
  * NaN where not evaluated), offsets[ncells + 1] (may be NULL). */
 int  smx_job_get_voxels(smx_job *job, uint32_t *words, float *rho, int64_t *offsets);
+/* Download the rows of cells [cell0, cell1) only (contiguous in CSR order). */
+int  smx_job_get_voxel_rows(smx_job *job, long long cell0, long long cell1, uint32_t *words, float *rho);
 /* Replace the cube (CSR order, lengths from the current NumOfHeight). */
 int  smx_job_set_voxels(smx_job *job, const uint32_t *words);
+
+/* Launch shape of the SMX_PREC_FAST kernel (tile of cells per workgroup,
+ * LDS rows per image, register budget): index in [0, smx_fast_num_configs()).
+ * -1 restores the default. The autotuner in setsm_amd/autotune.py picks one. */
+int  smx_fast_num_configs(void);
+int  smx_job_set_fast_config(smx_job *job, int config);
+/* Describe config `config` as "tx ty lds_rows o4" into buf. */
+int  smx_fast_config_desc(int config, char *buf, int buflen);
 
 /* Milliseconds the last smx_vertical_line_locus spent in its VLL kernel
  * (HIP events on the job's stream) and the number of kernel launches. */

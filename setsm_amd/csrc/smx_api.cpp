@@ -80,7 +80,7 @@ static void jfree(smx_job *j, T *&p)
     p = nullptr;
 }
 
-static int scan(smx_job *j, const int64_t *in, int64_t *out, long n)
+int job_scan(smx_job *j, const int64_t *in, int64_t *out, long n)
 {
     size_t need = 0;
     HIPCHK(scan_i64(in, out, n, nullptr, need, j->stream));
@@ -92,6 +92,25 @@ static int scan(smx_job *j, const int64_t *in, int64_t *out, long n)
     }
     size_t have = j->scan_tmp_bytes;
     HIPCHK(scan_i64(in, out, n, j->scan_tmp, have, j->stream));
+    return 0;
+}
+
+int fast_scratch(smx_job *j, long ntiles, long nruns)
+{
+    if (ntiles + 1 > j->tiles_cap) {
+        jfree(j, j->d_tile_count);
+        jfree(j, j->d_tile_off);
+        int r = jalloc(j, &j->d_tile_count, (size_t)ntiles + 1);
+        if (!r) r = jalloc(j, &j->d_tile_off, (size_t)ntiles + 1);
+        if (r) return r;
+        j->tiles_cap = ntiles + 1;
+    }
+    if (nruns > j->tile_runs_cap) {
+        jfree(j, j->d_tile_runs);
+        int r = jalloc(j, &j->d_tile_runs, (size_t)nruns);
+        if (r) return r;
+        j->tile_runs_cap = nruns;
+    }
     return 0;
 }
 
@@ -110,6 +129,10 @@ static void free_level_state(smx_job *j)
     jfree(j, j->d_action);
     jfree(j, j->d_new_len);
     jfree(j, j->d_new_off);
+    jfree(j, j->d_tile_count);
+    jfree(j, j->d_tile_off);
+    jfree(j, j->d_tile_runs);
+    j->tiles_cap = j->tile_runs_cap = 0;
     j->runs_cap = 0;
     j->vox_total = 0;
     j->level_set = false;
@@ -479,7 +502,7 @@ int smx_initialize_voxel(smx_job *j, int iteration)
     A.old_vox = j->d_vox;
     A.old_rho = j->d_rho;
     HIPCHK(launch_init_voxel(j->d_params, A, j->stream));
-    r = scan(j, j->d_new_len, j->d_new_off, j->ncells + 1);
+    r = job_scan(j, j->d_new_len, j->d_new_off, j->ncells + 1);
     if (r) return r;
     HIPCHK(hipMemcpyAsync(j->h_small, j->d_new_off + j->ncells, sizeof(int64_t), hipMemcpyDeviceToHost, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
@@ -549,7 +572,7 @@ long long smx_vertical_line_locus(smx_job *j, int iteration, int precision, int 
     A.evaluated = j->d_counters + 1;
     HIPCHK(hipMemsetAsync(j->d_counters, 0, 4 * sizeof(unsigned long long), j->stream));
     HIPCHK(launch_vll_gate(j->d_params, A, j->stream));
-    r = scan(j, j->d_run_count, j->d_run_off, j->ncells + 1);
+    r = job_scan(j, j->d_run_count, j->d_run_off, j->ncells + 1);
     if (r) return r;
     HIPCHK(hipMemcpyAsync(j->h_small, j->d_run_off + j->ncells, sizeof(int64_t), hipMemcpyDeviceToHost, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
@@ -563,14 +586,14 @@ long long smx_vertical_line_locus(smx_job *j, int iteration, int precision, int 
     A.runs = j->d_runs;
     A.nruns = nruns;
     HIPCHK(launch_vll_fill_runs(A, j->stream));
-    HIPCHK(hipEventRecord(j->ev0, j->stream));
-    if (precision == SMX_PREC_FP64)
+    if (precision == SMX_PREC_FP64) {
+        HIPCHK(hipEventRecord(j->ev0, j->stream));
         HIPCHK(launch_vll_fp64(half, j->d_params, A, j->stream));
-    else {
-        r = launch_vll_fast(j, half, A);
+        HIPCHK(hipEventRecord(j->ev1, j->stream));
+    } else {
+        r = launch_vll_fast(j, half, A, j->ev0, j->ev1);
         if (r) return r;
     }
-    HIPCHK(hipEventRecord(j->ev1, j->stream));
     HIPCHK(launch_vll_finalize(A, j->stream));
     HIPCHK(hipMemcpyAsync(j->h_small, j->d_counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
@@ -615,12 +638,48 @@ int smx_job_get_voxels(smx_job *j, uint32_t *words, float *rho, int64_t *offsets
     return 0;
 }
 
+int smx_job_get_voxel_rows(smx_job *j, long long cell0, long long cell1, uint32_t *words, float *rho)
+{
+    if (!j || !j->level_set) return fail(SMX_E_STATE, "no level set");
+    if (cell0 < 0 || cell1 < cell0 || cell1 > j->ncells) return fail(SMX_E_ARG, "bad cell range");
+    HIPCHK(hipSetDevice(j->device));
+    HIPCHK(hipMemcpyAsync(j->h_small, j->d_vox_off + cell0, sizeof(int64_t), hipMemcpyDeviceToHost, j->stream));
+    HIPCHK(hipMemcpyAsync(j->h_small + 1, j->d_vox_off + cell1, sizeof(int64_t), hipMemcpyDeviceToHost, j->stream));
+    HIPCHK(hipStreamSynchronize(j->stream));
+    const long long a = j->h_small[0], n = j->h_small[1] - j->h_small[0];
+    if (n > 0 && words)
+        HIPCHK(hipMemcpyAsync(words, j->d_vox + a, (size_t)n * 4, hipMemcpyDeviceToHost, j->stream));
+    if (n > 0 && rho) {
+        if (j->d_rho)
+            HIPCHK(hipMemcpyAsync(rho, j->d_rho + a, (size_t)n * 4, hipMemcpyDeviceToHost, j->stream));
+        else
+            for (long long k = 0; k < n; k++) rho[k] = NAN;
+    }
+    HIPCHK(hipStreamSynchronize(j->stream));
+    return 0;
+}
+
 int smx_job_set_voxels(smx_job *j, const uint32_t *words)
 {
     if (!j || !words || !j->level_set) return fail(SMX_E_STATE, "no level set");
     if (j->vox_total)
         HIPCHK(hipMemcpyAsync(j->d_vox, words, (size_t)j->vox_total * 4, hipMemcpyHostToDevice, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
+    return 0;
+}
+
+int smx_fast_num_configs(void) { return fast_num_configs(); }
+
+int smx_job_set_fast_config(smx_job *j, int config)
+{
+    if (!j || config < -1 || config >= fast_num_configs()) return fail(SMX_E_ARG, "fast config %d out of range", config);
+    j->fast_cfg = config;
+    return 0;
+}
+
+int smx_fast_config_desc(int config, char *buf, int buflen)
+{
+    if (!buf || fast_config_desc(config, buf, buflen) < 0) return fail(SMX_E_ARG, "fast config %d out of range", config);
     return 0;
 }
 

@@ -62,8 +62,9 @@ __device__ __forceinline__ void rpc_ground_to_image(const smx_rpc &rpc, const do
         gy = gy > 0 ? gy - 360 : gy + 360;
         P = (gy - rpc.off[3]) / rpc.scale[3];
     }
+    // One polynomial at a time keeps only its 20 coefficients live.
     double Coeff[4];
-#pragma unroll
+#pragma unroll 1
     for (int j = 0; j < 4; j++) {
         const double *c = rpc.coef[j];
         Coeff[j] = c[0] * 1.0 + c[1] * L + c[2] * P

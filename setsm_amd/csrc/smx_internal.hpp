@@ -61,8 +61,11 @@ struct smx_job {
     void *scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
 
-    // fast-kernel launch shape (autotuned)
+    // fast-kernel launch shape (autotuned) and its tile run lists
     int fast_cfg = -1;
+    int64_t *d_tile_count = nullptr, *d_tile_off = nullptr;
+    int2 *d_tile_runs = nullptr;
+    long tiles_cap = 0, tile_runs_cap = 0;
 
     double last_ms = 0;
     long long last_evaluated = 0;
@@ -91,6 +94,10 @@ hipError_t launch_vll_finalize(const CellArgs &A, hipStream_t s);
 hipError_t launch_init_voxel(const VllParams *dP, const InitArgs &A, hipStream_t s);
 hipError_t launch_relayout(const InitArgs &A, hipStream_t s);
 hipError_t launch_awncc(const VllParams *dP, const PeakArgs &A, hipStream_t s);
-int launch_vll_fast(smx_job *j, int half, const CellArgs &A);
+int launch_vll_fast(smx_job *j, int half, const CellArgs &A, hipEvent_t ev0, hipEvent_t ev1);
+int fast_num_configs();
+int fast_config_desc(int i, char *buf, int n);
+int fast_scratch(smx_job *j, long ntiles, long nruns);
+int job_scan(smx_job *j, const int64_t *in, int64_t *out, long n);
 
 }  // namespace smx

@@ -67,6 +67,19 @@ __host__ __device__ constexpr int window_bits_ce(int half, int row, int col)
                        : 0));
 }
 
+// Samples of ring k (0 = outer only, 1 = middle only, 2 = inner window).
+__host__ __device__ constexpr int ring_count_ce(int half, int ring)
+{
+    int n = 0;
+    for (int r = -half; r <= half; r++)
+        for (int c = -half; c <= half; c++) {
+            const int b = window_bits_ce(half, r, c);
+            const int k = (b & 4) ? 2 : ((b & 2) ? 1 : 0);
+            if (b && k == ring) n++;
+        }
+    return n;
+}
+
 __device__ __forceinline__ unsigned long long order_key(double v)
 {
     const unsigned long long b = (unsigned long long)__double_as_longlong(v);

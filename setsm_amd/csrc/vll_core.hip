@@ -137,7 +137,7 @@ __device__ __forceinline__ bool multi_ncc_fp64(const DevView &Lv, const DevView 
 {
     double sL[3] = {0, 0, 0}, sLm[3] = {0, 0, 0}, sR[3] = {0, 0, 0}, sRm[3] = {0, 0, 0};
     int cnt[3] = {0, 0, 0};
-#pragma unroll
+#pragma unroll 1
     for (int row = -H; row <= H; row++) {
 #pragma unroll
         for (int col = -H; col <= H; col++) {
@@ -165,7 +165,7 @@ __device__ __forceinline__ bool multi_ncc_fp64(const DevView &Lv, const DevView 
     }
     double LR[3] = {0, 0, 0}, L2[3] = {0, 0, 0}, R2[3] = {0, 0, 0};
     double mLR[3] = {0, 0, 0}, mL2[3] = {0, 0, 0}, mR2[3] = {0, 0, 0};
-#pragma unroll
+#pragma unroll 1
     for (int row = -H; row <= H; row++) {
 #pragma unroll
         for (int col = -H; col <= H; col++) {

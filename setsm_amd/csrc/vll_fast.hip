@@ -1,11 +1,593 @@
-// vll_fast.hip — precision form of VerticalLineLocus (placeholder until the
-// LDS kernel lands).
+// vll_fast.hip — the precision form of VerticalLineLocus for gfx950.
+//
+// Same hypotheses, gates and cube semantics as the FP64 kernel (the height,
+// RPC pair, pyramid coordinates, image tests and orientation are computed in
+// FP64 with identical operand order, so exactly the same (cell, height) pairs
+// are evaluated). What changes is the arithmetic of the window loop:
+//
+//  * a workgroup owns a tile of TX x TY cells; it projects the tile's height
+//    bands through both RPCs, takes the bounding box of every texel the
+//    tile's windows can touch, and stages it in LDS once as float2
+//    (intensity - pivot, magnitude - pivot). The pivot is one texel value of
+//    the box, so moments are accumulated on small, exactly representable
+//    numbers (NCC is shift invariant);
+//  * one lane evaluates one height hypothesis; taps are LDS reads at
+//    compile-time offsets (left window) or at rotated offsets (right window);
+//  * bilinear taps and the 5 moments per channel pair are FP32; each window
+//    row's partial sums are added into per-ring totals with Kahan
+//    compensation; rings (outer / middle / inner window) are combined and
+//    the six correlations finished in FP64;
+//  * a tile whose box does not fit the LDS budget reads texels straight from
+//    HBM with the same arithmetic.
+//
+// The per-hypothesis result is the mean NCC of SETSM within 1e-5 of the FP64
+// value; INCC is its (short)(rho*1000) truncation.
+#include <cstdio>
+
 #include "smx_internal.hpp"
 
 namespace smx {
-int launch_vll_fast(smx_job *j, int half, const CellArgs &A)
+
+struct FastArgs {
+    const int64_t *tile_run_off;   // ntiles + 1
+    const int2 *tile_runs;         // (cell, first height), grouped by tile
+    int64_t *tile_run_count;       // ntiles + 1
+    int tiles_x, tiles_y;
+    int tx, ty;                    // cells per tile
+    uint32_t grid_w, grid_h;
+    int lds_rows;                  // LDS rows available per image
+};
+
+// ------------------------------------------------------ tile run lists
+
+__global__ void __launch_bounds__(256) k_tile_count(CellArgs A, FastArgs F)
 {
-    (void)j; (void)half; (void)A;
-    return fail(SMX_E_UNSUPPORTED, "SMX_PREC_FAST not built yet");
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ntiles = (long)F.tiles_x * F.tiles_y;
+    if (t >= ntiles) return;
+    const int tcx = (int)(t % F.tiles_x), tcy = (int)(t / F.tiles_x);
+    int64_t n = 0;
+    for (int y = 0; y < F.ty; y++) {
+        const long row = (long)tcy * F.ty + y;
+        if (row >= F.grid_h) break;
+        for (int x = 0; x < F.tx; x++) {
+            const long col = (long)tcx * F.tx + x;
+            if (col >= F.grid_w) break;
+            n += A.run_count[row * F.grid_w + col];
+        }
+    }
+    F.tile_run_count[t] = n;
 }
+
+__global__ void __launch_bounds__(256) k_tile_fill(CellArgs A, FastArgs F, int2 *out)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ntiles = (long)F.tiles_x * F.tiles_y;
+    if (t >= ntiles) return;
+    const int tcx = (int)(t % F.tiles_x), tcy = (int)(t / F.tiles_x);
+    long k = F.tile_run_off[t];
+    for (int y = 0; y < F.ty; y++) {
+        const long row = (long)tcy * F.ty + y;
+        if (row >= F.grid_h) break;
+        for (int x = 0; x < F.tx; x++) {
+            const long col = (long)tcx * F.tx + x;
+            if (col >= F.grid_w) break;
+            const long c = row * F.grid_w + col;
+            const int n = (int)A.run_count[c];
+            for (int i = 0; i < n; i++) out[k++] = make_int2((int)c, i * kRun);
+        }
+    }
+}
+
+// ------------------------------------------------------------- samplers
+
+// Texels of one image staged in LDS: float2 per texel, row pitch P.
+template <int P>
+struct LdsImg {
+    const float2 *base;   // texel (0, 0) of the box
+    __device__ __forceinline__ float2 at(int row, int col) const { return base[row * P + col]; }
+};
+
+// Texels read from HBM (packed u16 pair), shifted by a pivot on the fly.
+// Coordinates are clamped into the image so that edge taps stay in bounds;
+// samples that need such taps are rejected by the per-sample test.
+struct GlobalImg {
+    const uint32_t *texel;
+    int width, height;
+    int x0, y0;   // image position of at(0, 0)
+    float piv_i, piv_m;
+    __device__ __forceinline__ float2 at(int row, int col) const
+    {
+        const int x = min(max(x0 + col, 0), width - 1);
+        const int y = min(max(y0 + row, 0), height - 1);
+        const uint32_t t = texel[(long)y * width + x];
+        return make_float2((float)(t & 0xffffu) - piv_i, (float)(t >> 16) - piv_m);
+    }
+};
+
+struct Acc5 {
+    float a, b, aa, bb, ab;
+};
+
+struct RingAcc {
+    Acc5 i, m;   // intensity pair, magnitude pair
+};
+
+__device__ __forceinline__ void acc_add(Acc5 &s, float l, float r)
+{
+    s.a += l;
+    s.b += r;
+    s.aa = fmaf(l, l, s.aa);
+    s.bb = fmaf(r, r, s.bb);
+    s.ab = fmaf(l, r, s.ab);
+}
+
+__device__ __forceinline__ void kahan(float &sum, float &comp, float x)
+{
+    const float y = x - comp;
+    const float t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+__device__ __forceinline__ float lerp2(float a, float b, float w) { return fmaf(w, b - a, a); }
+
+__device__ __forceinline__ float2 bilerp(float2 t00, float2 t01, float2 t10, float2 t11, float wx, float wy)
+{
+    const float i0 = lerp2(t00.x, t01.x, wx), i1 = lerp2(t10.x, t11.x, wx);
+    const float m0 = lerp2(t00.y, t01.y, wx), m1 = lerp2(t10.y, t11.y, wx);
+    return make_float2(lerp2(i0, i1, wy), lerp2(m0, m1, wy));
+}
+
+// Correlation of one window / channel from pivot-shifted moments (FP64
+// finish). Returns -99 for an undefined correlation, like Correlate.
+__device__ __forceinline__ double corr_moments(double n, double a, double b, double aa, double bb, double ab)
+{
+    const double va = aa - a * a / n;
+    const double vb = bb - b * b / n;
+    const double cv = ab - a * b / n;
+    if (va > 1e-8 && vb > 1e-8) return cv / sqrt(va * vb);
+    return -99.0;
+}
+
+// One hypothesis' window loop. L/R are samplers positioned so that at(0,0)
+// is the texel (center - (H+2)) in both axes. (lx, ly) are the integer parts
+// of the left centre relative to that origin plus H+2, (fxl, fyl) its
+// fractional part; same for the right centre. Check = per-sample image-edge
+// and nodata tests (SubFunctions.cpp:3252-3257); otherwise every sample is
+// known to be valid. Returns false if a window count is zero.
+template <int H, bool kCheck, bool kKahan, class SL, class SR>
+__device__ __forceinline__ bool fast_window(const SL &L, const SR &R, float fxl, float fyl, float ftx, float fty,
+                                         float cos0, float sin0, float piv_l, float piv_r, float lim_lx0,
+                                         float lim_lx1, float lim_ly0, float lim_ly1, float lim_rx0,
+                                         float lim_rx1, float lim_ry0, float lim_ry1, double &rho)
+{
+    constexpr int O = H + 2;   // origin offset
+    RingAcc tot[3];
+    float comp[3][6];
+    float cnt[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        tot[k].i = Acc5{0, 0, 0, 0, 0};
+        tot[k].m = Acc5{0, 0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < 6; q++) comp[k][q] = 0.f;
+    }
+    // Per-hypothesis pivots: the window-centre samples. Moments are taken on
+    // sample - pivot, so a low-contrast patch far from the tile's staging
+    // pivot does not lose its variance to cancellation.
+    const float2 pvl = bilerp(L.at(O, O), L.at(O, O + 1), L.at(O + 1, O), L.at(O + 1, O + 1), fxl, fyl);
+    const float2 pvr = bilerp(R.at(O, O), R.at(O, O + 1), R.at(O + 1, O), R.at(O + 1, O + 1), ftx, fty);
+    // Rows run as a loop (the row index is wave-uniform, so the window
+    // membership tests below are scalar); columns are unrolled so left taps
+    // are LDS reads at immediate offsets.
+#pragma unroll 1
+    for (int row = -H; row <= H; row++) {
+        RingAcc part[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            part[k].i = Acc5{0, 0, 0, 0, 0};
+            part[k].m = Acc5{0, 0, 0, 0, 0};
+        }
+        const float frow = (float)row;
+        const float rs = frow * sin0, rc = frow * cos0;
+#pragma unroll
+        for (int col = -H; col <= H; col++) {
+            const int bits = window_bits_ce(H, row, col);
+            if (!bits) continue;
+            const int ring = (bits & 4) ? 2 : ((bits & 2) ? 1 : 0);
+            // left: integer offsets, shared fractional weights
+            const float2 l = bilerp(L.at(O + row, O + col), L.at(O + row, O + col + 1), L.at(O + row + 1, O + col),
+                                    L.at(O + row + 1, O + col + 1), fxl, fyl);
+            // right: rotated offsets (setsm_code.cpp:6437-6439)
+            const float x = ftx + fmaf((float)col, cos0, -rs);
+            const float y = fty + fmaf((float)col, sin0, rc);
+            const float flx = floorf(x), fly = floorf(y);
+            const int ix = (int)flx + O, iy = (int)fly + O;
+            const float wx = x - flx, wy = y - fly;
+            const float2 r = bilerp(R.at(iy, ix), R.at(iy, ix + 1), R.at(iy + 1, ix), R.at(iy + 1, ix + 1), wx, wy);
+            bool ok = true;
+            if (kCheck) {
+                // pos >= 0 && pos + 1 < size on both images, then L > 0 && R > 0
+                ok = (float)col >= lim_lx0 && (float)col < lim_lx1 && frow >= lim_ly0 && frow < lim_ly1 &&
+                     x >= lim_rx0 && x < lim_rx1 && y >= lim_ry0 && y < lim_ry1 && l.x > -piv_l && r.x > -piv_r;
+            }
+            if (ok) {
+                const float li = l.x - pvl.x, lm = l.y - pvl.y, ri = r.x - pvr.x, rm = r.y - pvr.y;
+                if (ring == 0) {
+                    acc_add(part[0].i, li, ri);
+                    acc_add(part[0].m, lm, rm);
+                } else if (ring == 1) {
+                    acc_add(part[1].i, li, ri);
+                    acc_add(part[1].m, lm, rm);
+                } else {
+                    acc_add(part[2].i, li, ri);
+                    acc_add(part[2].m, lm, rm);
+                }
+                if (kCheck) cnt[ring] += 1.f;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            tot[k].i.a += part[k].i.a;
+            tot[k].i.b += part[k].i.b;
+            tot[k].m.a += part[k].m.a;
+            tot[k].m.b += part[k].m.b;
+            if (kKahan) {
+                kahan(tot[k].i.aa, comp[k][0], part[k].i.aa);
+                kahan(tot[k].i.bb, comp[k][1], part[k].i.bb);
+                kahan(tot[k].i.ab, comp[k][2], part[k].i.ab);
+                kahan(tot[k].m.aa, comp[k][3], part[k].m.aa);
+                kahan(tot[k].m.bb, comp[k][4], part[k].m.bb);
+                kahan(tot[k].m.ab, comp[k][5], part[k].m.ab);
+            } else {
+                tot[k].i.aa += part[k].i.aa;
+                tot[k].i.bb += part[k].i.bb;
+                tot[k].i.ab += part[k].i.ab;
+                tot[k].m.aa += part[k].m.aa;
+                tot[k].m.bb += part[k].m.bb;
+                tot[k].m.ab += part[k].m.ab;
+            }
+        }
+    }
+    // window totals: inner = ring 2, middle = ring 1 + 2, outer = all rings
+    double n[3], S[3][10];
+#pragma unroll
+    for (int w = 0; w < 3; w++) {
+        double s[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        double nn = 0;
+#pragma unroll
+        for (int k = w; k < 3; k++) {
+            // ring k belongs to windows 0..k
+            s[0] += tot[k].i.a;
+            s[1] += tot[k].i.b;
+            s[2] += (double)tot[k].i.aa - (double)comp[k][0];
+            s[3] += (double)tot[k].i.bb - (double)comp[k][1];
+            s[4] += (double)tot[k].i.ab - (double)comp[k][2];
+            s[5] += tot[k].m.a;
+            s[6] += tot[k].m.b;
+            s[7] += (double)tot[k].m.aa - (double)comp[k][3];
+            s[8] += (double)tot[k].m.bb - (double)comp[k][4];
+            s[9] += (double)tot[k].m.ab - (double)comp[k][5];
+            nn += kCheck ? (double)cnt[k] : (double)ring_count_ce(H, k);
+        }
+        // windows are nested: window w = rings w..2
+        n[w] = nn;
+#pragma unroll
+        for (int q = 0; q < 10; q++) S[w][q] = s[q];
+    }
+    if (!(n[0] > 0 && n[1] > 0 && n[2] > 0)) return false;
+    double temp_roh = 0, count_roh = 0;
+#pragma unroll
+    for (int w = 0; w < 3; w++) {
+        const double c1 = corr_moments(n[w], S[w][0], S[w][1], S[w][2], S[w][3], S[w][4]);
+        if (c1 != -99) { count_roh++; temp_roh += c1; }
+        const double c2 = corr_moments(n[w], S[w][5], S[w][6], S[w][7], S[w][8], S[w][9]);
+        if (c2 != -99) { count_roh++; temp_roh += c2; }
+    }
+    rho = count_roh > 0 ? temp_roh / count_roh : 1e300;   // 1e300 = no valid correlation
+    return true;
+}
+
+struct TileBox {
+    int x0[2], y0[2], w[2], h[2];
+    float piv_i[2], piv_m[2];
+    int fits, nodata;
+};
+
+// Window of one hypothesis with taps straight from HBM (tiles whose box
+// does not fit LDS, and the level-1 term of the combined INCC). Pivots are
+// the centre texels; every sample is tested like SetVecKernelValue.
+template <int H, bool kKahan>
+__device__ __forceinline__ bool global_window(const DevView &V0, const DevView &V1, double rpx, double rpy, double tpx,
+                                              double tpy, float cos0, float sin0, double &rho)
+{
+    constexpr int O = H + 2;
+    const double flx = floor(rpx), fly = floor(rpy), ftx = floor(tpx), fty = floor(tpy);
+    const int ilx = (int)flx, ily = (int)fly, itx = (int)ftx, ity = (int)fty;
+    const int clx = min(max(ilx, 0), (int)V0.width - 1), cly = min(max(ily, 0), (int)V0.height - 1);
+    const int ctx = min(max(itx, 0), (int)V1.width - 1), cty = min(max(ity, 0), (int)V1.height - 1);
+    const uint32_t pl = V0.texel[(long)cly * V0.width + clx], pr = V1.texel[(long)cty * V1.width + ctx];
+    const GlobalImg Lg{V0.texel, (int)V0.width, (int)V0.height, ilx - O, ily - O, (float)(pl & 0xffffu),
+                       (float)(pl >> 16)};
+    const GlobalImg Rg{V1.texel, (int)V1.width, (int)V1.height, itx - O, ity - O, (float)(pr & 0xffffu),
+                       (float)(pr >> 16)};
+    return fast_window<H, true, kKahan>(Lg, Rg, (float)(rpx - flx), (float)(rpy - fly), (float)(tpx - ftx),
+                                        (float)(tpy - fty), cos0, sin0, Lg.piv_i, Rg.piv_i, -(float)rpx,
+                                        (float)((double)V0.width - 1.0 - rpx), -(float)rpy,
+                                        (float)((double)V0.height - 1.0 - rpy), -(float)itx,
+                                        (float)((int)V1.width - 1 - itx), -(float)ity,
+                                        (float)((int)V1.height - 1 - ity), rho);
+}
+
+template <int H, int PITCH, bool kKahan>
+__device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, CellArgs &A, const FastArgs &F,
+                                              float2 *lds)
+{
+    __shared__ int sbox[2][4];
+    __shared__ TileBox tb;
+    const VllParams &P = *Pp;
+    const long tile = blockIdx.x;
+    const long r0 = F.tile_run_off[tile], r1 = F.tile_run_off[tile + 1];
+    if (r0 == r1) return;
+    constexpr int O = H + 2;
+
+    if (threadIdx.x < 2) {
+        sbox[threadIdx.x][0] = 0x7fffffff;
+        sbox[threadIdx.x][1] = 0x7fffffff;
+        sbox[threadIdx.x][2] = -0x7fffffff;
+        sbox[threadIdx.x][3] = -0x7fffffff;
+    }
+    __syncthreads();
+    // Bounding box of the tile's windows: project each active cell's band ends.
+    const int tcx = (int)(tile % F.tiles_x), tcy = (int)(tile / F.tiles_x);
+    for (int t = threadIdx.x; t < F.tx * F.ty; t += blockDim.x) {
+        const long col = (long)tcx * F.tx + t % F.tx, row = (long)tcy * F.ty + t / F.tx;
+        if (col >= F.grid_w || row >= F.grid_h) continue;
+        const long c = row * F.grid_w + col;
+        if (A.run_count[c] == 0) continue;
+        const smx_ugrid g = A.gridpt3[c];
+        const double lon = A.grid_wgs[2 * c], lat = A.grid_wgs[2 * c + 1];
+        for (int e = 0; e < 2; e++) {
+            const double hh = e == 0 ? (double)g.minHeight : (double)g.maxHeight;
+            for (int ti = 0; ti < 2; ti++) {
+                double x, y, px, py;
+                rpc_ground_to_image(P.rpc[ti], P.adjust[ti], lon, lat, hh, x, y);
+                to_pyramid(x, y, P.view[ti], P.level, px, py);
+                const int fx = (int)floor(px), fy = (int)floor(py);
+                atomicMin(&sbox[ti][0], fx);
+                atomicMin(&sbox[ti][1], fy);
+                atomicMax(&sbox[ti][2], fx);
+                atomicMax(&sbox[ti][3], fy);
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int fits = 1;
+        for (int ti = 0; ti < 2; ti++) {
+            const DevView &v = P.view[ti];
+            // window radius H+1, bilinear +1, 2 px for the band's curvature
+            int x0 = sbox[ti][0] - O - 2, y0 = sbox[ti][1] - O - 2;
+            int x1 = sbox[ti][2] + O + 3, y1 = sbox[ti][3] + O + 3;
+            x0 = max(x0, 0);
+            y0 = max(y0, 0);
+            x1 = min(x1, (int)v.width - 1);
+            y1 = min(y1, (int)v.height - 1);
+            tb.x0[ti] = x0;
+            tb.y0[ti] = y0;
+            tb.w[ti] = max(x1 - x0 + 1, 1);
+            tb.h[ti] = max(y1 - y0 + 1, 1);
+            if (tb.w[ti] > PITCH || tb.h[ti] > F.lds_rows) fits = 0;
+            const long cx = min(max((x0 + x1) / 2, 0), (int)v.width - 1);
+            const long cy = min(max((y0 + y1) / 2, 0), (int)v.height - 1);
+            const uint32_t t = v.texel[cy * v.width + cx];
+            tb.piv_i[ti] = (float)(t & 0xffffu);
+            tb.piv_m[ti] = (float)(t >> 16);
+        }
+        tb.fits = fits;
+        tb.nodata = 0;
+    }
+    __syncthreads();
+    const bool fits = tb.fits != 0;
+    if (fits) {
+        int nod = 0;
+        for (int ti = 0; ti < 2; ti++) {
+            const DevView &v = P.view[ti];
+            float2 *dst = lds + ti * PITCH * F.lds_rows;
+            const int w = tb.w[ti], h = tb.h[ti];
+            const float pi = tb.piv_i[ti], pm = tb.piv_m[ti];
+            for (int k = threadIdx.x; k < w * h; k += blockDim.x) {
+                const int rr = k / w, cc = k % w;
+                const uint32_t t = v.texel[(long)(tb.y0[ti] + rr) * v.width + tb.x0[ti] + cc];
+                nod |= (t & 0xffffu) == 0;
+                dst[rr * PITCH + cc] = make_float2((float)(t & 0xffffu) - pi, (float)(t >> 16) - pm);
+            }
+        }
+        if (nod) tb.nodata = 1;   // benign race: every writer stores 1
+    }
+    __syncthreads();
+    const bool nodata = tb.nodata != 0;
+
+    const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6, lane = threadIdx.x & 63;
+    for (long ri = r0 + wave; ri < r1; ri += nwaves) {
+        const int2 run = F.tile_runs[ri];
+        const long cell = run.x;
+        const int hidx = run.y + lane;
+        const smx_nccresult nr = A.ncc[cell];
+        const smx_ugrid g = A.gridpt3[cell];
+        Hyp hy{};
+        const bool act = project_hypothesis(P, A, cell, hidx, nr, g.minHeight, g.maxHeight, hy);
+        double temp_rho = -100.0;
+        unsigned evaluated = 0;
+        bool do_ncc = false;
+        long wi = 0;
+        uint32_t word = 0;
+        if (act) {
+            wi = A.vox_off[cell] + hidx;
+            word = A.vox[wi];
+            if (!hy.orient_ok) A.vox[wi] = word & ~0xffu;
+            else do_ncc = true;
+        }
+        // integer / fractional split of both centres (FP64 -> FP32 fraction)
+        const double flx = floor(hy.rpx), fly = floor(hy.rpy), ftx = floor(hy.tpx), fty = floor(hy.tpy);
+        const int ilx = (int)flx, ily = (int)fly, itx = (int)ftx, ity = (int)fty;
+        const float cos0 = (float)hy.cos0, sin0 = (float)hy.sin0;
+        const DevView &V0 = P.view[0], &V1 = P.view[1];
+        // interior: every tap of the disc is inside both images
+        const bool interior = ilx - O >= 0 && ilx + O + 1 < (int)V0.width && ily - O >= 0 &&
+                              ily + O + 1 < (int)V0.height && itx - O >= 0 && itx + O + 1 < (int)V1.width &&
+                              ity - O >= 0 && ity + O + 1 < (int)V1.height;
+        bool in_box = fits;
+        if (fits) {
+            in_box = ilx - O >= tb.x0[0] && ilx + O + 1 < tb.x0[0] + tb.w[0] && ily - O >= tb.y0[0] &&
+                     ily + O + 1 < tb.y0[0] + tb.h[0] && itx - O >= tb.x0[1] && itx + O + 1 < tb.x0[1] + tb.w[1] &&
+                     ity - O >= tb.y0[1] && ity + O + 1 < tb.y0[1] + tb.h[1];
+        }
+        double rho = 0;
+        bool all3 = false;
+        const bool use_lds = __all(!do_ncc || in_box);
+        const bool fast_ok = !nodata && __all(!do_ncc || interior);
+        if (do_ncc) {
+            if (use_lds) {
+                const float fxl = (float)(hy.rpx - flx), fyl = (float)(hy.rpy - fly);
+                const float fxt = (float)(hy.tpx - ftx), fyt = (float)(hy.tpy - fty);
+                const LdsImg<PITCH> Lw{lds + (long)(ily - O - tb.y0[0]) * PITCH + (ilx - O - tb.x0[0])};
+                const LdsImg<PITCH> Rw{lds + (long)PITCH * F.lds_rows + (long)(ity - O - tb.y0[1]) * PITCH +
+                                       (itx - O - tb.x0[1])};
+                if (fast_ok)
+                    all3 = fast_window<H, false, kKahan>(Lw, Rw, fxl, fyl, fxt, fyt, cos0, sin0, tb.piv_i[0],
+                                                         tb.piv_i[1], 0, 0, 0, 0, 0, 0, 0, 0, rho);
+                else
+                    all3 = fast_window<H, true, kKahan>(
+                        Lw, Rw, fxl, fyl, fxt, fyt, cos0, sin0, tb.piv_i[0], tb.piv_i[1], -(float)hy.rpx,
+                        (float)((double)V0.width - 1.0 - hy.rpx), -(float)hy.rpy,
+                        (float)((double)V0.height - 1.0 - hy.rpy), -(float)itx, (float)((int)V1.width - 1 - itx),
+                        -(float)ity, (float)((int)V1.height - 1 - ity), rho);
+            } else {
+                all3 = global_window<H, kKahan>(V0, V1, hy.rpx, hy.rpy, hy.tpx, hy.tpy, cos0, sin0, rho);
+            }
+        }
+        // setsm_code.cpp:6456-6493: average of the level term and, on even
+        // iterations above level 0, the level-1 term.
+        double sum = 0, cnt = 0;
+        if (do_ncc && all3 && rho != 1e300) {
+            sum += rho;
+            cnt += 1;
+        }
+        if (do_ncc && P.combined_incc) {
+            double rho_n = 0;
+            const bool all3n = global_window<H, kKahan>(P.next[0], P.next[1], hy.rnx, hy.rny, hy.tnx, hy.tny, cos0,
+                                                        sin0, rho_n);
+            if (all3n && rho_n != 1e300) {
+                sum += rho_n;
+                cnt += 1;
+            }
+        }
+        if (do_ncc) {
+            const double rr = cnt > 0 ? sum / cnt : -1.0;
+            word = (word & 0xffu) | (all3 ? 1u : 0u);
+            word |= (uint32_t)(uint16_t)q_voxel(rr) << 16;
+            A.vox[wi] = word;
+            if (A.rho) A.rho[wi] = (float)rr;
+            temp_rho = rr;
+            evaluated = 1;
+        }
+        wave_max_commit(temp_rho, evaluated, &A.max_bits[cell], A.evaluated);
+    }
+}
+
+// Two register budgets of the same body: the compiler's own choice, and one
+// capped so that four waves fit per SIMD. The autotuner picks per shape.
+template <int H, int PITCH, bool kKahan>
+__global__ void __launch_bounds__(256) k_vll_fast(const VllParams *__restrict__ Pp, CellArgs A, FastArgs F)
+{
+    extern __shared__ float2 lds[];
+    vll_fast_body<H, PITCH, kKahan>(Pp, A, F, lds);
+}
+
+template <int H, int PITCH, bool kKahan>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_vll_fast_o4(const VllParams *__restrict__ Pp, CellArgs A, FastArgs F)
+{
+    extern __shared__ float2 lds[];
+    vll_fast_body<H, PITCH, kKahan>(Pp, A, F, lds);
+}
+
+}  // namespace smx
+
+// Host launcher ------------------------------------------------------------
+namespace smx {
+
+struct FastCfg {
+    int tx, ty, lds_rows, o4;
+};
+
+// Launch shapes the autotuner chooses from (cells per tile, LDS rows per
+// image, register cap for four waves per SIMD).
+static const FastCfg kFastCfgs[] = {
+    {4, 4, 48, 0}, {4, 4, 48, 1}, {8, 4, 48, 0}, {8, 4, 48, 1}, {4, 8, 64, 0},
+    {4, 8, 64, 1}, {8, 8, 64, 0}, {8, 8, 64, 1}, {2, 4, 40, 0}, {2, 4, 40, 1},
+};
+constexpr int kNumFastCfgs = sizeof(kFastCfgs) / sizeof(kFastCfgs[0]);
+constexpr int kPitch = 64;
+
+template <int H>
+static hipError_t launch_fast_h(bool o4, dim3 grid, size_t lds, hipStream_t s, const VllParams *dP,
+                                const CellArgs &A, const FastArgs &F)
+{
+    if (o4)
+        hipLaunchKernelGGL((k_vll_fast_o4<H, kPitch, true>), grid, dim3(256), lds, s, dP, A, F);
+    else
+        hipLaunchKernelGGL((k_vll_fast<H, kPitch, true>), grid, dim3(256), lds, s, dP, A, F);
+    return hipGetLastError();
+}
+
+int launch_vll_fast(smx_job *j, int half, const CellArgs &A, hipEvent_t ev0, hipEvent_t ev1)
+{
+    const int ci = (j->fast_cfg >= 0 && j->fast_cfg < kNumFastCfgs) ? j->fast_cfg : 0;
+    const FastCfg cfg = kFastCfgs[ci];
+    FastArgs F{};
+    F.tx = cfg.tx;
+    F.ty = cfg.ty;
+    F.grid_w = j->lv.grid_w;
+    F.grid_h = j->lv.grid_h;
+    F.tiles_x = (int)((F.grid_w + F.tx - 1) / F.tx);
+    F.tiles_y = (int)((F.grid_h + F.ty - 1) / F.ty);
+    F.lds_rows = cfg.lds_rows;
+    const long ntiles = (long)F.tiles_x * F.tiles_y;
+    int r = fast_scratch(j, ntiles, A.nruns);
+    if (r) return r;
+    F.tile_run_count = j->d_tile_count;
+    F.tile_run_off = j->d_tile_off;
+    F.tile_runs = j->d_tile_runs;
+    hipError_t e = hipMemsetAsync(j->d_tile_count + ntiles, 0, sizeof(int64_t), j->stream);
+    if (e != hipSuccess) return fail(SMX_E_HIP, "memset: %s", hipGetErrorString(e));
+    const unsigned nb = (unsigned)((ntiles + 255) / 256);
+    hipLaunchKernelGGL(k_tile_count, dim3(nb), dim3(256), 0, j->stream, A, F);
+    r = job_scan(j, j->d_tile_count, j->d_tile_off, ntiles + 1);
+    if (r) return r;
+    hipLaunchKernelGGL(k_tile_fill, dim3(nb), dim3(256), 0, j->stream, A, F, j->d_tile_runs);
+    const size_t lds = (size_t)2 * kPitch * F.lds_rows * sizeof(float2);
+    if (ev0) (void)hipEventRecord(ev0, j->stream);
+    switch (half) {
+    case 5: e = launch_fast_h<5>(cfg.o4, dim3((unsigned)ntiles), lds, j->stream, j->d_params, A, F); break;
+    case 6: e = launch_fast_h<6>(cfg.o4, dim3((unsigned)ntiles), lds, j->stream, j->d_params, A, F); break;
+    case 7: e = launch_fast_h<7>(cfg.o4, dim3((unsigned)ntiles), lds, j->stream, j->d_params, A, F); break;
+    default: return fail(SMX_E_UNSUPPORTED, "half %d", half);
+    }
+    if (ev1) (void)hipEventRecord(ev1, j->stream);
+    if (e != hipSuccess) return fail(SMX_E_HIP, "k_vll_fast: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int fast_num_configs() { return kNumFastCfgs; }
+
+int fast_config_desc(int i, char *buf, int n)
+{
+    if (i < 0 || i >= kNumFastCfgs) return -1;
+    return snprintf(buf, (size_t)n, "%d %d %d %d", kFastCfgs[i].tx, kFastCfgs[i].ty, kFastCfgs[i].lds_rows,
+                    kFastCfgs[i].o4);
+}
+
 }  // namespace smx

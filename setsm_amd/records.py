@@ -118,3 +118,20 @@ class LevelInputs:
         L.grid_pts, L.grid_wgs = gp.ctypes.data, gw.ctypes.data
         self.level = L
         self.ncells = gh * gwid
+
+
+def window_bits(half: int, row: int, col: int) -> int:
+    """Nested-window membership of a window offset: bit 0 outer disc
+    (setsm_code.cpp:6434), bits 1/2 the middle / inner windows
+    (SubFunctions.cpp:3274-3312)."""
+    r2 = row * row + col * col
+    if abs(row) > half or abs(col) > half or r2 > (half + 1) ** 2:
+        return 0
+    bits = 1
+    s1 = half // 2
+    if r2 <= (half - s1 + 1) ** 2 and -half + s1 <= row <= half - s1 and -half + s1 <= col <= half - s1:
+        bits |= 2
+    s2 = s1 + int(s1 / 2.0 + 0.5)
+    if r2 <= (half - s2 + 1) ** 2 and -half + s2 <= row <= half - s2 and -half + s2 <= col <= half - s2:
+        bits |= 4
+    return bits

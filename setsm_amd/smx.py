@@ -31,8 +31,9 @@ EXPORTS = [
     "smx_job_device_bytes", "smx_job_set_budget", "smx_job_set_view", "smx_job_build_pyramid",
     "smx_job_get_view", "smx_job_set_level", "smx_job_set_searchspace", "smx_job_get_searchspace",
     "smx_job_get_nccresult", "smx_job_set_nccresult", "smx_initialize_voxel", "smx_vertical_line_locus",
-    "smx_awncc", "smx_job_num_hypotheses", "smx_job_get_voxels", "smx_job_set_voxels",
-    "smx_job_last_kernel_ms", "smx_job_last_evaluated", "smx_job_sync",
+    "smx_awncc", "smx_job_num_hypotheses", "smx_job_get_voxels", "smx_job_get_voxel_rows", "smx_job_set_voxels",
+    "smx_job_last_kernel_ms", "smx_job_last_evaluated", "smx_job_sync", "smx_fast_num_configs",
+    "smx_job_set_fast_config", "smx_fast_config_desc",
 ]
 
 
@@ -76,10 +77,14 @@ def lib():
             "smx_awncc": (i, [vp, i]),
             "smx_job_num_hypotheses": (ll, [vp]),
             "smx_job_get_voxels": (i, [vp, vp, vp, vp]),
+            "smx_job_get_voxel_rows": (i, [vp, ll, ll, vp, vp]),
             "smx_job_set_voxels": (i, [vp, vp]),
             "smx_job_last_kernel_ms": (C.c_double, [vp]),
             "smx_job_last_evaluated": (ll, [vp]),
             "smx_job_sync": (i, [vp]),
+            "smx_fast_num_configs": (i, []),
+            "smx_job_set_fast_config": (i, [vp, i]),
+            "smx_fast_config_desc": (i, [i, C.c_char_p, i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -198,6 +203,12 @@ class Job:
         _check(lib().smx_job_get_voxels(self.h, _p(words), _p(rho), _p(off)))
         return words[:n], (rho[:n] if with_rho else None), off
 
+    def get_voxel_rows(self, cell0, cell1, nwords, with_rho=False):
+        words = np.zeros(max(nwords, 1), np.uint32)
+        rho = np.zeros(max(nwords, 1), np.float32) if with_rho else None
+        _check(lib().smx_job_get_voxel_rows(self.h, int(cell0), int(cell1), _p(words), _p(rho)))
+        return words[:nwords], (rho[:nwords] if with_rho else None)
+
     def set_voxels(self, words):
         w = np.ascontiguousarray(words, np.uint32)
         assert w.shape[0] == self.num_hypotheses
@@ -215,5 +226,18 @@ class Job:
     def device_bytes(self) -> int:
         return int(lib().smx_job_device_bytes(self.h))
 
+    def set_fast_config(self, config: int):
+        _check(lib().smx_job_set_fast_config(self.h, int(config)))
+
     def sync(self):
         _check(lib().smx_job_sync(self.h))
+
+
+def fast_configs():
+    """[(index, (tx, ty, lds_rows, o4))] of the SMX_PREC_FAST launch shapes."""
+    out = []
+    buf = C.create_string_buffer(64)
+    for i in range(lib().smx_fast_num_configs()):
+        _check(lib().smx_fast_config_desc(i, buf, 64))
+        out.append((i, tuple(int(x) for x in buf.value.split())))
+    return out

@@ -98,3 +98,87 @@ def test_upper_levels_fp64_bit_exact(require_gpu, level):
         _compare(ro, rg, f"L{level} it{it}")
         ug = ro["ug"]
     job.close()
+
+
+def _fast_vs_fp64(o, g, ncells, tag, tol=1e-5):
+    """FAST path against the FP64 oracle: same hypotheses evaluated, same
+    flags, mean NCC within tol, INCC within one quantum, and the per-cell
+    argmax of the mean NCC equal except near-ties (top-two gap < tol)."""
+    ev = np.isfinite(o["rho"])
+    assert np.array_equal(ev, np.isfinite(g["rho"])), f"{tag}: evaluated sets differ"
+    assert np.array_equal(o["words"] & 0xFF, g["words"] & 0xFF), f"{tag}: flag_cal differs"
+    d = np.abs(g["rho"][ev].astype(np.float64) - o["rho"][ev])
+    incc_o = (o["words"] >> 16).astype(np.uint16).view(np.int16).astype(np.int32)
+    incc_g = (g["words"] >> 16).astype(np.uint16).view(np.int16).astype(np.int32)
+    nflip = int(np.sum(incc_o != incc_g))
+    # per-cell argmax of rho (first maximum) with near-tie accounting
+    off = g["off"]
+    flips = ties = cells = 0
+    for c in range(ncells):
+        a, b = off[c], off[c + 1]
+        if b <= a:
+            continue
+        ro = np.where(np.isfinite(o["rho"][a:b]), o["rho"][a:b], -np.inf)
+        rg = np.where(np.isfinite(g["rho"][a:b]), g["rho"][a:b].astype(np.float64), -np.inf)
+        if not np.isfinite(ro).any():
+            continue
+        cells += 1
+        io, ig = int(np.argmax(ro)), int(np.argmax(rg))
+        if io != ig:
+            if ro[io] - ro[ig] < tol:
+                ties += 1
+            else:
+                flips += 1
+    print(f"[{tag}] n={ev.sum()} max|drho|={d.max():.2e} rms={np.sqrt(np.mean(d * d)):.2e} "
+          f"incc_diff={nflip} ({nflip / max(1, ev.sum()):.2%}) cells={cells} peak_ties={ties} peak_flips={flips}", flush=True)
+    assert d.max() <= tol, f"{tag}: max |rho_fast - rho_fp64| = {d.max():.3e}"
+    assert np.all(np.abs(incc_o - incc_g) <= 1), tag
+    assert flips == 0, f"{tag}: {flips} peak flips beyond near-ties"
+
+
+@pytest.mark.parametrize("level,iters", [(0, (1, 2)), (1, (1, 2)), (3, (1, 2)), (4, (1,))])
+def test_fast_precision(require_gpu, level, iters):
+    """SMX_PREC_FAST: FP32 compensated moments, FP64 RPC, LDS-staged taps."""
+    from setsm_amd.smx import PREC_FAST
+    if level == 0:
+        sc, pyr = scene(image_px=512, levels=1)
+        g = level_grid(sc, 0, 72)
+        nlev = 1
+    else:
+        sc, pyr = scene(image_px=1024, relief=120.0, seed=5, levels=5)
+        g = level_grid(sc, level, {1: 48, 3: 28, 4: 16}[level])
+        nlev = 5
+    o = OracleRun(pyr, sc.rpcs, level, g)
+    job = _job(nlev)
+    upload_pyramid(job, pyr)
+    gr = GpuRun(job, level, pyr, sc.rpcs, g)
+    ug = g["ugrid"]
+    for it in iters:
+        if it > 1:
+            ug = perturb_searchspace(ug, 300 + it, level)
+        ro = o.step(ug, it, g["minmax"])
+        rg = gr.step(ug, it, g["minmax"], precision=PREC_FAST)
+        assert ro["acc"] == rg["acc"]
+        _fast_vs_fp64(ro, rg, len(ug), f"FAST L{level} it{it}")
+        ug = ro["ug"]
+    job.close()
+
+
+def test_fast_configs_agree(require_gpu):
+    """Every FAST launch shape gives the same cube (tiling is not arithmetic)."""
+    from setsm_amd.smx import PREC_FAST, fast_configs
+    sc, pyr = scene(image_px=512, levels=1)
+    g = level_grid(sc, 0, 40)
+    job = _job(1)
+    upload_pyramid(job, pyr)
+    gr = GpuRun(job, 0, pyr, sc.rpcs, g)
+    ref = None
+    for i, desc in fast_configs():
+        job.set_fast_config(i)
+        r = gr.step(g["ugrid"], 1, g["minmax"], precision=PREC_FAST)
+        print(f"config {i} {desc}: {r['ms']:.3f} ms", flush=True)
+        if ref is None:
+            ref = r
+        else:
+            assert np.array_equal(ref["words"], r["words"]), f"config {i} differs"
+    job.close()
