@@ -71,6 +71,7 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=64, help="cell rows of the CPU baseline stripe")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--autotune", action="store_true", help="time every FAST launch shape and store the best")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -146,9 +147,24 @@ def main():
 
     fast = None
     fast_err = None
+    tuned = None
     try:
+        from setsm_amd import autotune
+        key = f"vll_L0_{args.image_px}px_{cells}cells_h128"
+        if args.autotune:
+            best, rows = autotune.tune_fast(job, 1, log=log)
+            os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+            autotune.save(key, best, rows, path=os.path.join(ROOT, "gpurun_out", "tuning.json"))
+            tuned = best
+        else:
+            entry = autotune.load(key)
+            if entry:
+                job.set_fast_config(entry["best"]["config"])
+                tuned = entry["best"]
+        log(f"FAST launch shape: {tuned}")
         log("timing FAST kernel")
         fast = run(smx.PREC_FAST, args.steps, args.warmup, keep_rho=False)
+        fast["stats"] = job.last_stats
         job.vertical_line_locus(1, precision=smx.PREC_FAST, keep_rho=True)
         wf, rhof = job.get_voxel_rows(c0, c1, nw, with_rho=True)
         ev = np.isfinite(rho64)
@@ -220,6 +236,7 @@ def main():
         "speedup_vs_plain_fp64": (value / (p64["evaluated"] / p64["wall_s"])) if fast is not None else 1.0,
         "fast_vs_fp64_error": fast_err,
         "parity_stripe": parity,
+        "fast_launch": (dict(tuned or {}, **fast.get("stats", {})) if fast else None),
         "kernel_ms": best["kernel_ms"],
     }
     print(json.dumps(out), flush=True)

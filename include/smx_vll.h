@@ -140,6 +140,9 @@ int  smx_fast_config_desc(int config, char *buf, int buflen);
 double smx_job_last_kernel_ms(const smx_job *job);
 /* Hypotheses the last smx_vertical_line_locus evaluated (passed every gate). */
 long long smx_job_last_evaluated(const smx_job *job);
+/* out4 = {evaluated hypotheses, FAST tiles staged in LDS, FAST waves that
+ * read their taps from HBM, FAST launch shape index} of the last call. */
+int  smx_job_last_stats(const smx_job *job, long long *out4);
 
 /* Wait for all work queued on the job's stream. */
 int  smx_job_sync(smx_job *job);

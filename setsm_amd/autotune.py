@@ -1,0 +1,64 @@
+"""Autotuner for the SMX_PREC_FAST launch shape.
+
+The FAST kernel's shape is the cell tile a workgroup owns (which sets the LDS
+footprint, the texel reuse and the number of workgroups), the LDS
+reserved for the two images' boxes, and the register budget (compiler's choice, or capped
+for four waves per SIMD). heights-per-wavefront is fixed at 64: one run of
+consecutive heights of one cell per wavefront, so lanes of a wave read
+neighbouring texels along the epipolar line.
+
+`tune_fast` times every shape on the job's current level and keeps the
+fastest; results are stored per workload in setsm_amd/tuning.json so the
+benchmark, the profiler runs and production calls use the same shape.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TUNING_PATH = os.path.join(HERE, "tuning.json")
+
+
+def tune_fast(job, iteration=1, reps=2, log=None, only_kahan=False):
+    from .smx import PREC_FAST, fast_configs
+    rows = []
+    for i, desc in fast_configs():
+        if only_kahan and not desc[5]:
+            continue
+        job.set_fast_config(i)
+        job.vertical_line_locus(iteration, precision=PREC_FAST)   # warm-up
+        best = None
+        for _ in range(reps):
+            job.vertical_line_locus(iteration, precision=PREC_FAST)
+            ms = job.last_kernel_ms
+            best = ms if best is None else min(best, ms)
+        st = job.last_stats
+        rows.append(dict(config=i, tx=desc[0], ty=desc[1], lds_texels=desc[2], occ=desc[3], stage=desc[4], kahan=desc[5],
+                         kernel_ms=best, tiles_lds=st["tiles_lds"], waves_hbm=st["waves_hbm"]))
+        if log:
+            log(f"tune config {i} {desc}: {best:.2f} ms (tiles in LDS {st['tiles_lds']}, HBM waves {st['waves_hbm']})")
+    rows.sort(key=lambda r: r["kernel_ms"])
+    pick = [r for r in rows if r["kahan"]] or rows   # compensated summation is the contract
+    job.set_fast_config(pick[0]["config"])
+    return pick[0], rows
+
+
+def load(key):
+    try:
+        with open(TUNING_PATH) as f:
+            return json.load(f).get(key)
+    except (OSError, ValueError):
+        return None
+
+
+def save(key, best, rows, path=TUNING_PATH):
+    data = {}
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        pass
+    data[key] = dict(best=best, table=rows)
+    with open(path, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)

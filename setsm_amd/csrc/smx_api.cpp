@@ -595,8 +595,10 @@ long long smx_vertical_line_locus(smx_job *j, int iteration, int precision, int 
         if (r) return r;
     }
     HIPCHK(launch_vll_finalize(A, j->stream));
-    HIPCHK(hipMemcpyAsync(j->h_small, j->d_counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, j->stream));
+    HIPCHK(hipMemcpyAsync(j->h_small, j->d_counters, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
+    j->last_tiles_lds = j->h_small[2];
+    j->last_waves_hbm = j->h_small[3];
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, j->ev0, j->ev1));
     j->last_ms = ms;
@@ -684,6 +686,16 @@ int smx_fast_config_desc(int config, char *buf, int buflen)
 }
 
 double smx_job_last_kernel_ms(const smx_job *j) { return j ? j->last_ms : 0.0; }
+
+int smx_job_last_stats(const smx_job *j, long long *out4)
+{
+    if (!j || !out4) return fail(SMX_E_ARG, "null argument");
+    out4[0] = j->last_evaluated;
+    out4[1] = j->last_tiles_lds;
+    out4[2] = j->last_waves_hbm;
+    out4[3] = (long long)j->fast_cfg;
+    return 0;
+}
 long long smx_job_last_evaluated(const smx_job *j) { return j ? j->last_evaluated : 0; }
 
 int smx_job_sync(smx_job *j)

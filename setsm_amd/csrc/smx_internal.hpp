@@ -69,6 +69,8 @@ struct smx_job {
 
     double last_ms = 0;
     long long last_evaluated = 0;
+    long long last_tiles_lds = 0;   // FAST: tiles whose footprint was staged in LDS
+    long long last_waves_hbm = 0;   // FAST: waves that read taps from HBM
 };
 
 namespace smx {

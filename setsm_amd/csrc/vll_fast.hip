@@ -35,7 +35,8 @@ struct FastArgs {
     int tiles_x, tiles_y;
     int tx, ty;                    // cells per tile
     uint32_t grid_w, grid_h;
-    int lds_rows;                  // LDS rows available per image
+    int lds_texels;                // float2 texels of LDS for both images' boxes
+    int stage;                     // 0: read every tap from HBM
 };
 
 // ------------------------------------------------------ tile run lists
@@ -81,11 +82,12 @@ __global__ void __launch_bounds__(256) k_tile_fill(CellArgs A, FastArgs F, int2 
 
 // ------------------------------------------------------------- samplers
 
-// Texels of one image staged in LDS: float2 per texel, row pitch P.
-template <int P>
+// Texels of one image staged in LDS: float2 per texel, rows of the box
+// packed back to back (pitch = box width).
 struct LdsImg {
-    const float2 *base;   // texel (0, 0) of the box
-    __device__ __forceinline__ float2 at(int row, int col) const { return base[row * P + col]; }
+    const float2 *base;   // texel (0, 0) of the window origin
+    int pitch;
+    __device__ __forceinline__ float2 at(int row, int col) const { return base[row * pitch + col]; }
 };
 
 // Texels read from HBM (packed u16 pair), shifted by a pivot on the fly.
@@ -320,7 +322,7 @@ __device__ __forceinline__ bool global_window(const DevView &V0, const DevView &
                                         (float)((int)V1.height - 1 - ity), rho);
 }
 
-template <int H, int PITCH, bool kKahan>
+template <int H, bool kKahan>
 __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, CellArgs &A, const FastArgs &F,
                                               float2 *lds)
 {
@@ -378,30 +380,32 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
             tb.y0[ti] = y0;
             tb.w[ti] = max(x1 - x0 + 1, 1);
             tb.h[ti] = max(y1 - y0 + 1, 1);
-            if (tb.w[ti] > PITCH || tb.h[ti] > F.lds_rows) fits = 0;
+            if (!F.stage) fits = 0;
             const long cx = min(max((x0 + x1) / 2, 0), (int)v.width - 1);
             const long cy = min(max((y0 + y1) / 2, 0), (int)v.height - 1);
             const uint32_t t = v.texel[cy * v.width + cx];
             tb.piv_i[ti] = (float)(t & 0xffffu);
             tb.piv_m[ti] = (float)(t >> 16);
         }
+        if (tb.w[0] * tb.h[0] + tb.w[1] * tb.h[1] > F.lds_texels) fits = 0;
         tb.fits = fits;
         tb.nodata = 0;
     }
     __syncthreads();
     const bool fits = tb.fits != 0;
+    if (threadIdx.x == 0 && fits) atomicAdd(A.accessible + 2, 1ull);
     if (fits) {
         int nod = 0;
         for (int ti = 0; ti < 2; ti++) {
             const DevView &v = P.view[ti];
-            float2 *dst = lds + ti * PITCH * F.lds_rows;
+            float2 *dst = lds + (ti ? tb.w[0] * tb.h[0] : 0);
             const int w = tb.w[ti], h = tb.h[ti];
             const float pi = tb.piv_i[ti], pm = tb.piv_m[ti];
             for (int k = threadIdx.x; k < w * h; k += blockDim.x) {
                 const int rr = k / w, cc = k % w;
                 const uint32_t t = v.texel[(long)(tb.y0[ti] + rr) * v.width + tb.x0[ti] + cc];
                 nod |= (t & 0xffffu) == 0;
-                dst[rr * PITCH + cc] = make_float2((float)(t & 0xffffu) - pi, (float)(t >> 16) - pm);
+                dst[k] = make_float2((float)(t & 0xffffu) - pi, (float)(t >> 16) - pm);
             }
         }
         if (nod) tb.nodata = 1;   // benign race: every writer stores 1
@@ -452,9 +456,9 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
             if (use_lds) {
                 const float fxl = (float)(hy.rpx - flx), fyl = (float)(hy.rpy - fly);
                 const float fxt = (float)(hy.tpx - ftx), fyt = (float)(hy.tpy - fty);
-                const LdsImg<PITCH> Lw{lds + (long)(ily - O - tb.y0[0]) * PITCH + (ilx - O - tb.x0[0])};
-                const LdsImg<PITCH> Rw{lds + (long)PITCH * F.lds_rows + (long)(ity - O - tb.y0[1]) * PITCH +
-                                       (itx - O - tb.x0[1])};
+                const LdsImg Lw{lds + (ily - O - tb.y0[0]) * tb.w[0] + (ilx - O - tb.x0[0]), tb.w[0]};
+                const LdsImg Rw{lds + tb.w[0] * tb.h[0] + (ity - O - tb.y0[1]) * tb.w[1] + (itx - O - tb.x0[1]),
+                                tb.w[1]};
                 if (fast_ok)
                     all3 = fast_window<H, false, kKahan>(Lw, Rw, fxl, fyl, fxt, fyt, cos0, sin0, tb.piv_i[0],
                                                          tb.piv_i[1], 0, 0, 0, 0, 0, 0, 0, 0, rho);
@@ -468,6 +472,7 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
                 all3 = global_window<H, kKahan>(V0, V1, hy.rpx, hy.rpy, hy.tpx, hy.tpy, cos0, sin0, rho);
             }
         }
+        if (lane == 0 && !use_lds && __any(do_ncc)) atomicAdd(A.accessible + 3, 1ull);
         // setsm_code.cpp:6456-6493: average of the level term and, on even
         // iterations above level 0, the level-1 term.
         double sum = 0, cnt = 0;
@@ -499,19 +504,27 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
 
 // Two register budgets of the same body: the compiler's own choice, and one
 // capped so that four waves fit per SIMD. The autotuner picks per shape.
-template <int H, int PITCH, bool kKahan>
+template <int H, bool kKahan>
 __global__ void __launch_bounds__(256) k_vll_fast(const VllParams *__restrict__ Pp, CellArgs A, FastArgs F)
 {
     extern __shared__ float2 lds[];
-    vll_fast_body<H, PITCH, kKahan>(Pp, A, F, lds);
+    vll_fast_body<H, kKahan>(Pp, A, F, lds);
 }
 
-template <int H, int PITCH, bool kKahan>
+template <int H, bool kKahan>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_vll_fast_o4(const VllParams *__restrict__ Pp, CellArgs A, FastArgs F)
 {
     extern __shared__ float2 lds[];
-    vll_fast_body<H, PITCH, kKahan>(Pp, A, F, lds);
+    vll_fast_body<H, kKahan>(Pp, A, F, lds);
+}
+
+template <int H, bool kKahan>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
+k_vll_fast_o3(const VllParams *__restrict__ Pp, CellArgs A, FastArgs F)
+{
+    extern __shared__ float2 lds[];
+    vll_fast_body<H, kKahan>(Pp, A, F, lds);
 }
 
 }  // namespace smx
@@ -520,27 +533,39 @@ k_vll_fast_o4(const VllParams *__restrict__ Pp, CellArgs A, FastArgs F)
 namespace smx {
 
 struct FastCfg {
-    int tx, ty, lds_rows, o4;
+    int tx, ty, lds_texels, occ, stage, kahan;
 };
 
 // Launch shapes the autotuner chooses from (cells per tile, LDS rows per
 // image, register cap for four waves per SIMD).
 static const FastCfg kFastCfgs[] = {
-    {4, 4, 48, 0}, {4, 4, 48, 1}, {8, 4, 48, 0}, {8, 4, 48, 1}, {4, 8, 64, 0},
-    {4, 8, 64, 1}, {8, 8, 64, 0}, {8, 8, 64, 1}, {2, 4, 40, 0}, {2, 4, 40, 1},
+    {1, 4, 3072, 0, 1, 1}, {2, 4, 3584, 0, 1, 1}, {2, 2, 3072, 0, 1, 1}, {4, 4, 4608, 0, 1, 1},
+    {1, 8, 4096, 0, 1, 1}, {2, 8, 4608, 0, 1, 1}, {4, 8, 6144, 0, 1, 1}, {1, 16, 6144, 0, 1, 1},
+    {1, 4, 3072, 3, 1, 1}, {2, 4, 3584, 3, 1, 1}, {4, 4, 4608, 3, 1, 1}, {1, 4, 3072, 4, 1, 1},
+    {2, 4, 3584, 0, 0, 1}, {1, 4, 3072, 0, 1, 0}, {2, 4, 3584, 3, 1, 0}, {2, 8, 4608, 3, 1, 1},
+    {1, 8, 4096, 3, 1, 1}, {4, 2, 3584, 3, 1, 1}, {2, 2, 3072, 3, 1, 1}, {2, 4, 3584, 4, 1, 1},
 };
 constexpr int kNumFastCfgs = sizeof(kFastCfgs) / sizeof(kFastCfgs[0]);
-constexpr int kPitch = 64;
+
+template <int H, bool kKahan>
+static hipError_t launch_fast_hk(int occ, dim3 grid, size_t lds, hipStream_t s, const VllParams *dP,
+                                 const CellArgs &A, const FastArgs &F)
+{
+    if (occ == 4)
+        hipLaunchKernelGGL((k_vll_fast_o4<H, kKahan>), grid, dim3(256), lds, s, dP, A, F);
+    else if (occ == 3)
+        hipLaunchKernelGGL((k_vll_fast_o3<H, kKahan>), grid, dim3(256), lds, s, dP, A, F);
+    else
+        hipLaunchKernelGGL((k_vll_fast<H, kKahan>), grid, dim3(256), lds, s, dP, A, F);
+    return hipGetLastError();
+}
 
 template <int H>
-static hipError_t launch_fast_h(bool o4, dim3 grid, size_t lds, hipStream_t s, const VllParams *dP,
+static hipError_t launch_fast_h(const FastCfg &cfg, dim3 grid, size_t lds, hipStream_t s, const VllParams *dP,
                                 const CellArgs &A, const FastArgs &F)
 {
-    if (o4)
-        hipLaunchKernelGGL((k_vll_fast_o4<H, kPitch, true>), grid, dim3(256), lds, s, dP, A, F);
-    else
-        hipLaunchKernelGGL((k_vll_fast<H, kPitch, true>), grid, dim3(256), lds, s, dP, A, F);
-    return hipGetLastError();
+    return cfg.kahan ? launch_fast_hk<H, true>(cfg.occ, grid, lds, s, dP, A, F)
+                     : launch_fast_hk<H, false>(cfg.occ, grid, lds, s, dP, A, F);
 }
 
 int launch_vll_fast(smx_job *j, int half, const CellArgs &A, hipEvent_t ev0, hipEvent_t ev1)
@@ -554,7 +579,8 @@ int launch_vll_fast(smx_job *j, int half, const CellArgs &A, hipEvent_t ev0, hip
     F.grid_h = j->lv.grid_h;
     F.tiles_x = (int)((F.grid_w + F.tx - 1) / F.tx);
     F.tiles_y = (int)((F.grid_h + F.ty - 1) / F.ty);
-    F.lds_rows = cfg.lds_rows;
+    F.lds_texels = cfg.lds_texels;
+    F.stage = cfg.stage;
     const long ntiles = (long)F.tiles_x * F.tiles_y;
     int r = fast_scratch(j, ntiles, A.nruns);
     if (r) return r;
@@ -568,12 +594,12 @@ int launch_vll_fast(smx_job *j, int half, const CellArgs &A, hipEvent_t ev0, hip
     r = job_scan(j, j->d_tile_count, j->d_tile_off, ntiles + 1);
     if (r) return r;
     hipLaunchKernelGGL(k_tile_fill, dim3(nb), dim3(256), 0, j->stream, A, F, j->d_tile_runs);
-    const size_t lds = (size_t)2 * kPitch * F.lds_rows * sizeof(float2);
+    const size_t lds = cfg.stage ? (size_t)F.lds_texels * sizeof(float2) : 0;
     if (ev0) (void)hipEventRecord(ev0, j->stream);
     switch (half) {
-    case 5: e = launch_fast_h<5>(cfg.o4, dim3((unsigned)ntiles), lds, j->stream, j->d_params, A, F); break;
-    case 6: e = launch_fast_h<6>(cfg.o4, dim3((unsigned)ntiles), lds, j->stream, j->d_params, A, F); break;
-    case 7: e = launch_fast_h<7>(cfg.o4, dim3((unsigned)ntiles), lds, j->stream, j->d_params, A, F); break;
+    case 5: e = launch_fast_h<5>(cfg, dim3((unsigned)ntiles), lds, j->stream, j->d_params, A, F); break;
+    case 6: e = launch_fast_h<6>(cfg, dim3((unsigned)ntiles), lds, j->stream, j->d_params, A, F); break;
+    case 7: e = launch_fast_h<7>(cfg, dim3((unsigned)ntiles), lds, j->stream, j->d_params, A, F); break;
     default: return fail(SMX_E_UNSUPPORTED, "half %d", half);
     }
     if (ev1) (void)hipEventRecord(ev1, j->stream);
@@ -586,8 +612,8 @@ int fast_num_configs() { return kNumFastCfgs; }
 int fast_config_desc(int i, char *buf, int n)
 {
     if (i < 0 || i >= kNumFastCfgs) return -1;
-    return snprintf(buf, (size_t)n, "%d %d %d %d", kFastCfgs[i].tx, kFastCfgs[i].ty, kFastCfgs[i].lds_rows,
-                    kFastCfgs[i].o4);
+    return snprintf(buf, (size_t)n, "%d %d %d %d %d %d", kFastCfgs[i].tx, kFastCfgs[i].ty, kFastCfgs[i].lds_texels,
+                    kFastCfgs[i].occ, kFastCfgs[i].stage, kFastCfgs[i].kahan);
 }
 
 }  // namespace smx

@@ -33,7 +33,7 @@ EXPORTS = [
     "smx_job_get_nccresult", "smx_job_set_nccresult", "smx_initialize_voxel", "smx_vertical_line_locus",
     "smx_awncc", "smx_job_num_hypotheses", "smx_job_get_voxels", "smx_job_get_voxel_rows", "smx_job_set_voxels",
     "smx_job_last_kernel_ms", "smx_job_last_evaluated", "smx_job_sync", "smx_fast_num_configs",
-    "smx_job_set_fast_config", "smx_fast_config_desc",
+    "smx_job_set_fast_config", "smx_fast_config_desc", "smx_job_last_stats",
 ]
 
 
@@ -85,6 +85,7 @@ def lib():
             "smx_fast_num_configs": (i, []),
             "smx_job_set_fast_config": (i, [vp, i]),
             "smx_fast_config_desc": (i, [i, C.c_char_p, i]),
+            "smx_job_last_stats": (i, [vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -225,6 +226,12 @@ class Job:
     @property
     def device_bytes(self) -> int:
         return int(lib().smx_job_device_bytes(self.h))
+
+    @property
+    def last_stats(self) -> dict:
+        out = np.zeros(4, np.int64)
+        _check(lib().smx_job_last_stats(self.h, _p(out)))
+        return dict(evaluated=int(out[0]), tiles_lds=int(out[1]), waves_hbm=int(out[2]), fast_config=int(out[3]))
 
     def set_fast_config(self, config: int):
         _check(lib().smx_job_set_fast_config(self.h, int(config)))

@@ -135,22 +135,30 @@ def _fbm(n, hurst, gen, device):
     f[0, 0] = 1.0
     amp = f ** (-(hurst + 1.0))
     amp[0, 0] = 0.0
-    re = torch.randn(f.shape, generator=gen).to(device)
-    im = torch.randn(f.shape, generator=gen).to(device)
+    re = torch.randn(f.shape, generator=gen, device=device)
+    im = torch.randn(f.shape, generator=gen, device=device)
     spec = torch.complex(re * amp, im * amp)
+    del re, im, amp, f
     z = torch.fft.irfft2(spec, s=(n, n))
     return z
 
 
 def _gauss_blur(x, sigma):
+    """Separable Gaussian blur (reflect padding) by shifted adds."""
     r = int(math.ceil(3 * sigma))
-    t = torch.arange(-r, r + 1, device=x.device, dtype=x.dtype)
+    t = torch.arange(-r, r + 1, dtype=torch.float64)
     k = torch.exp(-0.5 * (t / sigma) ** 2)
-    k = k / k.sum()
-    x = x[None, None]
-    x = torch.nn.functional.conv2d(torch.nn.functional.pad(x, (r, r, 0, 0), mode="reflect"), k.view(1, 1, 1, -1))
-    x = torch.nn.functional.conv2d(torch.nn.functional.pad(x, (0, 0, r, r), mode="reflect"), k.view(1, 1, -1, 1))
-    return x[0, 0]
+    k = (k / k.sum()).tolist()
+    h, w = x.shape
+    xp = torch.nn.functional.pad(x[None, None], (r, r, 0, 0), mode="reflect")[0, 0]
+    out = torch.zeros_like(x)
+    for i, kw in enumerate(k):
+        out += kw * xp[:, i:i + w]
+    xp = torch.nn.functional.pad(out[None, None], (0, 0, r, r), mode="reflect")[0, 0]
+    out = torch.zeros_like(x)
+    for i, kw in enumerate(k):
+        out += kw * xp[i:i + h, :]
+    return out
 
 
 def _sample(grid, origin, post, E, N):
@@ -172,7 +180,7 @@ def _sample(grid, origin, post, E, N):
 
 
 def make_scene(spec: SceneSpec, device="cpu") -> Scene:
-    gen = torch.Generator(device="cpu").manual_seed(spec.seed)
+    gen = torch.Generator(device=device).manual_seed(spec.seed)
     W = spec.image_px
     side = W * spec.gsd
     tan_max = max(abs(math.tan(math.radians(v))) for v in spec.view_deg)
@@ -192,7 +200,7 @@ def make_scene(spec: SceneSpec, device="cpu") -> Scene:
     # band-limited albedo on the image-pixel ground grid
     apost = spec.gsd
     an = int(math.ceil(ext / apost)) + 2
-    noise = torch.randn((an, an), generator=gen).to(device)
+    noise = torch.randn((an, an), generator=gen, device=device)
     alb = _gauss_blur(noise, spec.albedo_sigma_px)
     alb = (alb - alb.mean()) / alb.std()
     alb = (0.55 + 0.22 * alb).clamp(0.05, 1.0)
