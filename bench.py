@@ -56,6 +56,23 @@ def flops_per_hypothesis(half):
     return 24 * n0 + 20 * (n0 + n1 + n2)
 
 
+def save_scene(d, sc):
+    os.makedirs(d, exist_ok=True)
+    np.savez(os.path.join(d, "scene.npz"), img0=sc.images[0], img1=sc.images[1], rpc0=sc.rpcs[0], rpc1=sc.rpcs[1],
+             terrain=sc.terrain, origin=np.array(sc.terrain_origin), extent=np.array(sc.extent),
+             resid=np.array(sc.fit_residual_px))
+
+
+def load_scene(d, spec):
+    from setsm_amd.synth import Scene
+    path = os.path.join(d, "scene.npz")
+    if not os.path.exists(path):
+        return None
+    z = np.load(path)
+    return Scene(spec, [z["img0"], z["img1"]], [z["rpc0"], z["rpc1"]], z["terrain"], tuple(z["origin"]),
+                 tuple(z["extent"]), list(z["resid"]))
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -72,6 +89,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--autotune", action="store_true", help="time every FAST launch shape and store the best")
+    ap.add_argument("--scene-cache", default="", help="directory to load / save the rendered scene")
+    ap.add_argument("--copy-calib-gib", type=float, default=0.0,
+                    help="first run one device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -85,11 +105,25 @@ def main():
     from setsm_amd.synth import GridSpec, SceneSpec, make_grid, make_scene
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.copy_calib_gib > 0:
+        nbytes = int(args.copy_calib_gib * 2**30)
+        src = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {nbytes} bytes read, {nbytes} bytes written")
+        del src, dst
+        torch.cuda.empty_cache()
     t0 = time.time()
     spec = SceneSpec(image_px=args.image_px, gsd=0.5, relief=300.0, base=200.0, seed=args.seed)
-    sc = make_scene(spec, device="cuda")
-    torch.cuda.empty_cache()
-    log(f"scene {args.image_px}^2 x2 rendered in {time.time() - t0:.1f}s, RPC fit residual {max(sc.fit_residual_px):.2e} px")
+    sc = load_scene(args.scene_cache, spec) if args.scene_cache else None
+    if sc is None:
+        sc = make_scene(spec, device="cuda")
+        torch.cuda.empty_cache()
+        if args.scene_cache:
+            save_scene(args.scene_cache, sc)
+    log(f"scene {args.image_px}^2 x2 ready in {time.time() - t0:.1f}s, RPC fit residual {max(sc.fit_residual_px):.2e} px")
 
     side = args.image_px * spec.gsd
     grid_res = 2.0

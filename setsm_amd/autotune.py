@@ -45,11 +45,23 @@ def tune_fast(job, iteration=1, reps=2, log=None, only_kahan=False):
 
 
 def load(key):
+    """Stored best shape for `key`, with its current index in the library's
+    shape table (matched by shape, not by position)."""
     try:
         with open(TUNING_PATH) as f:
-            return json.load(f).get(key)
+            entry = json.load(f).get(key)
     except (OSError, ValueError):
         return None
+    if not entry:
+        return None
+    from .smx import fast_configs
+    b = entry["best"]
+    want = (b["tx"], b["ty"], b["lds_texels"], b["occ"], b["stage"], b["kahan"])
+    for i, desc in fast_configs():
+        if tuple(desc) == want:
+            entry["best"] = dict(b, config=i)
+            return entry
+    return None
 
 
 def save(key, best, rows, path=TUNING_PATH):

@@ -12,6 +12,17 @@ namespace smx {
 
 constexpr double kPi = 3.141592653589793;   // Typedefine.hpp:27
 
+// Pointers read out of argument structs are generic; casting them to the
+// global address space lets the compiler emit global_load (32-bit offsets,
+// no flat aperture check) instead of flat_load.
+template <class T>
+using gptr_t = const __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gptr_t<T> gptr(const T *p)
+{
+    return (gptr_t<T>)p;
+}
+
 struct DevView {
     const uint32_t *texel;   // (intensity | magnitude << 16) per pixel
     const uint8_t *ori;

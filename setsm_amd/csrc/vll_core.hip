@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(256) k_vll_finalize(CellArgs A)
 
 // SubFunctions.cpp:3317-3326 InterpolatePatch on both channels of a packed
 // texel quad.
-__device__ __forceinline__ void interp2(const uint32_t *T, long pos, uint32_t w, double dx, double dy, double &v,
+__device__ __forceinline__ void interp2(gptr_t<uint32_t> T, long pos, uint32_t w, double dx, double dy, double &v,
                                         double &m)
 {
     const uint32_t a = T[pos], b = T[pos + 1], c = T[pos + w], d = T[pos + 1 + w];
@@ -113,8 +113,8 @@ __device__ __forceinline__ bool sample_fp64(const DevView &Lv, const DevView &Rv
         lx + 1 < Lv.width) {
         const long pl = (long)lx + (long)ly * (long)Lv.width;
         const long pr = (long)rx + (long)ry * (long)Rv.width;
-        interp2(Lv.texel, pl, Lv.width, lx - floor(lx), ly - floor(ly), l, lm);
-        interp2(Rv.texel, pr, Rv.width, rx - floor(rx), ry - floor(ry), r, rm);
+        interp2(gptr(Lv.texel), pl, Lv.width, lx - floor(lx), ly - floor(ly), l, lm);
+        interp2(gptr(Rv.texel), pr, Rv.width, rx - floor(rx), ry - floor(ry), r, rm);
         return l > 0 && r > 0;
     }
     return false;

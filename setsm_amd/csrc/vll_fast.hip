@@ -82,19 +82,22 @@ __global__ void __launch_bounds__(256) k_tile_fill(CellArgs A, FastArgs F, int2 
 
 // ------------------------------------------------------------- samplers
 
+extern __shared__ float2 smx_lds[];
+
 // Texels of one image staged in LDS: float2 per texel, rows of the box
-// packed back to back (pitch = box width).
+// packed back to back (pitch = box width). Indexing the __shared__ array
+// itself keeps every tap a ds_read.
 struct LdsImg {
-    const float2 *base;   // texel (0, 0) of the window origin
+    int base;   // index of texel (0, 0) of the window origin
     int pitch;
-    __device__ __forceinline__ float2 at(int row, int col) const { return base[row * pitch + col]; }
+    __device__ __forceinline__ float2 at(int row, int col) const { return smx_lds[base + row * pitch + col]; }
 };
 
 // Texels read from HBM (packed u16 pair), shifted by a pivot on the fly.
 // Coordinates are clamped into the image so that edge taps stay in bounds;
 // samples that need such taps are rejected by the per-sample test.
 struct GlobalImg {
-    const uint32_t *texel;
+    gptr_t<uint32_t> texel;
     int width, height;
     int x0, y0;   // image position of at(0, 0)
     float piv_i, piv_m;
@@ -192,7 +195,8 @@ __device__ __forceinline__ bool fast_window(const SL &L, const SR &R, float fxl,
             part[k].m = Acc5{0, 0, 0, 0, 0};
         }
         const float frow = (float)row;
-        const float rs = frow * sin0, rc = frow * cos0;
+        // x = ftx + col*cos - row*sin, y = fty + col*sin + row*cos
+        const float xr = fmaf(-frow, sin0, ftx), yr = fmaf(frow, cos0, fty);
 #pragma unroll
         for (int col = -H; col <= H; col++) {
             const int bits = window_bits_ce(H, row, col);
@@ -202,8 +206,8 @@ __device__ __forceinline__ bool fast_window(const SL &L, const SR &R, float fxl,
             const float2 l = bilerp(L.at(O + row, O + col), L.at(O + row, O + col + 1), L.at(O + row + 1, O + col),
                                     L.at(O + row + 1, O + col + 1), fxl, fyl);
             // right: rotated offsets (setsm_code.cpp:6437-6439)
-            const float x = ftx + fmaf((float)col, cos0, -rs);
-            const float y = fty + fmaf((float)col, sin0, rc);
+            const float x = fmaf((float)col, cos0, xr);
+            const float y = fmaf((float)col, sin0, yr);
             const float flx = floorf(x), fly = floorf(y);
             const int ix = (int)flx + O, iy = (int)fly + O;
             const float wx = x - flx, wy = y - fly;
@@ -310,9 +314,9 @@ __device__ __forceinline__ bool global_window(const DevView &V0, const DevView &
     const int clx = min(max(ilx, 0), (int)V0.width - 1), cly = min(max(ily, 0), (int)V0.height - 1);
     const int ctx = min(max(itx, 0), (int)V1.width - 1), cty = min(max(ity, 0), (int)V1.height - 1);
     const uint32_t pl = V0.texel[(long)cly * V0.width + clx], pr = V1.texel[(long)cty * V1.width + ctx];
-    const GlobalImg Lg{V0.texel, (int)V0.width, (int)V0.height, ilx - O, ily - O, (float)(pl & 0xffffu),
+    const GlobalImg Lg{gptr(V0.texel), (int)V0.width, (int)V0.height, ilx - O, ily - O, (float)(pl & 0xffffu),
                        (float)(pl >> 16)};
-    const GlobalImg Rg{V1.texel, (int)V1.width, (int)V1.height, itx - O, ity - O, (float)(pr & 0xffffu),
+    const GlobalImg Rg{gptr(V1.texel), (int)V1.width, (int)V1.height, itx - O, ity - O, (float)(pr & 0xffffu),
                        (float)(pr >> 16)};
     return fast_window<H, true, kKahan>(Lg, Rg, (float)(rpx - flx), (float)(rpy - fly), (float)(tpx - ftx),
                                         (float)(tpy - fty), cos0, sin0, Lg.piv_i, Rg.piv_i, -(float)rpx,
@@ -323,8 +327,7 @@ __device__ __forceinline__ bool global_window(const DevView &V0, const DevView &
 }
 
 template <int H, bool kKahan>
-__device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, CellArgs &A, const FastArgs &F,
-                                              float2 *lds)
+__device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, CellArgs &A, const FastArgs &F)
 {
     __shared__ int sbox[2][4];
     __shared__ TileBox tb;
@@ -398,14 +401,14 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
         int nod = 0;
         for (int ti = 0; ti < 2; ti++) {
             const DevView &v = P.view[ti];
-            float2 *dst = lds + (ti ? tb.w[0] * tb.h[0] : 0);
+            const int dst = ti ? tb.w[0] * tb.h[0] : 0;
             const int w = tb.w[ti], h = tb.h[ti];
             const float pi = tb.piv_i[ti], pm = tb.piv_m[ti];
             for (int k = threadIdx.x; k < w * h; k += blockDim.x) {
                 const int rr = k / w, cc = k % w;
-                const uint32_t t = v.texel[(long)(tb.y0[ti] + rr) * v.width + tb.x0[ti] + cc];
+                const uint32_t t = gptr(v.texel)[(long)(tb.y0[ti] + rr) * v.width + tb.x0[ti] + cc];
                 nod |= (t & 0xffffu) == 0;
-                dst[k] = make_float2((float)(t & 0xffffu) - pi, (float)(t >> 16) - pm);
+                smx_lds[dst + k] = make_float2((float)(t & 0xffffu) - pi, (float)(t >> 16) - pm);
             }
         }
         if (nod) tb.nodata = 1;   // benign race: every writer stores 1
@@ -456,9 +459,8 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
             if (use_lds) {
                 const float fxl = (float)(hy.rpx - flx), fyl = (float)(hy.rpy - fly);
                 const float fxt = (float)(hy.tpx - ftx), fyt = (float)(hy.tpy - fty);
-                const LdsImg Lw{lds + (ily - O - tb.y0[0]) * tb.w[0] + (ilx - O - tb.x0[0]), tb.w[0]};
-                const LdsImg Rw{lds + tb.w[0] * tb.h[0] + (ity - O - tb.y0[1]) * tb.w[1] + (itx - O - tb.x0[1]),
-                                tb.w[1]};
+                const LdsImg Lw{(ily - O - tb.y0[0]) * tb.w[0] + (ilx - O - tb.x0[0]), tb.w[0]};
+                const LdsImg Rw{tb.w[0] * tb.h[0] + (ity - O - tb.y0[1]) * tb.w[1] + (itx - O - tb.x0[1]), tb.w[1]};
                 if (fast_ok)
                     all3 = fast_window<H, false, kKahan>(Lw, Rw, fxl, fyl, fxt, fyt, cos0, sin0, tb.piv_i[0],
                                                          tb.piv_i[1], 0, 0, 0, 0, 0, 0, 0, 0, rho);
@@ -507,24 +509,21 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
 template <int H, bool kKahan>
 __global__ void __launch_bounds__(256) k_vll_fast(const VllParams *__restrict__ Pp, CellArgs A, FastArgs F)
 {
-    extern __shared__ float2 lds[];
-    vll_fast_body<H, kKahan>(Pp, A, F, lds);
+    vll_fast_body<H, kKahan>(Pp, A, F);
 }
 
 template <int H, bool kKahan>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_vll_fast_o4(const VllParams *__restrict__ Pp, CellArgs A, FastArgs F)
 {
-    extern __shared__ float2 lds[];
-    vll_fast_body<H, kKahan>(Pp, A, F, lds);
+    vll_fast_body<H, kKahan>(Pp, A, F);
 }
 
 template <int H, bool kKahan>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
 k_vll_fast_o3(const VllParams *__restrict__ Pp, CellArgs A, FastArgs F)
 {
-    extern __shared__ float2 lds[];
-    vll_fast_body<H, kKahan>(Pp, A, F, lds);
+    vll_fast_body<H, kKahan>(Pp, A, F);
 }
 
 }  // namespace smx

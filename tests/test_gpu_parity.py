@@ -165,7 +165,9 @@ def test_fast_precision(require_gpu, level, iters):
 
 
 def test_fast_configs_agree(require_gpu):
-    """Every FAST launch shape gives the same cube (tiling is not arithmetic)."""
+    """Every FAST launch shape evaluates the same hypotheses with the same
+    flags; the mean NCC may differ only by FP32 rounding (the LDS staging
+    pivot depends on the tile box), far inside the 1e-5 contract."""
     from setsm_amd.smx import PREC_FAST, fast_configs
     sc, pyr = scene(image_px=512, levels=1)
     g = level_grid(sc, 0, 40)
@@ -180,5 +182,12 @@ def test_fast_configs_agree(require_gpu):
         if ref is None:
             ref = r
         else:
-            assert np.array_equal(ref["words"], r["words"]), f"config {i} differs"
+            assert np.array_equal(ref["words"] & 0xFF, r["words"] & 0xFF), f"config {i}: flags differ"
+            ev = np.isfinite(ref["rho"])
+            assert np.array_equal(ev, np.isfinite(r["rho"])), f"config {i}: evaluated sets differ"
+            d = np.abs(ref["rho"][ev].astype(np.float64) - r["rho"][ev])
+            dq = np.abs((ref["words"] >> 16).astype(np.uint16).view(np.int16).astype(int)
+                        - (r["words"] >> 16).astype(np.uint16).view(np.int16).astype(int))
+            print(f"  vs config 0: max|drho|={d.max():.2e} incc_diff={int((dq > 0).sum())}", flush=True)
+            assert d.max() < 2e-6 and dq.max() <= 1, f"config {i} differs beyond rounding"
     job.close()
