@@ -1,0 +1,44 @@
+"""The C ABI library loads and exports every entry point include/smx_vll.h
+declares (no GPU work is started here)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "smx_vll.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(smx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_lists_the_trio():
+    syms = declared_symbols()
+    for s in ("smx_initialize_voxel", "smx_vertical_line_locus", "smx_awncc", "smx_last_error"):
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol():
+    from setsm_amd import smx
+    if not os.path.exists(smx.LIB_PATH):
+        pytest.skip("libsmx.so not built")
+    lib = C.CDLL(smx.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, f"not exported: {missing}"
+    assert set(smx.EXPORTS) <= set(declared_symbols())
+    assert lib.smx_version() >= 10000
+
+
+def test_errors_without_gpu_work_are_reported():
+    """Argument errors come back as codes with a message, before any HIP call."""
+    from setsm_amd import smx
+    if not os.path.exists(smx.LIB_PATH):
+        pytest.skip("libsmx.so not built")
+    L = smx.lib()
+    h = C.c_void_p()
+    assert L.smx_job_create(C.byref(h), 0) == -1
+    assert b"num_levels" in L.smx_last_error()
+    assert L.smx_job_set_fast_config(None, 0) == -1
