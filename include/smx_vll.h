@@ -113,6 +113,14 @@ int  smx_initialize_voxel(smx_job *job, int iteration);
  * before quantisation (float) for precision studies. */
 long long smx_vertical_line_locus(smx_job *job, int iteration, int precision, int keep_rho);
 
+/* VerticalLineLocus over several jobs (tiles of different sizes, height
+ * ranges and pyramids) at one window size. SMX_PREC_FAST runs ONE persistent
+ * launch that takes the tiles of all jobs from a work queue;
+ * SMX_PREC_FP64 launches the plain kernel once per job. accessible[i]
+ * receives job i's Accessable_grid. Not for combined-level iterations. */
+int  smx_vertical_line_locus_batch(smx_job **jobs, int n, int iteration, int precision, int keep_rho,
+                                   long long *accessible);
+
 /* AWNCC peak scan (setsm_code.cpp:7511-7702) without SGM aggregation, i.e.
  * the path taken when Pyramid_step > SGM_py. */
 int  smx_awncc(smx_job *job, int iteration);

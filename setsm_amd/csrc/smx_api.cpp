@@ -255,6 +255,8 @@ int smx_job_destroy(smx_job *j)
             jfree(j, v.ori);
         }
     jfree(j, j->scan_tmp);
+    job_free(j, j->d_batch);
+    j->d_batch = nullptr;
     jfree(j, j->d_params);
     jfree(j, j->d_counters);
     for (auto &s : j->sizes) (void)hipFree(s.first);
@@ -528,7 +530,12 @@ int smx_initialize_voxel(smx_job *j, int iteration)
     return 0;
 }
 
-long long smx_vertical_line_locus(smx_job *j, int iteration, int precision, int keep_rho)
+}  // extern "C"
+
+namespace smx {
+
+// VerticalLineLocus up to the NCC kernel: checks, parameters, gate, run list.
+static int vll_prepare(smx_job *j, int iteration, int precision, int keep_rho, CellArgs &A)
 {
     if (!j || !j->level_set || !j->searchspace_set) return fail(SMX_E_STATE, "level and search space must be set");
     if (precision != SMX_PREC_FP64 && precision != SMX_PREC_FAST) return fail(SMX_E_ARG, "unknown precision %d", precision);
@@ -557,7 +564,7 @@ long long smx_vertical_line_locus(smx_job *j, int iteration, int precision, int 
         if (r) return r;
         HIPCHK(hipMemsetAsync(j->d_rho, 0xff, (size_t)j->vox_total * sizeof(float), j->stream));   // NaN
     }
-    CellArgs A{};
+    A = CellArgs{};
     A.ncells = j->ncells;
     A.gridpt3 = j->d_gridpt3;
     A.ncc = j->d_ncc;
@@ -586,24 +593,86 @@ long long smx_vertical_line_locus(smx_job *j, int iteration, int precision, int 
     A.runs = j->d_runs;
     A.nruns = nruns;
     HIPCHK(launch_vll_fill_runs(A, j->stream));
-    if (precision == SMX_PREC_FP64) {
-        HIPCHK(hipEventRecord(j->ev0, j->stream));
-        HIPCHK(launch_vll_fp64(half, j->d_params, A, j->stream));
-        HIPCHK(hipEventRecord(j->ev1, j->stream));
-    } else {
-        r = launch_vll_fast(j, half, A, j->ev0, j->ev1);
-        if (r) return r;
-    }
+    return 0;
+}
+
+// After the NCC kernel: max_WNCC, counters. Returns Accessable_grid.
+static long long vll_finish(smx_job *j, const CellArgs &A)
+{
     HIPCHK(launch_vll_finalize(A, j->stream));
     HIPCHK(hipMemcpyAsync(j->h_small, j->d_counters, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
+    j->last_evaluated = j->h_small[1];
     j->last_tiles_lds = j->h_small[2];
     j->last_waves_hbm = j->h_small[3];
+    return (long long)j->h_small[0];
+}
+
+}  // namespace smx
+
+extern "C" {
+
+long long smx_vertical_line_locus(smx_job *j, int iteration, int precision, int keep_rho)
+{
+    CellArgs A;
+    int r = vll_prepare(j, iteration, precision, keep_rho, A);
+    if (r) return r;
+    if (precision == SMX_PREC_FP64) {
+        HIPCHK(hipEventRecord(j->ev0, j->stream));
+        HIPCHK(launch_vll_fp64(j->hp.half, j->d_params, A, j->stream));
+        HIPCHK(hipEventRecord(j->ev1, j->stream));
+    } else {
+        r = launch_vll_fast(j, j->hp.half, A, j->ev0, j->ev1);
+        if (r) return r;
+    }
+    const long long acc = vll_finish(j, A);
+    if (acc < 0) return acc;
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, j->ev0, j->ev1));
     j->last_ms = ms;
-    j->last_evaluated = j->h_small[1];
-    return (long long)j->h_small[0];
+    return acc;
+}
+
+int smx_vertical_line_locus_batch(smx_job **jobs, int n, int iteration, int precision, int keep_rho,
+                                  long long *accessible)
+{
+    if (!jobs || n < 1 || !accessible) return fail(SMX_E_ARG, "smx_vertical_line_locus_batch: bad argument");
+    for (int i = 0; i < n; i++) {
+        if (!jobs[i]) return fail(SMX_E_ARG, "null job %d", i);
+        if (jobs[i]->device != jobs[0]->device) return fail(SMX_E_ARG, "jobs of one batch share a device");
+    }
+    if (precision == SMX_PREC_FP64) {
+        // the plain kernel: one tile per launch
+        for (int i = 0; i < n; i++) {
+            const long long a = smx_vertical_line_locus(jobs[i], iteration, precision, keep_rho);
+            if (a < 0) return (int)a;
+            accessible[i] = a;
+        }
+        return 0;
+    }
+    std::vector<CellArgs> As((size_t)n);
+    for (int i = 0; i < n; i++) {
+        int r = vll_prepare(jobs[i], iteration, precision, keep_rho, As[i]);
+        if (r) return r;
+        if (jobs[i]->hp.half != jobs[0]->hp.half)
+            return fail(SMX_E_ARG, "a batch holds one window size (job %d has half %d, job 0 %d)", i,
+                        jobs[i]->hp.half, jobs[0]->hp.half);
+        if (jobs[i]->hp.combined_incc)
+            return fail(SMX_E_UNSUPPORTED, "combined-level iterations are launched per job");
+    }
+    smx_job *j0 = jobs[0];
+    int r = launch_vll_fast_batch(jobs, n, As.data(), j0->hp.half, j0->ev0, j0->ev1);
+    if (r) return r;
+    HIPCHK(hipStreamSynchronize(j0->stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, j0->ev0, j0->ev1));
+    for (int i = 0; i < n; i++) {
+        const long long a = vll_finish(jobs[i], As[i]);
+        if (a < 0) return (int)a;
+        accessible[i] = a;
+        jobs[i]->last_ms = ms;
+    }
+    return 0;
 }
 
 int smx_awncc(smx_job *j, int iteration)

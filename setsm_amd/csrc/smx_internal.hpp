@@ -66,6 +66,8 @@ struct smx_job {
     int64_t *d_tile_count = nullptr, *d_tile_off = nullptr;
     int2 *d_tile_runs = nullptr;
     long tiles_cap = 0, tile_runs_cap = 0;
+    void *d_batch = nullptr;                     // batched-launch descriptors
+    size_t batch_bytes = 0;
 
     double last_ms = 0;
     long long last_evaluated = 0;
@@ -97,6 +99,7 @@ hipError_t launch_init_voxel(const VllParams *dP, const InitArgs &A, hipStream_t
 hipError_t launch_relayout(const InitArgs &A, hipStream_t s);
 hipError_t launch_awncc(const VllParams *dP, const PeakArgs &A, hipStream_t s);
 int launch_vll_fast(smx_job *j, int half, const CellArgs &A, hipEvent_t ev0, hipEvent_t ev1);
+int launch_vll_fast_batch(smx_job **jobs, int n, const CellArgs *As, int half, hipEvent_t ev0, hipEvent_t ev1);
 int fast_num_configs();
 int fast_config_desc(int i, char *buf, int n);
 int fast_scratch(smx_job *j, long ntiles, long nruns);

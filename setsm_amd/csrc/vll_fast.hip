@@ -23,6 +23,7 @@
 // The per-hypothesis result is the mean NCC of SETSM within 1e-5 of the FP64
 // value; INCC is its (short)(rho*1000) truncation.
 #include <cstdio>
+#include <vector>
 
 #include "smx_internal.hpp"
 
@@ -327,12 +328,12 @@ __device__ __forceinline__ bool global_window(const DevView &V0, const DevView &
 }
 
 template <int H, bool kKahan>
-__device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, CellArgs &A, const FastArgs &F)
+__device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, const CellArgs &A, const FastArgs &F,
+                                              const long tile)
 {
     __shared__ int sbox[2][4];
     __shared__ TileBox tb;
     const VllParams &P = *Pp;
-    const long tile = blockIdx.x;
     const long r0 = F.tile_run_off[tile], r1 = F.tile_run_off[tile + 1];
     if (r0 == r1) return;
     constexpr int O = H + 2;
@@ -504,26 +505,96 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
     }
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8), each
+// with its own L2. Mapping block b to tile (b % 8) * ceil(n / 8) + b / 8 hands
+// every XCD one contiguous run of spatially adjacent tiles, so the texels
+// neighbouring tiles share are fetched into one L2 instead of eight.
+__device__ __forceinline__ long xcd_tile(long b, long n)
+{
+    constexpr long kXcd = 8;
+    const long per = (n + kXcd - 1) / kXcd;
+    const long x = b % kXcd, i = b / kXcd;
+    const long full = n - (per - 1) * kXcd;   // XCDs whose range holds `per` tiles
+    const long start = x < full ? x * per : full * per + (x - full) * (per - 1);
+    return start + i;
+}
+
+// One job of a batched launch.
+struct BatchJob {
+    const VllParams *P;
+    CellArgs A;
+    FastArgs F;
+    long tile_begin;   // first tile of this job in the flattened tile space
+    long ntiles;
+};
+
+struct BatchArgs {
+    const BatchJob *jobs;
+    int njobs;
+    long total_tiles;
+    unsigned long long *next;   // 8 per-XCD work counters
+};
+
+// Persistent form over the tiles of many jobs (tiles of different sizes,
+// height ranges and pyramids): each workgroup takes the next tile of its
+// XCD's contiguous range from an atomic counter and, when that range is
+// drained, steals from the other XCDs' ranges.
+template <int H, bool kKahan>
+__device__ __forceinline__ void vll_fast_persistent(const BatchArgs &B)
+{
+    __shared__ long cur;
+    constexpr long kXcd = 8;
+    const long per = (B.total_tiles + kXcd - 1) / kXcd;
+    const long home = blockIdx.x % kXcd;
+    for (long q = 0; q < kXcd; q++) {
+        const long x = (home + q) % kXcd;
+        const long lo = x * per;
+        const long hi = min(B.total_tiles, lo + per);
+        for (;;) {
+            __syncthreads();
+            if (threadIdx.x == 0) cur = lo + (long)atomicAdd(&B.next[x], 1ull);
+            __syncthreads();
+            const long t = cur;
+            if (t >= hi) break;
+            int j = 0;
+            while (j + 1 < B.njobs && B.jobs[j + 1].tile_begin <= t) j++;
+            const BatchJob &J = B.jobs[j];
+            vll_fast_body<H, kKahan>(J.P, J.A, J.F, t - J.tile_begin);
+        }
+    }
+}
+
+template <int H, bool kKahan>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) k_vll_fast_batch(BatchArgs B)
+{
+    vll_fast_persistent<H, kKahan>(B);
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
 // Two register budgets of the same body: the compiler's own choice, and one
 // capped so that four waves fit per SIMD. The autotuner picks per shape.
 template <int H, bool kKahan>
 __global__ void __launch_bounds__(256) k_vll_fast(const VllParams *__restrict__ Pp, CellArgs A, FastArgs F)
 {
-    vll_fast_body<H, kKahan>(Pp, A, F);
+    vll_fast_body<H, kKahan>(Pp, A, F, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 template <int H, bool kKahan>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_vll_fast_o4(const VllParams *__restrict__ Pp, CellArgs A, FastArgs F)
 {
-    vll_fast_body<H, kKahan>(Pp, A, F);
+    vll_fast_body<H, kKahan>(Pp, A, F, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 template <int H, bool kKahan>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
 k_vll_fast_o3(const VllParams *__restrict__ Pp, CellArgs A, FastArgs F)
 {
-    vll_fast_body<H, kKahan>(Pp, A, F);
+    vll_fast_body<H, kKahan>(Pp, A, F, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 }  // namespace smx
@@ -545,6 +616,7 @@ static const FastCfg kFastCfgs[] = {
     {1, 8, 4096, 3, 1, 1}, {4, 2, 3584, 3, 1, 1}, {2, 2, 3072, 3, 1, 1}, {2, 4, 3584, 4, 1, 1},
 };
 constexpr int kNumFastCfgs = sizeof(kFastCfgs) / sizeof(kFastCfgs[0]);
+constexpr int kDefaultFastCfg = 15;   // 2 x 8 cells, 3 waves/SIMD, Kahan: the tuned 16384^2 level-0 shape
 
 template <int H, bool kKahan>
 static hipError_t launch_fast_hk(int occ, dim3 grid, size_t lds, hipStream_t s, const VllParams *dP,
@@ -567,11 +639,11 @@ static hipError_t launch_fast_h(const FastCfg &cfg, dim3 grid, size_t lds, hipSt
                      : launch_fast_hk<H, false>(cfg.occ, grid, lds, s, dP, A, F);
 }
 
-int launch_vll_fast(smx_job *j, int half, const CellArgs &A, hipEvent_t ev0, hipEvent_t ev1)
+// Per-tile run lists of one job for the FAST kernel (tile shape of config ci).
+int fast_prepare(smx_job *j, int ci, const CellArgs &A, FastArgs &F)
 {
-    const int ci = (j->fast_cfg >= 0 && j->fast_cfg < kNumFastCfgs) ? j->fast_cfg : 0;
     const FastCfg cfg = kFastCfgs[ci];
-    FastArgs F{};
+    F = FastArgs{};
     F.tx = cfg.tx;
     F.ty = cfg.ty;
     F.grid_w = j->lv.grid_w;
@@ -586,14 +658,32 @@ int launch_vll_fast(smx_job *j, int half, const CellArgs &A, hipEvent_t ev0, hip
     F.tile_run_count = j->d_tile_count;
     F.tile_run_off = j->d_tile_off;
     F.tile_runs = j->d_tile_runs;
-    hipError_t e = hipMemsetAsync(j->d_tile_count + ntiles, 0, sizeof(int64_t), j->stream);
-    if (e != hipSuccess) return fail(SMX_E_HIP, "memset: %s", hipGetErrorString(e));
     const unsigned nb = (unsigned)((ntiles + 255) / 256);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, j->stream, (unsigned long long *)(j->d_tile_count + ntiles), 1);
     hipLaunchKernelGGL(k_tile_count, dim3(nb), dim3(256), 0, j->stream, A, F);
     r = job_scan(j, j->d_tile_count, j->d_tile_off, ntiles + 1);
     if (r) return r;
     hipLaunchKernelGGL(k_tile_fill, dim3(nb), dim3(256), 0, j->stream, A, F, j->d_tile_runs);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(SMX_E_HIP, "tile lists: %s", hipGetErrorString(e));
+    return 0;
+}
+
+static int cfg_index(const smx_job *j)
+{
+    return (j->fast_cfg >= 0 && j->fast_cfg < kNumFastCfgs) ? j->fast_cfg : kDefaultFastCfg;
+}
+
+int launch_vll_fast(smx_job *j, int half, const CellArgs &A, hipEvent_t ev0, hipEvent_t ev1)
+{
+    const int ci = cfg_index(j);
+    const FastCfg cfg = kFastCfgs[ci];
+    FastArgs F;
+    int r = fast_prepare(j, ci, A, F);
+    if (r) return r;
+    const long ntiles = (long)F.tiles_x * F.tiles_y;
     const size_t lds = cfg.stage ? (size_t)F.lds_texels * sizeof(float2) : 0;
+    hipError_t e = hipSuccess;
     if (ev0) (void)hipEventRecord(ev0, j->stream);
     switch (half) {
     case 5: e = launch_fast_h<5>(cfg, dim3((unsigned)ntiles), lds, j->stream, j->d_params, A, F); break;
@@ -603,6 +693,75 @@ int launch_vll_fast(smx_job *j, int half, const CellArgs &A, hipEvent_t ev0, hip
     }
     if (ev1) (void)hipEventRecord(ev1, j->stream);
     if (e != hipSuccess) return fail(SMX_E_HIP, "k_vll_fast: %s", hipGetErrorString(e));
+    return 0;
+}
+
+template <int H>
+static hipError_t launch_batch_h(bool kahan, int grid, size_t lds, hipStream_t s, const BatchArgs &B)
+{
+    if (kahan)
+        hipLaunchKernelGGL((k_vll_fast_batch<H, true>), dim3(grid), dim3(256), lds, s, B);
+    else
+        hipLaunchKernelGGL((k_vll_fast_batch<H, false>), dim3(grid), dim3(256), lds, s, B);
+    return hipGetLastError();
+}
+
+// One persistent launch over the tiles of every job (all at the same window
+// half size). As[i] are the prepared cell arguments of jobs[i]; the launch
+// goes on jobs[0]'s stream after every job's preparation has completed.
+int launch_vll_fast_batch(smx_job **jobs, int n, const CellArgs *As, int half, hipEvent_t ev0, hipEvent_t ev1)
+{
+    smx_job *j0 = jobs[0];
+    const int ci = cfg_index(j0);
+    const FastCfg cfg = kFastCfgs[ci];
+    std::vector<BatchJob> hb((size_t)n);
+    long total = 0;
+    for (int i = 0; i < n; i++) {
+        FastArgs F;
+        int r = fast_prepare(jobs[i], ci, As[i], F);
+        if (r) return r;
+        hb[i] = BatchJob{jobs[i]->d_params, As[i], F, total, (long)F.tiles_x * F.tiles_y};
+        total += hb[i].ntiles;
+    }
+    for (int i = 0; i < n; i++) {
+        hipError_t e = hipStreamSynchronize(jobs[i]->stream);
+        if (e != hipSuccess) return fail(SMX_E_HIP, "batch prepare: %s", hipGetErrorString(e));
+    }
+    const size_t need = sizeof(BatchJob) * (size_t)n + 8 * sizeof(unsigned long long);
+    if (need > j0->batch_bytes) {
+        job_free(j0, j0->d_batch);
+        j0->d_batch = nullptr;
+        j0->batch_bytes = 0;
+        int r = job_alloc(j0, &j0->d_batch, need);
+        if (r) return r;
+        j0->batch_bytes = need;
+    }
+    BatchJob *dj = (BatchJob *)j0->d_batch;
+    unsigned long long *dnext = (unsigned long long *)(dj + n);
+    hipError_t e = hipMemcpyAsync(dj, hb.data(), sizeof(BatchJob) * (size_t)n, hipMemcpyHostToDevice, j0->stream);
+    if (e != hipSuccess) return fail(SMX_E_HIP, "batch upload: %s", hipGetErrorString(e));
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, j0->stream, dnext, 8);
+    BatchArgs B{dj, n, total, dnext};
+    const size_t lds = (size_t)cfg.lds_texels * sizeof(float2);
+    int cus = 0;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, j0->device);
+    int per_cu = 0;
+    switch (half) {
+    case 5: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vll_fast_batch<5, true>, 256, lds); break;
+    case 6: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vll_fast_batch<6, true>, 256, lds); break;
+    case 7: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vll_fast_batch<7, true>, 256, lds); break;
+    default: return fail(SMX_E_UNSUPPORTED, "half %d", half);
+    }
+    int grid = std::max(1, cus) * std::max(1, per_cu);
+    grid = (grid + 7) / 8 * 8;
+    if (ev0) (void)hipEventRecord(ev0, j0->stream);
+    switch (half) {
+    case 5: e = launch_batch_h<5>(cfg.kahan != 0, grid, lds, j0->stream, B); break;
+    case 6: e = launch_batch_h<6>(cfg.kahan != 0, grid, lds, j0->stream, B); break;
+    case 7: e = launch_batch_h<7>(cfg.kahan != 0, grid, lds, j0->stream, B); break;
+    }
+    if (ev1) (void)hipEventRecord(ev1, j0->stream);
+    if (e != hipSuccess) return fail(SMX_E_HIP, "k_vll_fast_batch: %s", hipGetErrorString(e));
     return 0;
 }
 

@@ -33,7 +33,7 @@ EXPORTS = [
     "smx_job_get_nccresult", "smx_job_set_nccresult", "smx_initialize_voxel", "smx_vertical_line_locus",
     "smx_awncc", "smx_job_num_hypotheses", "smx_job_get_voxels", "smx_job_get_voxel_rows", "smx_job_set_voxels",
     "smx_job_last_kernel_ms", "smx_job_last_evaluated", "smx_job_sync", "smx_fast_num_configs",
-    "smx_job_set_fast_config", "smx_fast_config_desc", "smx_job_last_stats",
+    "smx_job_set_fast_config", "smx_fast_config_desc", "smx_job_last_stats", "smx_vertical_line_locus_batch",
 ]
 
 
@@ -86,6 +86,7 @@ def lib():
             "smx_job_set_fast_config": (i, [vp, i]),
             "smx_fast_config_desc": (i, [i, C.c_char_p, i]),
             "smx_job_last_stats": (i, [vp, vp]),
+            "smx_vertical_line_locus_batch": (i, [vp, i, i, i, i, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -248,3 +249,12 @@ def fast_configs():
         _check(lib().smx_fast_config_desc(i, buf, 64))
         out.append((i, tuple(int(x) for x in buf.value.split())))
     return out
+
+
+def vertical_line_locus_batch(jobs, iteration: int, precision: int = PREC_FAST, keep_rho: bool = False):
+    """One batched VerticalLineLocus over `jobs`; returns Accessable_grid per job."""
+    arr = (C.c_void_p * len(jobs))(*[j.h.value for j in jobs])
+    acc = np.zeros(len(jobs), np.int64)
+    _check(lib().smx_vertical_line_locus_batch(arr, len(jobs), int(iteration), int(precision),
+                                               1 if keep_rho else 0, _p(acc)))
+    return acc
