@@ -73,6 +73,74 @@ def load_scene(d, spec):
                  tuple(z["extent"]), list(z["resid"]))
 
 
+def batched_run(pool, ntiles, seed, cfg):
+    """Tiles of mixed size (log-uniform 128..1024 cells per side) and height
+    range (32..256 per cell) from independently seeded scenes, one level:
+    ONE persistent FAST launch over all tiles vs one FAST launch per tile vs
+    the plain FP64 kernel one tile per launch."""
+    import math
+    from setsm_amd import smx
+    from setsm_amd.records import LevelInputs
+    from setsm_amd.synth import level_grid
+    rng = np.random.default_rng(seed + 5)
+    jobs = []
+    for k in range(ntiles):
+        sc = pool[k % len(pool)]
+        cells = int(round(math.exp(rng.uniform(math.log(128), math.log(1024)))))
+        side = sc.spec.image_px * sc.spec.gsd
+        cells = min(cells, int((side - 100.0) / 2.0))
+        nh = int(rng.integers(32, 257))
+        g = level_grid(sc, 0, cells, nh, seed=int(rng.integers(0, 1 << 20)))
+        j = smx.Job(1)
+        if cfg is not None:
+            j.set_fast_config(cfg)
+        for ti in range(2):
+            j.build_pyramid(ti, sc.images[ti])
+        stub = [[(sc.images[ti], sc.images[ti], np.zeros((1, 1), np.uint8))] for ti in range(2)]
+        li = LevelInputs(stub, 0, sc.rpcs, g, grid_resolution=g["grid_res"])
+        j.set_level(li.level)
+        j.set_searchspace(g["ugrid"], g["minmax"])
+        j.initialize_voxel(1)
+        jobs.append(j)
+
+    def timed(fn, reps=3):
+        fn()
+        best = None
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+        return best
+
+    t_batch = timed(lambda: smx.vertical_line_locus_batch(jobs, 1, precision=smx.PREC_FAST))
+    hyps = sum(j.last_evaluated for j in jobs)
+    t_each = timed(lambda: [j.vertical_line_locus(1, precision=smx.PREC_FAST) for j in jobs])
+    t_fp64 = timed(lambda: [j.vertical_line_locus(1, precision=smx.PREC_FP64) for j in jobs], reps=1)
+    for j in jobs:
+        j.close()
+    return dict(tiles=ntiles, hypotheses=hyps, batched_hyp_per_s=hyps / t_batch, per_tile_fast_hyp_per_s=hyps / t_each,
+                plain_fp64_hyp_per_s=hyps / t_fp64, speedup_vs_plain=t_fp64 / t_batch)
+
+
+def served_run(pool, nreq, rate, seed, cfg):
+    """Poisson streams of 4-level tile requests: continuous batching on the
+    FAST kernel vs the plain FP64 kernel serving one request at a time.
+    Run twice: at `rate` (latency under the stated load) and with every
+    request arriving within 0.25 s (saturation: throughput = capacity)."""
+    from setsm_amd import smx
+    from setsm_amd.serve import Server, make_stream
+    out = dict(requests=nreq, levels=4, budget_gib=200, sizes_cells=[96, 640], heights=[32, 256])
+    for tag, r in (("load", rate), ("saturated", nreq / 0.25)):
+        stream = make_stream(nreq, r, sizes=(96, 640), heights=(32, 256), n_scenes=len(pool), seed=seed + 9)
+        Server(pool, levels=4).prepare(stream)          # request payloads, built before the clock starts
+        _, fast = Server(pool, levels=4, fast_config=cfg).run(stream)
+        _, plain = Server(pool, levels=4, precision=smx.PREC_FP64, batching=False).run(stream)
+        out[tag] = dict(rate_per_s=r, fast=fast, plain_fp64=plain,
+                        speedup_vs_plain=fast["hyp_per_s"] / plain["hyp_per_s"])
+    return out
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -90,6 +158,9 @@ def main():
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--autotune", action="store_true", help="time every FAST launch shape and store the best")
     ap.add_argument("--scene-cache", default="", help="directory to load / save the rendered scene")
+    ap.add_argument("--batch-tiles", type=int, default=16, help="tiles of the batched run (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=80, help="requests of the served run (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="first run one device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -273,8 +344,21 @@ def main():
         "fast_launch": (dict(tuned or {}, **fast.get("stats", {})) if fast else None),
         "kernel_ms": best["kernel_ms"],
     }
-    print(json.dumps(out), flush=True)
     job.close()
+    del job
+    torch.cuda.empty_cache()
+    if args.batch_tiles > 0 or args.serve_requests > 0:
+        from setsm_amd.serve import make_pool
+        pool = make_pool(n_scenes=4, image_px=3072, device="cuda", seed=100 + args.seed)
+        torch.cuda.empty_cache()
+        tuned_cfg = tuned["config"] if tuned else None
+        if args.batch_tiles > 0:
+            out["batched"] = batched_run(pool, args.batch_tiles, args.seed, tuned_cfg)
+            log(f"batched: {out['batched']}")
+        if args.serve_requests > 0:
+            out["served"] = served_run(pool, args.serve_requests, args.serve_rate, args.seed, tuned_cfg)
+            log(f"served: {out['served']}")
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

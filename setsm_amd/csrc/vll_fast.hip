@@ -327,98 +327,90 @@ __device__ __forceinline__ bool global_window(const DevView &V0, const DevView &
                                         (float)((int)V1.height - 1 - ity), rho);
 }
 
-template <int H, bool kKahan>
-__device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, const CellArgs &A, const FastArgs &F,
-                                              const long tile)
+// Box helpers: the texels a set of hypotheses can touch in both images,
+// from their band ends projected through the RPCs.
+__device__ __forceinline__ void box_reset(int (&sbox)[2][4])
 {
-    __shared__ int sbox[2][4];
-    __shared__ TileBox tb;
-    const VllParams &P = *Pp;
-    const long r0 = F.tile_run_off[tile], r1 = F.tile_run_off[tile + 1];
-    if (r0 == r1) return;
-    constexpr int O = H + 2;
-
     if (threadIdx.x < 2) {
         sbox[threadIdx.x][0] = 0x7fffffff;
         sbox[threadIdx.x][1] = 0x7fffffff;
         sbox[threadIdx.x][2] = -0x7fffffff;
         sbox[threadIdx.x][3] = -0x7fffffff;
     }
-    __syncthreads();
-    // Bounding box of the tile's windows: project each active cell's band ends.
-    const int tcx = (int)(tile % F.tiles_x), tcy = (int)(tile / F.tiles_x);
-    for (int t = threadIdx.x; t < F.tx * F.ty; t += blockDim.x) {
-        const long col = (long)tcx * F.tx + t % F.tx, row = (long)tcy * F.ty + t / F.tx;
-        if (col >= F.grid_w || row >= F.grid_h) continue;
-        const long c = row * F.grid_w + col;
-        if (A.run_count[c] == 0) continue;
-        const smx_ugrid g = A.gridpt3[c];
-        const double lon = A.grid_wgs[2 * c], lat = A.grid_wgs[2 * c + 1];
-        for (int e = 0; e < 2; e++) {
-            const double hh = e == 0 ? (double)g.minHeight : (double)g.maxHeight;
-            for (int ti = 0; ti < 2; ti++) {
-                double x, y, px, py;
-                rpc_ground_to_image(P.rpc[ti], P.adjust[ti], lon, lat, hh, x, y);
-                to_pyramid(x, y, P.view[ti], P.level, px, py);
-                const int fx = (int)floor(px), fy = (int)floor(py);
-                atomicMin(&sbox[ti][0], fx);
-                atomicMin(&sbox[ti][1], fy);
-                atomicMax(&sbox[ti][2], fx);
-                atomicMax(&sbox[ti][3], fy);
-            }
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int fits = 1;
-        for (int ti = 0; ti < 2; ti++) {
-            const DevView &v = P.view[ti];
-            // window radius H+1, bilinear +1, 2 px for the band's curvature
-            int x0 = sbox[ti][0] - O - 2, y0 = sbox[ti][1] - O - 2;
-            int x1 = sbox[ti][2] + O + 3, y1 = sbox[ti][3] + O + 3;
-            x0 = max(x0, 0);
-            y0 = max(y0, 0);
-            x1 = min(x1, (int)v.width - 1);
-            y1 = min(y1, (int)v.height - 1);
-            tb.x0[ti] = x0;
-            tb.y0[ti] = y0;
-            tb.w[ti] = max(x1 - x0 + 1, 1);
-            tb.h[ti] = max(y1 - y0 + 1, 1);
-            if (!F.stage) fits = 0;
-            const long cx = min(max((x0 + x1) / 2, 0), (int)v.width - 1);
-            const long cy = min(max((y0 + y1) / 2, 0), (int)v.height - 1);
-            const uint32_t t = v.texel[cy * v.width + cx];
-            tb.piv_i[ti] = (float)(t & 0xffffu);
-            tb.piv_m[ti] = (float)(t >> 16);
-        }
-        if (tb.w[0] * tb.h[0] + tb.w[1] * tb.h[1] > F.lds_texels) fits = 0;
-        tb.fits = fits;
-        tb.nodata = 0;
-    }
-    __syncthreads();
-    const bool fits = tb.fits != 0;
-    if (threadIdx.x == 0 && fits) atomicAdd(A.accessible + 2, 1ull);
-    if (fits) {
-        int nod = 0;
-        for (int ti = 0; ti < 2; ti++) {
-            const DevView &v = P.view[ti];
-            const int dst = ti ? tb.w[0] * tb.h[0] : 0;
-            const int w = tb.w[ti], h = tb.h[ti];
-            const float pi = tb.piv_i[ti], pm = tb.piv_m[ti];
-            for (int k = threadIdx.x; k < w * h; k += blockDim.x) {
-                const int rr = k / w, cc = k % w;
-                const uint32_t t = gptr(v.texel)[(long)(tb.y0[ti] + rr) * v.width + tb.x0[ti] + cc];
-                nod |= (t & 0xffffu) == 0;
-                smx_lds[dst + k] = make_float2((float)(t & 0xffffu) - pi, (float)(t >> 16) - pm);
-            }
-        }
-        if (nod) tb.nodata = 1;   // benign race: every writer stores 1
-    }
-    __syncthreads();
-    const bool nodata = tb.nodata != 0;
+}
 
-    const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6, lane = threadIdx.x & 63;
-    for (long ri = r0 + wave; ri < r1; ri += nwaves) {
+__device__ __forceinline__ void box_add(const VllParams &P, int (&sbox)[2][4], double lon, double lat, double hh)
+{
+    for (int ti = 0; ti < 2; ti++) {
+        double x, y, px, py;
+        rpc_ground_to_image(P.rpc[ti], P.adjust[ti], lon, lat, hh, x, y);
+        to_pyramid(x, y, P.view[ti], P.level, px, py);
+        const int fx = (int)floor(px), fy = (int)floor(py);
+        atomicMin(&sbox[ti][0], fx);
+        atomicMin(&sbox[ti][1], fy);
+        atomicMax(&sbox[ti][2], fx);
+        atomicMax(&sbox[ti][3], fy);
+    }
+}
+
+// Thread 0: clamp the box to the images, add the window margin, pick the
+// staging pivots, decide whether it fits the LDS budget.
+template <int H>
+__device__ __forceinline__ void box_finalize(const VllParams &P, const FastArgs &F, const int (&sbox)[2][4], TileBox &tb)
+{
+    constexpr int O = H + 2;
+    int fits = F.stage ? 1 : 0;
+    for (int ti = 0; ti < 2; ti++) {
+        const DevView &v = P.view[ti];
+        // window radius H+1, bilinear +1, 2 px for the band's curvature
+        int x0 = sbox[ti][0] - O - 2, y0 = sbox[ti][1] - O - 2;
+        int x1 = sbox[ti][2] + O + 3, y1 = sbox[ti][3] + O + 3;
+        x0 = max(x0, 0);
+        y0 = max(y0, 0);
+        x1 = min(x1, (int)v.width - 1);
+        y1 = min(y1, (int)v.height - 1);
+        tb.x0[ti] = x0;
+        tb.y0[ti] = y0;
+        tb.w[ti] = max(x1 - x0 + 1, 1);
+        tb.h[ti] = max(y1 - y0 + 1, 1);
+        const long cx = min(max((x0 + x1) / 2, 0), (int)v.width - 1);
+        const long cy = min(max((y0 + y1) / 2, 0), (int)v.height - 1);
+        const uint32_t t = gptr(v.texel)[cy * v.width + cx];
+        tb.piv_i[ti] = (float)(t & 0xffffu);
+        tb.piv_m[ti] = (float)(t >> 16);
+    }
+    if (tb.w[0] * tb.h[0] + tb.w[1] * tb.h[1] > F.lds_texels) fits = 0;
+    tb.fits = fits;
+    tb.nodata = 0;
+}
+
+// All threads: copy the box into LDS as pivot-shifted float2 texels.
+__device__ __forceinline__ void box_stage(const VllParams &P, TileBox &tb)
+{
+    int nod = 0;
+    for (int ti = 0; ti < 2; ti++) {
+        const DevView &v = P.view[ti];
+        const int dst = ti ? tb.w[0] * tb.h[0] : 0;
+        const int w = tb.w[ti], h = tb.h[ti];
+        const float pi = tb.piv_i[ti], pm = tb.piv_m[ti];
+        for (int k = threadIdx.x; k < w * h; k += blockDim.x) {
+            const int rr = k / w, cc = k % w;
+            const uint32_t t = gptr(v.texel)[(long)(tb.y0[ti] + rr) * v.width + tb.x0[ti] + cc];
+            nod |= (t & 0xffffu) == 0;
+            smx_lds[dst + k] = make_float2((float)(t & 0xffffu) - pi, (float)(t >> 16) - pm);
+        }
+    }
+    if (nod) tb.nodata = 1;   // benign race: every writer stores 1
+}
+
+// One run (64 consecutive heights of one cell) on one wavefront.
+template <int H, bool kKahan>
+__device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs &A, const FastArgs &F,
+                                             const TileBox &tb, const bool fits, const bool nodata, const long ri,
+                                             const int lane)
+{
+    constexpr int O = H + 2;
+
         const int2 run = F.tile_runs[ri];
         const long cell = run.x;
         const int hidx = run.y + lane;
@@ -502,6 +494,74 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
             evaluated = 1;
         }
         wave_max_commit(temp_rho, evaluated, &A.max_bits[cell], A.evaluated);
+    }
+
+template <int H, bool kKahan>
+__device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, const CellArgs &A, const FastArgs &F,
+                                              const long tile)
+{
+    __shared__ int sbox[2][4];
+    __shared__ TileBox tb;
+    const VllParams &P = *Pp;
+    const long r0 = F.tile_run_off[tile], r1 = F.tile_run_off[tile + 1];
+    if (r0 == r1) return;
+    const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6, lane = threadIdx.x & 63;
+
+    // 1. One box for the whole tile: project each active cell's band ends.
+    box_reset(sbox);
+    __syncthreads();
+    const int tcx = (int)(tile % F.tiles_x), tcy = (int)(tile / F.tiles_x);
+    for (int t = threadIdx.x; t < F.tx * F.ty; t += blockDim.x) {
+        const long col = (long)tcx * F.tx + t % F.tx, row = (long)tcy * F.ty + t / F.tx;
+        if (col >= F.grid_w || row >= F.grid_h) continue;
+        const long c = row * F.grid_w + col;
+        if (A.run_count[c] == 0) continue;
+        const smx_ugrid g = A.gridpt3[c];
+        const double lon = A.grid_wgs[2 * c], lat = A.grid_wgs[2 * c + 1];
+        box_add(P, sbox, lon, lat, (double)g.minHeight);
+        box_add(P, sbox, lon, lat, (double)g.maxHeight);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) box_finalize<H>(P, F, sbox, tb);
+    __syncthreads();
+    if (tb.fits || !F.stage) {
+        const bool fits = tb.fits != 0;
+        if (threadIdx.x == 0 && fits) atomicAdd(A.accessible + 2, 1ull);
+        if (fits) box_stage(P, tb);
+        __syncthreads();
+        const bool nodata = tb.nodata != 0;
+        for (long ri = r0 + wave; ri < r1; ri += nwaves) vll_fast_run<H, kKahan>(P, A, F, tb, fits, nodata, ri, lane);
+        return;
+    }
+    // 2. Tall footprints (wide height bands, strong convergence): stage one
+    // box per chunk of runs the waves evaluate together; each run spans 64
+    // heights, so a chunk's box stays small.
+    for (long c0 = r0; c0 < r1; c0 += nwaves) {
+        __syncthreads();
+        box_reset(sbox);
+        __syncthreads();
+        if ((int)threadIdx.x < 2 * nwaves) {
+            const long ri = c0 + threadIdx.x / 2;
+            if (ri < r1) {
+                const int2 run = F.tile_runs[ri];
+                const long cell = run.x;
+                const smx_nccresult nr = A.ncc[cell];
+                const smx_ugrid g = A.gridpt3[cell];
+                const int hl = (threadIdx.x & 1) ? min(run.y + kRun - 1, (int)nr.NumOfHeight - 1) : run.y;
+                double hh = (double)(float)(nr.minHeight + hl * P.height_step);
+                hh = fmin(fmax(hh, (double)g.minHeight), (double)g.maxHeight);
+                box_add(P, sbox, A.grid_wgs[2 * cell], A.grid_wgs[2 * cell + 1], hh);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) box_finalize<H>(P, F, sbox, tb);
+        __syncthreads();
+        const bool fits = tb.fits != 0;
+        if (threadIdx.x == 0 && fits) atomicAdd(A.accessible + 2, 1ull);
+        if (fits) box_stage(P, tb);
+        __syncthreads();
+        const bool nodata = tb.nodata != 0;
+        if (c0 + wave < r1) vll_fast_run<H, kKahan>(P, A, F, tb, fits, nodata, c0 + wave, lane);
     }
 }
 

@@ -301,3 +301,21 @@ def make_grid(scene: Scene, gs: GridSpec):
         "ugrid": ug,
         "minmax": np.array([float(lo.min() - 1), float(hi.max() + 1)]),
     }
+
+
+def level_grid(scene: Scene, level: int, cells: int, nheights: int, seed: int = 7, resolution: float = 0.5):
+    """Centred cell grid at one pyramid level with `nheights` hypotheses per
+    cell: grid spacing max(2 m, GSD * 2^level), height step GetHeightStep,
+    band = nheights * step (whole metres) around the true height."""
+    from .records import height_step
+    grid_res = max(2.0, resolution * (1 << level))
+    hs = height_step(level, resolution)
+    band = float(max(1, int(round(nheights * hs))))
+    side = scene.spec.image_px * scene.spec.gsd
+    inset = (side - cells * grid_res) / 2.0
+    g = make_grid(scene, GridSpec(cells=cells, grid_res=grid_res, band=band, inset=inset, x_offset=0.0,
+                                  y_offset=0.0, seed=seed + level))
+    g["boundary"] = np.array([inset, inset, inset + cells * grid_res, inset + cells * grid_res])
+    g["shape"] = (cells, cells)
+    g["grid_res"] = grid_res
+    return g
