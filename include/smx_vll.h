@@ -132,7 +132,10 @@ long long smx_job_num_hypotheses(const smx_job *job);
 int  smx_job_get_voxels(smx_job *job, uint32_t *words, float *rho, int64_t *offsets);
 /* Download the rows of cells [cell0, cell1) only (contiguous in CSR order). */
 int  smx_job_get_voxel_rows(smx_job *job, long long cell0, long long cell1, uint32_t *words, float *rho);
-/* Replace the cube (CSR order, lengths from the current NumOfHeight). */
+/* Replace the cube: words in CSR order, row lengths taken from the
+ * NumOfHeight of the job's current NCCresult array (smx_job_set_nccresult),
+ * offsets rebuilt from them. Hands a host cube (SETSM's grid_voxel rows,
+ * setsm_code.cpp:5930-5948) to a job that did not build it. */
 int  smx_job_set_voxels(smx_job *job, const uint32_t *words);
 
 /* Launch shape of the SMX_PREC_FAST kernel (tile of cells per workgroup,

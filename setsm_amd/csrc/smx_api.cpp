@@ -733,8 +733,26 @@ int smx_job_get_voxel_rows(smx_job *j, long long cell0, long long cell1, uint32_
 int smx_job_set_voxels(smx_job *j, const uint32_t *words)
 {
     if (!j || !words || !j->level_set) return fail(SMX_E_STATE, "no level set");
-    if (j->vox_total)
-        HIPCHK(hipMemcpyAsync(j->d_vox, words, (size_t)j->vox_total * 4, hipMemcpyHostToDevice, j->stream));
+    HIPCHK(hipSetDevice(j->device));
+    // Rebuild the CSR offsets from the NCCresult array on the device, so a
+    // cube can be handed over without this job's own InitializeVoxel.
+    std::vector<smx_nccresult> ncc((size_t)j->ncells);
+    std::vector<int64_t> off((size_t)j->ncells + 1);
+    HIPCHK(hipMemcpyAsync(ncc.data(), j->d_ncc, ncc.size() * sizeof(smx_nccresult), hipMemcpyDeviceToHost, j->stream));
+    HIPCHK(hipStreamSynchronize(j->stream));
+    off[0] = 0;
+    for (long c = 0; c < j->ncells; c++) off[c + 1] = off[c] + ncc[c].NumOfHeight;
+    const long long total = off[j->ncells];
+    if (total != j->vox_total || !j->d_vox) {
+        jfree(j, j->d_vox);
+        int r = jalloc(j, &j->d_vox, (size_t)std::max(total, 1LL));
+        if (r) return r;
+        j->vox_total = total;
+    }
+    jfree(j, j->d_rho);   // no mean NCC for a handed-over cube
+    HIPCHK(hipMemcpyAsync(j->d_vox_off, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice, j->stream));
+    if (total)
+        HIPCHK(hipMemcpyAsync(j->d_vox, words, (size_t)total * 4, hipMemcpyHostToDevice, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
     return 0;
 }

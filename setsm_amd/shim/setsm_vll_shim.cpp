@@ -1,0 +1,311 @@
+// setsm_vll_shim.cpp — SETSM's InitializeVoxel / VerticalLineLocus / AWNCC
+// (setsm_code.hpp:75, 81, 91) with their exact C++ signatures and SETSM's own
+// types, running on the MI355X through libsmx (include/smx_vll.h).
+//
+// Drop-in use inside SETSM's build: add this file, link -lsmx, and either
+//   * compile it with -DSMX_SHIM_GLOBAL_NAMES and remove the three original
+//     definitions from setsm_code.cpp, or
+//   * keep the originals and call smx_setsm::InitializeVoxel / ... at the
+//     three call sites in Matching_SETSM (setsm_code.cpp:3827, 3829, 3844).
+//
+// State: one smx job per thread mirrors SETSM's per-level state. A new job is
+// made when the level's image buffers, sizes or origins change; a changed
+// level record (grid, RPCs, ImageAdjust, ...) restarts the level on the
+// device and re-uploads the host cube. The cube stays in HBM between the
+// calls of a level, and every call hands the host arrays back exactly as the
+// CPU path leaves them (grid_voxel rows re-allocated like InitializeVoxel
+// does, VOXEL bytes identical to the device cube). Call
+// smx_setsm::Release() when a tile is done, so that a later tile whose
+// buffers land at the same addresses is not taken for the same one.
+//
+// Errors follow SETSM's convention (setsm_code.cpp:6571-6575): a message on
+// stdout and exit(1). Inputs that need a term libsmx lacks (SGM aggregation
+// at Pyramid_step <= SGM_py, the ortho GNCC term, RA / check_matching_rate
+// mode, frame sensors, more than two images) are such errors.
+//
+// SMX_PRECISION=fp64 (read at every VerticalLineLocus call) selects the FP64
+// kernel, bit-identical to the CPU path; the default is the FAST kernel
+// (per-hypothesis mean NCC within 1e-5 of FP64).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "Typedefine.hpp"   // SETSM's ProInfo, LevelInfo, UGRID, NCCresult, VOXEL
+#include "smx_vll.h"
+
+static_assert(sizeof(UGRID) == sizeof(smx_ugrid), "UGRID layout");
+static_assert(sizeof(NCCresult) == sizeof(smx_nccresult), "NCCresult layout");
+static_assert(sizeof(VOXEL) == 4, "VOXEL layout");
+static_assert(sizeof(D2DPOINT) == 2 * sizeof(double), "D2DPOINT layout");
+
+namespace smx_setsm {
+
+namespace {
+
+struct State {
+    smx_job *job = nullptr;
+    int level = -1;
+    const void *views[2][6] = {};   // the pointers and geometry the job's views came from
+    double starts[2][4] = {};
+    int sizes[2][4] = {};
+    smx_level lv;                   // the level record last passed to smx_job_set_level
+    bool lv_set = false;
+};
+
+thread_local State g_state;
+
+void die(const char *what, int rc)
+{
+    printf("ERROR: %s failed (%d): %s\n", what, rc, smx_last_error());
+    exit(1);
+}
+
+void check(int rc, const char *what)
+{
+    if (rc < 0) die(what, rc);
+}
+
+void fill_rpc(smx_rpc &r, const double *const *rows)
+{
+    for (int k = 0; k < 5; k++) {
+        r.off[k] = rows[0][k];
+        r.scale[k] = rows[1][k];
+    }
+    for (int j = 0; j < 4; j++)
+        for (int k = 0; k < 20; k++) r.coef[j][k] = rows[j + 2][k];
+}
+
+void require_pair(const ProInfo *proinfo)
+{
+    if (proinfo->number_of_images != 2 || !proinfo->check_selected_image[0] || !proinfo->check_selected_image[1]) {
+        printf("ERROR: the MI355X VerticalLineLocus handles one stereo pair (2 selected images)\n");
+        exit(1);
+    }
+    if (proinfo->sensor_type != SB) {
+        printf("ERROR: the MI355X VerticalLineLocus handles RPC (SB) sensors only\n");
+        exit(1);
+    }
+}
+
+int precision_from_env()
+{
+    const char *pe = getenv("SMX_PRECISION");
+    return (pe && !strcmp(pe, "fp64")) ? SMX_PREC_FP64 : SMX_PREC_FAST;
+}
+
+// Point the thread's job at this call's tile and level. A new tile (other
+// image buffers, sizes or origins) gets a new job; a changed level record
+// (level, grid, RPCs, ImageAdjust, ...) is passed to smx_job_set_level, which
+// starts the level's device state afresh. Returns true when the device holds
+// no cube for this level yet, so that the caller uploads the host's.
+bool bind(const ProInfo *proinfo, LevelInfo &L)
+{
+    State &s = g_state;
+    require_pair(proinfo);
+    if (*L.NumOfIAparam != 2) {
+        printf("ERROR: the MI355X VerticalLineLocus takes the 2-parameter (shift) image adjustment\n");
+        exit(1);
+    }
+    const int level = *L.Pyramid_step;
+    const bool with_next = level > *L.Py_combined_level && L.py_Images_next;
+    const int ref = L.reference_id, tgt = ref == 0 ? 1 : 0;
+    const int ids[2] = {ref, tgt};
+
+    const void *views[2][6] = {};
+    double starts[2][4] = {};
+    int sizes[2][4] = {};
+    for (int k = 0; k < 2; k++) {
+        const int ti = ids[k];
+        views[k][0] = L.py_Images[ti];
+        views[k][1] = L.py_MagImages[ti];
+        views[k][2] = L.py_OriImages[ti];
+        starts[k][0] = L.py_Startpos[ti].m_X;
+        starts[k][1] = L.py_Startpos[ti].m_Y;
+        sizes[k][0] = L.py_Sizes[ti][level].width;
+        sizes[k][1] = L.py_Sizes[ti][level].height;
+        if (with_next) {
+            views[k][3] = L.py_Images_next[ti];
+            views[k][4] = L.py_MagImages_next[ti];
+            views[k][5] = L.py_OriImages_next[ti];
+            starts[k][2] = L.py_Startpos_next[ti].m_X;
+            starts[k][3] = L.py_Startpos_next[ti].m_Y;
+            sizes[k][2] = L.py_Sizes[ti][level - 1].width;
+            sizes[k][3] = L.py_Sizes[ti][level - 1].height;
+        }
+    }
+    const bool same_views = s.job && s.level == level && !memcmp(views, s.views, sizeof views) &&
+                            !memcmp(starts, s.starts, sizeof starts) && !memcmp(sizes, s.sizes, sizeof sizes);
+    if (!same_views) {
+        if (s.job) smx_job_destroy(s.job);
+        s = State{};
+        check(smx_job_create(&s.job, level + 1), "smx_job_create");
+        for (int k = 0; k < 2; k++) {
+            check(smx_job_set_view(s.job, k, level, L.py_Images[ids[k]], L.py_MagImages[ids[k]],
+                                   L.py_OriImages[ids[k]], sizes[k][0], sizes[k][1], starts[k][0], starts[k][1]),
+                  "smx_job_set_view");
+            if (with_next)
+                check(smx_job_set_view(s.job, k, level - 1, L.py_Images_next[ids[k]], L.py_MagImages_next[ids[k]],
+                                       L.py_OriImages_next[ids[k]], sizes[k][2], sizes[k][3], starts[k][2],
+                                       starts[k][3]),
+                      "smx_job_set_view(next)");
+        }
+        s.level = level;
+        memcpy(s.views, views, sizeof views);
+        memcpy(s.starts, starts, sizeof starts);
+        memcpy(s.sizes, sizes, sizeof sizes);
+    }
+
+    smx_level lv;
+    memset(&lv, 0, sizeof lv);
+    for (int k = 0; k < 2; k++) {
+        fill_rpc(lv.rpc[k], L.RPCs[ids[k]]);
+        lv.adjust[k][0] = L.ImageAdjust[ids[k]][0];
+        lv.adjust[k][1] = L.ImageAdjust[ids[k]][1];
+    }
+    lv.level = level;
+    lv.template_size = *L.Template_size;
+    lv.py_combined_level = *L.Py_combined_level;
+    lv.sgm_py = proinfo->SGM_py;
+    lv.is_ra = proinfo->IsRA;
+    lv.check_matchtag = proinfo->check_Matchtag;
+    lv.pre_dem = proinfo->pre_DEMtif;
+    lv.check_matching_rate = L.check_matching_rate ? *L.check_matching_rate : 0;
+    lv.resolution = proinfo->resolution;
+    lv.dem_resolution = proinfo->DEM_resolution;
+    lv.bin_angle = *L.bin_angle;
+    lv.height_step = *L.height_step;
+    lv.grid_resolution = *L.grid_resolution;
+    for (int k = 0; k < 4; k++) lv.boundary[k] = L.Boundary[k];
+    lv.grid_w = L.Size_Grid2D->width;
+    lv.grid_h = L.Size_Grid2D->height;
+    lv.grid_pts = (const double *)L.GridPts;
+    lv.grid_wgs = (const double *)L.Grid_wgs;
+    if (s.lv_set && !memcmp(&lv, &s.lv, sizeof lv)) return false;
+    check(smx_job_set_level(s.job, &lv), "smx_job_set_level");
+    s.lv = lv;
+    s.lv_set = true;
+    return true;
+}
+
+// Upload the host cube (rows of NumOfHeight VOXELs) after a fresh level set.
+void upload_rows(State &s, VOXEL *const *grid_voxel, const NCCresult *ncc, long n)
+{
+    std::vector<uint32_t> words;
+    for (long c = 0; c < n; c++)
+        for (int h = 0; h < ncc[c].NumOfHeight; h++) {
+            uint32_t w;
+            memcpy(&w, &grid_voxel[c][h], sizeof w);
+            words.push_back(w);
+        }
+    if (!words.empty()) check(smx_job_set_voxels(s.job, words.data()), "smx_job_set_voxels");
+}
+
+// Copy the device cube into SETSM's per-cell VOXEL rows.
+void download_rows(State &s, VOXEL **grid_voxel, const NCCresult *ncc, long n)
+{
+    const long long total = smx_job_num_hypotheses(s.job);
+    std::vector<uint32_t> words((size_t)(total > 0 ? total : 1));
+    std::vector<int64_t> off((size_t)n + 1);
+    check(smx_job_get_voxels(s.job, words.data(), nullptr, off.data()), "smx_job_get_voxels");
+    for (long c = 0; c < n; c++)
+        if (ncc[c].NumOfHeight > 0 && grid_voxel[c])
+            memcpy(grid_voxel[c], words.data() + off[c], sizeof(VOXEL) * ncc[c].NumOfHeight);
+}
+
+}  // namespace
+
+void InitializeVoxel(const ProInfo *proinfo, VOXEL **grid_voxel, LevelInfo &plevelinfo, UGRID *GridPT3,
+                     NCCresult *nccresult, const int iteration, const double *minmaxHeight)
+{
+    const bool fresh = bind(proinfo, plevelinfo);
+    State &s = g_state;
+    const long n = *plevelinfo.Grid_length;
+    std::vector<unsigned short> old_n((size_t)n);
+    for (long c = 0; c < n; c++) old_n[c] = nccresult[c].NumOfHeight;
+    check(smx_job_set_searchspace(s.job, (const smx_ugrid *)GridPT3, minmaxHeight), "smx_job_set_searchspace");
+    check(smx_job_set_nccresult(s.job, (const smx_nccresult *)nccresult), "smx_job_set_nccresult");
+    if (fresh) upload_rows(s, grid_voxel, nccresult, n);
+    check(smx_initialize_voxel(s.job, iteration), "smx_initialize_voxel");
+    check(smx_job_get_nccresult(s.job, (smx_nccresult *)nccresult), "smx_job_get_nccresult");
+    check(smx_job_get_searchspace(s.job, (smx_ugrid *)GridPT3), "smx_job_get_searchspace");
+    // setsm_code.cpp:5930-5948: rows of changed cells are re-allocated with
+    // flag_cal = true, INCC = -1000; rows of dropped cells are released.
+    for (long c = 0; c < n; c++) {
+        const NCCresult &r = nccresult[c];
+        if (r.check_height_change && r.NumOfHeight > 0) {
+            if (old_n[c] > 0 && grid_voxel[c]) free(grid_voxel[c]);
+            grid_voxel[c] = (VOXEL *)calloc(sizeof(VOXEL), r.NumOfHeight);
+            for (int h = 0; h < r.NumOfHeight; h++) {
+                grid_voxel[c][h].flag_cal = true;
+                grid_voxel[c][h].INCC = (short)(-1.0 * 1000.0);
+            }
+        } else if (r.NumOfHeight == 0 && old_n[c] > 0 && grid_voxel[c]) {
+            free(grid_voxel[c]);
+            grid_voxel[c] = nullptr;
+        }
+    }
+}
+
+int VerticalLineLocus(VOXEL **grid_voxel, const ProInfo *proinfo, NCCresult *nccresult, LevelInfo &plevelinfo,
+                      const UGRID *GridPT3, const uint8 iteration, const double *minmaxHeight)
+{
+    const bool fresh = bind(proinfo, plevelinfo);
+    State &s = g_state;
+    const long n = *plevelinfo.Grid_length;
+    check(smx_job_set_searchspace(s.job, (const smx_ugrid *)GridPT3, minmaxHeight), "smx_job_set_searchspace");
+    check(smx_job_set_nccresult(s.job, (const smx_nccresult *)nccresult), "smx_job_set_nccresult");
+    if (fresh) upload_rows(s, grid_voxel, nccresult, n);
+    const long long acc = smx_vertical_line_locus(s.job, iteration, precision_from_env(), 0);
+    if (acc < 0) die("smx_vertical_line_locus", (int)acc);
+    check(smx_job_get_nccresult(s.job, (smx_nccresult *)nccresult), "smx_job_get_nccresult");
+    download_rows(s, grid_voxel, nccresult, n);
+    return (int)acc;
+}
+
+void AWNCC(ProInfo *proinfo, VOXEL **grid_voxel, CSize Size_Grid2D, UGRID *GridPT3, NCCresult *nccresult,
+           double step_height, uint8 Pyramid_step, uint8 iteration, int MaxNumberofHeightVoxel)
+{
+    (void)grid_voxel;
+    (void)MaxNumberofHeightVoxel;
+    State &s = g_state;
+    if (!s.job || !s.lv_set || s.level != Pyramid_step || s.lv.grid_w != (unsigned)Size_Grid2D.width ||
+        s.lv.grid_h != (unsigned)Size_Grid2D.height) {
+        printf("ERROR: AWNCC called without the level's VerticalLineLocus on the MI355X\n");
+        exit(1);
+    }
+    (void)step_height;
+    (void)GridPT3;
+    check(smx_job_set_nccresult(s.job, (const smx_nccresult *)nccresult), "smx_job_set_nccresult");
+    check(smx_awncc(s.job, iteration), "smx_awncc");
+    check(smx_job_get_nccresult(s.job, (smx_nccresult *)nccresult), "smx_job_get_nccresult");
+    (void)proinfo;
+}
+
+// Release the thread's job (optional; it is reused across calls).
+void Release()
+{
+    if (g_state.job) smx_job_destroy(g_state.job);
+    g_state = State{};
+}
+
+}  // namespace smx_setsm
+
+#ifdef SMX_SHIM_GLOBAL_NAMES
+void InitializeVoxel(const ProInfo *proinfo, VOXEL **grid_voxel, LevelInfo &plevelinfo, UGRID *GridPT3,
+                     NCCresult *nccresult, const int iteration, const double *minmaxHeight)
+{
+    smx_setsm::InitializeVoxel(proinfo, grid_voxel, plevelinfo, GridPT3, nccresult, iteration, minmaxHeight);
+}
+int VerticalLineLocus(VOXEL **grid_voxel, const ProInfo *proinfo, NCCresult *nccresult, LevelInfo &plevelinfo,
+                      const UGRID *GridPT3, const uint8 iteration, const double *minmaxHeight)
+{
+    return smx_setsm::VerticalLineLocus(grid_voxel, proinfo, nccresult, plevelinfo, GridPT3, iteration, minmaxHeight);
+}
+void AWNCC(ProInfo *proinfo, VOXEL **grid_voxel, CSize Size_Grid2D, UGRID *GridPT3, NCCresult *nccresult,
+           double step_height, uint8 Pyramid_step, uint8 iteration, int MaxNumberofHeightVoxel)
+{
+    smx_setsm::AWNCC(proinfo, grid_voxel, Size_Grid2D, GridPT3, nccresult, step_height, Pyramid_step, iteration,
+                     MaxNumberofHeightVoxel);
+}
+#endif

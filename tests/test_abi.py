@@ -42,3 +42,26 @@ def test_errors_without_gpu_work_are_reported():
     assert L.smx_job_create(C.byref(h), 0) == -1
     assert b"num_levels" in L.smx_last_error()
     assert L.smx_job_set_fast_config(None, 0) == -1
+
+
+def test_setsm_typed_shim_exports_the_reference_symbols():
+    """libsmx_setsm.so defines InitializeVoxel / VerticalLineLocus / AWNCC
+    under the C++ names that SETSM's declarations (setsm_code.hpp:75, 81, 91)
+    mangle to: the test driver, compiled from those declarations against
+    SETSM's Typedefine.hpp, needs exactly these names and nothing else."""
+    import subprocess
+    shim = os.path.join(ROOT, "setsm_amd", "lib", "libsmx_setsm.so")
+    drv = os.path.join(ROOT, "tests", "shim", "_build", "libshim_driver.so")
+    if not (os.path.exists(shim) and os.path.exists(drv)):
+        pytest.skip("SETSM-typed shim not built (SETSM's Typedefine.hpp was absent at build time)")
+
+    def syms(path, flag):
+        out = subprocess.run(["nm", "-D", flag, path], check=True, capture_output=True, text=True).stdout
+        return {line.split()[-1] for line in out.splitlines() if line.strip()}
+
+    defined = syms(shim, "--defined-only")
+    wanted = {s for s in syms(drv, "--undefined-only") if s.startswith("_Z")
+              and any(k in s for k in ("InitializeVoxel", "VerticalLineLocus", "AWNCC"))}
+    assert len(wanted) == 3, wanted
+    assert wanted <= defined, f"missing: {wanted - defined}"
+    assert "_Z15InitializeVoxelPK11ProjectInfoPP12tagVoxelinfoR12taglevelinfoP10UpdateGridP12tagNCCresultiPKd" in wanted

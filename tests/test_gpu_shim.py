@@ -1,0 +1,136 @@
+"""SETSM-typed entry points (setsm_amd/shim/setsm_vll_shim.cpp) against the
+oracle: a C++ driver fills SETSM's own ProInfo / LevelInfo and host arrays
+(VOXEL **grid_voxel, NCCresult *, UGRID *) and calls InitializeVoxel,
+VerticalLineLocus and AWNCC through the reference signatures
+(setsm_code.hpp:75, 81, 91), iteration by iteration as Matching_SETSM does
+(setsm_code.cpp:3827-3844).
+
+The shim and its driver compile against SETSM's Typedefine.hpp, so they exist
+only where SETSM's sources were present at build time; without them these
+tests skip."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from setsm_amd.records import NCCRESULT_DTYPE, UGRID_DTYPE
+from tests.helpers import OracleRun, level_grid, perturb_searchspace, scene
+from tests.test_gpu_parity import NCC_FIELDS
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DRIVER = os.path.join(HERE, "shim", "_build", "libshim_driver.so")
+
+
+def _driver():
+    if not os.path.exists(DRIVER):
+        pytest.skip("SETSM-typed shim not built (SETSM's Typedefine.hpp was absent at build time)")
+    d = C.CDLL(DRIVER)
+    vp = C.c_void_p
+    d.drv_create.restype = vp
+    d.drv_create.argtypes = [vp, C.c_int]
+    d.drv_step.restype = C.c_longlong
+    d.drv_step.argtypes = [vp, vp, vp, C.c_int]
+    d.drv_num_hyps.restype = C.c_longlong
+    d.drv_num_hyps.argtypes = [vp]
+    d.drv_get.restype = None
+    d.drv_get.argtypes = [vp, vp, vp, vp]
+    d.drv_destroy.restype = None
+    d.drv_destroy.argtypes = [vp]
+    return d
+
+
+class ShimRun:
+    def __init__(self, drv, li, swap):
+        self.d = drv
+        self.li = li
+        self.h = drv.drv_create(C.addressof(li.level), int(swap))
+        self.n = li.ncells
+
+    def step(self, ug, iteration, minmax):
+        ug = np.ascontiguousarray(ug.copy())
+        mm = np.ascontiguousarray(minmax, dtype=np.float64)
+        acc = self.d.drv_step(self.h, ug.ctypes.data, mm.ctypes.data, iteration)
+        total = self.d.drv_num_hyps(self.h)
+        words = np.zeros(max(total, 1), np.uint32)
+        ncc_vll = np.zeros(self.n, NCCRESULT_DTYPE)
+        ncc = np.zeros(self.n, NCCRESULT_DTYPE)
+        self.d.drv_get(self.h, words.ctypes.data, ncc_vll.ctypes.data, ncc.ctypes.data)
+        return dict(acc=acc, ug=ug, words=words[:total], ncc_vll=ncc_vll, ncc=ncc)
+
+    def close(self):
+        self.d.drv_destroy(self.h)
+
+
+def _check(o, s, tag, exact):
+    print(f"[{tag}] acc oracle={o['acc']} shim={s['acc']} hyps={len(o['words'])}", flush=True)
+    assert o["acc"] == s["acc"], tag
+    assert np.array_equal(o["ug"]["minHeight"], s["ug"]["minHeight"]), tag
+    assert np.array_equal(o["ug"]["maxHeight"], s["ug"]["maxHeight"]), tag
+    assert len(o["words"]) == len(s["words"]), tag
+    assert np.array_equal(o["words"] & 0xFF, s["words"] & 0xFF), f"{tag}: flag_cal differs"
+    if exact:
+        bad = np.flatnonzero(o["words"] != s["words"])
+        assert bad.size == 0, f"{tag}: {bad.size} VOXEL words differ, first {bad[:5]}"
+        for key in ("ncc_vll", "ncc"):
+            for f in NCC_FIELDS:
+                a, b = o[key][f], s[key][f]
+                bad = np.flatnonzero(a != b)
+                assert bad.size == 0, f"{tag} {key}: field {f} differs at {bad[:5]}: {a[bad[:5]]} vs {b[bad[:5]]}"
+    else:
+        io = (o["words"] >> 16).astype(np.uint16).view(np.int16).astype(np.int32)
+        isx = (s["words"] >> 16).astype(np.uint16).view(np.int16).astype(np.int32)
+        assert np.all(np.abs(io - isx) <= 1), f"{tag}: INCC beyond one quantum"
+        for f in ("minHeight", "maxHeight", "NumOfHeight", "check_height_change"):
+            assert np.array_equal(o["ncc_vll"][f], s["ncc_vll"][f]), f"{tag}: {f}"
+
+
+@pytest.mark.parametrize("level,swap,iters", [(0, 0, (1, 2, 3)), (1, 1, (1, 2)), (3, 0, (1, 2))])
+def test_shim_fp64_bit_exact(require_gpu, monkeypatch, level, swap, iters):
+    """SMX_PRECISION=fp64: host VOXEL rows, NCCresult (after VerticalLineLocus
+    and after AWNCC) and GridPT3 identical to the oracle; swap=1 puts the
+    reference image at SETSM index 1 (LevelInfo.reference_id = 1)."""
+    from setsm_amd.records import LevelInputs
+    monkeypatch.setenv("SMX_PRECISION", "fp64")
+    drv = _driver()
+    if level == 0:
+        sc, pyr = scene(image_px=512, levels=1)
+        g = level_grid(sc, 0, 64)
+    else:
+        sc, pyr = scene(image_px=1024, relief=120.0, seed=5, levels=5)
+        g = level_grid(sc, level, {1: 48, 3: 28}[level])
+    o = OracleRun(pyr, sc.rpcs, level, g)
+    li = LevelInputs(pyr, level, sc.rpcs, g, grid_resolution=g["grid_res"])
+    s = ShimRun(drv, li, swap)
+    ug = g["ugrid"]
+    for it in iters:
+        if it > 1:
+            ug = perturb_searchspace(ug, 400 + it, level)
+        ro = o.step(ug, it, g["minmax"])
+        rs = s.step(ug, it, g["minmax"])
+        _check(ro, rs, f"shim L{level} it{it} swap{swap}", exact=True)
+        ug = ro["ug"]
+    s.close()
+
+
+def test_shim_fast(require_gpu, monkeypatch):
+    """Default precision (FAST) through the SETSM-typed entry points."""
+    from setsm_amd.records import LevelInputs
+    monkeypatch.delenv("SMX_PRECISION", raising=False)
+    drv = _driver()
+    sc, pyr = scene(image_px=512, levels=1)
+    g = level_grid(sc, 0, 64)
+    o = OracleRun(pyr, sc.rpcs, 0, g)
+    li = LevelInputs(pyr, 0, sc.rpcs, g, grid_resolution=g["grid_res"])
+    s = ShimRun(drv, li, 0)
+    ug = g["ugrid"]
+    for it in (1, 2):
+        if it > 1:
+            ug = perturb_searchspace(ug, 500 + it, 0)
+        ro = o.step(ug, it, g["minmax"])
+        rs = s.step(ug, it, g["minmax"])
+        _check(ro, rs, f"shim FAST it{it}", exact=False)
+        ug = ro["ug"]
+    s.close()
