@@ -121,9 +121,14 @@ long long smx_vertical_line_locus(smx_job *job, int iteration, int precision, in
 int  smx_vertical_line_locus_batch(smx_job **jobs, int n, int iteration, int precision, int keep_rho,
                                    long long *accessible);
 
-/* AWNCC peak scan (setsm_code.cpp:7511-7702) without SGM aggregation, i.e.
- * the path taken when Pyramid_step > SGM_py. */
+/* AWNCC (setsm_code.cpp:7511-7702, setsm_code.hpp:91): at levels
+ * <= SGM_py the SGM 8-path aggregation first (setsm_code.cpp:7017-7509,
+ * P1 = 0.3, P2 = 0.6) and the peak scan over SumCost; above SGM_py the peak
+ * scan over the INCC (or INCC/GNCC blend). Writes NCCresult result0..4. */
 int  smx_awncc(smx_job *job, int iteration);
+/* SumCost of the last SGM-level smx_awncc, float per hypothesis in CSR
+ * order (SETSM keeps it in AWNCC's local float **SumCost). */
+int  smx_job_get_sumcost(smx_job *job, float *sumcost);
 
 /* Number of hypotheses in the cube (sum of NumOfHeight). */
 long long smx_job_num_hypotheses(const smx_job *job);

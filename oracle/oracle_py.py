@@ -54,6 +54,8 @@ def lib():
         L.orc_vertical_line_locus.restype = C.c_long
         L.orc_vertical_line_locus.argtypes = [vp, vp, vp, vp, ip, vp]
         L.orc_awncc_peak.argtypes = [vp, vp, vp, vp, ip]
+        L.orc_awncc.argtypes = [vp, vp, vp, vp, ip, vp]
+        L.orc_sgm_aggregate.argtypes = [vp, vp, vp, vp, vp]
         L.orc_gauss_pyramid.argtypes = [vp, C.c_uint32, C.c_uint32, vp]
         L.orc_sobel.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp]
         L.orc_orientation.argtypes = [vp, vp, C.c_uint32, C.c_uint32, ip, vp]
@@ -158,7 +160,29 @@ class OracleState:
         return r
 
     def awncc(self, ugrid, iteration):
-        lib().orc_awncc_peak(self.L, self.vox, _p(ugrid), _p(self.ncc), iteration)
+        """AWNCC as the reference runs it: SGM aggregation + SGM peak mode at
+        levels <= SGM_py (SumCost kept in self.sumcost), plain scan above."""
+        self.sumcost = None
+        if self.li.level.level <= self.li.level.sgm_py:
+            total = int(self.ncc["NumOfHeight"].astype(np.int64).sum())
+            self.sumcost = np.zeros(max(total, 1), np.float32)
+            lib().orc_awncc(self.L, self.vox, _p(ugrid), _p(self.ncc), iteration, _p(self.sumcost))
+            self.sumcost = self.sumcost[:total]
+        else:
+            lib().orc_awncc(self.L, self.vox, _p(ugrid), _p(self.ncc), iteration, None)
+
+    def sgm_aggregate(self, ugrid):
+        """SumCost of the current cube (CSR order) without the peak scan."""
+        total = int(self.ncc["NumOfHeight"].astype(np.int64).sum())
+        out = np.zeros(max(total, 1), np.float32)
+        lib().orc_sgm_aggregate(self.L, self.vox, _p(ugrid), _p(self.ncc), _p(out))
+        return out[:total]
+
+    def import_voxels(self, ncc, words):
+        """Seed NCCresult and the voxel rows (CSR words) directly."""
+        self.ncc[:] = ncc
+        w = np.ascontiguousarray(words, np.uint32)
+        lib().orc_voxels_import(self.vox, _p(self.ncc), _p(w))
 
     def export(self):
         total = int(self.ncc["NumOfHeight"].astype(np.int64).sum())

@@ -805,10 +805,10 @@ long orc_vertical_line_locus(const smx_level *Lv, orc_voxels *vox, smx_nccresult
 
 /* -------------------------------------------------------- AWNCC peak */
 
-void orc_awncc_peak(const smx_level *Lv, const orc_voxels *vox, const smx_ugrid *GridPT3,
-                    smx_nccresult *nccresult, int iteration)
+static void awncc_impl(const smx_level *Lv, const orc_voxels *vox, const smx_ugrid *GridPT3,
+                       smx_nccresult *nccresult, int iteration, float *const *SumCost)
 {
-    /* setsm_code.cpp:7511-7702 with check_SGM_peak false */
+    /* setsm_code.cpp:7511-7702; SumCost != NULL is check_SGM_peak (Pyramid_step <= SGM_py) */
     const int Pyramid_step = Lv->level;
     const double step_height = Lv->height_step;
     int check_ortho;
@@ -828,7 +828,6 @@ void orc_awncc_peak(const smx_level *Lv, const orc_voxels *vox, const smx_ugrid 
         int check_rho = 0;
         double max_roh = 0;
         double temp_nccresult = -100.0, temp_nccresult_sec = -100.0;
-        (void)max_roh;
 
         nr->result0 = q_result(-1.0);
         nr->result2 = -1000;
@@ -847,7 +846,7 @@ void orc_awncc_peak(const smx_level *Lv, const orc_voxels *vox, const smx_ugrid 
             else
                 WNCC_sum = incc;
             double WNCC_temp_rho = WNCC_sum;
-            double temp_rho = WNCC_sum;
+            double temp_rho = SumCost ? (double)SumCost[pt_index][hs] : WNCC_sum;
 
             double diff_rho;
             int t_direction;
@@ -899,9 +898,203 @@ void orc_awncc_peak(const smx_level *Lv, const orc_voxels *vox, const smx_ugrid 
             pre_height = iter_height;
             direction = t_direction;
         }
-        nr->result0 = q_result(temp_nccresult);
-        nr->result1 = q_result(temp_nccresult_sec);
+        if (SumCost && Pyramid_step == 0 && iteration >= 2) {
+            nr->result0 = q_result(max_roh);
+            nr->result1 = q_result(-1.0);
+            nr->result3 = -9999;
+        } else {
+            nr->result0 = q_result(temp_nccresult);
+            nr->result1 = q_result(temp_nccresult_sec);
+        }
     }
+}
+
+
+void orc_awncc_peak(const smx_level *Lv, const orc_voxels *vox, const smx_ugrid *GridPT3,
+                    smx_nccresult *nccresult, int iteration)
+{
+    awncc_impl(Lv, vox, GridPT3, nccresult, iteration, NULL);
+}
+
+/* ------------------------------------------------------------ SGM */
+
+/* setsm_code.cpp:6755-6767 SGM_start_pos */
+static void sgm_start_pos(const smx_nccresult *ncc, const orc_voxels *vox, const smx_ugrid *G, long pt,
+                          float *LHcost_pre, float *const *SumCost, double step)
+{
+    for (int hs = 0; hs < ncc[pt].NumOfHeight; hs++) {
+        float iter_height = (float)(ncc[pt].minHeight + hs * step);
+        if (iter_height >= G[pt].minHeight && iter_height <= G[pt].maxHeight) {
+            double WNCC_sum = dq(SMX_VOXEL_INCC(vox->rows[pt][hs]));
+            LHcost_pre[hs] = (float)WNCC_sum;
+            SumCost[pt][hs] += LHcost_pre[hs];
+        }
+    }
+}
+
+static double max2(double a, double b) { return a > b ? a : b; }
+
+/* setsm_code.cpp:6769-6999 SGM_con_pos: the previous cell on the path is
+ * (row + v_row[d], col + u_col[d]); every branch reduces to
+ * max(candidates) - maxWNCC where the candidates are the previous cell's path
+ * costs at the same height (V3) and one step below / above minus P1 (V1, V2),
+ * each only if that voxel was computed, and maxWNCC - P2 (V4). */
+static void sgm_con_pos(int col, int row, int W, int d, double step, int P_HS_step, const int *u_col,
+                        const int *v_row, const smx_nccresult *ncc, const orc_voxels *vox, const smx_ugrid *G,
+                        long pt, double P1, double P2, const float *LHcost_pre, float *LHcost_curr,
+                        float *const *SumCost)
+{
+    for (int hs = 0; hs < ncc[pt].NumOfHeight; hs++) {
+        const float iter_height = (float)(ncc[pt].minHeight + hs * step);
+        double WNCC_sum = 0;
+        if (!(iter_height >= G[pt].minHeight && iter_height <= G[pt].maxHeight)) continue;
+        WNCC_sum = dq(SMX_VOXEL_INCC(vox->rows[pt][hs]));
+        double t_WNCC_sum;
+        const int t_col = col + u_col[d];
+        const int t_row = row + v_row[d];
+        const int t_index = t_row * W + t_col;
+        const float maxWNCC = (float)dq(ncc[t_index].max_WNCC);
+        const int nt = ncc[t_index].NumOfHeight;
+        const int i0 = (int)((iter_height - (float)ncc[t_index].minHeight) / step);
+        const int i1 = i0 - P_HS_step;
+        const int i2 = i0 + P_HS_step;
+        const uint32_t *tv = vox->rows[t_index];
+        const double V4 = maxWNCC - P2;
+        int use1 = 0, use2 = 0, use3 = 0, zero3 = 0;
+        if (i0 == 0 && i2 >= 0 && i2 < nt) {
+            use3 = SMX_VOXEL_FLAG(tv[i0]) != 0;
+            use2 = SMX_VOXEL_FLAG(tv[i2]) != 0;
+        } else if (i0 == nt - 1 && i1 >= 0 && i1 < nt) {
+            use3 = SMX_VOXEL_FLAG(tv[i0]) != 0;
+            use1 = SMX_VOXEL_FLAG(tv[i1]) != 0;
+        } else if (i0 > 0 && i0 < nt - 1 && i1 >= 0 && i1 < nt && i2 >= 0 && i2 < nt) {
+            use3 = SMX_VOXEL_FLAG(tv[i0]) != 0;
+            use1 = SMX_VOXEL_FLAG(tv[i1]) != 0;
+            use2 = SMX_VOXEL_FLAG(tv[i2]) != 0;
+            /* setsm_code.cpp:6917-6929: with both neighbours' voxels computed
+             * but not the centre one, the reference still compares V3, which
+             * holds its initial 0. */
+            zero3 = !use3 && use1 && use2;
+        }
+        /* The reference's branches take the maximum in the order V1, V2, V3,
+         * V4 with strict '>'; max over a set is order-free for values that
+         * are never NaN, so the result is the same double. */
+        double m = V4;
+        if (use1) m = max2(LHcost_pre[i1] - P1, m);
+        if (use2) m = max2(LHcost_pre[i2] - P1, m);
+        if (use3) m = max2((double)LHcost_pre[i0], m);
+        if (zero3) m = max2(0.0, m);
+        t_WNCC_sum = m - maxWNCC;
+        LHcost_curr[hs] = (float)(WNCC_sum + t_WNCC_sum);
+        SumCost[pt][hs] += LHcost_curr[hs];
+    }
+}
+
+/* One path: start cell (row, col), steps (dr, dc) until it leaves the grid. */
+static void sgm_path(int row, int col, int dr, int dc, int d, const smx_level *Lv, const orc_voxels *vox,
+                     const smx_ugrid *G, const smx_nccresult *ncc, float *const *SumCost, float *pre, float *cur)
+{
+    static const int v_row[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+    static const int u_col[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+    const int W = (int)Lv->grid_w, H = (int)Lv->grid_h;
+    const double step = Lv->height_step;
+    long pt = (long)row * W + col;
+    memset(pre, 0, ncc[pt].NumOfHeight * sizeof(float));
+    sgm_start_pos(ncc, vox, G, pt, pre, SumCost, step);
+    for (row += dr, col += dc; row >= 0 && row < H && col >= 0 && col < W; row += dr, col += dc) {
+        pt = (long)row * W + col;
+        memset(cur, 0, ncc[pt].NumOfHeight * sizeof(float));
+        sgm_con_pos(col, row, W, d, step, 1, u_col, v_row, ncc, vox, G, pt, 0.3, 0.6, pre, cur, SumCost);
+        float *t = pre;
+        pre = cur;
+        cur = t;
+    }
+}
+
+/* setsm_code.cpp:7017-7509: the 8 path directions in the reference's order.
+ * d 0..3: left->right, right->left, top->bottom, bottom->top, one path per
+ * row / column. d 4..7 (diagonals): a loop over paths starting on the left or
+ * right wall (one per row), then a loop over paths starting on the top or
+ * bottom wall (one per column); the corner path is in both loops, so its
+ * cells receive that direction twice, as in the reference. Paths of one loop
+ * are disjoint, so the loop runs in parallel like the reference's omp for. */
+void orc_sgm_aggregate_rows(const smx_level *Lv, const orc_voxels *vox, const smx_ugrid *G,
+                            const smx_nccresult *ncc, float *const *SumCost)
+{
+    const int W = (int)Lv->grid_w, H = (int)Lv->grid_h;
+    const long n = (long)W * H;
+    int maxH = 0;
+    for (long c = 0; c < n; c++)
+        if (maxH < ncc[c].NumOfHeight) maxH = ncc[c].NumOfHeight;
+    /* loop kinds: 0 = one path per row, 1 = one path per column */
+    struct { int d, kind, dr, dc; } loops[12] = {
+        {0, 0, 0, 1}, {1, 0, 0, -1}, {2, 1, 1, 0}, {3, 1, -1, 0},
+        {4, 0, 1, 1}, {4, 1, 1, 1}, {5, 0, 1, -1}, {5, 1, 1, -1},
+        {6, 0, -1, 1}, {6, 1, -1, 1}, {7, 0, -1, -1}, {7, 1, -1, -1},
+    };
+    for (int k = 0; k < 12; k++) {
+        const int d = loops[k].d, dr = loops[k].dr, dc = loops[k].dc;
+        const int npaths = loops[k].kind == 0 ? H : W;
+#pragma omp parallel num_threads(orc_get_threads())
+        {
+            float *pre = (float *)calloc((size_t)maxH + 1, sizeof(float));
+            float *cur = (float *)calloc((size_t)maxH + 1, sizeof(float));
+#pragma omp for
+            for (int p = 0; p < npaths; p++) {
+                int row, col;
+                if (loops[k].kind == 0) {          /* starts on the left (dc>0) or right (dc<0) wall, or a row start */
+                    row = p;
+                    col = dc > 0 ? 0 : (dc < 0 ? W - 1 : 0);
+                } else {                           /* starts on the top (dr>0) or bottom (dr<0) wall */
+                    col = p;
+                    row = dr > 0 ? 0 : H - 1;
+                }
+                sgm_path(row, col, dr, dc, d, Lv, vox, G, ncc, SumCost, pre, cur);
+            }
+            free(pre);
+            free(cur);
+        }
+    }
+}
+
+void orc_sgm_aggregate(const smx_level *Lv, const orc_voxels *vox, const smx_ugrid *G, const smx_nccresult *ncc,
+                       float *sum_csr)
+{
+    const long n = (long)Lv->grid_w * Lv->grid_h;
+    float **rows = (float **)calloc((size_t)n, sizeof(float *));
+    long off = 0;
+    for (long c = 0; c < n; c++) {
+        rows[c] = sum_csr + off;
+        for (int h = 0; h < ncc[c].NumOfHeight; h++) rows[c][h] = 0.0f;
+        off += ncc[c].NumOfHeight;
+    }
+    orc_sgm_aggregate_rows(Lv, vox, G, ncc, rows);
+    free(rows);
+}
+
+void orc_awncc(const smx_level *Lv, const orc_voxels *vox, const smx_ugrid *G, smx_nccresult *ncc, int iteration,
+               float *sum_csr)
+{
+    /* setsm_code.cpp:7511-7533: SGM aggregation when Pyramid_step <= SGM_py */
+    if (Lv->level > Lv->sgm_py) {
+        awncc_impl(Lv, vox, G, ncc, iteration, NULL);
+        return;
+    }
+    const long n = (long)Lv->grid_w * Lv->grid_h;
+    long total = 0;
+    for (long c = 0; c < n; c++) total += ncc[c].NumOfHeight;
+    float *own = sum_csr ? NULL : (float *)calloc((size_t)total + 1, sizeof(float));
+    float *buf = sum_csr ? sum_csr : own;
+    orc_sgm_aggregate(Lv, vox, G, ncc, buf);
+    float **rows = (float **)calloc((size_t)n, sizeof(float *));
+    long off = 0;
+    for (long c = 0; c < n; c++) {
+        rows[c] = buf + off;
+        off += ncc[c].NumOfHeight;
+    }
+    awncc_impl(Lv, vox, G, ncc, iteration, rows);
+    free(rows);
+    free(own);
 }
 
 /* ------------------------------------------------------------ pyramid */

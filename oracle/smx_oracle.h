@@ -56,9 +56,8 @@ double orc_ncc_alpha(int level, int iteration, int is_ra);
 void orc_initialize_voxel(const smx_level *L, orc_voxels *vox, smx_ugrid *gridpt3,
                           smx_nccresult *ncc, int iteration, const double minmax[2]);
 
-/* setsm_code.cpp:6049-6548 VerticalLineLocus (INCC term; the ortho GNCC term is
- * computed when gncc_coords is non-NULL, see orc_set_ortho_coords).
- * Returns Accessable_grid. OpenMP dynamic,1 over cells like the reference. */
+/* setsm_code.cpp:6049-6548 VerticalLineLocus (INCC term). Returns
+ * Accessable_grid. OpenMP dynamic,1 over cells like the reference. */
 long orc_vertical_line_locus(const smx_level *L, orc_voxels *vox, smx_nccresult *ncc,
                              const smx_ugrid *gridpt3, int iteration, const double minmax[2]);
 
@@ -66,6 +65,16 @@ long orc_vertical_line_locus(const smx_level *L, orc_voxels *vox, smx_nccresult 
  * (check_SGM false, i.e. Pyramid_step > SGM_py). */
 void orc_awncc_peak(const smx_level *L, const orc_voxels *vox, const smx_ugrid *gridpt3,
                     smx_nccresult *ncc, int iteration);
+/* setsm_code.cpp:7017-7509 SGM 8-path aggregation (SGM_start_pos :6755-6767,
+ * SGM_con_pos :6769-6999, P1 = 0.3, P2 = 0.6, P_HS_step = 1): SumCost per
+ * hypothesis in CSR order (prefix sum of NumOfHeight), zeroed first. */
+void orc_sgm_aggregate(const smx_level *L, const orc_voxels *vox, const smx_ugrid *gridpt3,
+                       const smx_nccresult *ncc, float *sumcost_csr);
+/* setsm_code.cpp:7511-7702 AWNCC as the reference runs it: SGM aggregation
+ * and the SGM peak mode when Pyramid_step <= SGM_py, the plain scan
+ * otherwise. sumcost_csr (may be NULL) receives SumCost in SGM mode. */
+void orc_awncc(const smx_level *L, const orc_voxels *vox, const smx_ugrid *gridpt3,
+               smx_nccresult *ncc, int iteration, float *sumcost_csr);
 
 /* Pyramid inputs. Template.hpp:45-132 (9x9, sigma 1.5 decimation),
  * SubFunctions.cpp:2418-2470 (Sobel), SubFunctions.cpp:2473-2571 (Orientation). */

@@ -132,6 +132,8 @@ static void free_level_state(smx_job *j)
     jfree(j, j->d_tile_count);
     jfree(j, j->d_tile_off);
     jfree(j, j->d_tile_runs);
+    jfree(j, j->d_sum);
+    j->sum_total = 0;
     j->tiles_cap = j->tile_runs_cap = 0;
     j->runs_cap = 0;
     j->vox_total = 0;
@@ -255,6 +257,7 @@ int smx_job_destroy(smx_job *j)
             jfree(j, v.ori);
         }
     jfree(j, j->scan_tmp);
+    jfree(j, j->d_sgm_scratch);
     job_free(j, j->d_batch);
     j->d_batch = nullptr;
     jfree(j, j->d_params);
@@ -675,16 +678,73 @@ int smx_vertical_line_locus_batch(smx_job **jobs, int n, int iteration, int prec
     return 0;
 }
 
+// setsm_code.cpp:7017-7509: SumCost of the current cube by the reference's
+// 12 path loops (levels <= SGM_py).
+static int sgm_aggregate(smx_job *j)
+{
+    const long long total = j->vox_total;
+    if (total != j->sum_total || !j->d_sum) {
+        jfree(j, j->d_sum);
+        int r = jalloc(j, &j->d_sum, (size_t)std::max(total, 1LL));
+        if (r) return r;
+        j->sum_total = total;
+    }
+    if (total) HIPCHK(hipMemsetAsync(j->d_sum, 0, (size_t)total * sizeof(float), j->stream));
+    // the longest height list sizes the per-path buffers
+    HIPCHK(hipMemsetAsync(j->d_counters + 2, 0, sizeof(unsigned long long), j->stream));
+    HIPCHK(launch_max_nh(j->d_ncc, j->ncells, j->d_counters + 2, j->stream));
+    HIPCHK(hipMemcpyAsync(j->h_small, j->d_counters + 2, sizeof(int64_t), hipMemcpyDeviceToHost, j->stream));
+    HIPCHK(hipStreamSynchronize(j->stream));
+    const int cap = (int)((std::max<int64_t>(j->h_small[0], 1) + 3) & ~3LL);
+    SgmLaunch L{};
+    L.W = (int)j->lv.grid_w;
+    L.H = (int)j->lv.grid_h;
+    L.step = j->lv.height_step;
+    L.ncc = j->d_ncc;
+    L.gridpt3 = j->d_gridpt3;
+    L.vox_off = j->d_vox_off;
+    L.vox = j->d_vox;
+    L.sum = j->d_sum;
+    L.cap = cap;
+    if ((size_t)10 * cap > sgm_lds_limit()) {
+        const size_t need = (size_t)10 * cap * (size_t)std::max(L.W, L.H);
+        if (need > j->sgm_scratch_bytes) {
+            jfree(j, j->d_sgm_scratch);
+            int r = jalloc(j, &j->d_sgm_scratch, need);
+            if (r) return r;
+            j->sgm_scratch_bytes = need;
+        }
+        L.gscratch = j->d_sgm_scratch;
+    }
+    HIPCHK(launch_sgm(L, j->stream));
+    return 0;
+}
+
 int smx_awncc(smx_job *j, int iteration)
 {
     if (!j || !j->level_set || !j->searchspace_set) return fail(SMX_E_STATE, "level and search space must be set");
-    if (j->lv.level <= j->lv.sgm_py)
-        return fail(SMX_E_UNSUPPORTED, "SGM aggregation (Pyramid_step <= SGM_py) is not implemented");
     HIPCHK(hipSetDevice(j->device));
     int r = upload_params(j, iteration);
     if (r) return r;
-    PeakArgs A{j->ncells, j->d_ncc, j->d_gridpt3, j->d_vox_off, j->d_vox};
+    const bool sgm = j->lv.level <= j->lv.sgm_py;
+    if (sgm) {
+        r = sgm_aggregate(j);
+        if (r) return r;
+    }
+    PeakArgs A{j->ncells, j->d_ncc, j->d_gridpt3, j->d_vox_off, j->d_vox, sgm ? j->d_sum : nullptr};
     HIPCHK(launch_awncc(j->d_params, A, j->stream));
+    HIPCHK(hipStreamSynchronize(j->stream));
+    return 0;
+}
+
+int smx_job_get_sumcost(smx_job *j, float *out)
+{
+    if (!j || !out || !j->level_set) return fail(SMX_E_STATE, "no level set");
+    if (!j->d_sum || j->sum_total != j->vox_total)
+        return fail(SMX_E_STATE, "no SumCost: AWNCC has not run at an SGM level on this cube");
+    HIPCHK(hipSetDevice(j->device));
+    if (j->sum_total)
+        HIPCHK(hipMemcpyAsync(out, j->d_sum, (size_t)j->sum_total * sizeof(float), hipMemcpyDeviceToHost, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
     return 0;
 }

@@ -49,6 +49,10 @@ struct smx_job {
     uint32_t *d_vox = nullptr;
     float *d_rho = nullptr;
     long long vox_total = 0;
+    float *d_sum = nullptr;                      // SGM SumCost (CSR), levels <= SGM_py
+    long long sum_total = 0;
+    unsigned char *d_sgm_scratch = nullptr;      // SGM path buffers when they exceed LDS
+    size_t sgm_scratch_bytes = 0;
 
     // scratch of the launch sequences
     int64_t *d_run_count = nullptr, *d_run_off = nullptr;
@@ -98,6 +102,20 @@ hipError_t launch_vll_finalize(const CellArgs &A, hipStream_t s);
 hipError_t launch_init_voxel(const VllParams *dP, const InitArgs &A, hipStream_t s);
 hipError_t launch_relayout(const InitArgs &A, hipStream_t s);
 hipError_t launch_awncc(const VllParams *dP, const PeakArgs &A, hipStream_t s);
+struct SgmLaunch {
+    int W, H;
+    double step;
+    const smx_nccresult *ncc;
+    const smx_ugrid *gridpt3;
+    const int64_t *vox_off;
+    const uint32_t *vox;
+    float *sum;
+    int cap;                      // heights per path buffer (>= max NumOfHeight, multiple of 4)
+    unsigned char *gscratch;      // null: buffers in LDS (10 * cap bytes per path)
+};
+hipError_t launch_sgm(const SgmLaunch &L, hipStream_t s);
+hipError_t launch_max_nh(const smx_nccresult *ncc, long n, unsigned long long *out, hipStream_t s);
+size_t sgm_lds_limit();
 int launch_vll_fast(smx_job *j, int half, const CellArgs &A, hipEvent_t ev0, hipEvent_t ev1);
 int launch_vll_fast_batch(smx_job **jobs, int n, const CellArgs *As, int half, hipEvent_t ev0, hipEvent_t ev1);
 int fast_num_configs();

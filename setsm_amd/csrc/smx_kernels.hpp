@@ -46,6 +46,7 @@ struct PeakArgs {
     const smx_ugrid *gridpt3;
     const int64_t *vox_off;
     const uint32_t *vox;
+    const float *sum;              // SGM SumCost (check_SGM_peak), or null
 };
 
 // SubFunctions.cpp:3274-3312 plus the outer disc test of setsm_code.cpp:6434:

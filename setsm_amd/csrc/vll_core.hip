@@ -414,8 +414,10 @@ __global__ void __launch_bounds__(256) k_relayout(InitArgs A)
 
 // ------------------------------------------------------------ AWNCC peak
 
-// setsm_code.cpp:7527-7702 with check_SGM_peak false; one thread per cell
-// scanning its heights in order.
+// setsm_code.cpp:7527-7702; one thread per cell scanning its heights in
+// order. With A.sum (check_SGM_peak, Pyramid_step <= SGM_py) the peaks are
+// those of SumCost, and at level 0 from iteration 2 on result0 is the INCC
+// (or blended) value at the chosen peak.
 __global__ void __launch_bounds__(256) k_awncc(const VllParams *__restrict__ P, PeakArgs A)
 {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -432,6 +434,7 @@ __global__ void __launch_bounds__(256) k_awncc(const VllParams *__restrict__ P, 
     float pre_height = 0.0f;
     int direction = 0;
     bool check_rho = false;
+    double max_roh = 0;
     double temp_nccresult = -100.0, temp_nccresult_sec = -100.0;
     nr.result0 = q_result(-1.0);
     nr.result2 = -1000;
@@ -450,7 +453,7 @@ __global__ void __launch_bounds__(256) k_awncc(const VllParams *__restrict__ P, 
         if (check_ortho && gncc > -1.0) WNCC_sum = incc * ncc_alpha + gncc * ncc_beta;
         else WNCC_sum = incc;
         const double WNCC_temp_rho = WNCC_sum;
-        const double temp_rho = WNCC_sum;
+        const double temp_rho = A.sum ? (double)A.sum[base + hs] : WNCC_sum;
         double diff_rho;
         int t_direction;
         if (temp_rho >= 0 && pre_rho >= 0) diff_rho = temp_rho - pre_rho;
@@ -472,6 +475,7 @@ __global__ void __launch_bounds__(256) k_awncc(const VllParams *__restrict__ P, 
                     nr.result2 = iter_height;
                     temp_nccresult_sec = -1.0;
                     nr.result3 = -9999;
+                    max_roh = WNCC_temp_rho;
                 }
             } else if (check_peak) {
                 if (nr.result4 < 10) nr.result4 += 1;
@@ -480,6 +484,7 @@ __global__ void __launch_bounds__(256) k_awncc(const VllParams *__restrict__ P, 
                     temp_nccresult = pre_rho;
                     const float temp_2 = nr.result2;
                     nr.result2 = pre_height;
+                    max_roh = pre_rho_WNCC;
                     if (temp_nccresult_sec < temp_1) {
                         temp_nccresult_sec = temp_1;
                         nr.result3 = temp_2;
@@ -495,9 +500,14 @@ __global__ void __launch_bounds__(256) k_awncc(const VllParams *__restrict__ P, 
         pre_height = iter_height;
         direction = t_direction;
     }
-    (void)pre_rho_WNCC;
-    nr.result0 = q_result(temp_nccresult);
-    nr.result1 = q_result(temp_nccresult_sec);
+    if (A.sum && Pyramid_step == 0 && iteration >= 2) {
+        nr.result0 = q_result(max_roh);
+        nr.result1 = q_result(-1.0);
+        nr.result3 = -9999;
+    } else {
+        nr.result0 = q_result(temp_nccresult);
+        nr.result1 = q_result(temp_nccresult_sec);
+    }
     A.ncc[c] = nr;
 }
 

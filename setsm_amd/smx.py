@@ -32,6 +32,7 @@ EXPORTS = [
     "smx_job_get_view", "smx_job_set_level", "smx_job_set_searchspace", "smx_job_get_searchspace",
     "smx_job_get_nccresult", "smx_job_set_nccresult", "smx_initialize_voxel", "smx_vertical_line_locus",
     "smx_awncc", "smx_job_num_hypotheses", "smx_job_get_voxels", "smx_job_get_voxel_rows", "smx_job_set_voxels",
+    "smx_job_get_sumcost",
     "smx_job_last_kernel_ms", "smx_job_last_evaluated", "smx_job_sync", "smx_fast_num_configs",
     "smx_job_set_fast_config", "smx_fast_config_desc", "smx_job_last_stats", "smx_vertical_line_locus_batch",
 ]
@@ -79,6 +80,7 @@ def lib():
             "smx_job_get_voxels": (i, [vp, vp, vp, vp]),
             "smx_job_get_voxel_rows": (i, [vp, ll, ll, vp, vp]),
             "smx_job_set_voxels": (i, [vp, vp]),
+            "smx_job_get_sumcost": (i, [vp, vp]),
             "smx_job_last_kernel_ms": (C.c_double, [vp]),
             "smx_job_last_evaluated": (ll, [vp]),
             "smx_job_sync": (i, [vp]),
@@ -213,8 +215,13 @@ class Job:
 
     def set_voxels(self, words):
         w = np.ascontiguousarray(words, np.uint32)
-        assert w.shape[0] == self.num_hypotheses
         _check(lib().smx_job_set_voxels(self.h, _p(w)))
+
+    def get_sumcost(self):
+        """SumCost (float32, CSR order) of the last SGM-level AWNCC."""
+        out = np.zeros(max(self.num_hypotheses, 1), np.float32)
+        _check(lib().smx_job_get_sumcost(self.h, _p(out)))
+        return out[:self.num_hypotheses]
 
     @property
     def last_kernel_ms(self) -> float:
