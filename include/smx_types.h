@@ -81,6 +81,14 @@ typedef struct smx_level {
     uint32_t grid_h;
     const double *grid_pts;       /* GridPts: (x, y) metres, 2 doubles per cell      */
     const double *grid_wgs;       /* Grid_wgs: (lon, lat) degrees, 2 doubles per cell*/
+    int32_t  reference_id;        /* LevelInfo.reference_id: SETSM's index of image 0;
+                                     the GNCC gate reads GridPT3.ortho_ncc[1 - reference_id] */
+    uint32_t ortho_w;             /* ortho grid of SetOrthoImageCoord at im_resolution =
+                                     resolution * 2^level: (Boundary[2]-Boundary[0])/im_res + 1 */
+    uint32_t ortho_h;             /*   columns, (Boundary[3]-Boundary[1])/im_res + 1 rows   */
+    const double *ortho_wgs;      /* (lon, lat) degrees of every ortho pixel (Boundary[0] +
+                                     col*im_res, Boundary[1] + row*im_res), row-major, as
+                                     ps2wgs_single gives them; NULL if GNCC is never active */
 } smx_level;
 
 /* UGRID (Typedefine.hpp:271-284) */

@@ -92,6 +92,13 @@ int  smx_job_get_view(smx_job *job, int img, int level, uint16_t *intensity,
  * setsm_code.cpp:3745, 3769). */
 int  smx_job_set_level(smx_job *job, const smx_level *params);
 
+/* Upload (or replace) the ortho grid lon/lat of the current level without
+ * resetting it: w x h (lon, lat) pairs as smx_level.ortho_wgs describes
+ * them. Needed before smx_vertical_line_locus whenever the GNCC term is
+ * active (check_ortho and some GridPT3.ortho_ncc above ortho_th,
+ * setsm_code.cpp:6100-6129, 6240). */
+int  smx_job_set_ortho_wgs(smx_job *job, const double *lonlat, uint32_t w, uint32_t h);
+
 /* Upload the search space GridPT3 for the current level (one smx_ugrid per
  * cell) and the level's minmaxHeight. */
 int  smx_job_set_searchspace(smx_job *job, const smx_ugrid *gridpt3, const double minmax[2]);

@@ -548,6 +548,107 @@ void orc_initialize_voxel(const smx_level *Lv, orc_voxels *vox, smx_ugrid *GridP
     }
 }
 
+/* ---------------------------------------------------------- GNCC */
+
+/* setsm_code.cpp:6550-6670 SetOrthoImageCoord, SB sensor, check_pyimage =
+ * OR. The ortho pixels' lon/lat (ps2wgs_single of the pixel's PS position)
+ * come from the caller in L->ortho_wgs. cd[k] / cdn[k] hold pyramid image
+ * coordinates (x, y) of image k (0 = reference, 1 = target) at the level and
+ * at level - 1; entries never written stay 0 (calloc). */
+static void set_ortho_image_coord(const smx_level *Lv, const smx_ugrid *GridPT3, int combined, double im_res,
+                                  double im_res_next, long sw, long sh, long swn, long shn, double *cd[2],
+                                  double *cdn[2])
+{
+    const long total = sw * sh;
+    const long W = Lv->grid_w, H = Lv->grid_h;
+    const long grid_length = W * H;
+#pragma omp parallel for schedule(guided) num_threads(orc_get_threads())
+    for (long it = 0; it < total; it++) {
+        const long pts_row = (long)floor((double)(it / sw));
+        const long pts_col = it % sw;
+        const double t_X = Lv->boundary[0] + pts_col * im_res;
+        const double t_Y = Lv->boundary[1] + pts_row * im_res;
+        const long t_col = (long)((t_X - Lv->boundary[0]) / Lv->grid_resolution);
+        const long t_row = (long)((t_Y - Lv->boundary[1]) / Lv->grid_resolution);
+        const long pt_index = t_row * W + t_col;
+        const long pt_index_im = pts_row * sw + pts_col;
+        long t_col_next = 0, t_row_next = 0, pt_index_im_next = 0;
+        if (combined) {
+            t_col_next = (long)((t_X - Lv->boundary[0]) / im_res_next);
+            t_row_next = (long)((t_Y - Lv->boundary[1]) / im_res_next);
+            pt_index_im_next = t_row_next * swn + t_col_next;
+        }
+        if (pt_index < grid_length && t_col < W && t_row < H) {
+            if (GridPT3[pt_index].Height != -1000) {
+                const double lon = Lv->ortho_wgs[2 * it], lat = Lv->ortho_wgs[2 * it + 1];
+                const double Z = (double)GridPT3[pt_index].Height;
+                for (int k = 0; k < 2; k++) {
+                    double ix, iy, px, py;
+                    orc_rpc_ground_to_image(&Lv->rpc[k], Lv->adjust[k], lon, lat, Z, &ix, &iy);
+                    to_pyramid(ix, iy, Lv->view[k].start_x, Lv->view[k].start_y, Lv->level, &px, &py);
+                    cd[k][2 * pt_index_im] = px;
+                    cd[k][2 * pt_index_im + 1] = py;
+                    if (combined) {
+                        to_pyramid(ix, iy, Lv->next[k].start_x, Lv->next[k].start_y, Lv->level - 1, &px, &py);
+                        if (pt_index_im_next < swn * shn && t_col_next < swn && t_row_next < shn) {
+                            cdn[k][2 * pt_index_im_next] = px;
+                            cdn[k][2 * pt_index_im_next + 1] = py;
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* setsm_code.cpp:6240-6305: the ortho multi-window NCC of one cell on the
+ * fixed ortho-grid geometry. Returns count_GNCC, adds to *sum. */
+static double gncc_cell(const smx_level *Lv, const smx_ugrid *GridPT3, long pt_index, int H, int combined,
+                        double im_res, double im_res_next, long sw, long sh, long swn, long shn,
+                        double *const cd[2], double *const cdn[2], patchvec *pv, patchvec *pvn, double *sum)
+{
+    const long W = Lv->grid_w, Hg = Lv->grid_h;
+    const long numofpts = W * Hg;
+    int Count_N_ortho[3] = {0, 0, 0}, Count_N_ortho_next[3] = {0, 0, 0};
+    double count_GNCC = 0;
+    for (int row = -H; row <= H; row++) {
+        for (int col = -H; col <= H; col++) {
+            const int radius2 = row * row + col * col;
+            if (radius2 > (H + 1) * (H + 1)) continue;
+            const double t_X = Lv->grid_pts[2 * pt_index] + col * im_res;
+            const double t_Y = Lv->grid_pts[2 * pt_index + 1] + row * im_res;
+            long t_col = (long)((t_X - Lv->boundary[0]) / im_res);
+            long t_row = (long)((t_Y - Lv->boundary[1]) / im_res);
+            long pt_index_temp = t_row * sw + t_col;
+            const long tt_col = (long)((t_X - Lv->boundary[0]) / Lv->grid_resolution);
+            const long tt_row = (long)((t_Y - Lv->boundary[1]) / Lv->grid_resolution);
+            const long pt_index_dem = tt_row * W + tt_col;
+            if (pt_index_temp >= 0 && pt_index_temp < sw * sh && t_col >= 0 && t_col < sw && t_row >= 0 &&
+                t_row < sh && pt_index_dem >= 0 && pt_index_dem < numofpts && tt_col >= 0 && tt_col < W &&
+                tt_row >= 0 && tt_row < Hg) {
+                if (GridPT3[pt_index_dem].Height != -1000) {
+                    set_vec_kernel_value(pv, H, &Lv->view[0], &Lv->view[1], row, col, cd[0][2 * pt_index_temp],
+                                         cd[0][2 * pt_index_temp + 1], cd[1][2 * pt_index_temp],
+                                         cd[1][2 * pt_index_temp + 1], radius2, Count_N_ortho);
+                    if (combined) {
+                        t_col = (long)((t_X - Lv->boundary[0]) / im_res_next);
+                        t_row = (long)((t_Y - Lv->boundary[1]) / im_res_next);
+                        pt_index_temp = t_row * swn + t_col;
+                        if (pt_index_temp >= 0 && pt_index_temp < swn * shn && t_col < swn && t_row < shn)
+                            set_vec_kernel_value(pvn, H, &Lv->next[0], &Lv->next[1], row, col,
+                                                 cdn[0][2 * pt_index_temp], cdn[0][2 * pt_index_temp + 1],
+                                                 cdn[1][2 * pt_index_temp], cdn[1][2 * pt_index_temp + 1], radius2,
+                                                 Count_N_ortho_next);
+                    }
+                }
+            }
+        }
+    }
+    compute_multi_ncc(pv, Count_N_ortho, &count_GNCC, sum);
+    if (combined) compute_multi_ncc(pvn, Count_N_ortho_next, &count_GNCC, sum);
+    return count_GNCC;
+}
+
 /* --------------------------------------------------- VerticalLineLocus */
 
 long orc_vertical_line_locus(const smx_level *Lv, orc_voxels *vox, smx_nccresult *nccresult,
@@ -573,11 +674,36 @@ long orc_vertical_line_locus(const smx_level *Lv, orc_voxels *vox, smx_nccresult
     if (Pyramid_step <= Lv->sgm_py) check_ortho = 0;
 
     const double ortho_th = 0.7 - (4 - Pyramid_step) * 0.10;
-    /* GNCC (setsm_code.cpp:6240-6305) is outside this restatement: refuse
-     * inputs where it would be computed. */
-    if (check_ortho) {
-        for (long c = 0; c < numofpts; c++)
-            if (dq(GridPT3[c].ortho_ncc[1]) > ortho_th) return -2;
+    const int ti_slot = 1 - Lv->reference_id;     /* GridPT3.ortho_ncc[ti] of the target image */
+    const double ncc_alpha = orc_ncc_alpha(Pyramid_step, iteration, IsRA);
+    const double ncc_beta = 1.0 - ncc_alpha;
+    const double im_resolution = Lv->resolution * (double)(1 << Pyramid_step);
+    const double im_resolution_next = check_combined_WNCC ? Lv->resolution * (double)(1 << (Pyramid_step - 1))
+                                                          : Lv->resolution;
+    /* setsm_code.cpp:6121-6129 ortho image coordinates, when some cell can
+     * take the GNCC term (setsm_code.cpp:6240) */
+    int gncc_on = 0;
+    if (check_ortho)
+        for (long c = 0; c < numofpts && !gncc_on; c++)
+            if (dq(GridPT3[c].ortho_ncc[ti_slot]) > ortho_th) gncc_on = 1;
+    long sw = 0, sh = 0, swn = 0, shn = 0;
+    double *cd[2] = {NULL, NULL}, *cdn[2] = {NULL, NULL};
+    if (gncc_on) {
+        if (!Lv->ortho_wgs) return -2;
+        sw = (long)((Lv->boundary[2] - Lv->boundary[0]) / im_resolution) + 1;
+        sh = (long)((Lv->boundary[3] - Lv->boundary[1]) / im_resolution) + 1;
+        if (sw != (long)Lv->ortho_w || sh != (long)Lv->ortho_h) return -4;
+        if (check_combined_WNCC) {
+            if (!Lv->next[0].img || !Lv->next[1].img) return -3;
+            swn = (long)((Lv->boundary[2] - Lv->boundary[0]) / im_resolution_next) + 1;
+            shn = (long)((Lv->boundary[3] - Lv->boundary[1]) / im_resolution_next) + 1;
+        }
+        for (int k = 0; k < 2; k++) {
+            cd[k] = (double *)calloc((size_t)(2 * sw * sh), sizeof(double));
+            if (check_combined_WNCC) cdn[k] = (double *)calloc((size_t)(2 * swn * shn), sizeof(double));
+        }
+        set_ortho_image_coord(Lv, GridPT3, check_combined_WNCC, im_resolution, im_resolution_next, sw, sh, swn, shn,
+                              cd, cdn);
     }
     if (check_combined_WNCC_INCC && (!Lv->next[0].img || !Lv->next[1].img)) return -3;
 
@@ -630,7 +756,16 @@ long orc_vertical_line_locus(const smx_level *Lv, orc_voxels *vox, smx_nccresult
 
             /* ti = 1 only (stereo pair) */
             Accessable_grid++;
-            nr->GNCC = q_result(-1.0);   /* GNCC term inactive (checked above) */
+            double sum_GNCC_multi = 0, count_GNCC = 0;
+            if (check_ortho && dq(GridPT3[pt_index].ortho_ncc[ti_slot]) > ortho_th) {
+                count_GNCC = gncc_cell(Lv, GridPT3, pt_index, Half_template_size, check_combined_WNCC, im_resolution,
+                                       im_resolution_next, sw, sh, swn, shn, cd, cdn, &pv, &pvn, &sum_GNCC_multi);
+                if (count_GNCC > 0)
+                    nr->GNCC = q_result(sum_GNCC_multi / count_GNCC);
+                else
+                    nr->GNCC = q_result(-1.0);
+            } else
+                nr->GNCC = q_result(-1.0);
 
             const int NumOfHeights = (int)((end_H - start_H) / height_step);
             if (IsRA || check_matching_rate) {
@@ -749,8 +884,12 @@ long orc_vertical_line_locus(const smx_level *Lv, orc_voxels *vox, smx_nccresult
                     }
 
                     double temp_rho;
-                    if (count_INCC > 0) temp_rho = sum_INCC_multi / count_INCC;
-                    else temp_rho = -1.0;
+                    if (check_ortho && count_GNCC > 0)   /* setsm_code.cpp:6482-6483 */
+                        temp_rho = sum_INCC_multi / count_INCC * ncc_alpha + sum_GNCC_multi / count_GNCC * ncc_beta;
+                    else if (count_INCC > 0)
+                        temp_rho = sum_INCC_multi / count_INCC;
+                    else
+                        temp_rho = -1.0;
                     if (max_WNCC < temp_rho) max_WNCC = temp_rho;
 
                     if (check_matching_rate) {
@@ -799,6 +938,10 @@ long orc_vertical_line_locus(const smx_level *Lv, orc_voxels *vox, smx_nccresult
         }
         pv_free(&pv);
         pv_free(&pvn);
+    }
+    for (int k = 0; k < 2; k++) {
+        free(cd[k]);
+        free(cdn[k]);
     }
     return Accessable_grid;
 }

@@ -134,6 +134,12 @@ static void free_level_state(smx_job *j)
     jfree(j, j->d_tile_runs);
     jfree(j, j->d_sum);
     j->sum_total = 0;
+    jfree(j, j->d_grid_pts);
+    jfree(j, j->d_ortho_wgs);
+    jfree(j, j->d_cd);
+    jfree(j, j->d_cdn);
+    jfree(j, j->d_gval);
+    j->cd_bytes = j->cdn_bytes = 0;
     j->tiles_cap = j->tile_runs_cap = 0;
     j->runs_cap = 0;
     j->vox_total = 0;
@@ -158,7 +164,7 @@ static int upload_params(smx_job *j, int iteration)
     for (int ti = 0; ti < 2; ti++) {
         const LevelView &v = j->views[ti][L.level];
         P.view[ti] = DevView{v.texel, v.ori, v.w, v.h, v.sx, v.sy};
-        if (P.combined_incc) {
+        if (combined_wncc && L.level >= 1 && j->views[ti][L.level - 1].texel) {
             const LevelView &n = j->views[ti][L.level - 1];
             P.next[ti] = DevView{n.texel, n.ori, n.w, n.h, n.sx, n.sy};
         } else {
@@ -414,6 +420,21 @@ int smx_job_set_level(smx_job *j, const smx_level *p)
     if (!r) r = jalloc(j, &j->d_new_off, (size_t)n + 1);
     if (r) return r;
     HIPCHK(hipMemcpyAsync(j->d_grid_wgs, p->grid_wgs, (size_t)n * 16, hipMemcpyHostToDevice, j->stream));
+    if (p->grid_pts) {
+        r = jalloc(j, &j->d_grid_pts, (size_t)n * 2);
+        if (r) return r;
+        HIPCHK(hipMemcpyAsync(j->d_grid_pts, p->grid_pts, (size_t)n * 16, hipMemcpyHostToDevice, j->stream));
+    }
+    j->lv.ortho_w = j->lv.ortho_h = 0;
+    j->lv.ortho_wgs = nullptr;
+    if (p->ortho_wgs) {
+        j->level_set = true;
+        r = smx_job_set_ortho_wgs(j, p->ortho_wgs, p->ortho_w, p->ortho_h);
+        if (r) {
+            j->level_set = false;
+            return r;
+        }
+    }
     // SETSM callocs nccresult per level (setsm_code.cpp:3769): all zero.
     HIPCHK(hipMemsetAsync(j->d_ncc, 0, (size_t)n * sizeof(smx_nccresult), j->stream));
     HIPCHK(hipMemsetAsync(j->d_vox_off, 0, ((size_t)n + 1) * sizeof(int64_t), j->stream));
@@ -448,14 +469,38 @@ int smx_job_set_level(smx_job *j, const smx_level *p)
     return 0;
 }
 
+int smx_job_set_ortho_wgs(smx_job *j, const double *lonlat, uint32_t w, uint32_t h)
+{
+    if (!j || !lonlat) return fail(SMX_E_ARG, "smx_job_set_ortho_wgs: null argument");
+    if (!j->level_set) return fail(SMX_E_STATE, "no level set");
+    const smx_level &p = j->lv;
+    // setsm_code.cpp:6552-6553: the ortho grid SetOrthoImageCoord walks
+    const double im_res = p.resolution * (double)(1 << p.level);
+    const long sw = (long)((p.boundary[2] - p.boundary[0]) / im_res) + 1;
+    const long sh = (long)((p.boundary[3] - p.boundary[1]) / im_res) + 1;
+    if (sw != (long)w || sh != (long)h)
+        return fail(SMX_E_ARG, "ortho grid is %ldx%ld at this level and boundary, got %ux%u", sw, sh, w, h);
+    HIPCHK(hipSetDevice(j->device));
+    jfree(j, j->d_ortho_wgs);
+    int r = jalloc(j, &j->d_ortho_wgs, (size_t)sw * sh * 2);
+    if (r) return r;
+    HIPCHK(hipMemcpyAsync(j->d_ortho_wgs, lonlat, (size_t)sw * sh * 16, hipMemcpyHostToDevice, j->stream));
+    HIPCHK(hipStreamSynchronize(j->stream));
+    j->lv.ortho_w = w;
+    j->lv.ortho_h = h;
+    return 0;
+}
+
 int smx_job_set_searchspace(smx_job *j, const smx_ugrid *g, const double minmax[2])
 {
     if (!j || !g || !minmax) return fail(SMX_E_ARG, "smx_job_set_searchspace: null argument");
     if (!j->level_set) return fail(SMX_E_STATE, "no level set");
     HIPCHK(hipSetDevice(j->device));
-    int16_t mx = -32768;
-    for (long c = 0; c < j->ncells; c++) mx = std::max(mx, g[c].ortho_ncc[1]);
-    j->max_ortho_ncc = mx;
+    int16_t mx[2] = {-32768, -32768};
+    for (long c = 0; c < j->ncells; c++)
+        for (int k = 0; k < 2; k++) mx[k] = std::max(mx[k], g[c].ortho_ncc[k]);
+    j->max_ortho_ncc[0] = mx[0];
+    j->max_ortho_ncc[1] = mx[1];
     j->minmax[0] = minmax[0];
     j->minmax[1] = minmax[1];
     HIPCHK(hipMemcpyAsync(j->d_gridpt3, g, (size_t)j->ncells * sizeof(smx_ugrid), hipMemcpyHostToDevice, j->stream));
@@ -545,15 +590,25 @@ static int vll_prepare(smx_job *j, int iteration, int precision, int keep_rho, C
     const smx_level &L = j->lv;
     if (L.is_ra || L.check_matching_rate)
         return fail(SMX_E_UNSUPPORTED, "RA / check_matching_rate mode (inline FindPeakNcc) is not implemented");
-    // setsm_code.cpp:6100-6112 check_ortho; the GNCC term it enables is not
-    // implemented, so refuse inputs where a cell would compute it.
+    // setsm_code.cpp:6100-6112 check_ortho, 6240 the per-cell GNCC gate
     bool check_ortho = !((L.level == 4 && iteration == 1) || L.is_ra);
     if (L.pre_dem && !(L.level == 4 && iteration == 1)) check_ortho = true;
     if (L.check_matchtag) check_ortho = true;
     if (L.level <= L.sgm_py) check_ortho = false;
     const double ortho_th = 0.7 - (4 - L.level) * 0.10;
-    if (check_ortho && (double)j->max_ortho_ncc / 1000.0 > ortho_th)
-        return fail(SMX_E_UNSUPPORTED, "ortho GNCC term would be active (ortho_ncc > %.2f): not implemented", ortho_th);
+    const int slot = 1 - L.reference_id;
+    if (slot < 0 || slot > 1) return fail(SMX_E_ARG, "reference_id must be 0 or 1");
+    const bool gncc_on = check_ortho && (double)j->max_ortho_ncc[slot] / 1000.0 > ortho_th;
+    const bool combined_wncc = !L.is_ra && (L.level > L.py_combined_level);
+    if (gncc_on) {
+        if (!j->d_ortho_wgs || !j->d_grid_pts)
+            return fail(SMX_E_ARG, "the GNCC term is active (ortho_ncc > %.2f) but the level has no ortho_wgs / "
+                                   "grid_pts", ortho_th);
+        if (combined_wncc)
+            for (int ti = 0; ti < 2; ti++)
+                if (!j->views[ti][L.level - 1].texel)
+                    return fail(SMX_E_STATE, "the combined GNCC term needs level %d", L.level - 1);
+    }
     const int half = j->hp.half;
     if (half < 5 || half > 7) return fail(SMX_E_UNSUPPORTED, "Half_template_size %d not built (5..7 are)", half);
     HIPCHK(hipSetDevice(j->device));
@@ -580,8 +635,65 @@ static int vll_prepare(smx_job *j, int iteration, int precision, int keep_rho, C
     A.max_bits = j->d_max_bits;
     A.accessible = j->d_counters;
     A.evaluated = j->d_counters + 1;
+    A.ncc_alpha = j->hp.ncc_alpha;
+    OrthoArgs O{};
+    if (gncc_on) {
+        // setsm_code.cpp:6121-6129: ortho image coordinates of this iteration's
+        // surface (GridPT3.Height)
+        const double im_res = L.resolution * (double)(1 << L.level);
+        const double im_res_next = combined_wncc ? L.resolution * (double)(1 << (L.level - 1)) : L.resolution;
+        O.W = L.grid_w;
+        O.H = L.grid_h;
+        O.sw = L.ortho_w;
+        O.sh = L.ortho_h;
+        if (combined_wncc) {
+            O.swn = (long)((L.boundary[2] - L.boundary[0]) / im_res_next) + 1;
+            O.shn = (long)((L.boundary[3] - L.boundary[1]) / im_res_next) + 1;
+        }
+        O.b0 = L.boundary[0];
+        O.b1 = L.boundary[1];
+        O.im_res = im_res;
+        O.im_res_next = im_res_next;
+        O.grid_res = L.grid_resolution;
+        O.combined = combined_wncc ? 1 : 0;
+        O.slot = slot;
+        O.ortho_th = ortho_th;
+        O.grid_pts = j->d_grid_pts;
+        O.ortho_wgs = j->d_ortho_wgs;
+        O.gridpt3 = j->d_gridpt3;
+        const size_t cdb = (size_t)O.sw * O.sh * 2 * 2, cdnb = (size_t)O.swn * O.shn * 2 * 2;
+        if (cdb > j->cd_bytes) {
+            jfree(j, j->d_cd);
+            int r2 = jalloc(j, &j->d_cd, cdb);
+            if (r2) return r2;
+            j->cd_bytes = cdb;
+        }
+        if (cdnb > j->cdn_bytes) {
+            jfree(j, j->d_cdn);
+            int r2 = jalloc(j, &j->d_cdn, cdnb);
+            if (r2) return r2;
+            j->cdn_bytes = cdnb;
+        }
+        if (!j->d_gval) {
+            int r2 = jalloc(j, &j->d_gval, (size_t)j->ncells);
+            if (r2) return r2;
+        }
+        O.cd[0] = j->d_cd;
+        O.cd[1] = j->d_cd + (size_t)O.sw * O.sh * 2;
+        O.cdn[0] = cdnb ? j->d_cdn : nullptr;
+        O.cdn[1] = cdnb ? j->d_cdn + (size_t)O.swn * O.shn * 2 : nullptr;
+        // calloc'd in the reference: pixels it skips read as (0, 0)
+        HIPCHK(hipMemsetAsync(j->d_cd, 0, cdb * sizeof(double), j->stream));
+        if (cdnb) HIPCHK(hipMemsetAsync(j->d_cdn, 0, cdnb * sizeof(double), j->stream));
+        HIPCHK(launch_ortho_coords(j->d_params, O, j->stream));
+        A.acc_flag = j->d_action;
+    }
     HIPCHK(hipMemsetAsync(j->d_counters, 0, 4 * sizeof(unsigned long long), j->stream));
     HIPCHK(launch_vll_gate(j->d_params, A, j->stream));
+    if (gncc_on) {
+        HIPCHK(launch_gncc(half, j->d_params, O, A, j->d_gval, j->stream));
+        A.gval = j->d_gval;
+    }
     r = job_scan(j, j->d_run_count, j->d_run_off, j->ncells + 1);
     if (r) return r;
     HIPCHK(hipMemcpyAsync(j->h_small, j->d_run_off + j->ncells, sizeof(int64_t), hipMemcpyDeviceToHost, j->stream));

@@ -41,8 +41,14 @@ struct smx_job {
     smx::VllParams *d_params = nullptr;
     long ncells = 0;
     double minmax[2] = {0, 0};
-    int16_t max_ortho_ncc = 0;
+    int16_t max_ortho_ncc[2] = {0, 0};            // max GridPT3.ortho_ncc[k] over cells
     double *d_grid_wgs = nullptr;
+    double *d_grid_pts = nullptr;                 // GridPts (GNCC windows)
+    double *d_ortho_wgs = nullptr;                // lon/lat per ortho pixel (GNCC)
+    double *d_cd = nullptr;                       // ortho image coords: 2 images x sw*sh (x, y)
+    double *d_cdn = nullptr;                      // the same at level - 1
+    size_t cd_bytes = 0, cdn_bytes = 0;
+    double *d_gval = nullptr;                     // per-cell GNCC value (NaN: none)
     smx_ugrid *d_gridpt3 = nullptr;
     smx_nccresult *d_ncc = nullptr;
     int64_t *d_vox_off = nullptr;
@@ -96,6 +102,9 @@ hipError_t launch_pack(const uint16_t *img, const uint16_t *mag, uint32_t *texel
 hipError_t launch_unpack(const uint32_t *texel, uint16_t *img, uint16_t *mag, long n, hipStream_t s);
 hipError_t launch_dir_probe(const int *gx, const int *gy, int16_t *out, long n, hipStream_t s);
 hipError_t launch_vll_gate(const VllParams *dP, const CellArgs &A, hipStream_t s);
+hipError_t launch_ortho_coords(const VllParams *dP, const OrthoArgs &O, hipStream_t s);
+hipError_t launch_gncc(int half, const VllParams *dP, const OrthoArgs &O, const CellArgs &A, double *gval,
+                       hipStream_t s);
 hipError_t launch_vll_fill_runs(const CellArgs &A, hipStream_t s);
 hipError_t launch_vll_fp64(int half, const VllParams *dP, const CellArgs &A, hipStream_t s);
 hipError_t launch_vll_finalize(const CellArgs &A, hipStream_t s);

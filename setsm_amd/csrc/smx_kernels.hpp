@@ -23,6 +23,41 @@ struct CellArgs {
     unsigned long long *max_bits;  // per cell, order_key of max_WNCC
     unsigned long long *accessible;
     unsigned long long *evaluated;
+    uint8_t *acc_flag;             // per cell: passed the gate (written when non-null)
+    const double *gval;            // per cell: sum_GNCC_multi / count_GNCC, NaN if no GNCC term; null: none
+    double ncc_alpha;              // SetNCC_alpha of the call (GNCC blend)
+};
+
+// setsm_code.cpp:6481-6490: the value max_WNCC takes for one hypothesis —
+// INCC blended with the cell's GNCC when that term exists (0/0 stays NaN,
+// which never raises the maximum, as in the reference).
+__device__ __forceinline__ double wncc_of(const CellArgs &A, long cell, double sum, double cnt, double rr)
+{
+#pragma clang fp contract(off)
+    if (A.gval) {
+        const double g = A.gval[cell];
+        if (g == g) {
+            const double t = sum / cnt * A.ncc_alpha + g * (1.0 - A.ncc_alpha);
+            return t == t ? t : -100.0;
+        }
+    }
+    return rr;
+}
+
+// SetOrthoImageCoord / GNCC geometry (setsm_code.cpp:6550-6670, 6240-6305).
+struct OrthoArgs {
+    long W, H;                     // Size_Grid2D
+    long sw, sh, swn, shn;         // ortho grid at im_resolution (and at the next level's)
+    double b0, b1;                 // Boundary[0], Boundary[1]
+    double im_res, im_res_next, grid_res;
+    int combined;                  // check_combined_WNCC
+    int slot;                      // GridPT3.ortho_ncc[] index of the target image
+    double ortho_th;
+    const double *grid_pts;        // GridPts (x, y)
+    const double *ortho_wgs;       // (lon, lat) per ortho pixel
+    const smx_ugrid *gridpt3;
+    double *cd[2];                 // pyramid image (x, y) per ortho pixel, image 0 / 1
+    double *cdn[2];                // the same at level - 1
 };
 
 struct InitArgs {

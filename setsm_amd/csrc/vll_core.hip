@@ -55,7 +55,7 @@ __device__ __forceinline__ bool gate_cell(const VllParams *__restrict__ P, CellA
         }
         if (!blunder) {
             accessible = true;
-            nr->GNCC = q_result(-1.0);   // ortho term inactive (checked on the host)
+            nr->GNCC = q_result(-1.0);   // k_gncc overwrites it where the ortho term applies
             if (nr->check_height_change || P->dem_lt2) {
                 nr->result0 = q_result(-1.0);
                 nr->result2 = -1000;
@@ -68,6 +68,7 @@ __device__ __forceinline__ bool gate_cell(const VllParams *__restrict__ P, CellA
         }
     }
     A.run_count[c] = runs;
+    if (A.acc_flag) A.acc_flag[c] = accessible ? 1 : 0;
     return accessible;
 }
 
@@ -130,10 +131,38 @@ __device__ __forceinline__ double corr_from(double sLR, double sL2, double sR2)
 // One level's multi-window NCC for one hypothesis: SetVecKernelValue over the
 // window (setsm_code.cpp:6429-6453) and ComputeMultiNCC (SubFunctions.cpp:
 // 3328-3359), two passes like Correlate. Returns true if all 3 counts > 0.
+// Sample positions of the hypothesis window: the left patch on the integer
+// grid around the left centre, the right patch rotated by -theta about the
+// right centre (setsm_code.cpp:6431-6436).
+struct AffinePos {
+    double rpx, rpy, tpx, tpy, cos0, sin0;
+    __device__ __forceinline__ bool operator()(int row, int col, double &lx, double &ly, double &rx,
+                                               double &ry) const
+    {
+        lx = rpx + col;
+        ly = rpy + row;
+        const double ox = cos0 * col - sin0 * row, oy = sin0 * col + cos0 * row;
+        rx = tpx + ox;
+        ry = tpy + oy;
+        return true;
+    }
+};
+
+template <int H, class Pos>
+__device__ __forceinline__ bool multi_ncc_fp64_at(const DevView &Lv, const DevView &Rv, const Pos &pos,
+                                                  double &sum_multi, double &count_multi);
+
 template <int H>
 __device__ __forceinline__ bool multi_ncc_fp64(const DevView &Lv, const DevView &Rv, double rpx, double rpy, double tpx,
                                                double tpy, double cos0, double sin0, double &sum_multi,
                                                double &count_multi)
+{
+    return multi_ncc_fp64_at<H>(Lv, Rv, AffinePos{rpx, rpy, tpx, tpy, cos0, sin0}, sum_multi, count_multi);
+}
+
+template <int H, class Pos>
+__device__ __forceinline__ bool multi_ncc_fp64_at(const DevView &Lv, const DevView &Rv, const Pos &pos,
+                                                  double &sum_multi, double &count_multi)
 {
     double sL[3] = {0, 0, 0}, sLm[3] = {0, 0, 0}, sR[3] = {0, 0, 0}, sRm[3] = {0, 0, 0};
     int cnt[3] = {0, 0, 0};
@@ -142,11 +171,10 @@ __device__ __forceinline__ bool multi_ncc_fp64(const DevView &Lv, const DevView 
 #pragma unroll
         for (int col = -H; col <= H; col++) {
             const int bits = window_bits_ce(H, row, col);
-            if (bits) {
-                const double lx = rpx + col, ly = rpy + row;
-                const double ox = cos0 * col - sin0 * row, oy = sin0 * col + cos0 * row;
+            double lx, ly, rx, ry;
+            if (bits && pos(row, col, lx, ly, rx, ry)) {
                 double l, lm, r, rm;
-                if (sample_fp64(Lv, Rv, lx, ly, tpx + ox, tpy + oy, l, lm, r, rm)) {
+                if (sample_fp64(Lv, Rv, lx, ly, rx, ry, l, lm, r, rm)) {
 #pragma unroll
                     for (int k = 0; k < 3; k++)
                         if (bits & (1 << k)) {
@@ -170,11 +198,10 @@ __device__ __forceinline__ bool multi_ncc_fp64(const DevView &Lv, const DevView 
 #pragma unroll
         for (int col = -H; col <= H; col++) {
             const int bits = window_bits_ce(H, row, col);
-            if (bits) {
-                const double lx = rpx + col, ly = rpy + row;
-                const double ox = cos0 * col - sin0 * row, oy = sin0 * col + cos0 * row;
+            double lx, ly, rx, ry;
+            if (bits && pos(row, col, lx, ly, rx, ry)) {
                 double l, lm, r, rm;
-                if (sample_fp64(Lv, Rv, lx, ly, tpx + ox, tpy + oy, l, lm, r, rm)) {
+                if (sample_fp64(Lv, Rv, lx, ly, rx, ry, l, lm, r, rm)) {
 #pragma unroll
                     for (int k = 0; k < 3; k++)
                         if (bits & (1 << k)) {
@@ -238,7 +265,7 @@ __global__ void __launch_bounds__(256) k_vll_fp64(const VllParams *__restrict__ 
             word |= (uint32_t)(uint16_t)q_voxel(rho) << 16;
             A.vox[wi] = word;
             if (A.rho) A.rho[wi] = (float)rho;
-            temp_rho = rho;
+            temp_rho = wncc_of(A, cell, sum_multi, count_multi, rho);
             evaluated = 1;
         }
     }
@@ -248,6 +275,114 @@ __global__ void __launch_bounds__(256) k_vll_fp64(const VllParams *__restrict__ 
 template __global__ void k_vll_fp64<5>(const VllParams *, CellArgs);
 template __global__ void k_vll_fp64<6>(const VllParams *, CellArgs);
 template __global__ void k_vll_fp64<7>(const VllParams *, CellArgs);
+
+// ------------------------------------------------------------- GNCC
+
+// setsm_code.cpp:6550-6670 SetOrthoImageCoord (SB sensor, OR pyramid): one
+// thread per ortho pixel; lon/lat of the pixel come from the caller.
+__global__ void __launch_bounds__(256) k_ortho_coords(const VllParams *__restrict__ P, OrthoArgs O)
+{
+    const long it = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= O.sw * O.sh) return;
+    const long pts_row = it / O.sw;
+    const long pts_col = it % O.sw;
+    const double t_X = O.b0 + pts_col * O.im_res;
+    const double t_Y = O.b1 + pts_row * O.im_res;
+    const long t_col = (long)((t_X - O.b0) / O.grid_res);
+    const long t_row = (long)((t_Y - O.b1) / O.grid_res);
+    const long pt_index = t_row * O.W + t_col;
+    const long pim = pts_row * O.sw + pts_col;
+    long t_col_next = 0, t_row_next = 0, pim_next = 0;
+    if (O.combined) {
+        t_col_next = (long)((t_X - O.b0) / O.im_res_next);
+        t_row_next = (long)((t_Y - O.b1) / O.im_res_next);
+        pim_next = t_row_next * O.swn + t_col_next;
+    }
+    if (!(pt_index < O.W * O.H && t_col < O.W && t_row < O.H)) return;
+    const float Hgt = O.gridpt3[pt_index].Height;
+    if (Hgt == -1000) return;
+    const double lon = O.ortho_wgs[2 * it], lat = O.ortho_wgs[2 * it + 1];
+    const double Z = (double)Hgt;
+#pragma unroll 1
+    for (int k = 0; k < 2; k++) {
+        double ix, iy, px, py;
+        rpc_ground_to_image(P->rpc[k], P->adjust[k], lon, lat, Z, ix, iy);
+        to_pyramid(ix, iy, P->view[k], P->level, px, py);
+        O.cd[k][2 * pim] = px;
+        O.cd[k][2 * pim + 1] = py;
+        if (O.combined) {
+            to_pyramid(ix, iy, P->next[k], P->level - 1, px, py);
+            if (pim_next < O.swn * O.shn && t_col_next < O.swn && t_row_next < O.shn) {
+                O.cdn[k][2 * pim_next] = px;
+                O.cdn[k][2 * pim_next + 1] = py;
+            }
+        }
+    }
+}
+
+// Sample positions of the GNCC window of one cell (setsm_code.cpp:
+// 6250-6289): window offsets on the ortho grid, positions from the ortho
+// image coordinates; absent where the reference skips SetVecKernelValue.
+struct OrthoPos {
+    const OrthoArgs *O;
+    long cell;
+    bool next;
+    __device__ __forceinline__ bool operator()(int row, int col, double &lx, double &ly, double &rx,
+                                               double &ry) const
+    {
+        const double t_X = O->grid_pts[2 * cell] + col * O->im_res;
+        const double t_Y = O->grid_pts[2 * cell + 1] + row * O->im_res;
+        long t_col = (long)((t_X - O->b0) / O->im_res);
+        long t_row = (long)((t_Y - O->b1) / O->im_res);
+        long pit = t_row * O->sw + t_col;
+        const long tt_col = (long)((t_X - O->b0) / O->grid_res);
+        const long tt_row = (long)((t_Y - O->b1) / O->grid_res);
+        const long pdem = tt_row * O->W + tt_col;
+        if (!(pit >= 0 && pit < O->sw * O->sh && t_col >= 0 && t_col < O->sw && t_row >= 0 && t_row < O->sh &&
+              pdem >= 0 && pdem < O->W * O->H && tt_col >= 0 && tt_col < O->W && tt_row >= 0 && tt_row < O->H))
+            return false;
+        if (O->gridpt3[pdem].Height == -1000) return false;
+        if (!next) {
+            lx = O->cd[0][2 * pit];
+            ly = O->cd[0][2 * pit + 1];
+            rx = O->cd[1][2 * pit];
+            ry = O->cd[1][2 * pit + 1];
+            return true;
+        }
+        t_col = (long)((t_X - O->b0) / O->im_res_next);
+        t_row = (long)((t_Y - O->b1) / O->im_res_next);
+        pit = t_row * O->swn + t_col;
+        if (!(pit >= 0 && pit < O->swn * O->shn && t_col < O->swn && t_row < O->shn)) return false;
+        lx = O->cdn[0][2 * pit];
+        ly = O->cdn[0][2 * pit + 1];
+        rx = O->cdn[1][2 * pit];
+        ry = O->cdn[1][2 * pit + 1];
+        return true;
+    }
+};
+
+// setsm_code.cpp:6240-6305: GNCC of every cell that passed the gate and whose
+// previous ortho NCC clears ortho_th; one thread per cell. Writes
+// NCCresult.GNCC and the unquantised value the INCC blend uses.
+template <int H>
+__global__ void __launch_bounds__(256) k_gncc(const VllParams *__restrict__ P, OrthoArgs O, CellArgs A, double *gval)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= A.ncells) return;
+    double g = __builtin_nan("");
+    if (A.acc_flag[c] && dq(A.gridpt3[c].ortho_ncc[O.slot]) > O.ortho_th) {
+        double sum = 0, count = 0;
+        multi_ncc_fp64_at<H>(P->view[0], P->view[1], OrthoPos{&O, c, false}, sum, count);
+        if (O.combined) multi_ncc_fp64_at<H>(P->next[0], P->next[1], OrthoPos{&O, c, true}, sum, count);
+        if (count > 0) {
+            g = sum / count;
+            A.ncc[c].GNCC = q_result(g);
+        } else {
+            A.ncc[c].GNCC = q_result(-1.0);
+        }
+    }
+    gval[c] = g;
+}
 
 // ------------------------------------------------------ InitializeVoxel
 
@@ -517,6 +652,25 @@ __global__ void __launch_bounds__(256) k_awncc(const VllParams *__restrict__ P, 
 namespace smx {
 
 static inline unsigned nblk(long n, int per = 256) { return (unsigned)((n + per - 1) / per); }
+
+hipError_t launch_ortho_coords(const VllParams *dP, const OrthoArgs &O, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ortho_coords, dim3(nblk(O.sw * O.sh)), dim3(256), 0, s, dP, O);
+    return hipGetLastError();
+}
+
+hipError_t launch_gncc(int half, const VllParams *dP, const OrthoArgs &O, const CellArgs &A, double *gval,
+                       hipStream_t s)
+{
+    const dim3 g(nblk(A.ncells)), b(256);
+    switch (half) {
+    case 5: hipLaunchKernelGGL(k_gncc<5>, g, b, 0, s, dP, O, A, gval); break;
+    case 6: hipLaunchKernelGGL(k_gncc<6>, g, b, 0, s, dP, O, A, gval); break;
+    case 7: hipLaunchKernelGGL(k_gncc<7>, g, b, 0, s, dP, O, A, gval); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_vll_gate(const VllParams *dP, const CellArgs &A, hipStream_t s)
 {

@@ -490,7 +490,7 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
             word |= (uint32_t)(uint16_t)q_voxel(rr) << 16;
             A.vox[wi] = word;
             if (A.rho) A.rho[wi] = (float)rr;
-            temp_rho = rr;
+            temp_rho = wncc_of(A, cell, sum, cnt, rr);
             evaluated = 1;
         }
         wave_max_commit(temp_rho, evaluated, &A.max_bits[cell], A.evaluated);

@@ -53,6 +53,8 @@ class SmxLevel(C.Structure):
         ("height_step", C.c_double), ("grid_resolution", C.c_double), ("boundary", C.c_double * 4),
         ("grid_w", C.c_uint32), ("grid_h", C.c_uint32),
         ("grid_pts", C.c_void_p), ("grid_wgs", C.c_void_p),
+        ("reference_id", C.c_int32), ("ortho_w", C.c_uint32), ("ortho_h", C.c_uint32),
+        ("ortho_wgs", C.c_void_p),
     ]
 
 
@@ -81,7 +83,7 @@ class LevelInputs:
     def __init__(self, pyr, level, rpcs, grid, *, resolution=0.5, dem_resolution=2.0,
                  grid_resolution=2.0, template_size=15, sgm_py=-1, py_combined_level=0,
                  start=((0.0, 0.0), (0.0, 0.0)), adjust=((0.0, 0.0), (0.0, 0.0)),
-                 is_ra=0, check_matchtag=0, pre_dem=0, check_matching_rate=0):
+                 is_ra=0, check_matchtag=0, pre_dem=0, check_matching_rate=0, ortho_wgs=None, reference_id=0):
         self.keep = []
         L = SmxLevel()
         for ti in range(2):
@@ -116,6 +118,13 @@ class LevelInputs:
         gh, gwid = grid.get("shape", (n, n))
         L.grid_w, L.grid_h = gwid, gh
         L.grid_pts, L.grid_wgs = gp.ctypes.data, gw.ctypes.data
+        L.reference_id = reference_id
+        if ortho_wgs is not None:
+            ow = np.ascontiguousarray(ortho_wgs, dtype=np.float64)
+            assert ow.ndim == 3 and ow.shape[2] == 2
+            self.keep.append(ow)
+            L.ortho_h, L.ortho_w = ow.shape[0], ow.shape[1]
+            L.ortho_wgs = ow.ctypes.data
         self.level = L
         self.ncells = gh * gwid
 

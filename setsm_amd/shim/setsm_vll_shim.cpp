@@ -19,9 +19,10 @@
 // buffers land at the same addresses is not taken for the same one.
 //
 // Errors follow SETSM's convention (setsm_code.cpp:6571-6575): a message on
-// stdout and exit(1). Inputs that need a term libsmx lacks (SGM aggregation
-// at Pyramid_step <= SGM_py, the ortho GNCC term, RA / check_matching_rate
-// mode, frame sensors, more than two images) are such errors.
+// stdout and exit(1). Inputs that need a mode libsmx lacks (RA /
+// check_matching_rate, frame sensors, more than two images) are such errors.
+// The GNCC term's ortho grid is computed here with SETSM's own
+// ps2wgs_single (CoordConversion.cpp), as SetOrthoImageCoord does.
 //
 // SMX_PRECISION=fp64 (read at every VerticalLineLocus call) selects the FP64
 // kernel, bit-identical to the CPU path; the default is the FAST kernel
@@ -31,7 +32,7 @@
 #include <cstring>
 #include <vector>
 
-#include "Typedefine.hpp"   // SETSM's ProInfo, LevelInfo, UGRID, NCCresult, VOXEL
+#include "CoordConversion.hpp"   // SETSM's records (Typedefine.hpp) and ps2wgs_single
 #include "smx_vll.h"
 
 static_assert(sizeof(UGRID) == sizeof(smx_ugrid), "UGRID layout");
@@ -51,6 +52,7 @@ struct State {
     int sizes[2][4] = {};
     smx_level lv;                   // the level record last passed to smx_job_set_level
     bool lv_set = false;
+    bool ortho_set = false;         // ortho grid lon/lat uploaded for this level record
 };
 
 thread_local State g_state;
@@ -181,11 +183,48 @@ bool bind(const ProInfo *proinfo, LevelInfo &L)
     lv.grid_h = L.Size_Grid2D->height;
     lv.grid_pts = (const double *)L.GridPts;
     lv.grid_wgs = (const double *)L.Grid_wgs;
+    lv.reference_id = ref;
     if (s.lv_set && !memcmp(&lv, &s.lv, sizeof lv)) return false;
     check(smx_job_set_level(s.job, &lv), "smx_job_set_level");
     s.lv = lv;
     s.lv_set = true;
+    s.ortho_set = false;
     return true;
+}
+
+// setsm_code.cpp:6100-6129, 6240: when some cell takes the GNCC term, hand
+// the level's ortho grid to the job as SETSM's SetOrthoImageCoord sees it:
+// ps2wgs_single of every ortho pixel (SB sensor).
+void ensure_ortho(State &s, const ProInfo *proinfo, LevelInfo &L, const UGRID *G, int iteration)
+{
+    if (s.ortho_set) return;
+    const int level = *L.Pyramid_step;
+    bool check_ortho = !((level == 4 && iteration == 1) || proinfo->IsRA);
+    if (proinfo->pre_DEMtif && !(level == 4 && iteration == 1)) check_ortho = true;
+    if (proinfo->check_Matchtag) check_ortho = true;
+    if (level <= proinfo->SGM_py) check_ortho = false;
+    if (!check_ortho) return;
+    const double ortho_th = 0.7 - (4 - level) * 0.10;
+    const int ti = L.reference_id == 0 ? 1 : 0;
+    const long n = *L.Grid_length;
+    bool any = false;
+    for (long c = 0; c < n && !any; c++) any = (double)(G[c].ortho_ncc[ti]) / 1000.0 > ortho_th;
+    if (!any) return;
+    const double im_res = proinfo->resolution * (double)(1 << level);
+    const long sw = (long)((L.Boundary[2] - L.Boundary[0]) / im_res) + 1;
+    const long sh = (long)((L.Boundary[3] - L.Boundary[1]) / im_res) + 1;
+    std::vector<double> lonlat((size_t)sw * sh * 2);
+#pragma omp parallel for schedule(guided)
+    for (long it = 0; it < sw * sh; it++) {
+        D2DPOINT p;
+        p.m_X = L.Boundary[0] + (it % sw) * im_res;
+        p.m_Y = L.Boundary[1] + (it / sw) * im_res;
+        const D2DPOINT w = ps2wgs_single(*L.param, p);
+        lonlat[2 * it] = w.m_X;
+        lonlat[2 * it + 1] = w.m_Y;
+    }
+    check(smx_job_set_ortho_wgs(s.job, lonlat.data(), (uint32_t)sw, (uint32_t)sh), "smx_job_set_ortho_wgs");
+    s.ortho_set = true;
 }
 
 // Upload the host cube (rows of NumOfHeight VOXELs) after a fresh level set.
@@ -256,6 +295,7 @@ int VerticalLineLocus(VOXEL **grid_voxel, const ProInfo *proinfo, NCCresult *ncc
     check(smx_job_set_searchspace(s.job, (const smx_ugrid *)GridPT3, minmaxHeight), "smx_job_set_searchspace");
     check(smx_job_set_nccresult(s.job, (const smx_nccresult *)nccresult), "smx_job_set_nccresult");
     if (fresh) upload_rows(s, grid_voxel, nccresult, n);
+    ensure_ortho(s, proinfo, plevelinfo, GridPT3, iteration);
     const long long acc = smx_vertical_line_locus(s.job, iteration, precision_from_env(), 0);
     if (acc < 0) die("smx_vertical_line_locus", (int)acc);
     check(smx_job_get_nccresult(s.job, (smx_nccresult *)nccresult), "smx_job_get_nccresult");

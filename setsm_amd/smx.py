@@ -32,7 +32,7 @@ EXPORTS = [
     "smx_job_get_view", "smx_job_set_level", "smx_job_set_searchspace", "smx_job_get_searchspace",
     "smx_job_get_nccresult", "smx_job_set_nccresult", "smx_initialize_voxel", "smx_vertical_line_locus",
     "smx_awncc", "smx_job_num_hypotheses", "smx_job_get_voxels", "smx_job_get_voxel_rows", "smx_job_set_voxels",
-    "smx_job_get_sumcost",
+    "smx_job_get_sumcost", "smx_job_set_ortho_wgs",
     "smx_job_last_kernel_ms", "smx_job_last_evaluated", "smx_job_sync", "smx_fast_num_configs",
     "smx_job_set_fast_config", "smx_fast_config_desc", "smx_job_last_stats", "smx_vertical_line_locus_batch",
 ]
@@ -81,6 +81,7 @@ def lib():
             "smx_job_get_voxel_rows": (i, [vp, ll, ll, vp, vp]),
             "smx_job_set_voxels": (i, [vp, vp]),
             "smx_job_get_sumcost": (i, [vp, vp]),
+            "smx_job_set_ortho_wgs": (i, [vp, vp, C.c_uint32, C.c_uint32]),
             "smx_job_last_kernel_ms": (C.c_double, [vp]),
             "smx_job_last_evaluated": (ll, [vp]),
             "smx_job_sync": (i, [vp]),
@@ -163,6 +164,11 @@ class Job:
     def set_level(self, level: SmxLevel):
         _check(lib().smx_job_set_level(self.h, C.addressof(level)))
         self.ncells = int(level.grid_w) * int(level.grid_h)
+
+    def set_ortho_wgs(self, lonlat):
+        """(rows, cols, 2) lon/lat of the level's ortho grid (GNCC term)."""
+        a = np.ascontiguousarray(lonlat, np.float64)
+        _check(lib().smx_job_set_ortho_wgs(self.h, _p(a), a.shape[1], a.shape[0]))
 
     def set_searchspace(self, ugrid, minmax):
         ug = np.ascontiguousarray(ugrid, UGRID_DTYPE)

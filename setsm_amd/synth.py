@@ -319,3 +319,20 @@ def level_grid(scene: Scene, level: int, cells: int, nheights: int, seed: int = 
     g["shape"] = (cells, cells)
     g["grid_res"] = grid_res
     return g
+
+
+def ortho_wgs(scene: Scene, boundary, level: int, resolution: float = 0.5, x_offset: float = 0.0,
+              y_offset: float = 0.0):
+    """(lon, lat) of every pixel of the ortho grid SETSM's SetOrthoImageCoord
+    walks at one level (setsm_code.cpp:6552-6553, 6593-6594): columns
+    (Boundary[2]-Boundary[0])/im_res + 1, rows likewise, pixel (row, col) at
+    (Boundary[0] + col*im_res, Boundary[1] + row*im_res), im_res =
+    resolution * 2^level. Returns an array (rows, cols, 2)."""
+    im_res = resolution * float(1 << level)
+    sw = int((boundary[2] - boundary[0]) / im_res) + 1
+    sh = int((boundary[3] - boundary[1]) / im_res) + 1
+    X = boundary[0] + np.arange(sw) * im_res
+    Y = boundary[1] + np.arange(sh) * im_res
+    XX, YY = np.meshgrid(X, Y)
+    lon, lat = scene.to_wgs(XX - x_offset, YY - y_offset)
+    return np.stack([lon, lat], -1)

@@ -5,13 +5,30 @@
 // iteration, on host arrays (VOXEL **grid_voxel, NCCresult *, UGRID *).
 // The three functions are called through their reference signatures
 // (setsm_code.hpp:75, 81, 91), which libsmx_setsm.so exports.
+#include <cmath>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
 
-#include "Typedefine.hpp"
+#include "CoordConversion.hpp"
 #include "smx_types.h"
+
+// Test double for SETSM's ps2wgs_single (CoordConversion.cpp:396), which the
+// shim calls for the GNCC ortho grid: the synthetic scenes map metres to
+// degrees with a local spherical mapping, lat = lat0 + N/R*180/pi and
+// lon = lon0 + E/D*180/pi (D = R cos(lat0)), evaluated in the same operation
+// order as the Python scene generator.
+static double g_lat0, g_lon0, g_R = 1.0, g_D = 1.0;
+
+D2DPOINT ps2wgs_single(const TransParam _param, const D2DPOINT _ps)
+{
+    (void)_param;
+    D2DPOINT w;
+    w.m_Y = g_lat0 + _ps.m_Y / g_R * 180.0 / M_PI;
+    w.m_X = g_lon0 + _ps.m_X / g_D * 180.0 / M_PI;
+    return w;
+}
 
 void InitializeVoxel(const ProInfo *proinfo, VOXEL **grid_voxel, LevelInfo &plevelinfo, UGRID *GridPT3,
                      NCCresult *nccresult, const int iteration, const double *minmaxHeight);
@@ -25,6 +42,7 @@ namespace {
 struct Driver {
     ProInfo pro;
     LevelInfo L;
+    TransParam param;
     int level, combined, reference_id;
     unsigned char template_size;
     unsigned char num_ia = 2;
@@ -51,6 +69,14 @@ struct Driver {
 }  // namespace
 
 extern "C" {
+
+void drv_set_projection(double lat0, double lon0, double R, double D)
+{
+    g_lat0 = lat0;
+    g_lon0 = lon0;
+    g_R = R;
+    g_D = D;
+}
 
 // swap_ids: put the smx_level's image 0 at SETSM image index 1, so the
 // shim's reference_id mapping is exercised (reference_id = 1).
@@ -134,6 +160,8 @@ void *drv_create(const smx_level *lv, int swap_ids)
     L.grid_resolution = &d->grid_resolution;
     L.Py_combined_level = &d->combined;
     L.check_matching_rate = &d->check_matching_rate;
+    memset(&d->param, 0, sizeof d->param);
+    L.param = &d->param;
     // setsm_code.cpp:3769-3770: calloc'd per level.
     d->ncc.assign((size_t)d->grid_length, NCCresult());
     memset(d->ncc.data(), 0, d->ncc.size() * sizeof(NCCresult));

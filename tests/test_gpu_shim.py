@@ -39,6 +39,8 @@ def _driver():
     d.drv_get.argtypes = [vp, vp, vp, vp]
     d.drv_destroy.restype = None
     d.drv_destroy.argtypes = [vp]
+    d.drv_set_projection.restype = None
+    d.drv_set_projection.argtypes = [C.c_double] * 4
     return d
 
 
@@ -132,5 +134,35 @@ def test_shim_fast(require_gpu, monkeypatch):
         ro = o.step(ug, it, g["minmax"])
         rs = s.step(ug, it, g["minmax"])
         _check(ro, rs, f"shim FAST it{it}", exact=False)
+        ug = ro["ug"]
+    s.close()
+
+
+def test_shim_gncc_fp64(require_gpu, monkeypatch):
+    """GNCC active at level 2 through the SETSM-typed entry points: the shim
+    builds the ortho grid with ps2wgs_single (here the scene's own mapping)
+    and the results match the oracle, which gets the same grid."""
+    import math
+
+    from setsm_amd.records import LevelInputs
+    from setsm_amd.synth import EARTH_R, ortho_wgs
+    monkeypatch.setenv("SMX_PRECISION", "fp64")
+    drv = _driver()
+    sc, pyr = scene(image_px=1024, relief=120.0, seed=5, levels=5)
+    drv.drv_set_projection(sc.spec.lat0, sc.spec.lon0, EARTH_R, EARTH_R * math.cos(math.radians(sc.spec.lat0)))
+    g = level_grid(sc, 2, 40)
+    g["ugrid"]["ortho_ncc"][:, 1] = 900
+    ow = ortho_wgs(sc, g["boundary"], 2)
+    o = OracleRun(pyr, sc.rpcs, 2, g, ortho_wgs=ow)
+    li = LevelInputs(pyr, 2, sc.rpcs, g, grid_resolution=g["grid_res"])
+    s = ShimRun(drv, li, 0)
+    ug = g["ugrid"]
+    for it in (1, 2):
+        if it > 1:
+            ug = perturb_searchspace(ug, 800 + it, 2)
+        ro = o.step(ug, it, g["minmax"])
+        rs = s.step(ug, it, g["minmax"])
+        assert (ro["ncc_vll"]["GNCC"] > -1000).sum() > 100
+        _check(ro, rs, f"shim GNCC it{it}", exact=True)
         ug = ro["ug"]
     s.close()
