@@ -64,7 +64,11 @@ __global__ void __launch_bounds__(64) k_sgm_path(SgmArgs A)
 
     // SGM_start_pos
     long c = (long)row * A.W + col;
-    int nh = A.ncc[c].NumOfHeight;
+    // heights of a cell: NumOfHeight, never past the cell's row of the cube
+    auto row_len = [&](long cc) {
+        return (int)min((long)A.ncc[cc].NumOfHeight, (long)(A.vox_off[cc + 1] - A.vox_off[cc]));
+    };
+    int nh = row_len(c);
     int minH = A.ncc[c].minHeight;
     short gmin = A.gridpt3[c].minHeight, gmax = A.gridpt3[c].maxHeight;
     int64_t off = A.vox_off[c];
@@ -89,7 +93,7 @@ __global__ void __launch_bounds__(64) k_sgm_path(SgmArgs A)
         // (row + v_row[d], col + u_col[d]) in the reference's tables
         // (setsm_code.cpp:7036-7037), is the previous cell.
         c = (long)row * A.W + col;
-        nh = A.ncc[c].NumOfHeight;
+        nh = row_len(c);
         minH = A.ncc[c].minHeight;
         gmin = A.gridpt3[c].minHeight;
         gmax = A.gridpt3[c].maxHeight;

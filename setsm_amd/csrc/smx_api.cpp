@@ -139,6 +139,8 @@ static void free_level_state(smx_job *j)
     jfree(j, j->d_cd);
     jfree(j, j->d_cdn);
     jfree(j, j->d_gval);
+    jfree(j, j->d_peak_rho);
+    j->peak_cap = 0;
     j->cd_bytes = j->cdn_bytes = 0;
     j->tiles_cap = j->tile_runs_cap = 0;
     j->runs_cap = 0;
@@ -588,8 +590,6 @@ static int vll_prepare(smx_job *j, int iteration, int precision, int keep_rho, C
     if (!j || !j->level_set || !j->searchspace_set) return fail(SMX_E_STATE, "level and search space must be set");
     if (precision != SMX_PREC_FP64 && precision != SMX_PREC_FAST) return fail(SMX_E_ARG, "unknown precision %d", precision);
     const smx_level &L = j->lv;
-    if (L.is_ra || L.check_matching_rate)
-        return fail(SMX_E_UNSUPPORTED, "RA / check_matching_rate mode (inline FindPeakNcc) is not implemented");
     // setsm_code.cpp:6100-6112 check_ortho, 6240 the per-cell GNCC gate
     bool check_ortho = !((L.level == 4 && iteration == 1) || L.is_ra);
     if (L.pre_dem && !(L.level == 4 && iteration == 1)) check_ortho = true;
@@ -708,6 +708,16 @@ static int vll_prepare(smx_job *j, int iteration, int precision, int keep_rho, C
     A.runs = j->d_runs;
     A.nruns = nruns;
     HIPCHK(launch_vll_fill_runs(A, j->stream));
+    if (L.is_ra || L.check_matching_rate) {
+        // RA / matching-rate pass: temp_rho per run lane for FindPeakNcc
+        if (nruns > j->peak_cap) {
+            jfree(j, j->d_peak_rho);
+            r = jalloc(j, &j->d_peak_rho, (size_t)std::max(nruns, 1L) * kRun);
+            if (r) return r;
+            j->peak_cap = nruns;
+        }
+        A.peak_rho = j->d_peak_rho;
+    }
     return 0;
 }
 
@@ -732,10 +742,12 @@ long long smx_vertical_line_locus(smx_job *j, int iteration, int precision, int 
     CellArgs A;
     int r = vll_prepare(j, iteration, precision, keep_rho, A);
     if (r) return r;
-    if (precision == SMX_PREC_FP64) {
+    if (precision == SMX_PREC_FP64 || A.peak_rho) {
+        // RA / matching-rate passes (no voxel cube) always take the FP64 kernel
         HIPCHK(hipEventRecord(j->ev0, j->stream));
         HIPCHK(launch_vll_fp64(j->hp.half, j->d_params, A, j->stream));
         HIPCHK(hipEventRecord(j->ev1, j->stream));
+        if (A.peak_rho && j->lv.check_matching_rate) HIPCHK(launch_find_peak(j->d_params, A, j->stream));
     } else {
         r = launch_vll_fast(j, j->hp.half, A, j->ev0, j->ev1);
         if (r) return r;
@@ -756,8 +768,10 @@ int smx_vertical_line_locus_batch(smx_job **jobs, int n, int iteration, int prec
         if (!jobs[i]) return fail(SMX_E_ARG, "null job %d", i);
         if (jobs[i]->device != jobs[0]->device) return fail(SMX_E_ARG, "jobs of one batch share a device");
     }
-    if (precision == SMX_PREC_FP64) {
-        // the plain kernel: one tile per launch
+    bool ra = false;
+    for (int i = 0; i < n; i++) ra = ra || jobs[i]->lv.is_ra || jobs[i]->lv.check_matching_rate;
+    if (precision == SMX_PREC_FP64 || ra) {
+        // the plain kernel: one tile per launch (RA / matching-rate passes included)
         for (int i = 0; i < n; i++) {
             const long long a = smx_vertical_line_locus(jobs[i], iteration, precision, keep_rho);
             if (a < 0) return (int)a;

@@ -49,6 +49,8 @@ struct smx_job {
     double *d_cdn = nullptr;                      // the same at level - 1
     size_t cd_bytes = 0, cdn_bytes = 0;
     double *d_gval = nullptr;                     // per-cell GNCC value (NaN: none)
+    double *d_peak_rho = nullptr;                 // RA / matching-rate pass: temp_rho per run lane
+    long peak_cap = 0;
     smx_ugrid *d_gridpt3 = nullptr;
     smx_nccresult *d_ncc = nullptr;
     int64_t *d_vox_off = nullptr;
@@ -102,6 +104,7 @@ hipError_t launch_pack(const uint16_t *img, const uint16_t *mag, uint32_t *texel
 hipError_t launch_unpack(const uint32_t *texel, uint16_t *img, uint16_t *mag, long n, hipStream_t s);
 hipError_t launch_dir_probe(const int *gx, const int *gy, int16_t *out, long n, hipStream_t s);
 hipError_t launch_vll_gate(const VllParams *dP, const CellArgs &A, hipStream_t s);
+hipError_t launch_find_peak(const VllParams *dP, const CellArgs &A, hipStream_t s);
 hipError_t launch_ortho_coords(const VllParams *dP, const OrthoArgs &O, hipStream_t s);
 hipError_t launch_gncc(int half, const VllParams *dP, const OrthoArgs &O, const CellArgs &A, double *gval,
                        hipStream_t s);

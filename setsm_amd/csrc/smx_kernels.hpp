@@ -7,6 +7,9 @@
 namespace smx {
 
 constexpr int kRun = 64;   // heights per run = one wavefront of hypotheses
+// RA / matching-rate passes: a NaN payload marking "no NCC at this height"
+// (FindPeakNcc is not called there), distinct from an evaluated NaN.
+constexpr unsigned long long kNotEvaluated = 0x7ff80000000bad01ull;
 
 struct CellArgs {
     long ncells;
@@ -26,6 +29,8 @@ struct CellArgs {
     uint8_t *acc_flag;             // per cell: passed the gate (written when non-null)
     const double *gval;            // per cell: sum_GNCC_multi / count_GNCC, NaN if no GNCC term; null: none
     double ncc_alpha;              // SetNCC_alpha of the call (GNCC blend)
+    double *peak_rho;              // RA / matching-rate pass: temp_rho per run lane (NaN: not evaluated);
+                                   // the cube is not touched
 };
 
 // setsm_code.cpp:6481-6490: the value max_WNCC takes for one hypothesis —
@@ -145,7 +150,10 @@ __device__ __forceinline__ bool project_hypothesis(const VllParams &P, const Cel
 {
 #pragma clang fp contract(off)
     if (hidx >= (int)nr.NumOfHeight) return false;
-    const float iter_height = (float)(nr.minHeight + hidx * P.height_step);
+    // setsm_code.cpp:6325-6328: RA / matching-rate passes count heights from
+    // the search-space bottom, the voxel passes from NCCresult.minHeight
+    const int base = (P.is_ra || P.check_matching_rate) ? start_H : (int)nr.minHeight;
+    const float iter_height = (float)(base + hidx * P.height_step);
     if (!(iter_height >= start_H && iter_height <= end_H)) return false;
     hy.height = iter_height;
     const double lon = A.grid_wgs[2 * cell], lat = A.grid_wgs[2 * cell + 1];

@@ -56,7 +56,13 @@ __device__ __forceinline__ bool gate_cell(const VllParams *__restrict__ P, CellA
         if (!blunder) {
             accessible = true;
             nr->GNCC = q_result(-1.0);   // k_gncc overwrites it where the ortho term applies
-            if (nr->check_height_change || P->dem_lt2) {
+            const bool ra = P->is_ra || P->check_matching_rate;
+            if (ra) {
+                // setsm_code.cpp:6308-6313
+                nr->check_height_change = 1;
+                nr->NumOfHeight = (uint16_t)(int)((end_H - start_H) / P->height_step);
+            }
+            if (ra || nr->check_height_change || P->dem_lt2) {
                 nr->result0 = q_result(-1.0);
                 nr->result2 = -1000;
                 nr->result1 = q_result(-1.0);
@@ -89,6 +95,65 @@ __global__ void __launch_bounds__(256) k_vll_finalize(CellArgs A)
     if (c >= A.ncells) return;
     if (A.run_count[c] > 0 && A.ncc[c].check_height_change)
         A.ncc[c].max_WNCC = q_result(order_value(A.max_bits[c]));
+}
+
+// setsm_code.cpp:6672-6752 FindPeakNcc over one cell's heights in order (the
+// matching-rate pass calls it inline per evaluated hypothesis); one thread
+// per cell over the temp_rho values the NCC kernel kept.
+__global__ void __launch_bounds__(256) k_find_peak(const VllParams *__restrict__ P, CellArgs A)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= A.ncells || A.run_count[c] == 0) return;
+    smx_nccresult nr = A.ncc[c];
+    const int start_H = A.gridpt3[c].minHeight;
+    const double *rr = A.peak_rho + A.run_offset[c] * kRun;
+    double pre_rho = -1.0;
+    float pre_height = 0.0f;
+    int direction = 0;
+    bool check_rho = false;
+    for (int h = 0; h < (int)nr.NumOfHeight; h++) {
+        const double temp_rho = rr[h];
+        if ((unsigned long long)__double_as_longlong(temp_rho) == kNotEvaluated) continue;
+        const float iter_height = (float)(start_H + h * P->height_step);
+        double diff_rho;
+        int t_direction;
+        if (temp_rho >= 0 && pre_rho >= 0) diff_rho = temp_rho - pre_rho;
+        else if (temp_rho < 0 && pre_rho < 0) diff_rho = -(fabs(temp_rho) - fabs(pre_rho));
+        else if (temp_rho >= 0 && pre_rho < 0) diff_rho = 1;
+        else diff_rho = -1;
+        if (diff_rho > 0) t_direction = 1;
+        else if (diff_rho < 0) t_direction = -1;
+        else t_direction = 0;
+        if (!check_rho) {
+            check_rho = true;
+            t_direction = -1;
+        } else if (pre_rho != -1) {
+            if (direction > 0 && t_direction < 0) {
+                if (nr.result4 < 10) nr.result4 += 1;
+                if (nr.result0 < q_result(pre_rho)) {
+                    const double temp_1 = dq(nr.result0);
+                    nr.result0 = q_result(pre_rho);
+                    const float temp_2 = nr.result2;
+                    nr.result2 = pre_height;
+                    if (nr.result1 < q_result(temp_1)) {
+                        nr.result1 = q_result(temp_1);
+                        nr.result3 = temp_2;
+                    }
+                } else if (nr.result1 < q_result(pre_rho)) {
+                    nr.result1 = q_result(pre_rho);
+                    nr.result3 = pre_height;
+                }
+            }
+        }
+        pre_rho = temp_rho;
+        pre_height = iter_height;
+        direction = t_direction;
+    }
+    A.ncc[c].result0 = nr.result0;
+    A.ncc[c].result1 = nr.result1;
+    A.ncc[c].result2 = nr.result2;
+    A.ncc[c].result3 = nr.result3;
+    A.ncc[c].result4 = nr.result4;
 }
 
 // ------------------------------------------------------ FP64 plain kernel
@@ -248,6 +313,30 @@ __global__ void __launch_bounds__(256) k_vll_fp64(const VllParams *__restrict__ 
     bool act = project_hypothesis(*P, A, cell, hidx, nr, start_H, end_H, hy);
     double temp_rho = -100.0;
     unsigned evaluated = 0;
+    if (A.peak_rho) {
+        // RA / matching-rate pass (setsm_code.cpp:6468-6496): no voxel cube;
+        // temp_rho kept for the per-cell FindPeakNcc scan
+        double pr = __longlong_as_double(kNotEvaluated);
+        if (act && hy.orient_ok) {
+            double sum_multi = 0, count_multi = 0;
+            multi_ncc_fp64<H>(P->view[0], P->view[1], hy.rpx, hy.rpy, hy.tpx, hy.tpy, hy.cos0, hy.sin0, sum_multi,
+                              count_multi);
+            if (P->combined_incc)
+                multi_ncc_fp64<H>(P->next[0], P->next[1], hy.rnx, hy.rny, hy.tnx, hy.tny, hy.cos0, hy.sin0,
+                                  sum_multi, count_multi);
+            const double rho = count_multi > 0 ? sum_multi / count_multi : -1.0;
+            // the value FindPeakNcc receives (INCC/GNCC blend where the term
+            // exists; an evaluated NaN stays NaN), and max_WNCC's
+            pr = A.gval && A.gval[cell] == A.gval[cell]
+                     ? sum_multi / count_multi * A.ncc_alpha + A.gval[cell] * (1.0 - A.ncc_alpha)
+                     : rho;
+            temp_rho = wncc_of(A, cell, sum_multi, count_multi, rho);
+            evaluated = 1;
+        }
+        A.peak_rho[wave * kRun + lane] = pr;
+        wave_max_commit(temp_rho, evaluated, &A.max_bits[cell], A.evaluated);
+        return;
+    }
     if (act) {
         const long wi = A.vox_off[cell] + hidx;
         uint32_t word = A.vox[wi];
@@ -579,8 +668,12 @@ __global__ void __launch_bounds__(256) k_awncc(const VllParams *__restrict__ P, 
     const bool max_mode = (iteration <= 3 && Pyramid_step == 4) || (iteration <= 2 && Pyramid_step == 3) ||
                           (iteration <= 1 && Pyramid_step == 2);
     const long base = A.vox_off[c];
+    // an RA pass may leave NumOfHeight above the row InitializeVoxel laid
+    // out; never read past the row (the reference would read past its heap
+    // row there)
+    const long nh = min((long)nr.NumOfHeight, (long)(A.vox_off[c + 1] - base));
     const double gncc = dq(nr.GNCC);
-    for (long hs = 0; hs < nr.NumOfHeight; hs++) {
+    for (long hs = 0; hs < nh; hs++) {
         const float iter_height = (float)(nr.minHeight + hs * P->height_step);
         if (!(iter_height >= g.minHeight && iter_height <= g.maxHeight)) continue;
         const double incc = dq(SMX_VOXEL_INCC(A.vox[base + hs]));
@@ -669,6 +762,12 @@ hipError_t launch_gncc(int half, const VllParams *dP, const OrthoArgs &O, const 
     case 7: hipLaunchKernelGGL(k_gncc<7>, g, b, 0, s, dP, O, A, gval); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_find_peak(const VllParams *dP, const CellArgs &A, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_find_peak, dim3(nblk(A.ncells)), dim3(256), 0, s, dP, A);
     return hipGetLastError();
 }
 

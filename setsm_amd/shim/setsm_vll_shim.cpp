@@ -19,8 +19,8 @@
 // buffers land at the same addresses is not taken for the same one.
 //
 // Errors follow SETSM's convention (setsm_code.cpp:6571-6575): a message on
-// stdout and exit(1). Inputs that need a mode libsmx lacks (RA /
-// check_matching_rate, frame sensors, more than two images) are such errors.
+// stdout and exit(1). Inputs that need a mode libsmx lacks (frame sensors,
+// more than two images, affine image adjustment) are such errors.
 // The GNCC term's ortho grid is computed here with SETSM's own
 // ps2wgs_single (CoordConversion.cpp), as SetOrthoImageCoord does.
 //
@@ -299,7 +299,9 @@ int VerticalLineLocus(VOXEL **grid_voxel, const ProInfo *proinfo, NCCresult *ncc
     const long long acc = smx_vertical_line_locus(s.job, iteration, precision_from_env(), 0);
     if (acc < 0) die("smx_vertical_line_locus", (int)acc);
     check(smx_job_get_nccresult(s.job, (smx_nccresult *)nccresult), "smx_job_get_nccresult");
-    download_rows(s, grid_voxel, nccresult, n);
+    // RA and matching-rate passes leave grid_voxel alone (setsm_code.cpp:6470)
+    const bool cmr = plevelinfo.check_matching_rate && *plevelinfo.check_matching_rate;
+    if (!proinfo->IsRA && !cmr) download_rows(s, grid_voxel, nccresult, n);
     return (int)acc;
 }
 
