@@ -145,6 +145,20 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic():
+    """HBM bytes per FAST launch at this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated); the
+    counters cannot be read from inside the timed run."""
+    path = os.path.join(ROOT, "profiles", "vll_c1_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -313,7 +327,8 @@ def main():
     if best_prec == "fast":
         achieved = best["evaluated"] * fph / kernel_s / 1e12
         roof = dict(bound="valu", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / PEAK_FP32_TFLOPS, traffic=None)
+                    frac=achieved / PEAK_FP32_TFLOPS,
+                    traffic=measured_traffic() if (args.image_px, cells) == (16384, 4000) else None)
     else:
         achieved = best["evaluated"] * fph / kernel_s / 1e12
         roof = dict(bound="valu", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
