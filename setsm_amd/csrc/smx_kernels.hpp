@@ -121,6 +121,16 @@ __host__ __device__ constexpr int ring_count_ce(int half, int ring)
     return n;
 }
 
+// Samples of window w (0 = outer disc, 1 = middle, 2 = inner).
+__host__ __device__ constexpr int window_count_ce(int half, int w)
+{
+    int n = 0;
+    for (int r = -half; r <= half; r++)
+        for (int c = -half; c <= half; c++)
+            if (window_bits_ce(half, r, c) & (1 << w)) n++;
+    return n;
+}
+
 __device__ __forceinline__ unsigned long long order_key(double v)
 {
     const unsigned long long b = (unsigned long long)__double_as_longlong(v);

@@ -92,6 +92,18 @@ struct LdsImg {
     int base;   // index of texel (0, 0) of the window origin
     int pitch;
     __device__ __forceinline__ float2 at(int row, int col) const { return smx_lds[base + row * pitch + col]; }
+    // The 2x2 taps at (row, col) .. (row + 1, col + 1) for integer-valued
+    // float coordinates: the offset is formed with one exact FP32 fma
+    // (offsets stay far below 2^24).
+    __device__ __forceinline__ void quad(float row, float col, float2 &t00, float2 &t01, float2 &t10,
+                                         float2 &t11) const
+    {
+        const int o = base + (int)fmaf(row, (float)pitch, col);
+        t00 = smx_lds[o];
+        t01 = smx_lds[o + 1];
+        t10 = smx_lds[o + pitch];
+        t11 = smx_lds[o + pitch + 1];
+    }
 };
 
 // Texels read from HBM (packed u16 pair), shifted by a pivot on the fly.
@@ -108,6 +120,15 @@ struct GlobalImg {
         const int y = min(max(y0 + row, 0), height - 1);
         const uint32_t t = texel[(long)y * width + x];
         return make_float2((float)(t & 0xffffu) - piv_i, (float)(t >> 16) - piv_m);
+    }
+    __device__ __forceinline__ void quad(float row, float col, float2 &t00, float2 &t01, float2 &t10,
+                                         float2 &t11) const
+    {
+        const int r = (int)row, c = (int)col;
+        t00 = at(r, c);
+        t01 = at(r, c + 1);
+        t10 = at(r + 1, c);
+        t11 = at(r + 1, c + 1);
     }
 };
 
@@ -156,12 +177,106 @@ __device__ __forceinline__ double corr_moments(double n, double a, double b, dou
     return -99.0;
 }
 
+// Half-width of window w (0 outer disc, 1 middle, 2 inner) on row `row`:
+// the largest |col| inside it, -1 if the row misses the window. Membership of
+// every window is symmetric and contiguous in col on a row.
+__host__ __device__ constexpr int half_width_ce(int H, int row, int w)
+{
+    int hw = -1;
+    for (int c = 0; c <= H; c++)
+        if (window_bits_ce(H, row, c) & (1 << w)) hw = c;
+    return hw;
+}
+
+template <int H>
+struct RowWidths {
+    int w[2 * H + 1][3];
+    constexpr RowWidths() : w()
+    {
+        for (int r = -H; r <= H; r++)
+            for (int k = 0; k < 3; k++) w[r + H][k] = half_width_ce(H, r, k);
+    }
+};
+
+// Distinct (outer, middle, inner) half-width triples over the rows of a
+// window, and the triple of each row: rows sharing a triple share code.
+template <int H>
+struct RowSigs {
+    int n;
+    int w[2 * H + 1][3];
+    int of_row[2 * H + 1];
+    constexpr RowSigs() : n(0), w(), of_row()
+    {
+        for (int r = -H; r <= H; r++) {
+            const int a = half_width_ce(H, r, 0), b = half_width_ce(H, r, 1), c = half_width_ce(H, r, 2);
+            int k = 0;
+            while (k < n && !(w[k][0] == a && w[k][1] == b && w[k][2] == c)) k++;
+            if (k == n) {
+                w[n][0] = a;
+                w[n][1] = b;
+                w[n][2] = c;
+                n++;
+            }
+            of_row[r + H] = k;
+        }
+    }
+};
+
+struct Mom {
+    float a, b, aa, bb, ab;   // intensity pair
+    float c, d, cc, dd, cd;   // magnitude pair
+};
+
+__device__ __forceinline__ void mom_add(Mom &s, float li, float ri, float lm, float rm)
+{
+    s.a += li;
+    s.b += ri;
+    s.aa = fmaf(li, li, s.aa);
+    s.bb = fmaf(ri, ri, s.bb);
+    s.ab = fmaf(li, ri, s.ab);
+    s.c += lm;
+    s.d += rm;
+    s.cc = fmaf(lm, lm, s.cc);
+    s.dd = fmaf(rm, rm, s.dd);
+    s.cd = fmaf(lm, rm, s.cd);
+}
+
+template <bool kKahan>
+__device__ __forceinline__ void mom_fold(Mom &t, float (&comp)[6], float &tn, const Mom &r, float rn)
+{
+    t.a += r.a;
+    t.b += r.b;
+    t.c += r.c;
+    t.d += r.d;
+    tn += rn;
+    if (kKahan) {
+        kahan(t.aa, comp[0], r.aa);
+        kahan(t.bb, comp[1], r.bb);
+        kahan(t.ab, comp[2], r.ab);
+        kahan(t.cc, comp[3], r.cc);
+        kahan(t.dd, comp[4], r.dd);
+        kahan(t.cd, comp[5], r.cd);
+    } else {
+        t.aa += r.aa;
+        t.bb += r.bb;
+        t.ab += r.ab;
+        t.cc += r.cc;
+        t.dd += r.dd;
+        t.cd += r.cd;
+    }
+}
+
 // One hypothesis' window loop. L/R are samplers positioned so that at(0,0)
-// is the texel (center - (H+2)) in both axes. (lx, ly) are the integer parts
-// of the left centre relative to that origin plus H+2, (fxl, fyl) its
-// fractional part; same for the right centre. Check = per-sample image-edge
-// and nodata tests (SubFunctions.cpp:3252-3257); otherwise every sample is
-// known to be valid. Returns false if a window count is zero.
+// is the texel (center - (H+2)) in both axes. (fxl, fyl) is the fractional
+// part of the left centre, (ftx, fty) of the right one. Check = per-sample
+// image-edge and nodata tests (SubFunctions.cpp:3252-3257); otherwise every
+// sample is known to be valid. Returns false if a window count is zero.
+//
+// Each row is walked from its centre column outwards into one running sum;
+// the sum is folded into the inner / middle / outer window totals when the
+// walk passes that window's half-width on the row (the windows are nested
+// and each is a symmetric column range on a row), so every sample updates
+// one fixed set of registers.
 template <int H, bool kCheck, bool kKahan, class SL, class SR>
 __device__ __forceinline__ bool fast_window(const SL &L, const SR &R, float fxl, float fyl, float ftx, float fty,
                                          float cos0, float sin0, float piv_l, float piv_r, float lim_lx0,
@@ -169,13 +284,13 @@ __device__ __forceinline__ bool fast_window(const SL &L, const SR &R, float fxl,
                                          float lim_rx1, float lim_ry0, float lim_ry1, double &rho)
 {
     constexpr int O = H + 2;   // origin offset
-    RingAcc tot[3];
+    constexpr RowWidths<H> kW{};
+    Mom tot[3];
     float comp[3][6];
-    float cnt[3] = {0.f, 0.f, 0.f};
+    float tn[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        tot[k].i = Acc5{0, 0, 0, 0, 0};
-        tot[k].m = Acc5{0, 0, 0, 0, 0};
+        tot[k] = Mom{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int q = 0; q < 6; q++) comp[k][q] = 0.f;
     }
@@ -184,35 +299,25 @@ __device__ __forceinline__ bool fast_window(const SL &L, const SR &R, float fxl,
     // pivot does not lose its variance to cancellation.
     const float2 pvl = bilerp(L.at(O, O), L.at(O, O + 1), L.at(O + 1, O), L.at(O + 1, O + 1), fxl, fyl);
     const float2 pvr = bilerp(R.at(O, O), R.at(O, O + 1), R.at(O + 1, O), R.at(O + 1, O + 1), ftx, fty);
-    // Rows run as a loop (the row index is wave-uniform, so the window
-    // membership tests below are scalar); columns are unrolled so left taps
-    // are LDS reads at immediate offsets.
 #pragma unroll 1
     for (int row = -H; row <= H; row++) {
-        RingAcc part[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            part[k].i = Acc5{0, 0, 0, 0, 0};
-            part[k].m = Acc5{0, 0, 0, 0, 0};
-        }
+        const int w0 = kW.w[row + H][0], w1 = kW.w[row + H][1], w2 = kW.w[row + H][2];
         const float frow = (float)row;
         // x = ftx + col*cos - row*sin, y = fty + col*sin + row*cos
         const float xr = fmaf(-frow, sin0, ftx), yr = fmaf(frow, cos0, fty);
-#pragma unroll
-        for (int col = -H; col <= H; col++) {
-            const int bits = window_bits_ce(H, row, col);
-            if (!bits) continue;
-            const int ring = (bits & 4) ? 2 : ((bits & 2) ? 1 : 0);
-            // left: integer offsets, shared fractional weights
+        Mom run = Mom{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        float rn = 0.f;
+        auto sample = [&](const int col) {
             const float2 l = bilerp(L.at(O + row, O + col), L.at(O + row, O + col + 1), L.at(O + row + 1, O + col),
                                     L.at(O + row + 1, O + col + 1), fxl, fyl);
             // right: rotated offsets (setsm_code.cpp:6437-6439)
             const float x = fmaf((float)col, cos0, xr);
             const float y = fmaf((float)col, sin0, yr);
             const float flx = floorf(x), fly = floorf(y);
-            const int ix = (int)flx + O, iy = (int)fly + O;
             const float wx = x - flx, wy = y - fly;
-            const float2 r = bilerp(R.at(iy, ix), R.at(iy, ix + 1), R.at(iy + 1, ix), R.at(iy + 1, ix + 1), wx, wy);
+            float2 r00, r01, r10, r11;
+            R.quad(fly + (float)O, flx + (float)O, r00, r01, r10, r11);
+            const float2 r = bilerp(r00, r01, r10, r11, wx, wy);
             bool ok = true;
             if (kCheck) {
                 // pos >= 0 && pos + 1 < size on both images, then L > 0 && R > 0
@@ -220,68 +325,63 @@ __device__ __forceinline__ bool fast_window(const SL &L, const SR &R, float fxl,
                      x >= lim_rx0 && x < lim_rx1 && y >= lim_ry0 && y < lim_ry1 && l.x > -piv_l && r.x > -piv_r;
             }
             if (ok) {
-                const float li = l.x - pvl.x, lm = l.y - pvl.y, ri = r.x - pvr.x, rm = r.y - pvr.y;
-                if (ring == 0) {
-                    acc_add(part[0].i, li, ri);
-                    acc_add(part[0].m, lm, rm);
-                } else if (ring == 1) {
-                    acc_add(part[1].i, li, ri);
-                    acc_add(part[1].m, lm, rm);
-                } else {
-                    acc_add(part[2].i, li, ri);
-                    acc_add(part[2].m, lm, rm);
-                }
-                if (kCheck) cnt[ring] += 1.f;
+                mom_add(run, l.x - pvl.x, r.x - pvr.x, l.y - pvl.y, r.y - pvr.y);
+                if (kCheck) rn += 1.f;
             }
-        }
+        };
+        if constexpr (!kCheck) {
+            // every sample valid: straight-line code per half-width triple
+            auto walk = [&](auto W0c, auto W1c, auto W2c) {
+                constexpr int W0 = decltype(W0c)::value, W1 = decltype(W1c)::value, W2 = decltype(W2c)::value;
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            tot[k].i.a += part[k].i.a;
-            tot[k].i.b += part[k].i.b;
-            tot[k].m.a += part[k].m.a;
-            tot[k].m.b += part[k].m.b;
-            if (kKahan) {
-                kahan(tot[k].i.aa, comp[k][0], part[k].i.aa);
-                kahan(tot[k].i.bb, comp[k][1], part[k].i.bb);
-                kahan(tot[k].i.ab, comp[k][2], part[k].i.ab);
-                kahan(tot[k].m.aa, comp[k][3], part[k].m.aa);
-                kahan(tot[k].m.bb, comp[k][4], part[k].m.bb);
-                kahan(tot[k].m.ab, comp[k][5], part[k].m.ab);
-            } else {
-                tot[k].i.aa += part[k].i.aa;
-                tot[k].i.bb += part[k].i.bb;
-                tot[k].i.ab += part[k].i.ab;
-                tot[k].m.aa += part[k].m.aa;
-                tot[k].m.bb += part[k].m.bb;
-                tot[k].m.ab += part[k].m.ab;
+                for (int k = 0; k <= W0; k++) {
+                    sample(k);
+                    if (k > 0) sample(-k);
+                    if (k == W2) mom_fold<kKahan>(tot[2], comp[2], tn[2], run, rn);
+                    if (k == W1) mom_fold<kKahan>(tot[1], comp[1], tn[1], run, rn);
+                }
+                if (W0 >= 0) mom_fold<kKahan>(tot[0], comp[0], tn[0], run, rn);
+            };
+            constexpr RowSigs<H> kS{};
+#define SMX_SIG(I)                                                                                                  \
+    case I:                                                                                                         \
+        if constexpr (I < kS.n)                                                                                     \
+            walk(std::integral_constant<int, kS.w[I][0]>{}, std::integral_constant<int, kS.w[I][1]>{},             \
+                 std::integral_constant<int, kS.w[I][2]>{});                                                        \
+        break;
+            switch (kS.of_row[row + H]) {
+                SMX_SIG(0) SMX_SIG(1) SMX_SIG(2) SMX_SIG(3) SMX_SIG(4) SMX_SIG(5)
+            default: break;
             }
+#undef SMX_SIG
+        } else {
+#pragma unroll
+            for (int k = 0; k <= H; k++) {
+                if (k <= w0) {
+                    sample(k);
+                    if (k > 0) sample(-k);
+                }
+                if (k == w2) mom_fold<kKahan>(tot[2], comp[2], tn[2], run, rn);
+                if (k == w1) mom_fold<kKahan>(tot[1], comp[1], tn[1], run, rn);
+            }
+            if (w0 >= 0) mom_fold<kKahan>(tot[0], comp[0], tn[0], run, rn);
         }
     }
-    // window totals: inner = ring 2, middle = ring 1 + 2, outer = all rings
+    // window totals (Kahan compensation folded back in FP64)
     double n[3], S[3][10];
 #pragma unroll
     for (int w = 0; w < 3; w++) {
-        double s[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        double nn = 0;
-#pragma unroll
-        for (int k = w; k < 3; k++) {
-            // ring k belongs to windows 0..k
-            s[0] += tot[k].i.a;
-            s[1] += tot[k].i.b;
-            s[2] += (double)tot[k].i.aa - (double)comp[k][0];
-            s[3] += (double)tot[k].i.bb - (double)comp[k][1];
-            s[4] += (double)tot[k].i.ab - (double)comp[k][2];
-            s[5] += tot[k].m.a;
-            s[6] += tot[k].m.b;
-            s[7] += (double)tot[k].m.aa - (double)comp[k][3];
-            s[8] += (double)tot[k].m.bb - (double)comp[k][4];
-            s[9] += (double)tot[k].m.ab - (double)comp[k][5];
-            nn += kCheck ? (double)cnt[k] : (double)ring_count_ce(H, k);
-        }
-        // windows are nested: window w = rings w..2
-        n[w] = nn;
-#pragma unroll
-        for (int q = 0; q < 10; q++) S[w][q] = s[q];
+        n[w] = kCheck ? (double)tn[w] : (double)window_count_ce(H, w);
+        S[w][0] = tot[w].a;
+        S[w][1] = tot[w].b;
+        S[w][2] = (double)tot[w].aa - (double)comp[w][0];
+        S[w][3] = (double)tot[w].bb - (double)comp[w][1];
+        S[w][4] = (double)tot[w].ab - (double)comp[w][2];
+        S[w][5] = tot[w].c;
+        S[w][6] = tot[w].d;
+        S[w][7] = (double)tot[w].cc - (double)comp[w][3];
+        S[w][8] = (double)tot[w].dd - (double)comp[w][4];
+        S[w][9] = (double)tot[w].cd - (double)comp[w][5];
     }
     if (!(n[0] > 0 && n[1] > 0 && n[2] > 0)) return false;
     double temp_roh = 0, count_roh = 0;
