@@ -25,6 +25,8 @@
 #include <cstdio>
 #include <vector>
 
+#include <type_traits>
+
 #include "smx_internal.hpp"
 
 namespace smx {
@@ -168,11 +170,11 @@ __device__ __forceinline__ float2 bilerp(float2 t00, float2 t01, float2 t10, flo
 
 // Correlation of one window / channel from pivot-shifted moments (FP64
 // finish). Returns -99 for an undefined correlation, like Correlate.
-__device__ __forceinline__ double corr_moments(double n, double a, double b, double aa, double bb, double ab)
+__device__ __forceinline__ double corr_moments(double inv_n, double a, double b, double aa, double bb, double ab)
 {
-    const double va = aa - a * a / n;
-    const double vb = bb - b * b / n;
-    const double cv = ab - a * b / n;
+    const double va = aa - a * a * inv_n;
+    const double vb = bb - b * b * inv_n;
+    const double cv = ab - a * b * inv_n;
     if (va > 1e-8 && vb > 1e-8) return cv / sqrt(va * vb);
     return -99.0;
 }
@@ -241,29 +243,38 @@ __device__ __forceinline__ void mom_add(Mom &s, float li, float ri, float lm, fl
     s.cd = fmaf(lm, rm, s.cd);
 }
 
+// Window totals: sums of the row partials; the squared / product terms in
+// FP64 when compensated (the row partials themselves are short FP32 sums),
+// FP32 otherwise.
 template <bool kKahan>
-__device__ __forceinline__ void mom_fold(Mom &t, float (&comp)[6], float &tn, const Mom &r, float rn)
+struct Tot {
+    using Q = typename std::conditional<kKahan, double, float>::type;
+    float a, b, c, d, n;
+    Q aa, bb, ab, cc, dd, cd;
+};
+
+template <bool kKahan>
+__device__ __forceinline__ void tot_zero(Tot<kKahan> &t)
 {
+    t.a = t.b = t.c = t.d = t.n = 0.f;
+    t.aa = t.bb = t.ab = t.cc = t.dd = t.cd = 0;
+}
+
+template <bool kKahan>
+__device__ __forceinline__ void mom_fold(Tot<kKahan> &t, const Mom &r, float rn)
+{
+    using Q = typename Tot<kKahan>::Q;
     t.a += r.a;
     t.b += r.b;
     t.c += r.c;
     t.d += r.d;
-    tn += rn;
-    if (kKahan) {
-        kahan(t.aa, comp[0], r.aa);
-        kahan(t.bb, comp[1], r.bb);
-        kahan(t.ab, comp[2], r.ab);
-        kahan(t.cc, comp[3], r.cc);
-        kahan(t.dd, comp[4], r.dd);
-        kahan(t.cd, comp[5], r.cd);
-    } else {
-        t.aa += r.aa;
-        t.bb += r.bb;
-        t.ab += r.ab;
-        t.cc += r.cc;
-        t.dd += r.dd;
-        t.cd += r.cd;
-    }
+    t.n += rn;
+    t.aa += (Q)r.aa;
+    t.bb += (Q)r.bb;
+    t.ab += (Q)r.ab;
+    t.cc += (Q)r.cc;
+    t.dd += (Q)r.dd;
+    t.cd += (Q)r.cd;
 }
 
 // One hypothesis' window loop. L/R are samplers positioned so that at(0,0)
@@ -285,20 +296,20 @@ __device__ __forceinline__ bool fast_window(const SL &L, const SR &R, float fxl,
 {
     constexpr int O = H + 2;   // origin offset
     constexpr RowWidths<H> kW{};
-    Mom tot[3];
-    float comp[3][6];
-    float tn[3] = {0.f, 0.f, 0.f};
+    Tot<kKahan> tot[3];
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-        tot[k] = Mom{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-        for (int q = 0; q < 6; q++) comp[k][q] = 0.f;
-    }
+    for (int k = 0; k < 3; k++) tot_zero(tot[k]);
     // Per-hypothesis pivots: the window-centre samples. Moments are taken on
     // sample - pivot, so a low-contrast patch far from the tile's staging
     // pivot does not lose its variance to cancellation.
     const float2 pvl = bilerp(L.at(O, O), L.at(O, O + 1), L.at(O + 1, O), L.at(O + 1, O + 1), fxl, fyl);
     const float2 pvr = bilerp(R.at(O, O), R.at(O, O + 1), R.at(O + 1, O), R.at(O + 1, O + 1), ftx, fty);
+    // the left patch shares the centre's fractional offset: one set of weights
+    const float lw11 = fxl * fyl, lw10 = fyl - lw11, lw01 = fxl - lw11, lw00 = (1.f - fxl) - lw10;
+    // L > 0 / R > 0 in pivot-shifted units
+    const float thr_l = -piv_l - pvl.x, thr_r = -piv_r - pvr.x;
+    (void)thr_l;
+    (void)thr_r;
 #pragma unroll 1
     for (int row = -H; row <= H; row++) {
         const int w0 = kW.w[row + H][0], w1 = kW.w[row + H][1], w2 = kW.w[row + H][2];
@@ -307,9 +318,13 @@ __device__ __forceinline__ bool fast_window(const SL &L, const SR &R, float fxl,
         const float xr = fmaf(-frow, sin0, ftx), yr = fmaf(frow, cos0, fty);
         Mom run = Mom{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         float rn = 0.f;
-        auto sample = [&](const int col) {
-            const float2 l = bilerp(L.at(O + row, O + col), L.at(O + row, O + col + 1), L.at(O + row + 1, O + col),
-                                    L.at(O + row + 1, O + col + 1), fxl, fyl);
+        auto sample = [&](const int col, const bool first) {
+            // left: integer offsets, the hypothesis' bilinear weights; the
+            // pivot enters as the first term of the weighted sum
+            const float2 a = L.at(O + row, O + col), b = L.at(O + row, O + col + 1);
+            const float2 c = L.at(O + row + 1, O + col), d = L.at(O + row + 1, O + col + 1);
+            const float li = fmaf(lw11, d.x, fmaf(lw10, c.x, fmaf(lw01, b.x, fmaf(lw00, a.x, -pvl.x))));
+            const float lm = fmaf(lw11, d.y, fmaf(lw10, c.y, fmaf(lw01, b.y, fmaf(lw00, a.y, -pvl.y))));
             // right: rotated offsets (setsm_code.cpp:6437-6439)
             const float x = fmaf((float)col, cos0, xr);
             const float y = fmaf((float)col, sin0, yr);
@@ -317,15 +332,19 @@ __device__ __forceinline__ bool fast_window(const SL &L, const SR &R, float fxl,
             const float wx = x - flx, wy = y - fly;
             float2 r00, r01, r10, r11;
             R.quad(fly + (float)O, flx + (float)O, r00, r01, r10, r11);
-            const float2 r = bilerp(r00, r01, r10, r11, wx, wy);
+            const float w11 = wx * wy, w10 = wy - w11, w01 = wx - w11, w00 = (1.f - wx) - w10;
+            const float ri = fmaf(w11, r11.x, fmaf(w10, r10.x, fmaf(w01, r01.x, fmaf(w00, r00.x, -pvr.x))));
+            const float rm = fmaf(w11, r11.y, fmaf(w10, r10.y, fmaf(w01, r01.y, fmaf(w00, r00.y, -pvr.y))));
             bool ok = true;
             if (kCheck) {
                 // pos >= 0 && pos + 1 < size on both images, then L > 0 && R > 0
                 ok = (float)col >= lim_lx0 && (float)col < lim_lx1 && frow >= lim_ly0 && frow < lim_ly1 &&
-                     x >= lim_rx0 && x < lim_rx1 && y >= lim_ry0 && y < lim_ry1 && l.x > -piv_l && r.x > -piv_r;
+                     x >= lim_rx0 && x < lim_rx1 && y >= lim_ry0 && y < lim_ry1 && li > thr_l && ri > thr_r;
             }
-            if (ok) {
-                mom_add(run, l.x - pvl.x, r.x - pvr.x, l.y - pvl.y, r.y - pvr.y);
+            if (!kCheck && first) {
+                run = Mom{li, ri, li * li, ri * ri, li * ri, lm, rm, lm * lm, rm * rm, lm * rm};
+            } else if (ok) {
+                mom_add(run, li, ri, lm, rm);
                 if (kCheck) rn += 1.f;
             }
         };
@@ -335,12 +354,12 @@ __device__ __forceinline__ bool fast_window(const SL &L, const SR &R, float fxl,
                 constexpr int W0 = decltype(W0c)::value, W1 = decltype(W1c)::value, W2 = decltype(W2c)::value;
 #pragma unroll
                 for (int k = 0; k <= W0; k++) {
-                    sample(k);
-                    if (k > 0) sample(-k);
-                    if (k == W2) mom_fold<kKahan>(tot[2], comp[2], tn[2], run, rn);
-                    if (k == W1) mom_fold<kKahan>(tot[1], comp[1], tn[1], run, rn);
+                    sample(k, k == 0);
+                    if (k > 0) sample(-k, false);
+                    if (k == W2) mom_fold(tot[2], run, rn);
+                    if (k == W1) mom_fold(tot[1], run, rn);
                 }
-                if (W0 >= 0) mom_fold<kKahan>(tot[0], comp[0], tn[0], run, rn);
+                if (W0 >= 0) mom_fold(tot[0], run, rn);
             };
             constexpr RowSigs<H> kS{};
 #define SMX_SIG(I)                                                                                                  \
@@ -358,38 +377,39 @@ __device__ __forceinline__ bool fast_window(const SL &L, const SR &R, float fxl,
 #pragma unroll
             for (int k = 0; k <= H; k++) {
                 if (k <= w0) {
-                    sample(k);
-                    if (k > 0) sample(-k);
+                    sample(k, false);
+                    if (k > 0) sample(-k, false);
                 }
-                if (k == w2) mom_fold<kKahan>(tot[2], comp[2], tn[2], run, rn);
-                if (k == w1) mom_fold<kKahan>(tot[1], comp[1], tn[1], run, rn);
+                if (k == w2) mom_fold(tot[2], run, rn);
+                if (k == w1) mom_fold(tot[1], run, rn);
             }
-            if (w0 >= 0) mom_fold<kKahan>(tot[0], comp[0], tn[0], run, rn);
+            if (w0 >= 0) mom_fold(tot[0], run, rn);
         }
     }
     // window totals (Kahan compensation folded back in FP64)
     double n[3], S[3][10];
 #pragma unroll
     for (int w = 0; w < 3; w++) {
-        n[w] = kCheck ? (double)tn[w] : (double)window_count_ce(H, w);
+        n[w] = kCheck ? (double)tot[w].n : (double)window_count_ce(H, w);
         S[w][0] = tot[w].a;
         S[w][1] = tot[w].b;
-        S[w][2] = (double)tot[w].aa - (double)comp[w][0];
-        S[w][3] = (double)tot[w].bb - (double)comp[w][1];
-        S[w][4] = (double)tot[w].ab - (double)comp[w][2];
+        S[w][2] = (double)tot[w].aa;
+        S[w][3] = (double)tot[w].bb;
+        S[w][4] = (double)tot[w].ab;
         S[w][5] = tot[w].c;
         S[w][6] = tot[w].d;
-        S[w][7] = (double)tot[w].cc - (double)comp[w][3];
-        S[w][8] = (double)tot[w].dd - (double)comp[w][4];
-        S[w][9] = (double)tot[w].cd - (double)comp[w][5];
+        S[w][7] = (double)tot[w].cc;
+        S[w][8] = (double)tot[w].dd;
+        S[w][9] = (double)tot[w].cd;
     }
     if (!(n[0] > 0 && n[1] > 0 && n[2] > 0)) return false;
     double temp_roh = 0, count_roh = 0;
 #pragma unroll
     for (int w = 0; w < 3; w++) {
-        const double c1 = corr_moments(n[w], S[w][0], S[w][1], S[w][2], S[w][3], S[w][4]);
+        const double inv = kCheck ? 1.0 / n[w] : 1.0 / (double)window_count_ce(H, w);
+        const double c1 = corr_moments(inv, S[w][0], S[w][1], S[w][2], S[w][3], S[w][4]);
         if (c1 != -99) { count_roh++; temp_roh += c1; }
-        const double c2 = corr_moments(n[w], S[w][5], S[w][6], S[w][7], S[w][8], S[w][9]);
+        const double c2 = corr_moments(inv, S[w][5], S[w][6], S[w][7], S[w][8], S[w][9]);
         if (c2 != -99) { count_roh++; temp_roh += c2; }
     }
     rho = count_roh > 0 ? temp_roh / count_roh : 1e300;   // 1e300 = no valid correlation
@@ -398,6 +418,8 @@ __device__ __forceinline__ bool fast_window(const SL &L, const SR &R, float fxl,
 
 struct TileBox {
     int x0[2], y0[2], w[2], h[2];
+    int p[2];   // LDS row pitch: the width rounded up to odd, so that the rows of a
+                // float2 box fall on different LDS banks
     float piv_i[2], piv_m[2];
     int fits, nodata;
 };
@@ -473,13 +495,14 @@ __device__ __forceinline__ void box_finalize(const VllParams &P, const FastArgs 
         tb.y0[ti] = y0;
         tb.w[ti] = max(x1 - x0 + 1, 1);
         tb.h[ti] = max(y1 - y0 + 1, 1);
+        tb.p[ti] = tb.w[ti] | 1;
         const long cx = min(max((x0 + x1) / 2, 0), (int)v.width - 1);
         const long cy = min(max((y0 + y1) / 2, 0), (int)v.height - 1);
         const uint32_t t = gptr(v.texel)[cy * v.width + cx];
         tb.piv_i[ti] = (float)(t & 0xffffu);
         tb.piv_m[ti] = (float)(t >> 16);
     }
-    if (tb.w[0] * tb.h[0] + tb.w[1] * tb.h[1] > F.lds_texels) fits = 0;
+    if (tb.p[0] * tb.h[0] + tb.p[1] * tb.h[1] > F.lds_texels) fits = 0;
     tb.fits = fits;
     tb.nodata = 0;
 }
@@ -490,14 +513,14 @@ __device__ __forceinline__ void box_stage(const VllParams &P, TileBox &tb)
     int nod = 0;
     for (int ti = 0; ti < 2; ti++) {
         const DevView &v = P.view[ti];
-        const int dst = ti ? tb.w[0] * tb.h[0] : 0;
-        const int w = tb.w[ti], h = tb.h[ti];
+        const int dst = ti ? tb.p[0] * tb.h[0] : 0;
+        const int w = tb.w[ti], h = tb.h[ti], pitch = tb.p[ti];
         const float pi = tb.piv_i[ti], pm = tb.piv_m[ti];
         for (int k = threadIdx.x; k < w * h; k += blockDim.x) {
             const int rr = k / w, cc = k % w;
             const uint32_t t = gptr(v.texel)[(long)(tb.y0[ti] + rr) * v.width + tb.x0[ti] + cc];
             nod |= (t & 0xffffu) == 0;
-            smx_lds[dst + k] = make_float2((float)(t & 0xffffu) - pi, (float)(t >> 16) - pm);
+            smx_lds[dst + rr * pitch + cc] = make_float2((float)(t & 0xffffu) - pi, (float)(t >> 16) - pm);
         }
     }
     if (nod) tb.nodata = 1;   // benign race: every writer stores 1
@@ -552,8 +575,8 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
             if (use_lds) {
                 const float fxl = (float)(hy.rpx - flx), fyl = (float)(hy.rpy - fly);
                 const float fxt = (float)(hy.tpx - ftx), fyt = (float)(hy.tpy - fty);
-                const LdsImg Lw{(ily - O - tb.y0[0]) * tb.w[0] + (ilx - O - tb.x0[0]), tb.w[0]};
-                const LdsImg Rw{tb.w[0] * tb.h[0] + (ity - O - tb.y0[1]) * tb.w[1] + (itx - O - tb.x0[1]), tb.w[1]};
+                const LdsImg Lw{(ily - O - tb.y0[0]) * tb.p[0] + (ilx - O - tb.x0[0]), tb.p[0]};
+                const LdsImg Rw{tb.p[0] * tb.h[0] + (ity - O - tb.y0[1]) * tb.p[1] + (itx - O - tb.x0[1]), tb.p[1]};
                 if (fast_ok)
                     all3 = fast_window<H, false, kKahan>(Lw, Rw, fxl, fyl, fxt, fyt, cos0, sin0, tb.piv_i[0],
                                                          tb.piv_i[1], 0, 0, 0, 0, 0, 0, 0, 0, rho);

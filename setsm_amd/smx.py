@@ -21,7 +21,8 @@ import numpy as np
 from .records import NCCRESULT_DTYPE, UGRID_DTYPE, SmxLevel
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libsmx.so")
+# SMX_LIBRARY names another build of the same library (e.g. a kernel variant)
+LIB_PATH = os.environ.get("SMX_LIBRARY") or os.path.join(HERE, "lib", "libsmx.so")
 
 PREC_FP64 = 0
 PREC_FAST = 1
