@@ -9,6 +9,8 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <mutex>
+#include <map>
 
 #include "smx_vll.h"
 #include "smx_kernels.hpp"
@@ -39,31 +41,79 @@ int fail(int code, const char *fmt, ...)
 
 static unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
+// Device blocks are recycled across jobs and levels of one process: a job
+// allocates its per-level arrays at every level (SETSM allocates them per
+// level too), and hipMalloc / hipFree serialise the host threads of a
+// serving loop. Blocks are kept per (device, size class); a freed block is
+// cached only once its job's stream is idle, and the cache holds at most
+// kCacheCap bytes, beyond which blocks go back to the runtime.
+namespace {
+constexpr size_t kCacheCap = 64ull << 30;
+std::mutex g_cache_mu;
+std::map<std::pair<int, size_t>, std::vector<void *>> g_cache;
+size_t g_cached = 0;
+
+size_t size_class(size_t b)
+{
+    if (b <= (1u << 20)) {
+        size_t c = 256;
+        while (c < b) c <<= 1;
+        return c;
+    }
+    const size_t g = 2u << 20;   // 2 MiB granules above 1 MiB
+    return (b + g - 1) / g * g;
+}
+}  // namespace
+
 int job_alloc(smx_job *j, void **p, size_t bytes)
 {
     *p = nullptr;
     if (bytes == 0) return 0;
-    if (j->budget > 0 && j->device_bytes + (long long)bytes > j->budget)
-        return fail(SMX_E_NOMEM, "device budget of %lld bytes exceeded (need %zu more)", j->budget, bytes);
-    hipError_t e = hipMalloc(p, bytes);
-    if (e != hipSuccess) {
-        *p = nullptr;
-        return fail(SMX_E_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    const size_t cls = size_class(bytes);
+    if (j->budget > 0 && j->device_bytes + (long long)cls > j->budget)
+        return fail(SMX_E_NOMEM, "device budget of %lld bytes exceeded (need %zu more)", j->budget, cls);
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto it = g_cache.find({j->device, cls});
+        if (it != g_cache.end() && !it->second.empty()) {
+            *p = it->second.back();
+            it->second.pop_back();
+            g_cached -= cls;
+        }
     }
-    j->device_bytes += (long long)bytes;
-    j->sizes.push_back({*p, bytes});
+    if (!*p) {
+        hipError_t e = hipMalloc(p, cls);
+        if (e != hipSuccess) {
+            *p = nullptr;
+            return fail(SMX_E_NOMEM, "hipMalloc(%zu): %s", cls, hipGetErrorString(e));
+        }
+    }
+    j->device_bytes += (long long)cls;
+    j->sizes.push_back({*p, cls});
     return 0;
 }
 
 void job_free(smx_job *j, void *p)
 {
     if (!p) return;
+    size_t cls = 0;
     for (size_t i = 0; i < j->sizes.size(); i++)
         if (j->sizes[i].first == p) {
-            j->device_bytes -= (long long)j->sizes[i].second;
+            cls = j->sizes[i].second;
+            j->device_bytes -= (long long)cls;
             j->sizes.erase(j->sizes.begin() + (long)i);
             break;
         }
+    // kernels of this job may still read the block
+    if (j->stream) (void)hipStreamSynchronize(j->stream);
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        if (cls && g_cached + cls <= kCacheCap) {
+            g_cache[{j->device, cls}].push_back(p);
+            g_cached += cls;
+            return;
+        }
+    }
     (void)hipFree(p);
 }
 

@@ -92,6 +92,7 @@ class Server:
         self.batching = batching
         self.fast_config = fast_config
         self.in_use = 0
+        self.idle_jobs = []   # finished jobs, reused by later requests (their HBM stays with the process)
 
     # ------------------------------------------------------------ admit
     def _plan(self, req: Request):
@@ -129,11 +130,12 @@ class Server:
             req.plan = plan
         return stream
 
-    def _admit(self, req: Request, now: float):
+    def _admit(self, req: Request, now: float, job=None):
         sc: Scene = self.pool[req.scene]
-        job = smx.Job(self.levels)
-        if self.fast_config is not None:
-            job.set_fast_config(self.fast_config)
+        if job is None:
+            job = smx.Job(self.levels)
+            if self.fast_config is not None:
+                job.set_fast_config(self.fast_config)
         for ti in range(2):
             job.build_pyramid(ti, sc.images[ti])
         a = _Active(req, job, req.plan, admitted=now)
@@ -176,7 +178,8 @@ class Server:
                     break
                 self.in_use += need
                 admit.append(pending.pop(0))
-            active += list(ex.map(lambda r: self._admit(r, now), admit))
+            reuse = [self.idle_jobs.pop() if self.idle_jobs else None for _ in admit]
+            active += list(ex.map(lambda rj: self._admit(rj[0], now, rj[1]), zip(admit, reuse)))
             if not active:
                 time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
                 continue
@@ -202,13 +205,16 @@ class Server:
                 done.append(dict(rid=a.req.rid, arrival=a.req.arrival, admitted=a.admitted, done=t,
                                  latency=t - a.req.arrival, evaluated=a.evaluated, cells=a.req.cells,
                                  nheights=a.req.nheights, ncc=a.results))
-                a.job.close()
+                self.idle_jobs.append(a.job)
                 self.in_use -= a.reserved
                 active.remove(a)
                 if log:
                     log(f"request {a.req.rid} done: {a.req.cells}^2 cells x {a.req.nheights} h, "
                         f"latency {1e3 * (t - a.req.arrival):.1f} ms")
         wall = time.perf_counter() - t0
+        for j in self.idle_jobs:
+            j.close()
+        self.idle_jobs = []
         lat = np.array([d["latency"] for d in done])
         hyps = sum(d["evaluated"] for d in done)
         return done, dict(requests=len(done), wall_s=wall, hyps=hyps, hyp_per_s=hyps / wall,
