@@ -191,6 +191,8 @@ static void free_level_state(smx_job *j)
     jfree(j, j->d_gval);
     jfree(j, j->d_peak_rho);
     j->peak_cap = 0;
+    jfree(j, j->d_redo);
+    j->redo_cap = 0;
     j->cd_bytes = j->cdn_bytes = 0;
     j->tiles_cap = j->tile_runs_cap = 0;
     j->runs_cap = 0;
@@ -294,7 +296,7 @@ int smx_job_create(smx_job **out, int num_levels)
         return fail(SMX_E_HIP, "job setup: %s", hipGetErrorString(e));
     }
     int r = jalloc(j, &j->d_params, 1);
-    if (!r) r = jalloc(j, &j->d_counters, 4);
+    if (!r) r = jalloc(j, &j->d_counters, 8);
     if (r) {
         smx_job_destroy(j);
         return r;
@@ -738,7 +740,7 @@ static int vll_prepare(smx_job *j, int iteration, int precision, int keep_rho, C
         HIPCHK(launch_ortho_coords(j->d_params, O, j->stream));
         A.acc_flag = j->d_action;
     }
-    HIPCHK(hipMemsetAsync(j->d_counters, 0, 4 * sizeof(unsigned long long), j->stream));
+    HIPCHK(hipMemsetAsync(j->d_counters, 0, 8 * sizeof(unsigned long long), j->stream));
     HIPCHK(launch_vll_gate(j->d_params, A, j->stream));
     if (gncc_on) {
         HIPCHK(launch_gncc(half, j->d_params, O, A, j->d_gval, j->stream));
@@ -758,6 +760,17 @@ static int vll_prepare(smx_job *j, int iteration, int precision, int keep_rho, C
     A.runs = j->d_runs;
     A.nruns = nruns;
     HIPCHK(launch_vll_fill_runs(A, j->stream));
+    A.redo_n = j->d_counters + 4;
+    if (precision != SMX_PREC_FP64 && !L.is_ra && !L.check_matching_rate) {
+        // FAST: runs handed to the FP64 pass (each at most once)
+        if (nruns > j->redo_cap) {
+            jfree(j, j->d_redo);
+            r = jalloc(j, &j->d_redo, (size_t)std::max(nruns, 1L));
+            if (r) return r;
+            j->redo_cap = nruns;
+        }
+        A.redo = j->d_redo;
+    }
     if (L.is_ra || L.check_matching_rate) {
         // RA / matching-rate pass: temp_rho per run lane for FindPeakNcc
         if (nruns > j->peak_cap) {
@@ -775,11 +788,12 @@ static int vll_prepare(smx_job *j, int iteration, int precision, int keep_rho, C
 static long long vll_finish(smx_job *j, const CellArgs &A)
 {
     HIPCHK(launch_vll_finalize(A, j->stream));
-    HIPCHK(hipMemcpyAsync(j->h_small, j->d_counters, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, j->stream));
+    HIPCHK(hipMemcpyAsync(j->h_small, j->d_counters, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
     j->last_evaluated = j->h_small[1];
     j->last_tiles_lds = j->h_small[2];
     j->last_waves_hbm = j->h_small[3];
+    j->last_redo_runs = j->h_small[4];
     return (long long)j->h_small[0];
 }
 
@@ -799,8 +813,10 @@ long long smx_vertical_line_locus(smx_job *j, int iteration, int precision, int 
         HIPCHK(hipEventRecord(j->ev1, j->stream));
         if (A.peak_rho && j->lv.check_matching_rate) HIPCHK(launch_find_peak(j->d_params, A, j->stream));
     } else {
-        r = launch_vll_fast(j, j->hp.half, A, j->ev0, j->ev1);
+        r = launch_vll_fast(j, j->hp.half, A, j->ev0, nullptr);
         if (r) return r;
+        HIPCHK(launch_vll_redo(j->hp.half, j->d_params, A, j->stream));
+        HIPCHK(hipEventRecord(j->ev1, j->stream));
     }
     const long long acc = vll_finish(j, A);
     if (acc < 0) return acc;
@@ -840,8 +856,10 @@ int smx_vertical_line_locus_batch(smx_job **jobs, int n, int iteration, int prec
             return fail(SMX_E_UNSUPPORTED, "combined-level iterations are launched per job");
     }
     smx_job *j0 = jobs[0];
-    int r = launch_vll_fast_batch(jobs, n, As.data(), j0->hp.half, j0->ev0, j0->ev1);
+    int r = launch_vll_fast_batch(jobs, n, As.data(), j0->hp.half, j0->ev0, nullptr);
     if (r) return r;
+    for (int i = 0; i < n; i++) HIPCHK(launch_vll_redo(j0->hp.half, jobs[i]->d_params, As[i], j0->stream));
+    HIPCHK(hipEventRecord(j0->ev1, j0->stream));
     HIPCHK(hipStreamSynchronize(j0->stream));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, j0->ev0, j0->ev1));
@@ -1019,6 +1037,7 @@ int smx_job_last_stats(const smx_job *j, long long *out4)
     out4[3] = (long long)j->fast_cfg;
     return 0;
 }
+long long smx_job_last_redo_runs(const smx_job *j) { return j ? j->last_redo_runs : 0; }
 long long smx_job_last_evaluated(const smx_job *j) { return j ? j->last_evaluated : 0; }
 
 int smx_job_sync(smx_job *j)

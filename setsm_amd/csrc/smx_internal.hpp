@@ -51,6 +51,8 @@ struct smx_job {
     double *d_gval = nullptr;                     // per-cell GNCC value (NaN: none)
     double *d_peak_rho = nullptr;                 // RA / matching-rate pass: temp_rho per run lane
     long peak_cap = 0;
+    int2 *d_redo = nullptr;                      // FAST: runs handed to the FP64 pass
+    long redo_cap = 0;
     smx_ugrid *d_gridpt3 = nullptr;
     smx_nccresult *d_ncc = nullptr;
     int64_t *d_vox_off = nullptr;
@@ -85,6 +87,7 @@ struct smx_job {
     long long last_evaluated = 0;
     long long last_tiles_lds = 0;   // FAST: tiles whose footprint was staged in LDS
     long long last_waves_hbm = 0;   // FAST: waves that read taps from HBM
+    long long last_redo_runs = 0;   // FAST: runs recomputed by the FP64 pass
 };
 
 namespace smx {
@@ -110,6 +113,7 @@ hipError_t launch_gncc(int half, const VllParams *dP, const OrthoArgs &O, const 
                        hipStream_t s);
 hipError_t launch_vll_fill_runs(const CellArgs &A, hipStream_t s);
 hipError_t launch_vll_fp64(int half, const VllParams *dP, const CellArgs &A, hipStream_t s);
+hipError_t launch_vll_redo(int half, const VllParams *dP, const CellArgs &A, hipStream_t s);
 hipError_t launch_vll_finalize(const CellArgs &A, hipStream_t s);
 hipError_t launch_init_voxel(const VllParams *dP, const InitArgs &A, hipStream_t s);
 hipError_t launch_relayout(const InitArgs &A, hipStream_t s);

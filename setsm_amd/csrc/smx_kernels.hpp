@@ -3,6 +3,7 @@
 #pragma once
 
 #include "smx_device.hpp"
+#include "smx_window.hpp"
 
 namespace smx {
 
@@ -31,6 +32,8 @@ struct CellArgs {
     double ncc_alpha;              // SetNCC_alpha of the call (GNCC blend)
     double *peak_rho;              // RA / matching-rate pass: temp_rho per run lane (NaN: not evaluated);
                                    // the cube is not touched
+    int2 *redo;                    // FAST: runs handed to the FP64 pass (capacity nruns)
+    unsigned long long *redo_n;    // FAST: their count
 };
 
 // setsm_code.cpp:6481-6490: the value max_WNCC takes for one hypothesis —
@@ -88,48 +91,6 @@ struct PeakArgs {
     const uint32_t *vox;
     const float *sum;              // SGM SumCost (check_SGM_peak), or null
 };
-
-// SubFunctions.cpp:3274-3312 plus the outer disc test of setsm_code.cpp:6434:
-// bit 0 = 11x11 disc, bit 1 = middle window, bit 2 = inner window.
-__host__ __device__ constexpr int window_bits_ce(int half, int row, int col)
-{
-    const int r2 = row * row + col * col;
-    const int s1 = half / 2;
-    const int s2 = s1 + (int)((s1 / 2.0) + 0.5);
-    return (row < -half || row > half || col < -half || col > half || r2 > (half + 1) * (half + 1))
-               ? 0
-               : (1 | ((r2 <= (half - s1 + 1) * (half - s1 + 1) && row >= -half + s1 && row <= half - s1 &&
-                        col >= -half + s1 && col <= half - s1)
-                           ? 2
-                           : 0) |
-                  ((r2 <= (half - s2 + 1) * (half - s2 + 1) && row >= -half + s2 && row <= half - s2 &&
-                    col >= -half + s2 && col <= half - s2)
-                       ? 4
-                       : 0));
-}
-
-// Samples of ring k (0 = outer only, 1 = middle only, 2 = inner window).
-__host__ __device__ constexpr int ring_count_ce(int half, int ring)
-{
-    int n = 0;
-    for (int r = -half; r <= half; r++)
-        for (int c = -half; c <= half; c++) {
-            const int b = window_bits_ce(half, r, c);
-            const int k = (b & 4) ? 2 : ((b & 2) ? 1 : 0);
-            if (b && k == ring) n++;
-        }
-    return n;
-}
-
-// Samples of window w (0 = outer disc, 1 = middle, 2 = inner).
-__host__ __device__ constexpr int window_count_ce(int half, int w)
-{
-    int n = 0;
-    for (int r = -half; r <= half; r++)
-        for (int c = -half; c <= half; c++)
-            if (window_bits_ce(half, r, c) & (1 << w)) n++;
-    return n;
-}
 
 __device__ __forceinline__ unsigned long long order_key(double v)
 {

@@ -10,6 +10,8 @@
 // sum turns them into a flat list of (cell, first height) runs of 64 heights;
 // one wavefront evaluates one run, one lane per height hypothesis.
 
+#include <algorithm>
+
 #include "smx_kernels.hpp"
 
 namespace smx {
@@ -296,13 +298,11 @@ __device__ __forceinline__ bool multi_ncc_fp64_at(const DevView &Lv, const DevVi
     return true;
 }
 
+// One run (64 consecutive heights of one cell) on one wavefront, FP64.
 template <int H>
-__global__ void __launch_bounds__(256) k_vll_fp64(const VllParams *__restrict__ P, CellArgs A)
+__device__ __forceinline__ void vll_fp64_run(const VllParams *__restrict__ P, const CellArgs &A, const int2 run,
+                                             const long wave, const int lane)
 {
-    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    if (wave >= A.nruns) return;
-    const int2 run = A.runs[wave];
     const long cell = run.x;
     const int hidx = run.y + lane;
     const smx_nccresult nr = A.ncc[cell];
@@ -359,6 +359,27 @@ __global__ void __launch_bounds__(256) k_vll_fp64(const VllParams *__restrict__ 
         }
     }
     wave_max_commit(temp_rho, evaluated, &A.max_bits[cell], A.evaluated);
+}
+
+template <int H>
+__global__ void __launch_bounds__(256) k_vll_fp64(const VllParams *__restrict__ P, CellArgs A)
+{
+    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (wave >= A.nruns) return;
+    vll_fp64_run<H>(P, A, A.runs[wave], wave, threadIdx.x & 63);
+}
+
+// The runs the FAST kernel handed over (a window whose spread it cannot tell
+// from FP32 rounding), recomputed in full. The count is read on the device,
+// so the launch needs no host round trip; an empty queue costs one load per
+// wave.
+template <int H>
+__global__ void __launch_bounds__(256) k_vll_redo(const VllParams *__restrict__ P, CellArgs A)
+{
+    const long nw = ((long)gridDim.x * blockDim.x) >> 6;
+    const long n = (long)*A.redo_n;
+    for (long q = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < n; q += nw)
+        vll_fp64_run<H>(P, A, A.redo[q], q, threadIdx.x & 63);
 }
 
 template __global__ void k_vll_fp64<5>(const VllParams *, CellArgs);
@@ -791,6 +812,20 @@ hipError_t launch_vll_fp64(int half, const VllParams *dP, const CellArgs &A, hip
     case 5: hipLaunchKernelGGL(k_vll_fp64<5>, grid, block, 0, s, dP, A); break;
     case 6: hipLaunchKernelGGL(k_vll_fp64<6>, grid, block, 0, s, dP, A); break;
     case 7: hipLaunchKernelGGL(k_vll_fp64<7>, grid, block, 0, s, dP, A); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_vll_redo(int half, const VllParams *dP, const CellArgs &A, hipStream_t s)
+{
+    if (A.nruns == 0 || !A.redo) return hipSuccess;
+    // up to 32 waves per CU of the FP64 body over the queue
+    const dim3 grid((unsigned)std::min<long>(nblk(A.nruns, 4), 2048)), block(256);
+    switch (half) {
+    case 5: hipLaunchKernelGGL(k_vll_redo<5>, grid, block, 0, s, dP, A); break;
+    case 6: hipLaunchKernelGGL(k_vll_redo<6>, grid, block, 0, s, dP, A); break;
+    case 7: hipLaunchKernelGGL(k_vll_redo<7>, grid, block, 0, s, dP, A); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

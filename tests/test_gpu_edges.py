@@ -1,0 +1,125 @@
+"""Edge cases on the GPU against the oracle: nodata (DN 0) regions in both
+images, flat (constant) patches, grids reaching the image borders,
+degenerate search spaces (empty and single-height bands, every cell
+blundered). FP64 bit for bit; FAST within the 1e-5 contract with the same
+evaluated set (its degenerate windows decided by the FP64 pass)."""
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from tests.helpers import GpuRun, OracleRun, level_grid, scene
+from tests.test_gpu_parity import _compare, _fast_vs_fp64
+
+pytestmark = pytest.mark.gpu
+
+
+def _nodata_pyramid(sc, levels, seed=3, value=None, patches=6):
+    """Rectangles of DN 0 (value None) or of one constant DN per rectangle."""
+    rng = np.random.default_rng(seed)
+    imgs = []
+    for ti in range(2):
+        im = sc.images[ti].copy()
+        h, w = im.shape
+        for _ in range(patches):
+            y0, x0 = rng.integers(0, h - 40), rng.integers(0, w - 40)
+            dn = 0 if value is None else int(rng.integers(*value))
+            im[y0:y0 + rng.integers(8, 40), x0:x0 + rng.integers(8, 40)] = dn
+        imgs.append(im)
+    return imgs, [O.build_pyramid(im, levels) for im in imgs]
+
+
+def _job_with(pyr):
+    from setsm_amd.smx import Job
+    job = Job(len(pyr[0]))
+    for ti in range(2):
+        for lv, (img, mag, ori) in enumerate(pyr[ti]):
+            job.set_view(ti, lv, img, mag, ori)
+    return job
+
+
+@pytest.mark.parametrize("level", [0, 1])
+def test_nodata_regions(require_gpu, level):
+    from setsm_amd.smx import PREC_FAST
+    sc, _ = scene(image_px=512, levels=2) if level else scene(image_px=512, levels=1)
+    _, pyr = _nodata_pyramid(sc, level + 1)
+    g = level_grid(sc, level, 64 if level == 0 else 48)
+    o = OracleRun(pyr, sc.rpcs, level, g)
+    ro = o.step(g["ugrid"], 1, g["minmax"])
+    job = _job_with(pyr)
+    rg = GpuRun(job, level, pyr, sc.rpcs, g).step(g["ugrid"], 1, g["minmax"])
+    _compare(ro, rg, f"nodata L{level}")
+    o2 = OracleRun(pyr, sc.rpcs, level, g)
+    ro2 = o2.step(g["ugrid"], 1, g["minmax"])
+    job2 = _job_with(pyr)
+    rf = GpuRun(job2, level, pyr, sc.rpcs, g).step(g["ugrid"], 1, g["minmax"], precision=PREC_FAST)
+    _fast_vs_fp64(ro2, rf, len(g["ugrid"]), f"FAST nodata L{level}")
+    print(f"[FAST nodata L{level}] runs redone in FP64: {job2.last_stats['redo_runs']}", flush=True)
+    job.close()
+    job2.close()
+
+
+def test_flat_patches_fast(require_gpu):
+    """Constant non-zero patches: the reference's window variance is an exact
+    zero there; the FAST kernel hands those runs to the FP64 pass instead of
+    correlating FP32 rounding noise."""
+    from setsm_amd.smx import PREC_FAST
+    sc, _ = scene(image_px=512, levels=1)
+    _, pyr = _nodata_pyramid(sc, 1, seed=5, value=(50, 3000), patches=10)
+    g = level_grid(sc, 0, 64)
+    o = OracleRun(pyr, sc.rpcs, 0, g)
+    ro = o.step(g["ugrid"], 1, g["minmax"])
+    job = _job_with(pyr)
+    rf = GpuRun(job, 0, pyr, sc.rpcs, g).step(g["ugrid"], 1, g["minmax"], precision=PREC_FAST)
+    redo = job.last_stats["redo_runs"]
+    print(f"[FAST flat patches] runs redone in FP64: {redo}", flush=True)
+    assert redo > 0
+    _fast_vs_fp64(ro, rf, len(g["ugrid"]), "FAST flat patches")
+    job.close()
+
+
+def test_grid_at_image_borders(require_gpu):
+    """Cells whose windows reach the image edge: per-sample bounds tests in
+    both kernels, the FAST kernel's checked and HBM paths."""
+    from setsm_amd.smx import PREC_FAST
+    sc, pyr = scene(image_px=512, levels=1)
+    side = sc.spec.image_px * sc.spec.gsd
+    g = level_grid(sc, 0, int((side - 4.0) / 2.0), inset=2.0)
+    o = OracleRun(pyr, sc.rpcs, 0, g)
+    ro = o.step(g["ugrid"], 1, g["minmax"])
+    job = _job_with(pyr)
+    gr = GpuRun(job, 0, pyr, sc.rpcs, g)
+    rg = gr.step(g["ugrid"], 1, g["minmax"])
+    _compare(ro, rg, "borders FP64")
+    o2 = OracleRun(pyr, sc.rpcs, 0, g)
+    ro2 = o2.step(g["ugrid"], 1, g["minmax"])
+    job2 = _job_with(pyr)
+    rf = GpuRun(job2, 0, pyr, sc.rpcs, g).step(g["ugrid"], 1, g["minmax"], precision=PREC_FAST)
+    _fast_vs_fp64(ro2, rf, len(g["ugrid"]), "FAST borders")
+    job.close()
+    job2.close()
+
+
+@pytest.mark.parametrize("case", ["empty_bands", "all_blunders"])
+def test_degenerate_search_spaces(require_gpu, case):
+    from setsm_amd.smx import PREC_FAST
+    sc, pyr = scene(image_px=512, levels=1)
+    g = level_grid(sc, 0, 40)
+    ug = g["ugrid"].copy()
+    if case == "empty_bands":
+        ug["maxHeight"][::2] = ug["minHeight"][::2]          # NumOfHeight 0
+        ug["maxHeight"][1::4] = ug["minHeight"][1::4] + 1     # 4 heights
+    else:
+        ug["Matched_flag"][:] = 0                             # level 0: every cell is a blunder cell
+    for prec in (0, PREC_FAST):
+        o = OracleRun(pyr, sc.rpcs, 0, g)
+        ro = o.step(ug, 1, g["minmax"])
+        job = _job_with(pyr)
+        rg = GpuRun(job, 0, pyr, sc.rpcs, g).step(ug, 1, g["minmax"], precision=prec)
+        if case == "all_blunders":
+            assert ro["acc"] == 0 and rg["acc"] == 0
+        if prec == 0:
+            _compare(ro, rg, f"{case} FP64")
+        else:
+            assert ro["acc"] == rg["acc"]
+            _fast_vs_fp64(ro, rg, len(ug), f"{case} FAST")
+        job.close()

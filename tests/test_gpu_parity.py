@@ -108,6 +108,8 @@ def _fast_vs_fp64(o, g, ncells, tag, tol=1e-5):
     assert np.array_equal(ev, np.isfinite(g["rho"])), f"{tag}: evaluated sets differ"
     assert np.array_equal(o["words"] & 0xFF, g["words"] & 0xFF), f"{tag}: flag_cal differs"
     d = np.abs(g["rho"][ev].astype(np.float64) - o["rho"][ev])
+    if d.size == 0:
+        d = np.zeros(1)
     incc_o = (o["words"] >> 16).astype(np.uint16).view(np.int16).astype(np.int32)
     incc_g = (g["words"] >> 16).astype(np.uint16).view(np.int16).astype(np.int32)
     nflip = int(np.sum(incc_o != incc_g))
