@@ -331,7 +331,12 @@ def main():
     value = best["evaluated"] / best["wall_s"]
     kernel_s = best["kernel_ms"] / 1e3
     if best_prec == "fast":
-        achieved = best["evaluated"] * fph / kernel_s / 1e12
+        # the FAST window kernel alone: the hypotheses of the runs it handed to
+        # the FP64 pass, and that pass's time, are not its own
+        redo = best.get("stats", {})
+        log(f"FP64 pass over handed-over runs: {redo.get('redo_runs')} runs, {redo.get('redo_hyps')} hypotheses, "
+            f"{redo.get('redo_ms', 0.0):.2f} ms")
+        achieved = (best["evaluated"] - redo.get("redo_hyps", 0)) * fph / kernel_s / 1e12
         roof = dict(bound="valu", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
                     frac=achieved / PEAK_FP32_TFLOPS,
                     traffic=measured_traffic() if (args.image_px, cells) == (16384, 4000) else None)

@@ -158,20 +158,22 @@ int  smx_job_set_fast_config(smx_job *job, int config);
 /* Describe config `config` as "tx ty lds_rows o4" into buf. */
 int  smx_fast_config_desc(int config, char *buf, int buflen);
 
-/* Milliseconds the last smx_vertical_line_locus spent in its VLL kernels
- * (HIP events on the job's stream; FAST: the window kernel and the FP64
- * pass over the runs it handed over). */
+/* Milliseconds the last smx_vertical_line_locus spent in its VLL kernel
+ * (HIP events on the job's stream; FAST: the window kernel, without the FP64
+ * pass over the runs it hands over, see smx_job_last_redo). */
 double smx_job_last_kernel_ms(const smx_job *job);
 /* Hypotheses the last smx_vertical_line_locus evaluated (passed every gate). */
 long long smx_job_last_evaluated(const smx_job *job);
 /* out4 = {evaluated hypotheses, FAST tiles staged in LDS, FAST waves that
  * read their taps from HBM, FAST launch shape index} of the last call. */
 int  smx_job_last_stats(const smx_job *job, long long *out4);
-/* Runs (64 heights of one cell) of the last SMX_PREC_FAST call that the FP64
- * kernel recomputed because a window's spread was not clear of FP32
- * rounding (flat patches, windows left with a few valid samples). No
- * counterpart in the reference: a statistic of this library's FAST path. */
-long long smx_job_last_redo_runs(const smx_job *job);
+/* The last SMX_PREC_FAST call's FP64 pass: runs (64 heights of one cell) the
+ * FAST kernel handed over because a window's spread was not clear of FP32
+ * rounding (flat patches, windows left with a few valid samples), the
+ * hypotheses that pass evaluated (part of smx_job_last_evaluated) and its
+ * milliseconds (not part of smx_job_last_kernel_ms). No counterpart in the
+ * reference: statistics of this library's FAST path. */
+int  smx_job_last_redo(const smx_job *job, long long *runs, long long *hypotheses, double *ms);
 
 /* Wait for all work queued on the job's stream. */
 int  smx_job_sync(smx_job *job);

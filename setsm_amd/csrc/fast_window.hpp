@@ -55,15 +55,17 @@ constexpr double kRedoRho = 3e300;
 // Whether one window / channel's variance (sum of squared deviations v over
 // n samples, second moment ss about the pivot p) is clear of FP32 rounding.
 // Each sample carries a few roundings of its taps' magnitude (about |p|), and
-// the moments a few of ss; a variance within 2^14 times that floor, or below
-// twice the 1e-8 cut of Correlate, may be noise where the reference's
-// two-pass FP64 Pearson (SubFunctions.cpp:974-1018) sees an exact zero (a
-// flat patch, a window left with one valid sample) — or a spread so small
-// that the noise would swamp the correlation.
+// the moments a few of ss; a variance not above the sum of 2^14 times that
+// floor, ss / 128 and twice the 1e-8 cut of Correlate may be noise where the
+// reference's two-pass FP64 Pearson (SubFunctions.cpp:974-1018) sees an
+// exact zero (a flat patch, a window left with one valid sample) — or a
+// spread so small that the noise would swamp the correlation. (The sum, one
+// fused test per window channel, costs half the instructions of three
+// separate tests.)
 SMX_FN bool clear_of_noise(double n, double v, double ss, float p)
 {
     const double pp = (double)p * (double)p;
-    return v > 2e-8 && v > n * pp * 0x1p-16 && v > ss * 0x1p-7;
+    return v > fma(ss, 0x1p-7, fma(n * 0x1p-16, pp, 2e-8));
 }
 
 // Half-width of window w (0 outer disc, 1 middle, 2 inner) on row `row`:

@@ -290,6 +290,7 @@ int smx_job_create(smx_job **out, int num_levels)
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&j->ev0);
     if (e == hipSuccess) e = hipEventCreate(&j->ev1);
+    if (e == hipSuccess) e = hipEventCreate(&j->ev2);
     if (e == hipSuccess) e = hipHostMalloc((void **)&j->h_small, 64 * sizeof(int64_t), hipHostMallocDefault);
     if (e != hipSuccess) {
         delete j;
@@ -326,6 +327,7 @@ int smx_job_destroy(smx_job *j)
     if (j->h_small) (void)hipHostFree(j->h_small);
     if (j->ev0) (void)hipEventDestroy(j->ev0);
     if (j->ev1) (void)hipEventDestroy(j->ev1);
+    if (j->ev2) (void)hipEventDestroy(j->ev2);
     if (j->stream) (void)hipStreamDestroy(j->stream);
     delete j;
     return 0;
@@ -790,7 +792,8 @@ static long long vll_finish(smx_job *j, const CellArgs &A)
     HIPCHK(launch_vll_finalize(A, j->stream));
     HIPCHK(hipMemcpyAsync(j->h_small, j->d_counters, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
-    j->last_evaluated = j->h_small[1];
+    j->last_evaluated = j->h_small[1] + j->h_small[5];
+    j->last_redo_hyps = j->h_small[5];
     j->last_tiles_lds = j->h_small[2];
     j->last_waves_hbm = j->h_small[3];
     j->last_redo_runs = j->h_small[4];
@@ -813,16 +816,18 @@ long long smx_vertical_line_locus(smx_job *j, int iteration, int precision, int 
         HIPCHK(hipEventRecord(j->ev1, j->stream));
         if (A.peak_rho && j->lv.check_matching_rate) HIPCHK(launch_find_peak(j->d_params, A, j->stream));
     } else {
-        r = launch_vll_fast(j, j->hp.half, A, j->ev0, nullptr);
+        r = launch_vll_fast(j, j->hp.half, A, j->ev0, j->ev1);
         if (r) return r;
         HIPCHK(launch_vll_redo(j->hp.half, j->d_params, A, j->stream));
-        HIPCHK(hipEventRecord(j->ev1, j->stream));
+        HIPCHK(hipEventRecord(j->ev2, j->stream));
     }
     const long long acc = vll_finish(j, A);
     if (acc < 0) return acc;
-    float ms = 0;
+    float ms = 0, rms = 0;
     HIPCHK(hipEventElapsedTime(&ms, j->ev0, j->ev1));
+    if (precision != SMX_PREC_FP64 && !A.peak_rho) HIPCHK(hipEventElapsedTime(&rms, j->ev1, j->ev2));
     j->last_ms = ms;
+    j->last_redo_ms = rms;
     return acc;
 }
 
@@ -856,18 +861,20 @@ int smx_vertical_line_locus_batch(smx_job **jobs, int n, int iteration, int prec
             return fail(SMX_E_UNSUPPORTED, "combined-level iterations are launched per job");
     }
     smx_job *j0 = jobs[0];
-    int r = launch_vll_fast_batch(jobs, n, As.data(), j0->hp.half, j0->ev0, nullptr);
+    int r = launch_vll_fast_batch(jobs, n, As.data(), j0->hp.half, j0->ev0, j0->ev1);
     if (r) return r;
     for (int i = 0; i < n; i++) HIPCHK(launch_vll_redo(j0->hp.half, jobs[i]->d_params, As[i], j0->stream));
-    HIPCHK(hipEventRecord(j0->ev1, j0->stream));
+    HIPCHK(hipEventRecord(j0->ev2, j0->stream));
     HIPCHK(hipStreamSynchronize(j0->stream));
-    float ms = 0;
+    float ms = 0, rms = 0;
     HIPCHK(hipEventElapsedTime(&ms, j0->ev0, j0->ev1));
+    HIPCHK(hipEventElapsedTime(&rms, j0->ev1, j0->ev2));
     for (int i = 0; i < n; i++) {
         const long long a = vll_finish(jobs[i], As[i]);
         if (a < 0) return (int)a;
         accessible[i] = a;
         jobs[i]->last_ms = ms;
+        jobs[i]->last_redo_ms = rms;
     }
     return 0;
 }
@@ -1037,7 +1044,14 @@ int smx_job_last_stats(const smx_job *j, long long *out4)
     out4[3] = (long long)j->fast_cfg;
     return 0;
 }
-long long smx_job_last_redo_runs(const smx_job *j) { return j ? j->last_redo_runs : 0; }
+int smx_job_last_redo(const smx_job *j, long long *runs, long long *hypotheses, double *ms)
+{
+    if (!j || !runs || !hypotheses || !ms) return fail(SMX_E_ARG, "null argument");
+    *runs = j->last_redo_runs;
+    *hypotheses = j->last_redo_hyps;
+    *ms = j->last_redo_ms;
+    return 0;
+}
 long long smx_job_last_evaluated(const smx_job *j) { return j ? j->last_evaluated : 0; }
 
 int smx_job_sync(smx_job *j)

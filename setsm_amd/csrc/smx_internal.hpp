@@ -26,7 +26,7 @@ struct smx_job {
     int device = 0;
     int num_levels = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     int64_t *h_small = nullptr;                  // pinned readback slots
     std::vector<smx::LevelView> views[2];
     long long device_bytes = 0;
@@ -88,6 +88,8 @@ struct smx_job {
     long long last_tiles_lds = 0;   // FAST: tiles whose footprint was staged in LDS
     long long last_waves_hbm = 0;   // FAST: waves that read taps from HBM
     long long last_redo_runs = 0;   // FAST: runs recomputed by the FP64 pass
+    long long last_redo_hyps = 0;   // FAST: hypotheses that pass evaluated
+    double last_redo_ms = 0;        // FAST: its time
 };
 
 namespace smx {

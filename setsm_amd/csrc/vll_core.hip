@@ -820,12 +820,15 @@ hipError_t launch_vll_fp64(int half, const VllParams *dP, const CellArgs &A, hip
 hipError_t launch_vll_redo(int half, const VllParams *dP, const CellArgs &A, hipStream_t s)
 {
     if (A.nruns == 0 || !A.redo) return hipSuccess;
-    // up to 32 waves per CU of the FP64 body over the queue
+    // up to 32 waves per CU of the FP64 body over the queue; its hypotheses
+    // are counted in the slot after the queue length
+    CellArgs R = A;
+    R.evaluated = A.redo_n + 1;
     const dim3 grid((unsigned)std::min<long>(nblk(A.nruns, 4), 2048)), block(256);
     switch (half) {
-    case 5: hipLaunchKernelGGL(k_vll_redo<5>, grid, block, 0, s, dP, A); break;
-    case 6: hipLaunchKernelGGL(k_vll_redo<6>, grid, block, 0, s, dP, A); break;
-    case 7: hipLaunchKernelGGL(k_vll_redo<7>, grid, block, 0, s, dP, A); break;
+    case 5: hipLaunchKernelGGL(k_vll_redo<5>, grid, block, 0, s, dP, R); break;
+    case 6: hipLaunchKernelGGL(k_vll_redo<6>, grid, block, 0, s, dP, R); break;
+    case 7: hipLaunchKernelGGL(k_vll_redo<7>, grid, block, 0, s, dP, R); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

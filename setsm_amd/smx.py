@@ -34,7 +34,7 @@ EXPORTS = [
     "smx_job_get_nccresult", "smx_job_set_nccresult", "smx_initialize_voxel", "smx_vertical_line_locus",
     "smx_awncc", "smx_job_num_hypotheses", "smx_job_get_voxels", "smx_job_get_voxel_rows", "smx_job_set_voxels",
     "smx_job_get_sumcost", "smx_job_set_ortho_wgs",
-    "smx_job_last_kernel_ms", "smx_job_last_evaluated", "smx_job_last_redo_runs", "smx_job_sync", "smx_fast_num_configs",
+    "smx_job_last_kernel_ms", "smx_job_last_evaluated", "smx_job_last_redo", "smx_job_sync", "smx_fast_num_configs",
     "smx_job_set_fast_config", "smx_fast_config_desc", "smx_job_last_stats", "smx_vertical_line_locus_batch",
 ]
 
@@ -85,7 +85,7 @@ def lib():
             "smx_job_set_ortho_wgs": (i, [vp, vp, C.c_uint32, C.c_uint32]),
             "smx_job_last_kernel_ms": (C.c_double, [vp]),
             "smx_job_last_evaluated": (ll, [vp]),
-            "smx_job_last_redo_runs": (ll, [vp]),
+            "smx_job_last_redo": (i, [vp, vp, vp, vp]),
             "smx_job_sync": (i, [vp]),
             "smx_fast_num_configs": (i, []),
             "smx_job_set_fast_config": (i, [vp, i]),
@@ -247,8 +247,10 @@ class Job:
     def last_stats(self) -> dict:
         out = np.zeros(4, np.int64)
         _check(lib().smx_job_last_stats(self.h, _p(out)))
+        runs, hyps, ms = C.c_longlong(), C.c_longlong(), C.c_double()
+        _check(lib().smx_job_last_redo(self.h, C.byref(runs), C.byref(hyps), C.byref(ms)))
         return dict(evaluated=int(out[0]), tiles_lds=int(out[1]), waves_hbm=int(out[2]), fast_config=int(out[3]),
-                    redo_runs=int(lib().smx_job_last_redo_runs(self.h)))
+                    redo_runs=runs.value, redo_hyps=hyps.value, redo_ms=ms.value)
 
     def set_fast_config(self, config: int):
         _check(lib().smx_job_set_fast_config(self.h, int(config)))
