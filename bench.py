@@ -171,6 +171,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--autotune", action="store_true", help="time every FAST launch shape and store the best")
+    ap.add_argument("--tuning-out", default=None,
+                    help="file the --autotune table is written to (default: the package's tuning.json)")
     ap.add_argument("--scene-cache", default="", help="directory to load / save the rendered scene")
     ap.add_argument("--batch-tiles", type=int, default=16, help="tiles of the batched run (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=80, help="requests of the served run (0 = skip)")
@@ -276,8 +278,9 @@ def main():
         key = f"vll_L0_{args.image_px}px_{cells}cells_h128"
         if args.autotune:
             best, rows = autotune.tune_fast(job, 1, log=log)
-            os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
-            autotune.save(key, best, rows, path=os.path.join(ROOT, "gpurun_out", "tuning.json"))
+            out_path = args.tuning_out or autotune.TUNING_PATH
+            os.makedirs(os.path.dirname(os.path.abspath(out_path)), exist_ok=True)
+            autotune.save(key, best, rows, path=out_path)
             tuned = best
         else:
             entry = autotune.load(key)
