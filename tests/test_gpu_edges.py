@@ -123,3 +123,64 @@ def test_degenerate_search_spaces(require_gpu, case):
             assert ro["acc"] == rg["acc"]
             _fast_vs_fp64(ro, rg, len(ug), f"{case} FAST")
         job.close()
+
+
+def test_max_height_band(require_gpu):
+    """Bands near the reference's 1000-height limit at level 0: long voxel
+    rows (16 runs of 64 per cell), many heights projecting off the images."""
+    from setsm_amd.smx import PREC_FAST
+    sc, pyr = scene(image_px=512, levels=1)
+    g = level_grid(sc, 0, 16, band=249.0)
+    nh = g["ugrid"]["maxHeight"].astype(int) - g["ugrid"]["minHeight"].astype(int)
+    print(f"[max band] band {nh.min()}..{nh.max()} m", flush=True)
+    o = OracleRun(pyr, sc.rpcs, 0, g)
+    ro = o.step(g["ugrid"], 1, g["minmax"])
+    assert ro["ncc"]["NumOfHeight"].max() > 900
+    job = _job_with(pyr)
+    rg = GpuRun(job, 0, pyr, sc.rpcs, g).step(g["ugrid"], 1, g["minmax"])
+    _compare(ro, rg, "max band FP64")
+    o2 = OracleRun(pyr, sc.rpcs, 0, g)
+    ro2 = o2.step(g["ugrid"], 1, g["minmax"])
+    job2 = _job_with(pyr)
+    rf = GpuRun(job2, 0, pyr, sc.rpcs, g).step(g["ugrid"], 1, g["minmax"], precision=PREC_FAST)
+    _fast_vs_fp64(ro2, rf, len(g["ugrid"]), "FAST max band")
+    job.close()
+    job2.close()
+
+
+def test_batch_with_degenerate_windows(require_gpu):
+    """A batched launch over jobs with nodata and flat patches: each job's cube
+    equals the one it gets alone (the FP64 pass over handed-over runs
+    included), and the oracle within the contract."""
+    from setsm_amd import smx
+    sc, _ = scene(image_px=512, levels=1)
+    pyrs = [_nodata_pyramid(sc, 1, seed=3)[1], _nodata_pyramid(sc, 1, seed=5, value=(50, 3000), patches=10)[1]]
+    grids = [level_grid(sc, 0, 48, seed=11), level_grid(sc, 0, 40, band=16.0, seed=12)]
+    jobs, alone = [], []
+    for pyr, g in zip(pyrs, grids):
+        job = _job_with(pyr)
+        GpuRun(job, 0, pyr, sc.rpcs, g)
+        job.set_searchspace(g["ugrid"], g["minmax"])
+        job.initialize_voxel(1)
+        job.vertical_line_locus(1, precision=smx.PREC_FAST, keep_rho=True)
+        alone.append((job.get_voxels(with_rho=True), job.last_stats["redo_runs"]))
+        jobs.append(job)
+    acc = smx.vertical_line_locus_batch(jobs, 1, precision=smx.PREC_FAST, keep_rho=True)
+    for k, (job, pyr, g) in enumerate(zip(jobs, pyrs, grids)):
+        (w0, rho0, _), redo0 = alone[k]
+        w, rho, _ = job.get_voxels(with_rho=True)
+        redo = job.last_stats["redo_runs"]
+        print(f"[batch degenerate] job {k}: {len(w)} hypotheses, redo runs alone {redo0} batched {redo}", flush=True)
+        assert redo == redo0
+        assert np.array_equal(w, w0)
+        ev = ~np.isnan(rho)
+        assert np.array_equal(ev, ~np.isnan(rho0))
+        assert np.array_equal(rho[ev], rho0[ev])
+        o = OracleRun(pyr, sc.rpcs, 0, g)
+        ro = o.step(g["ugrid"], 1, g["minmax"])
+        assert acc[k] == ro["acc"]
+        assert np.array_equal(np.isfinite(ro["rho"]), ev)
+        assert np.max(np.abs(rho[ev].astype(np.float64) - ro["rho"][ev])) <= 1e-5
+    assert sum(a[1] for a in alone) > 0
+    for job in jobs:
+        job.close()
