@@ -393,11 +393,11 @@ int smx_job_build_pyramid(smx_job *j, int img, const uint16_t *level0, uint32_t 
     HIPCHK(hipSetDevice(j->device));
     const size_t n0 = (size_t)width * height;
     uint16_t *cur = nullptr, *nxt = nullptr, *mag = nullptr;
-    int16_t *dir = nullptr;
+    uint32_t *ow = nullptr;   // Sobel magnitude + direction bin per pixel
     int r = jalloc(j, &cur, n0);
     if (!r) r = jalloc(j, &nxt, n0 / 4 + 1);
     if (!r) r = jalloc(j, &mag, n0);
-    if (!r) r = jalloc(j, &dir, n0);
+    if (!r) r = jalloc(j, &ow, n0);
     if (r) goto done;
     HIPCHK(hipMemcpyAsync(cur, level0, n0 * 2, hipMemcpyHostToDevice, j->stream));
     {
@@ -413,8 +413,8 @@ int smx_job_build_pyramid(smx_job *j, int img, const uint16_t *level0, uint32_t 
             r = put_view(j, img, lv, w, h, start_x0 / (double)(1 << lv), start_y0 / (double)(1 << lv));
             if (r) goto done;
             LevelView &v = j->views[img][lv];
-            HIPCHK(launch_sobel(cur, w, h, mag, dir, j->stream));
-            HIPCHK(launch_orientation(mag, dir, w, h, v.ori, j->stream));
+            HIPCHK(launch_sobel(cur, w, h, mag, ow, j->stream));
+            HIPCHK(launch_orientation(ow, w, h, v.ori, j->stream));
             HIPCHK(launch_pack(cur, mag, v.texel, (long)w * h, j->stream));
         }
     }
@@ -423,7 +423,7 @@ done:
     jfree(j, cur);
     jfree(j, nxt);
     jfree(j, mag);
-    jfree(j, dir);
+    jfree(j, ow);
     return r;
 }
 

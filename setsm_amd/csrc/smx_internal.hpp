@@ -102,9 +102,8 @@ int vll_half_template(int level, int template_size, double resolution);
 
 hipError_t scan_i64(const int64_t *in, int64_t *out, long n, void *tmp, size_t &tmp_bytes, hipStream_t s);
 hipError_t launch_gauss_decimate(const uint16_t *in, uint32_t w, uint32_t h, uint16_t *out, hipStream_t s);
-hipError_t launch_sobel(const uint16_t *in, uint32_t w, uint32_t h, uint16_t *mag, int16_t *dir, hipStream_t s);
-hipError_t launch_orientation(const uint16_t *mag, const int16_t *dir, uint32_t w, uint32_t h, uint8_t *ori,
-                              hipStream_t s);
+hipError_t launch_sobel(const uint16_t *in, uint32_t w, uint32_t h, uint16_t *mag, uint32_t *ow, hipStream_t s);
+hipError_t launch_orientation(const uint32_t *ow, uint32_t w, uint32_t h, uint8_t *ori, hipStream_t s);
 hipError_t launch_pack(const uint16_t *img, const uint16_t *mag, uint32_t *texel, long n, hipStream_t s);
 hipError_t launch_unpack(const uint32_t *texel, uint16_t *img, uint16_t *mag, long n, hipStream_t s);
 hipError_t launch_dir_probe(const int *gx, const int *gy, int16_t *out, long n, hipStream_t s);

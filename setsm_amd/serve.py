@@ -167,6 +167,7 @@ class Server:
         done = []
         t0 = time.perf_counter()
         gpu_busy = 0.0
+        phase = dict(admit_s=0.0, begin_s=0.0, end_s=0.0)
         while pending or active:
             now = time.perf_counter() - t0
             admit = []
@@ -179,15 +180,19 @@ class Server:
                 self.in_use += need
                 admit.append(pending.pop(0))
             reuse = [self.idle_jobs.pop() if self.idle_jobs else None for _ in admit]
+            tk = time.perf_counter()
             active += list(ex.map(lambda rj: self._admit(rj[0], now, rj[1]), zip(admit, reuse)))
+            phase["admit_s"] += time.perf_counter() - tk
             if not active:
                 time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
                 continue
             step = active if self.batching else active[:1]
             groups = {}
+            tk = time.perf_counter()
             for a, lv in zip(step, ex.map(self._begin_level, step)):
                 half = 7 if lv >= 4 else (6 if lv == 3 else 5)
                 groups.setdefault(half, []).append(a)
+            phase["begin_s"] += time.perf_counter() - tk
             for half, grp in groups.items():
                 tk = time.perf_counter()
                 if self.batching and self.precision == smx.PREC_FAST:
@@ -198,7 +203,9 @@ class Server:
                     for a in grp:
                         a.job.vertical_line_locus(1, precision=self.precision)
                 gpu_busy += time.perf_counter() - tk
+            tk = time.perf_counter()
             list(ex.map(self._end_level, step))
+            phase["end_s"] += time.perf_counter() - tk
             t = time.perf_counter() - t0
             for a in [a for a in step if a.stage == len(a.plan)]:
                 a.results = a.job.get_nccresult()
@@ -219,4 +226,4 @@ class Server:
         hyps = sum(d["evaluated"] for d in done)
         return done, dict(requests=len(done), wall_s=wall, hyps=hyps, hyp_per_s=hyps / wall,
                           p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
-                          gpu_call_s=gpu_busy)
+                          gpu_call_s=gpu_busy, **phase)
