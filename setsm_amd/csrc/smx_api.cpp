@@ -145,21 +145,26 @@ int job_scan(smx_job *j, const int64_t *in, int64_t *out, long n)
     return 0;
 }
 
-int fast_scratch(smx_job *j, long ntiles, long nruns)
+int fast_scratch(smx_job *j, int slot, long ntiles, long nruns)
 {
-    if (ntiles + 1 > j->tiles_cap) {
-        jfree(j, j->d_tile_count);
-        jfree(j, j->d_tile_off);
-        int r = jalloc(j, &j->d_tile_count, (size_t)ntiles + 1);
-        if (!r) r = jalloc(j, &j->d_tile_off, (size_t)ntiles + 1);
+    int64_t *&count = slot ? j->d_tile2_count : j->d_tile_count;
+    int64_t *&off = slot ? j->d_tile2_off : j->d_tile_off;
+    int2 *&runs = slot ? j->d_tile2_runs : j->d_tile_runs;
+    long &tcap = slot ? j->tiles2_cap : j->tiles_cap;
+    long &rcap = slot ? j->tile2_runs_cap : j->tile_runs_cap;
+    if (ntiles + 1 > tcap) {
+        jfree(j, count);
+        jfree(j, off);
+        int r = jalloc(j, &count, (size_t)ntiles + 1);
+        if (!r) r = jalloc(j, &off, (size_t)ntiles + 1);
         if (r) return r;
-        j->tiles_cap = ntiles + 1;
+        tcap = ntiles + 1;
     }
-    if (nruns > j->tile_runs_cap) {
-        jfree(j, j->d_tile_runs);
-        int r = jalloc(j, &j->d_tile_runs, (size_t)nruns);
+    if (nruns > rcap) {
+        jfree(j, runs);
+        int r = jalloc(j, &runs, (size_t)nruns);
         if (r) return r;
-        j->tile_runs_cap = nruns;
+        rcap = nruns;
     }
     return 0;
 }
@@ -182,6 +187,10 @@ static void free_level_state(smx_job *j)
     jfree(j, j->d_tile_count);
     jfree(j, j->d_tile_off);
     jfree(j, j->d_tile_runs);
+    jfree(j, j->d_tile2_count);
+    jfree(j, j->d_tile2_off);
+    jfree(j, j->d_tile2_runs);
+    j->tiles2_cap = j->tile2_runs_cap = 0;
     jfree(j, j->d_sum);
     j->sum_total = 0;
     jfree(j, j->d_grid_pts);
@@ -193,6 +202,8 @@ static void free_level_state(smx_job *j)
     j->peak_cap = 0;
     jfree(j, j->d_redo);
     j->redo_cap = 0;
+    jfree(j, j->d_part);
+    j->part_cap = 0;
     j->cd_bytes = j->cdn_bytes = 0;
     j->tiles_cap = j->tile_runs_cap = 0;
     j->runs_cap = 0;
@@ -772,6 +783,16 @@ static int vll_prepare(smx_job *j, int iteration, int precision, int keep_rho, C
             j->redo_cap = nruns;
         }
         A.redo = j->d_redo;
+        if (j->hp.combined_incc) {
+            // the level term per hypothesis between the two passes
+            if (j->vox_total > j->part_cap || !j->d_part) {
+                jfree(j, j->d_part);
+                r = jalloc(j, &j->d_part, (size_t)std::max(j->vox_total, 1LL));
+                if (r) return r;
+                j->part_cap = j->vox_total;
+            }
+            A.part = j->d_part;
+        }
     }
     if (L.is_ra || L.check_matching_rate) {
         // RA / matching-rate pass: temp_rho per run lane for FindPeakNcc

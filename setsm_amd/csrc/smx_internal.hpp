@@ -53,6 +53,8 @@ struct smx_job {
     long peak_cap = 0;
     int2 *d_redo = nullptr;                      // FAST: runs handed to the FP64 pass
     long redo_cap = 0;
+    double *d_part = nullptr;                    // FAST, combined iterations: level term per hypothesis
+    long long part_cap = 0;
     smx_ugrid *d_gridpt3 = nullptr;
     smx_nccresult *d_ncc = nullptr;
     int64_t *d_vox_off = nullptr;
@@ -80,6 +82,10 @@ struct smx_job {
     int64_t *d_tile_count = nullptr, *d_tile_off = nullptr;
     int2 *d_tile_runs = nullptr;
     long tiles_cap = 0, tile_runs_cap = 0;
+    // second tile lists: the level-below pass of a combined iteration
+    int64_t *d_tile2_count = nullptr, *d_tile2_off = nullptr;
+    int2 *d_tile2_runs = nullptr;
+    long tiles2_cap = 0, tile2_runs_cap = 0;
     void *d_batch = nullptr;                     // batched-launch descriptors
     size_t batch_bytes = 0;
 
@@ -137,7 +143,7 @@ int launch_vll_fast(smx_job *j, int half, const CellArgs &A, hipEvent_t ev0, hip
 int launch_vll_fast_batch(smx_job **jobs, int n, const CellArgs *As, int half, hipEvent_t ev0, hipEvent_t ev1);
 int fast_num_configs();
 int fast_config_desc(int i, char *buf, int n);
-int fast_scratch(smx_job *j, long ntiles, long nruns);
+int fast_scratch(smx_job *j, int slot, long ntiles, long nruns);
 int job_scan(smx_job *j, const int64_t *in, int64_t *out, long n);
 
 }  // namespace smx

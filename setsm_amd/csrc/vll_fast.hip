@@ -22,6 +22,7 @@
 //
 // The per-hypothesis result is the mean NCC of SETSM within 1e-5 of the FP64
 // value; INCC is its (short)(rho*1000) truncation.
+#include <algorithm>
 #include <cstdio>
 #include <vector>
 
@@ -42,6 +43,16 @@ struct FastArgs {
     int lds_texels;                // float2 texels of LDS for both images' boxes
     int stage;                     // 0: read every tap from HBM
 };
+
+// Launch modes of the FAST kernel. kSingle: the whole hypothesis in one pass
+// (the level-below term of a combined iteration, if any, from HBM). A
+// combined iteration in two passes over the same runs: kLevelPass computes
+// the level term and keeps it per hypothesis (CellArgs::part); kBelowPass
+// stages the level below in LDS, adds its term and writes the results.
+constexpr int kSingle = 0, kLevelPass = 1, kBelowPass = 2;
+// CellArgs::part codes besides a correlation in [-1, 1]
+constexpr double kPartNoWindow = -2e300;   // a window count was zero
+constexpr double kPartNoCorr = 1e300;      // windows counted, no valid correlation
 
 // ------------------------------------------------------ tile run lists
 
@@ -180,12 +191,13 @@ __device__ __forceinline__ void box_reset(int (&sbox)[2][4])
     }
 }
 
-__device__ __forceinline__ void box_add(const VllParams &P, int (&sbox)[2][4], double lon, double lat, double hh)
+__device__ __forceinline__ void box_add(const VllParams &P, const DevView *views, int level, int (&sbox)[2][4],
+                                        double lon, double lat, double hh)
 {
     for (int ti = 0; ti < 2; ti++) {
         double x, y, px, py;
         rpc_ground_to_image(P.rpc[ti], P.adjust[ti], lon, lat, hh, x, y);
-        to_pyramid(x, y, P.view[ti], P.level, px, py);
+        to_pyramid(x, y, views[ti], level, px, py);
         const int fx = (int)floor(px), fy = (int)floor(py);
         atomicMin(&sbox[ti][0], fx);
         atomicMin(&sbox[ti][1], fy);
@@ -197,12 +209,13 @@ __device__ __forceinline__ void box_add(const VllParams &P, int (&sbox)[2][4], d
 // Thread 0: clamp the box to the images, add the window margin, pick the
 // staging pivots, decide whether it fits the LDS budget.
 template <int H>
-__device__ __forceinline__ void box_finalize(const VllParams &P, const FastArgs &F, const int (&sbox)[2][4], TileBox &tb)
+__device__ __forceinline__ void box_finalize(const DevView *views, const FastArgs &F, const int (&sbox)[2][4],
+                                             TileBox &tb)
 {
     constexpr int O = H + 2;
     int fits = F.stage ? 1 : 0;
     for (int ti = 0; ti < 2; ti++) {
-        const DevView &v = P.view[ti];
+        const DevView &v = views[ti];
         // window radius H+1, bilinear +1, 2 px for the band's curvature
         int x0 = sbox[ti][0] - O - 2, y0 = sbox[ti][1] - O - 2;
         int x1 = sbox[ti][2] + O + 3, y1 = sbox[ti][3] + O + 3;
@@ -227,11 +240,11 @@ __device__ __forceinline__ void box_finalize(const VllParams &P, const FastArgs 
 }
 
 // All threads: copy the box into LDS as pivot-shifted float2 texels.
-__device__ __forceinline__ void box_stage(const VllParams &P, TileBox &tb)
+__device__ __forceinline__ void box_stage(const DevView *views, TileBox &tb)
 {
     int nod = 0;
     for (int ti = 0; ti < 2; ti++) {
-        const DevView &v = P.view[ti];
+        const DevView &v = views[ti];
         const int dst = ti ? tb.p[0] * tb.h[0] : 0;
         const int w = tb.w[ti], h = tb.h[ti], pitch = tb.p[ti];
         const float pi = tb.piv_i[ti], pm = tb.piv_m[ti];
@@ -246,12 +259,13 @@ __device__ __forceinline__ void box_stage(const VllParams &P, TileBox &tb)
 }
 
 // One run (64 consecutive heights of one cell) on one wavefront.
-template <int H, bool kKahan>
+template <int H, bool kKahan, int kMode>
 __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs &A, const FastArgs &F,
                                              const TileBox &tb, const bool fits, const bool nodata, const long ri,
                                              const int lane)
 {
     constexpr int O = H + 2;
+    constexpr bool kBelow = kMode == kBelowPass;   // window on the level below
 
         const int2 run = F.tile_runs[ri];
         const long cell = run.x;
@@ -271,11 +285,19 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
             if (!hy.orient_ok) A.vox[wi] = word & ~0xffu;
             else do_ncc = true;
         }
+        double part = 0;
+        if constexpr (kBelow) {
+            // the level pass handed this run to the FP64 pass: nothing to add
+            if (do_ncc) part = A.part[wi];
+            if (__any(do_ncc && part == kRedoRho)) return;
+        }
+        const double rpx = kBelow ? hy.rnx : hy.rpx, rpy = kBelow ? hy.rny : hy.rpy;
+        const double tpx = kBelow ? hy.tnx : hy.tpx, tpy = kBelow ? hy.tny : hy.tpy;
         // integer / fractional split of both centres (FP64 -> FP32 fraction)
-        const double flx = floor(hy.rpx), fly = floor(hy.rpy), ftx = floor(hy.tpx), fty = floor(hy.tpy);
+        const double flx = floor(rpx), fly = floor(rpy), ftx = floor(tpx), fty = floor(tpy);
         const int ilx = (int)flx, ily = (int)fly, itx = (int)ftx, ity = (int)fty;
         const float cos0 = (float)hy.cos0, sin0 = (float)hy.sin0;
-        const DevView &V0 = P.view[0], &V1 = P.view[1];
+        const DevView &V0 = kBelow ? P.next[0] : P.view[0], &V1 = kBelow ? P.next[1] : P.view[1];
         // interior: every tap of the disc is inside both images
         const bool interior = ilx - O >= 0 && ilx + O + 1 < (int)V0.width && ily - O >= 0 &&
                               ily + O + 1 < (int)V0.height && itx - O >= 0 && itx + O + 1 < (int)V1.width &&
@@ -292,8 +314,8 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
         const bool fast_ok = !nodata && __all(!do_ncc || interior);
         if (do_ncc) {
             if (use_lds) {
-                const float fxl = (float)(hy.rpx - flx), fyl = (float)(hy.rpy - fly);
-                const float fxt = (float)(hy.tpx - ftx), fyt = (float)(hy.tpy - fty);
+                const float fxl = (float)(rpx - flx), fyl = (float)(rpy - fly);
+                const float fxt = (float)(tpx - ftx), fyt = (float)(tpy - fty);
                 const LdsImg Lw{(ily - O - tb.y0[0]) * tb.p[0] + (ilx - O - tb.x0[0]), tb.p[0]};
                 const LdsImg Rw{tb.p[0] * tb.h[0] + (ity - O - tb.y0[1]) * tb.p[1] + (itx - O - tb.x0[1]), tb.p[1]};
                 if (fast_ok)
@@ -301,36 +323,55 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
                                                          tb.piv_i[1], 0, 0, 0, 0, 0, 0, 0, 0, rho);
                 else
                     all3 = fast_window<H, true, kKahan>(
-                        Lw, Rw, fxl, fyl, fxt, fyt, cos0, sin0, tb.piv_i[0], tb.piv_i[1], -(float)hy.rpx,
-                        (float)((double)V0.width - 1.0 - hy.rpx), -(float)hy.rpy,
-                        (float)((double)V0.height - 1.0 - hy.rpy), -(float)itx, (float)((int)V1.width - 1 - itx),
+                        Lw, Rw, fxl, fyl, fxt, fyt, cos0, sin0, tb.piv_i[0], tb.piv_i[1], -(float)rpx,
+                        (float)((double)V0.width - 1.0 - rpx), -(float)rpy,
+                        (float)((double)V0.height - 1.0 - rpy), -(float)itx, (float)((int)V1.width - 1 - itx),
                         -(float)ity, (float)((int)V1.height - 1 - ity), rho);
             } else {
-                all3 = global_window<H, kKahan>(V0, V1, hy.rpx, hy.rpy, hy.tpx, hy.tpy, cos0, sin0, rho);
+                all3 = global_window<H, kKahan>(V0, V1, rpx, rpy, tpx, tpy, cos0, sin0, rho);
             }
         }
         if (lane == 0 && !use_lds && __any(do_ncc)) atomicAdd(A.accessible + 3, 1ull);
         double rho_n = 0;
         bool all3n = false;
-        if (do_ncc && P.combined_incc)
-            all3n = global_window<H, kKahan>(P.next[0], P.next[1], hy.rnx, hy.rny, hy.tnx, hy.tny, cos0, sin0,
-                                             rho_n);
+        if constexpr (kMode == kSingle) {
+            if (do_ncc && P.combined_incc)
+                all3n = global_window<H, kKahan>(P.next[0], P.next[1], hy.rnx, hy.rny, hy.tnx, hy.tny, cos0, sin0,
+                                                 rho_n);
+        }
         // a window this path cannot decide: the FP64 pass redoes the whole
         // run; nothing of it is written here (the cleared flags of
         // orientation-rejected lanes are rewritten identically there)
         const bool redo = (do_ncc && all3 && rho == kRedoRho) || (all3n && rho_n == kRedoRho);
         if (__any(redo)) {
             if (lane == 0) A.redo[atomicAdd(A.redo_n, 1ull)] = run;
+            if constexpr (kMode == kLevelPass) {
+                if (do_ncc) A.part[wi] = kRedoRho;   // the level-below pass skips the run
+            }
+            return;
+        }
+        if constexpr (kMode == kLevelPass) {
+            // the level term, kept for the level-below pass
+            if (do_ncc) A.part[wi] = !all3 ? kPartNoWindow : rho == 1e300 ? kPartNoCorr : rho;
             return;
         }
         // setsm_code.cpp:6456-6493: average of the level term and, on even
-        // iterations above level 0, the level-1 term.
+        // iterations above level 0, the level-1 term (in that order).
         double sum = 0, cnt = 0;
-        if (do_ncc && all3 && rho != 1e300) {
+        if constexpr (kBelow) {
+            // the level term from the level pass, then this pass's
+            if (do_ncc && part != kPartNoWindow && part != kPartNoCorr) {
+                sum += part;
+                cnt += 1;
+            }
+            rho_n = rho;
+            all3n = all3;
+            all3 = do_ncc && part != kPartNoWindow;
+        } else if (do_ncc && all3 && rho != 1e300) {
             sum += rho;
             cnt += 1;
         }
-        if (all3n && rho_n != 1e300) {
+        if (do_ncc && all3n && rho_n != 1e300) {
             sum += rho_n;
             cnt += 1;
         }
@@ -346,13 +387,16 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
         wave_max_commit(temp_rho, evaluated, &A.max_bits[cell], A.evaluated);
     }
 
-template <int H, bool kKahan>
+template <int H, bool kKahan, int kMode = kSingle>
 __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, const CellArgs &A, const FastArgs &F,
                                               const long tile)
 {
     __shared__ int sbox[2][4];
     __shared__ TileBox tb;
     const VllParams &P = *Pp;
+    // the images this launch's windows read: the level's, or the level below
+    const DevView *views = kMode == kBelowPass ? P.next : P.view;
+    const int level = kMode == kBelowPass ? P.level - 1 : P.level;
     const long r0 = F.tile_run_off[tile], r1 = F.tile_run_off[tile + 1];
     if (r0 == r1) return;
     const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6, lane = threadIdx.x & 63;
@@ -368,19 +412,20 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
         if (A.run_count[c] == 0) continue;
         const smx_ugrid g = A.gridpt3[c];
         const double lon = A.grid_wgs[2 * c], lat = A.grid_wgs[2 * c + 1];
-        box_add(P, sbox, lon, lat, (double)g.minHeight);
-        box_add(P, sbox, lon, lat, (double)g.maxHeight);
+        box_add(P, views, level, sbox, lon, lat, (double)g.minHeight);
+        box_add(P, views, level, sbox, lon, lat, (double)g.maxHeight);
     }
     __syncthreads();
-    if (threadIdx.x == 0) box_finalize<H>(P, F, sbox, tb);
+    if (threadIdx.x == 0) box_finalize<H>(views, F, sbox, tb);
     __syncthreads();
     if (tb.fits || !F.stage) {
         const bool fits = tb.fits != 0;
         if (threadIdx.x == 0 && fits) atomicAdd(A.accessible + 2, 1ull);
-        if (fits) box_stage(P, tb);
+        if (fits) box_stage(views, tb);
         __syncthreads();
         const bool nodata = tb.nodata != 0;
-        for (long ri = r0 + wave; ri < r1; ri += nwaves) vll_fast_run<H, kKahan>(P, A, F, tb, fits, nodata, ri, lane);
+        for (long ri = r0 + wave; ri < r1; ri += nwaves)
+            vll_fast_run<H, kKahan, kMode>(P, A, F, tb, fits, nodata, ri, lane);
         return;
     }
     // 2. Tall footprints (wide height bands, strong convergence): stage one
@@ -400,18 +445,18 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
                 const int hl = (threadIdx.x & 1) ? min(run.y + kRun - 1, (int)nr.NumOfHeight - 1) : run.y;
                 double hh = (double)(float)(nr.minHeight + hl * P.height_step);
                 hh = fmin(fmax(hh, (double)g.minHeight), (double)g.maxHeight);
-                box_add(P, sbox, A.grid_wgs[2 * cell], A.grid_wgs[2 * cell + 1], hh);
+                box_add(P, views, level, sbox, A.grid_wgs[2 * cell], A.grid_wgs[2 * cell + 1], hh);
             }
         }
         __syncthreads();
-        if (threadIdx.x == 0) box_finalize<H>(P, F, sbox, tb);
+        if (threadIdx.x == 0) box_finalize<H>(views, F, sbox, tb);
         __syncthreads();
         const bool fits = tb.fits != 0;
         if (threadIdx.x == 0 && fits) atomicAdd(A.accessible + 2, 1ull);
-        if (fits) box_stage(P, tb);
+        if (fits) box_stage(views, tb);
         __syncthreads();
         const bool nodata = tb.nodata != 0;
-        if (c0 + wave < r1) vll_fast_run<H, kKahan>(P, A, F, tb, fits, nodata, c0 + wave, lane);
+        if (c0 + wave < r1) vll_fast_run<H, kKahan, kMode>(P, A, F, tb, fits, nodata, c0 + wave, lane);
     }
 }
 
@@ -493,6 +538,13 @@ __global__ void __launch_bounds__(256) k_vll_fast(const VllParams *__restrict__ 
     vll_fast_body<H, kKahan>(Pp, A, F, xcd_tile(blockIdx.x, gridDim.x));
 }
 
+// The two passes of a combined iteration (kLevelPass, kBelowPass).
+template <int H, bool kKahan, int kMode>
+__global__ void __launch_bounds__(256) k_vll_fast_pass(const VllParams *__restrict__ Pp, CellArgs A, FastArgs F)
+{
+    vll_fast_body<H, kKahan, kMode>(Pp, A, F, xcd_tile(blockIdx.x, gridDim.x));
+}
+
 template <int H, bool kKahan>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_vll_fast_o4(const VllParams *__restrict__ Pp, CellArgs A, FastArgs F)
@@ -549,13 +601,15 @@ static hipError_t launch_fast_h(const FastCfg &cfg, dim3 grid, size_t lds, hipSt
                      : launch_fast_hk<H, false>(cfg.occ, grid, lds, s, dP, A, F);
 }
 
-// Per-tile run lists of one job for the FAST kernel (tile shape of config ci).
-int fast_prepare(smx_job *j, int ci, const CellArgs &A, FastArgs &F)
+// Per-tile run lists of one job for the FAST kernel: config ci's launch
+// shape, with tiles of tx x ty cells (config ci's own when 0), in tile-list
+// slot `slot` (1: the level-below pass of a combined iteration).
+int fast_prepare(smx_job *j, int ci, const CellArgs &A, FastArgs &F, int slot = 0, int tx = 0, int ty = 0)
 {
     const FastCfg cfg = kFastCfgs[ci];
     F = FastArgs{};
-    F.tx = cfg.tx;
-    F.ty = cfg.ty;
+    F.tx = tx > 0 ? tx : cfg.tx;
+    F.ty = ty > 0 ? ty : cfg.ty;
     F.grid_w = j->lv.grid_w;
     F.grid_h = j->lv.grid_h;
     F.tiles_x = (int)((F.grid_w + F.tx - 1) / F.tx);
@@ -563,17 +617,20 @@ int fast_prepare(smx_job *j, int ci, const CellArgs &A, FastArgs &F)
     F.lds_texels = cfg.lds_texels;
     F.stage = cfg.stage;
     const long ntiles = (long)F.tiles_x * F.tiles_y;
-    int r = fast_scratch(j, ntiles, A.nruns);
+    int r = fast_scratch(j, slot, ntiles, A.nruns);
     if (r) return r;
-    F.tile_run_count = j->d_tile_count;
-    F.tile_run_off = j->d_tile_off;
-    F.tile_runs = j->d_tile_runs;
+    int64_t *count = slot ? j->d_tile2_count : j->d_tile_count;
+    int64_t *off = slot ? j->d_tile2_off : j->d_tile_off;
+    int2 *runs = slot ? j->d_tile2_runs : j->d_tile_runs;
+    F.tile_run_count = count;
+    F.tile_run_off = off;
+    F.tile_runs = runs;
     const unsigned nb = (unsigned)((ntiles + 255) / 256);
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, j->stream, (unsigned long long *)(j->d_tile_count + ntiles), 1);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, j->stream, (unsigned long long *)(count + ntiles), 1);
     hipLaunchKernelGGL(k_tile_count, dim3(nb), dim3(256), 0, j->stream, A, F);
-    r = job_scan(j, j->d_tile_count, j->d_tile_off, ntiles + 1);
+    r = job_scan(j, count, off, ntiles + 1);
     if (r) return r;
-    hipLaunchKernelGGL(k_tile_fill, dim3(nb), dim3(256), 0, j->stream, A, F, j->d_tile_runs);
+    hipLaunchKernelGGL(k_tile_fill, dim3(nb), dim3(256), 0, j->stream, A, F, runs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(SMX_E_HIP, "tile lists: %s", hipGetErrorString(e));
     return 0;
@@ -584,10 +641,54 @@ static int cfg_index(const smx_job *j)
     return (j->fast_cfg >= 0 && j->fast_cfg < kNumFastCfgs) ? j->fast_cfg : kDefaultFastCfg;
 }
 
+template <int H, bool kKahan>
+static hipError_t launch_passes_hk(dim3 g1, dim3 g2, size_t lds, hipStream_t s, const VllParams *dP,
+                                   const CellArgs &A, const FastArgs &F1, const FastArgs &F2)
+{
+    hipLaunchKernelGGL((k_vll_fast_pass<H, kKahan, kLevelPass>), g1, dim3(256), lds, s, dP, A, F1);
+    hipLaunchKernelGGL((k_vll_fast_pass<H, kKahan, kBelowPass>), g2, dim3(256), lds, s, dP, A, F2);
+    return hipGetLastError();
+}
+
+template <int H>
+static hipError_t launch_passes_h(bool kahan, dim3 g1, dim3 g2, size_t lds, hipStream_t s, const VllParams *dP,
+                                  const CellArgs &A, const FastArgs &F1, const FastArgs &F2)
+{
+    return kahan ? launch_passes_hk<H, true>(g1, g2, lds, s, dP, A, F1, F2)
+                 : launch_passes_hk<H, false>(g1, g2, lds, s, dP, A, F1, F2);
+}
+
+// A combined iteration (the level-below term added): two passes over the
+// same runs, the second on tiles of half the cells per side, so that the
+// level below's box (twice the texels per side) fits the same LDS budget.
+static int launch_vll_fast_combined(smx_job *j, int ci, int half, const CellArgs &A, hipEvent_t ev0,
+                                    hipEvent_t ev1)
+{
+    const FastCfg cfg = kFastCfgs[ci];
+    FastArgs F1, F2;
+    int r = fast_prepare(j, ci, A, F1);
+    if (!r) r = fast_prepare(j, ci, A, F2, 1, std::max(1, cfg.tx / 2), std::max(1, cfg.ty / 2));
+    if (r) return r;
+    const dim3 g1((unsigned)((long)F1.tiles_x * F1.tiles_y)), g2((unsigned)((long)F2.tiles_x * F2.tiles_y));
+    const size_t lds = (size_t)cfg.lds_texels * sizeof(float2);
+    hipError_t e = hipSuccess;
+    if (ev0) (void)hipEventRecord(ev0, j->stream);
+    switch (half) {
+    case 5: e = launch_passes_h<5>(cfg.kahan != 0, g1, g2, lds, j->stream, j->d_params, A, F1, F2); break;
+    case 6: e = launch_passes_h<6>(cfg.kahan != 0, g1, g2, lds, j->stream, j->d_params, A, F1, F2); break;
+    case 7: e = launch_passes_h<7>(cfg.kahan != 0, g1, g2, lds, j->stream, j->d_params, A, F1, F2); break;
+    default: return fail(SMX_E_UNSUPPORTED, "half %d", half);
+    }
+    if (ev1) (void)hipEventRecord(ev1, j->stream);
+    if (e != hipSuccess) return fail(SMX_E_HIP, "k_vll_fast_pass: %s", hipGetErrorString(e));
+    return 0;
+}
+
 int launch_vll_fast(smx_job *j, int half, const CellArgs &A, hipEvent_t ev0, hipEvent_t ev1)
 {
     const int ci = cfg_index(j);
     const FastCfg cfg = kFastCfgs[ci];
+    if (j->hp.combined_incc && A.part && cfg.stage) return launch_vll_fast_combined(j, ci, half, A, ev0, ev1);
     FastArgs F;
     int r = fast_prepare(j, ci, A, F);
     if (r) return r;
