@@ -217,7 +217,7 @@ SMX_FN bool fast_window(const SL &L, const SR &R, float fxl, float fyl, float ft
             const float flx = floorf(x), fly = floorf(y);
             const float wx = x - flx, wy = y - fly;
             float2 r00, r01, r10, r11;
-            R.quad(fly + (float)O, flx + (float)O, r00, r01, r10, r11);
+            R.quad(fly, flx, O, r00, r01, r10, r11);
             const float w11 = wx * wy, w10 = wy - w11, w01 = wx - w11, w00 = (1.f - wx) - w10;
             const float ri = fmaf(w11, r11.x, fmaf(w10, r10.x, fmaf(w01, r01.x, fmaf(w00, r00.x, -pvr.x))));
             const float rm = fmaf(w11, r11.y, fmaf(w10, r10.y, fmaf(w01, r01.y, fmaf(w00, r00.y, -pvr.y))));

@@ -106,13 +106,14 @@ struct LdsImg {
     int base;   // index of texel (0, 0) of the window origin
     int pitch;
     __device__ __forceinline__ float2 at(int row, int col) const { return smx_lds[base + row * pitch + col]; }
-    // The 2x2 taps at (row, col) .. (row + 1, col + 1) for integer-valued
-    // float coordinates: the offset is formed with one exact FP32 fma
-    // (offsets stay far below 2^24).
-    __device__ __forceinline__ void quad(float row, float col, float2 &t00, float2 &t01, float2 &t10,
+    // The 2x2 taps at (row + off, col + off) .. (row + off + 1, col + off + 1)
+    // for integer-valued float coordinates: the offset is formed with one
+    // exact FP32 fma (offsets stay far below 2^24); `off` (a constant) folds
+    // into the per-hypothesis base.
+    __device__ __forceinline__ void quad(float row, float col, int off, float2 &t00, float2 &t01, float2 &t10,
                                          float2 &t11) const
     {
-        const int o = base + (int)fmaf(row, (float)pitch, col);
+        const int o = (base + off * pitch + off) + (int)fmaf(row, (float)pitch, col);
         t00 = smx_lds[o];
         t01 = smx_lds[o + 1];
         t10 = smx_lds[o + pitch];
@@ -135,10 +136,10 @@ struct GlobalImg {
         const uint32_t t = texel[(long)y * width + x];
         return make_float2((float)(t & 0xffffu) - piv_i, (float)(t >> 16) - piv_m);
     }
-    __device__ __forceinline__ void quad(float row, float col, float2 &t00, float2 &t01, float2 &t10,
+    __device__ __forceinline__ void quad(float row, float col, int off, float2 &t00, float2 &t01, float2 &t10,
                                          float2 &t11) const
     {
-        const int r = (int)row, c = (int)col;
+        const int r = (int)row + off, c = (int)col + off;
         t00 = at(r, c);
         t01 = at(r, c + 1);
         t10 = at(r + 1, c);
