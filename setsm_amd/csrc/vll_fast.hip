@@ -577,9 +577,11 @@ static const FastCfg kFastCfgs[] = {
     {1, 4, 3072, 3, 1, 1}, {2, 4, 3584, 3, 1, 1}, {4, 4, 4608, 3, 1, 1}, {1, 4, 3072, 4, 1, 1},
     {2, 4, 3584, 0, 0, 1}, {1, 4, 3072, 0, 1, 0}, {2, 4, 3584, 3, 1, 0}, {2, 8, 4608, 3, 1, 1},
     {1, 8, 4096, 3, 1, 1}, {4, 2, 3584, 3, 1, 1}, {2, 2, 3072, 3, 1, 1}, {2, 4, 3584, 4, 1, 1},
+    {8, 4, 6144, 0, 1, 1}, {4, 16, 7680, 0, 1, 1}, {8, 8, 7680, 0, 1, 1}, {4, 8, 6144, 3, 1, 1},
 };
 constexpr int kNumFastCfgs = sizeof(kFastCfgs) / sizeof(kFastCfgs[0]);
-constexpr int kDefaultFastCfg = 15;   // 2 x 8 cells, 3 waves/SIMD, Kahan: the tuned 16384^2 level-0 shape
+constexpr int kDefaultFastCfg = 6;   // 4 x 8 cells, compiler's register budget, Kahan: fastest at the
+                                     // 16384^2 level-0 workload when tuned
 
 template <int H, bool kKahan>
 static hipError_t launch_fast_hk(int occ, dim3 grid, size_t lds, hipStream_t s, const VllParams *dP,
