@@ -414,12 +414,15 @@ def main():
         from setsm_amd.serve import make_pool
         pool = make_pool(n_scenes=4, image_px=3072, device="cuda", seed=100 + args.seed)
         torch.cuda.empty_cache()
-        tuned_cfg = tuned["config"] if tuned else None
+        # mixed small tiles (96..1024 cells per side) take the library's default
+        # shape (4 x 8 cells): the level-0 tuned shape is sized for 4000^2 cells
+        # and leaves these launches with too few, too large tiles
+        mixed_cfg = None
         if args.batch_tiles > 0:
-            out["batched"] = batched_run(pool, args.batch_tiles, args.seed, tuned_cfg)
+            out["batched"] = batched_run(pool, args.batch_tiles, args.seed, mixed_cfg)
             log(f"batched: {out['batched']}")
         if args.serve_requests > 0:
-            out["served"] = served_run(pool, args.serve_requests, args.serve_rate, args.seed, tuned_cfg)
+            out["served"] = served_run(pool, args.serve_requests, args.serve_rate, args.seed, mixed_cfg)
             log(f"served: {out['served']}")
     print(json.dumps(out), flush=True)
 
