@@ -184,3 +184,33 @@ def test_batch_with_degenerate_windows(require_gpu):
     assert sum(a[1] for a in alone) > 0
     for job in jobs:
         job.close()
+
+
+def test_nodata_combined_iteration(require_gpu):
+    """Level 1, iterations 1 and 2 (the second adds the level-0 term: two FAST
+    passes, the level-0 window staged on its own tiles) over pyramids with
+    nodata rectangles and flat patches: the FAST contract against the oracle,
+    the degenerate windows of either pass decided by the FP64 pass."""
+    from setsm_amd.smx import PREC_FAST
+    from tests.helpers import perturb_searchspace
+    sc, _ = scene(image_px=512, levels=2)
+    imgs, _ = _nodata_pyramid(sc, 2, seed=7)
+    rng = np.random.default_rng(9)
+    for im in imgs:
+        for _ in range(6):
+            y0, x0 = rng.integers(0, 472, 2)
+            im[y0:y0 + rng.integers(10, 40), x0:x0 + rng.integers(10, 40)] = int(rng.integers(50, 3000))
+    pyr = [O.build_pyramid(im, 2) for im in imgs]
+    g = level_grid(sc, 1, 48)
+    o = OracleRun(pyr, sc.rpcs, 1, g)
+    job = _job_with(pyr)
+    gr = GpuRun(job, 1, pyr, sc.rpcs, g)
+    ug = g["ugrid"]
+    for it in (1, 2):
+        if it > 1:
+            ug = perturb_searchspace(ug, 200 + it, 1)
+        ro = o.step(ug, it, g["minmax"])
+        rf = gr.step(ug, it, g["minmax"], precision=PREC_FAST)
+        print(f"[FAST nodata combined] it{it}: runs redone in FP64 {job.last_stats['redo_runs']}", flush=True)
+        _fast_vs_fp64(ro, rf, len(ug), f"FAST nodata L1 it{it}")
+    job.close()
