@@ -73,11 +73,12 @@ def load_scene(d, spec):
                  tuple(z["extent"]), list(z["resid"]))
 
 
-def batched_run(pool, ntiles, seed, cfg):
+def batched_run(pool, ntiles, seed, cfg, tune_key=None, tuning_out=None):
     """Tiles of mixed size (log-uniform 128..1024 cells per side) and height
     range (32..256 per cell) from independently seeded scenes, one level:
     ONE persistent FAST launch over all tiles vs one FAST launch per tile vs
-    the plain FP64 kernel one tile per launch."""
+    the plain FP64 kernel one tile per launch. With tune_key, the launch shape
+    is first tuned on these tiles and stored under that key."""
     import math
     from setsm_amd import smx
     from setsm_amd.records import LevelInputs
@@ -113,6 +114,11 @@ def batched_run(pool, ntiles, seed, cfg):
             best = dt if best is None else min(best, dt)
         return best
 
+    if tune_key:
+        from setsm_amd import autotune
+        best, rows = autotune.tune_batch(jobs, log=log)
+        autotune.save(tune_key, best, rows, path=tuning_out or autotune.TUNING_PATH)
+        cfg = best["config"]
     t_batch = timed(lambda: smx.vertical_line_locus_batch(jobs, 1, precision=smx.PREC_FAST))
     hyps = sum(j.last_evaluated for j in jobs)
     t_each = timed(lambda: [j.vertical_line_locus(1, precision=smx.PREC_FAST) for j in jobs])
@@ -120,7 +126,7 @@ def batched_run(pool, ntiles, seed, cfg):
     for j in jobs:
         j.close()
     return dict(tiles=ntiles, hypotheses=hyps, batched_hyp_per_s=hyps / t_batch, per_tile_fast_hyp_per_s=hyps / t_each,
-                plain_fp64_hyp_per_s=hyps / t_fp64, speedup_vs_plain=t_fp64 / t_batch)
+                plain_fp64_hyp_per_s=hyps / t_fp64, speedup_vs_plain=t_fp64 / t_batch, fast_config=cfg)
 
 
 def peak_heights(rho64, rhof, lens, step, tol=1e-5):
@@ -414,12 +420,17 @@ def main():
         from setsm_amd.serve import make_pool
         pool = make_pool(n_scenes=4, image_px=3072, device="cuda", seed=100 + args.seed)
         torch.cuda.empty_cache()
-        # mixed small tiles (96..1024 cells per side) take the library's default
-        # shape (4 x 8 cells): the level-0 tuned shape is sized for 4000^2 cells
-        # and leaves these launches with too few, too large tiles
-        mixed_cfg = None
+        # mixed small tiles (96..1024 cells per side) have their own tuned shape:
+        # the level-0 one is sized for 4000^2 cells and leaves these launches
+        # with too few, too large tiles
+        from setsm_amd import autotune
+        mixed_key = "vll_mixed_tiles_3072px"
+        entry = autotune.load(mixed_key)
+        mixed_cfg = entry["best"]["config"] if entry else None
         if args.batch_tiles > 0:
-            out["batched"] = batched_run(pool, args.batch_tiles, args.seed, mixed_cfg)
+            out["batched"] = batched_run(pool, args.batch_tiles, args.seed, mixed_cfg,
+                                         tune_key=mixed_key if args.autotune else None, tuning_out=args.tuning_out)
+            mixed_cfg = out["batched"]["fast_config"]
             log(f"batched: {out['batched']}")
         if args.serve_requests > 0:
             out["served"] = served_run(pool, args.serve_requests, args.serve_rate, args.seed, mixed_cfg)

@@ -44,6 +44,32 @@ def tune_fast(job, iteration=1, reps=2, log=None, only_kahan=False):
     return pick[0], rows
 
 
+def tune_batch(jobs, iteration=1, reps=2, log=None):
+    """The same table for one persistent launch over several jobs (mixed tile
+    sizes): every compensated, LDS-staged shape, timed by the batched call."""
+    from .smx import PREC_FAST, fast_configs, vertical_line_locus_batch
+    rows = []
+    for i, desc in fast_configs():
+        if not desc[5] or not desc[4]:
+            continue
+        for j in jobs:
+            j.set_fast_config(i)
+        vertical_line_locus_batch(jobs, iteration, precision=PREC_FAST)   # warm-up
+        best = None
+        for _ in range(reps):
+            vertical_line_locus_batch(jobs, iteration, precision=PREC_FAST)
+            ms = jobs[0].last_kernel_ms
+            best = ms if best is None else min(best, ms)
+        rows.append(dict(config=i, tx=desc[0], ty=desc[1], lds_texels=desc[2], occ=desc[3], stage=desc[4], kahan=desc[5],
+                         kernel_ms=best))
+        if log:
+            log(f"batch tune config {i} {desc}: {best:.2f} ms")
+    rows.sort(key=lambda r: r["kernel_ms"])
+    for j in jobs:
+        j.set_fast_config(rows[0]["config"])
+    return rows[0], rows
+
+
 def load(key):
     """Stored best shape for `key`, with its current index in the library's
     shape table (matched by shape, not by position)."""
