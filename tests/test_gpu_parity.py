@@ -193,3 +193,25 @@ def test_fast_configs_agree(require_gpu):
             print(f"  vs config 0: max|drho|={d.max():.2e} incc_diff={int((dq > 0).sum())}", flush=True)
             assert d.max() < 2e-6 and dq.max() <= 1, f"config {i} differs beyond rounding"
     job.close()
+
+
+def test_fast_configs_combined_iteration(require_gpu):
+    """Every FAST launch shape on a combined-level iteration (level 1,
+    iteration 2: the level pass on the shape's tiles, the level-below pass on
+    tiles of half the cells per side) meets the contract against the oracle."""
+    from setsm_amd.smx import PREC_FAST, fast_configs
+    sc, pyr = scene(image_px=512, levels=2)
+    g = level_grid(sc, 1, 40)
+    ug2 = perturb_searchspace(g["ugrid"], 302, 1)
+    o = OracleRun(pyr, sc.rpcs, 1, g)
+    o.step(g["ugrid"], 1, g["minmax"])
+    ro = o.step(ug2, 2, g["minmax"])
+    for i, desc in fast_configs():
+        job = _job(2)
+        upload_pyramid(job, pyr)
+        gr = GpuRun(job, 1, pyr, sc.rpcs, g)
+        job.set_fast_config(i)
+        gr.step(g["ugrid"], 1, g["minmax"], precision=PREC_FAST)
+        r = gr.step(ug2, 2, g["minmax"], precision=PREC_FAST)
+        _fast_vs_fp64(ro, r, len(ug2), f"config {i} {desc} L1 it2")
+        job.close()
