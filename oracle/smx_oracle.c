@@ -401,7 +401,7 @@ static void compute_multi_ncc(const patchvec *pv, const int *Count_N, double *co
 /* ------------------------------------------------------ InitializeVoxel */
 
 void orc_initialize_voxel(const smx_level *Lv, orc_voxels *vox, smx_ugrid *GridPT3,
-                          smx_nccresult *nccresult, int iteration, const double *minmaxHeight)
+                          smx_nccresult *nccresult, int iteration, const double minmaxHeight[2])
 {
     /* setsm_code.cpp:5759-6012 (check_image_boundary with H_template_size 7) */
     const double height_step = Lv->height_step;
@@ -652,7 +652,7 @@ static double gncc_cell(const smx_level *Lv, const smx_ugrid *GridPT3, long pt_i
 /* --------------------------------------------------- VerticalLineLocus */
 
 long orc_vertical_line_locus(const smx_level *Lv, orc_voxels *vox, smx_nccresult *nccresult,
-                             const smx_ugrid *GridPT3, int iteration, const double *minmaxHeight)
+                             const smx_ugrid *GridPT3, int iteration, const double minmaxHeight[2])
 {
     /* setsm_code.cpp:6049-6548 */
     const int check_matchtag = Lv->check_matchtag;
