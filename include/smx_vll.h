@@ -167,13 +167,12 @@ long long smx_job_last_evaluated(const smx_job *job);
 /* out4 = {evaluated hypotheses, FAST tiles staged in LDS, FAST waves that
  * read their taps from HBM, FAST launch shape index} of the last call. */
 int  smx_job_last_stats(const smx_job *job, long long *out4);
-/* The last SMX_PREC_FAST call's FP64 pass: the runs (64 heights of one cell)
- * holding hypotheses the FAST kernel handed over because a window's spread
- * was not clear of FP32 rounding (flat patches, windows left with a few
- * valid samples), the hypotheses that pass evaluated (part of
- * smx_job_last_evaluated) and its milliseconds (not part of
- * smx_job_last_kernel_ms). No counterpart in the reference: statistics of
- * this library's FAST path. */
+/* The last SMX_PREC_FAST call's FP64 pass: runs (64 heights of one cell) the
+ * FAST kernel handed over because a window's spread was not clear of FP32
+ * rounding (flat patches, windows left with a few valid samples), the
+ * hypotheses that pass evaluated (part of smx_job_last_evaluated) and its
+ * milliseconds (not part of smx_job_last_kernel_ms). No counterpart in the
+ * reference: statistics of this library's FAST path. */
 int  smx_job_last_redo(const smx_job *job, long long *runs, long long *hypotheses, double *ms);
 
 /* Wait for all work queued on the job's stream. */

@@ -201,7 +201,6 @@ static void free_level_state(smx_job *j)
     jfree(j, j->d_peak_rho);
     j->peak_cap = 0;
     jfree(j, j->d_redo);
-    jfree(j, j->d_redo_mask);
     j->redo_cap = 0;
     jfree(j, j->d_part);
     j->part_cap = 0;
@@ -779,14 +778,11 @@ static int vll_prepare(smx_job *j, int iteration, int precision, int keep_rho, C
         // FAST: runs handed to the FP64 pass (each at most once)
         if (nruns > j->redo_cap) {
             jfree(j, j->d_redo);
-            jfree(j, j->d_redo_mask);
             r = jalloc(j, &j->d_redo, (size_t)std::max(nruns, 1L));
-            if (!r) r = jalloc(j, &j->d_redo_mask, (size_t)std::max(nruns, 1L));
             if (r) return r;
             j->redo_cap = nruns;
         }
         A.redo = j->d_redo;
-        A.redo_mask = j->d_redo_mask;
         if (j->hp.combined_incc) {
             // the level term per hypothesis between the two passes
             if (j->vox_total > j->part_cap || !j->d_part) {

@@ -52,7 +52,6 @@ struct smx_job {
     double *d_peak_rho = nullptr;                 // RA / matching-rate pass: temp_rho per run lane
     long peak_cap = 0;
     int2 *d_redo = nullptr;                      // FAST: runs handed to the FP64 pass
-    unsigned long long *d_redo_mask = nullptr;   // FAST: their lanes
     long redo_cap = 0;
     double *d_part = nullptr;                    // FAST, combined iterations: level term per hypothesis
     long long part_cap = 0;

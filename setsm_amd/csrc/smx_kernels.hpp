@@ -33,7 +33,6 @@ struct CellArgs {
     double *peak_rho;              // RA / matching-rate pass: temp_rho per run lane (NaN: not evaluated);
                                    // the cube is not touched
     int2 *redo;                    // FAST: runs handed to the FP64 pass (capacity nruns)
-    unsigned long long *redo_mask; // FAST: their hypotheses (lanes) the FP64 pass decides
     double *part;                  // FAST, combined iterations: the level term per hypothesis
     unsigned long long *redo_n;    // FAST: their count
 };

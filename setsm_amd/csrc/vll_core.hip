@@ -298,39 +298,6 @@ __device__ __forceinline__ bool multi_ncc_fp64_at(const DevView &Lv, const DevVi
     return true;
 }
 
-// One hypothesis of a voxel pass in FP64 (setsm_code.cpp:6329-6507): its
-// cube word (and mean NCC) written, the value it offers to max_WNCC in
-// temp_rho. Returns whether its NCC was evaluated.
-template <int H>
-__device__ __forceinline__ bool fp64_hypothesis(const VllParams *__restrict__ P, const CellArgs &A, const long cell,
-                                                const int hidx, double &temp_rho)
-{
-    const smx_nccresult nr = A.ncc[cell];
-    const smx_ugrid g = A.gridpt3[cell];
-    Hyp hy;
-    temp_rho = -100.0;
-    if (!project_hypothesis(*P, A, cell, hidx, nr, g.minHeight, g.maxHeight, hy)) return false;
-    const long wi = A.vox_off[cell] + hidx;
-    uint32_t word = A.vox[wi];
-    if (!hy.orient_ok) {
-        A.vox[wi] = word & ~0xffu;   // flag_cal = false (setsm_code.cpp:6503-6506)
-        return false;
-    }
-    double sum_multi = 0, count_multi = 0;
-    const bool all3 = multi_ncc_fp64<H>(P->view[0], P->view[1], hy.rpx, hy.rpy, hy.tpx, hy.tpy, hy.cos0, hy.sin0,
-                                        sum_multi, count_multi);
-    if (P->combined_incc)
-        multi_ncc_fp64<H>(P->next[0], P->next[1], hy.rnx, hy.rny, hy.tnx, hy.tny, hy.cos0, hy.sin0, sum_multi,
-                          count_multi);
-    const double rho = count_multi > 0 ? sum_multi / count_multi : -1.0;
-    word = (word & 0xffu) | (all3 ? 1u : 0u);
-    word |= (uint32_t)(uint16_t)q_voxel(rho) << 16;
-    A.vox[wi] = word;
-    if (A.rho) A.rho[wi] = (float)rho;
-    temp_rho = wncc_of(A, cell, sum_multi, count_multi, rho);
-    return true;
-}
-
 // One run (64 consecutive heights of one cell) on one wavefront, FP64.
 template <int H>
 __device__ __forceinline__ void vll_fp64_run(const VllParams *__restrict__ P, const CellArgs &A, const int2 run,
@@ -338,13 +305,15 @@ __device__ __forceinline__ void vll_fp64_run(const VllParams *__restrict__ P, co
 {
     const long cell = run.x;
     const int hidx = run.y + lane;
+    const smx_nccresult nr = A.ncc[cell];
+    const smx_ugrid g = A.gridpt3[cell];
+    const int start_H = g.minHeight, end_H = g.maxHeight;
+
+    Hyp hy;
+    bool act = project_hypothesis(*P, A, cell, hidx, nr, start_H, end_H, hy);
     double temp_rho = -100.0;
     unsigned evaluated = 0;
     if (A.peak_rho) {
-        const smx_nccresult nr = A.ncc[cell];
-        const smx_ugrid g = A.gridpt3[cell];
-        Hyp hy;
-        const bool act = project_hypothesis(*P, A, cell, hidx, nr, g.minHeight, g.maxHeight, hy);
         // RA / matching-rate pass (setsm_code.cpp:6468-6496): no voxel cube;
         // temp_rho kept for the per-cell FindPeakNcc scan
         double pr = __longlong_as_double(kNotEvaluated);
@@ -368,7 +337,27 @@ __device__ __forceinline__ void vll_fp64_run(const VllParams *__restrict__ P, co
         wave_max_commit(temp_rho, evaluated, &A.max_bits[cell], A.evaluated);
         return;
     }
-    evaluated = fp64_hypothesis<H>(P, A, cell, hidx, temp_rho) ? 1u : 0u;
+    if (act) {
+        const long wi = A.vox_off[cell] + hidx;
+        uint32_t word = A.vox[wi];
+        if (!hy.orient_ok) {
+            A.vox[wi] = word & ~0xffu;   // flag_cal = false (setsm_code.cpp:6503-6506)
+        } else {
+            double sum_multi = 0, count_multi = 0;
+            const bool all3 = multi_ncc_fp64<H>(P->view[0], P->view[1], hy.rpx, hy.rpy, hy.tpx, hy.tpy, hy.cos0,
+                                                hy.sin0, sum_multi, count_multi);
+            if (P->combined_incc)
+                multi_ncc_fp64<H>(P->next[0], P->next[1], hy.rnx, hy.rny, hy.tnx, hy.tny, hy.cos0, hy.sin0,
+                                  sum_multi, count_multi);
+            const double rho = count_multi > 0 ? sum_multi / count_multi : -1.0;
+            word = (word & 0xffu) | (all3 ? 1u : 0u);
+            word |= (uint32_t)(uint16_t)q_voxel(rho) << 16;
+            A.vox[wi] = word;
+            if (A.rho) A.rho[wi] = (float)rho;
+            temp_rho = wncc_of(A, cell, sum_multi, count_multi, rho);
+            evaluated = 1;
+        }
+    }
     wave_max_commit(temp_rho, evaluated, &A.max_bits[cell], A.evaluated);
 }
 
@@ -380,77 +369,17 @@ __global__ void __launch_bounds__(256) k_vll_fp64(const VllParams *__restrict__ 
     vll_fp64_run<H>(P, A, A.runs[wave], wave, threadIdx.x & 63);
 }
 
-// Position of the r-th set bit (r counted from 0) of m, which has more than r.
-__device__ __forceinline__ int nth_set_bit(unsigned long long m, int r)
-{
-    int pos = 0;
-#pragma unroll
-    for (int w = 32; w >= 1; w >>= 1) {
-        const unsigned long long low = (1ull << w) - 1ull;
-        const int c = __popcll(m & low);
-        if (r >= c) {
-            r -= c;
-            m >>= w;
-            pos += w;
-        } else {
-            m &= low;
-        }
-    }
-    return pos;
-}
-
-// The hypotheses the FAST kernel handed over (a window whose spread it cannot
-// tell from FP32 rounding): queued as (run, lane mask), decided here in FP64.
-// A wavefront takes kRedoChunk queued runs and packs their set lanes densely,
-// 64 hypotheses per step, so the FP64 work is per hypothesis, not per run
-// (about a quarter of a handed-over run's lanes are set at level 0). The
-// queue length is read on the device (no host round trip); an empty queue
-// costs one load per wave.
-constexpr int kRedoChunk = 8;
-
+// The runs the FAST kernel handed over (a window whose spread it cannot tell
+// from FP32 rounding), recomputed in full. The count is read on the device,
+// so the launch needs no host round trip; an empty queue costs one load per
+// wave.
 template <int H>
 __global__ void __launch_bounds__(256) k_vll_redo(const VllParams *__restrict__ P, CellArgs A)
 {
-    const int lane = threadIdx.x & 63;
     const long nw = ((long)gridDim.x * blockDim.x) >> 6;
     const long n = (long)*A.redo_n;
-    for (long c0 = (((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * kRedoChunk; c0 < n;
-         c0 += nw * kRedoChunk) {
-        const long q = c0 + lane;
-        const unsigned long long m = (lane < kRedoChunk && q < n) ? A.redo_mask[q] : 0ull;
-        const int2 run = (lane < kRedoChunk && q < n) ? A.redo[q] : make_int2(0, 0);
-        const int cnt = __popcll(m);
-        int incl = cnt;   // inclusive prefix sum of the lane counts
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int v = __shfl_up(incl, d, 64);
-            if (lane >= d) incl += v;
-        }
-        const int total = __shfl(incl, 63, 64);
-        for (int t = 0; t < total; t += 64) {
-            const int k = t + lane;   // this lane's hypothesis among the chunk's
-            int lo = 0, hi = 63;      // first queued run whose prefix passes k
-#pragma unroll
-            for (int s = 0; s < 6; s++) {
-                const int mid = (lo + hi) >> 1;
-                if (__shfl(incl, mid, 64) > k) hi = mid;
-                else lo = mid + 1;
-            }
-            const int j = lo < 63 ? lo : 63;
-            const unsigned long long mj = __shfl(m, j, 64);
-            const int before = __shfl(incl, j, 64) - __popcll(mj);
-            const int rx = __shfl(run.x, j, 64), ry = __shfl(run.y, j, 64);
-            double temp_rho = -100.0;
-            bool ev = false;
-            if (k < total) {
-                const long cell = rx;
-                ev = fp64_hypothesis<H>(P, A, cell, ry + nth_set_bit(mj, k - before), temp_rho);
-                if (ev) atomicMax(&A.max_bits[cell], order_key(temp_rho));
-            }
-            const unsigned long long any = __ballot(ev);
-            if (lane == 0 && any) atomicAdd(A.evaluated, (unsigned long long)__popcll(any));
-        }
-    }
+    for (long q = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < n; q += nw)
+        vll_fp64_run<H>(P, A, A.redo[q], q, threadIdx.x & 63);
 }
 
 template __global__ void k_vll_fp64<5>(const VllParams *, CellArgs);
@@ -891,11 +820,11 @@ hipError_t launch_vll_fp64(int half, const VllParams *dP, const CellArgs &A, hip
 hipError_t launch_vll_redo(int half, const VllParams *dP, const CellArgs &A, hipStream_t s)
 {
     if (A.nruns == 0 || !A.redo) return hipSuccess;
-    // up to 32 waves per CU, kRedoChunk queued runs per wave and step; the
-    // hypotheses are counted in the slot after the queue length
+    // up to 32 waves per CU of the FP64 body over the queue; its hypotheses
+    // are counted in the slot after the queue length
     CellArgs R = A;
     R.evaluated = A.redo_n + 1;
-    const dim3 grid((unsigned)std::min<long>(nblk(A.nruns, 4 * kRedoChunk), 2048)), block(256);
+    const dim3 grid((unsigned)std::min<long>(nblk(A.nruns, 4), 2048)), block(256);
     switch (half) {
     case 5: hipLaunchKernelGGL(k_vll_redo<5>, grid, block, 0, s, dP, R); break;
     case 6: hipLaunchKernelGGL(k_vll_redo<6>, grid, block, 0, s, dP, R); break;

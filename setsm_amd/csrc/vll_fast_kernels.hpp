@@ -223,10 +223,10 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
             else do_ncc = true;
         }
         double part = 0;
-        bool deferred = false;   // the level pass handed this hypothesis to the FP64 pass
         if constexpr (kBelow) {
+            // the level pass handed this run to the FP64 pass: nothing to add
             if (do_ncc) part = A.part[wi];
-            deferred = do_ncc && part == kRedoRho;
+            if (__any(do_ncc && part == kRedoRho)) return;
         }
         const double rpx = kBelow ? hy.rnx : hy.rpx, rpy = kBelow ? hy.rny : hy.rpy;
         const double tpx = kBelow ? hy.tnx : hy.tpx, tpy = kBelow ? hy.tny : hy.tpy;
@@ -249,7 +249,7 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
         bool all3 = false;
         const bool use_lds = __all(!do_ncc || in_box);
         const bool fast_ok = !nodata && __all(!do_ncc || interior);
-        if (do_ncc && !deferred) {
+        if (do_ncc) {
             if (use_lds) {
                 const float fxl = (float)(rpx - flx), fyl = (float)(rpy - fly);
                 const float fxt = (float)(tpx - ftx), fyt = (float)(tpy - fty);
@@ -276,20 +276,21 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
                 all3n = global_window<H, kKahan>(P.next[0], P.next[1], hy.rnx, hy.rny, hy.tnx, hy.tny, cos0, sin0,
                                                  rho_n);
         }
-        // a hypothesis with a window this path cannot decide goes to the FP64
-        // pass, which writes it; the rest of the run is written here
-        const bool redo = deferred || (do_ncc && all3 && rho == kRedoRho) || (all3n && rho_n == kRedoRho);
-        if constexpr (kMode == kLevelPass) {
-            // the level term, kept for the level-below pass (kRedoRho: the
-            // FP64 pass decides the hypothesis)
-            if (do_ncc) A.part[wi] = redo ? kRedoRho : !all3 ? kPartNoWindow : rho == 1e300 ? kPartNoCorr : rho;
+        // a window this path cannot decide: the FP64 pass redoes the whole
+        // run; nothing of it is written here (the cleared flags of
+        // orientation-rejected lanes are rewritten identically there)
+        const bool redo = (do_ncc && all3 && rho == kRedoRho) || (all3n && rho_n == kRedoRho);
+        if (__any(redo)) {
+            if (lane == 0) A.redo[atomicAdd(A.redo_n, 1ull)] = run;
+            if constexpr (kMode == kLevelPass) {
+                if (do_ncc) A.part[wi] = kRedoRho;   // the level-below pass skips the run
+            }
             return;
         }
-        const unsigned long long rmask = __ballot(redo);
-        if (rmask && lane == 0) {
-            const unsigned long long q = atomicAdd(A.redo_n, 1ull);
-            A.redo[q] = run;
-            A.redo_mask[q] = rmask;
+        if constexpr (kMode == kLevelPass) {
+            // the level term, kept for the level-below pass
+            if (do_ncc) A.part[wi] = !all3 ? kPartNoWindow : rho == 1e300 ? kPartNoCorr : rho;
+            return;
         }
         // setsm_code.cpp:6456-6493: average of the level term and, on even
         // iterations above level 0, the level-1 term (in that order).
@@ -311,7 +312,7 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
             sum += rho_n;
             cnt += 1;
         }
-        if (do_ncc && !redo) {
+        if (do_ncc) {
             const double rr = cnt > 0 ? sum / cnt : -1.0;
             word = (word & 0xffu) | (all3 ? 1u : 0u);
             word |= (uint32_t)(uint16_t)q_voxel(rr) << 16;
