@@ -195,8 +195,10 @@ __device__ __forceinline__ void box_stage(const DevView *views, TileBox &tb)
     if (nod) tb.nodata = 1;   // benign race: every writer stores 1
 }
 
-// One run (64 consecutive heights of one cell) on one wavefront.
-template <int H, bool kKahan, int kMode>
+// One run (64 consecutive heights of one cell) on one wavefront. kCombined:
+// a single-pass launch may meet a combined-level iteration (the persistent
+// kernel never does: batches hold no such iteration).
+template <int H, bool kKahan, int kMode, bool kCombined = true>
 __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs &A, const FastArgs &F,
                                              const TileBox &tb, const bool fits, const bool nodata, const long ri,
                                              const int lane)
@@ -271,7 +273,7 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
         if (lane == 0 && !use_lds && __any(do_ncc)) atomicAdd(A.accessible + 3, 1ull);
         double rho_n = 0;
         bool all3n = false;
-        if constexpr (kMode == kSingle) {
+        if constexpr (kMode == kSingle && kCombined) {
             if (do_ncc && P.combined_incc)
                 all3n = global_window<H, kKahan>(P.next[0], P.next[1], hy.rnx, hy.rny, hy.tnx, hy.tny, cos0, sin0,
                                                  rho_n);
@@ -324,7 +326,7 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
         wave_max_commit(temp_rho, evaluated, &A.max_bits[cell], A.evaluated);
     }
 
-template <int H, bool kKahan, int kMode = kSingle>
+template <int H, bool kKahan, int kMode = kSingle, bool kCombined = true>
 __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, const CellArgs &A, const FastArgs &F,
                                               const long tile)
 {
@@ -362,7 +364,7 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
         __syncthreads();
         const bool nodata = tb.nodata != 0;
         for (long ri = r0 + wave; ri < r1; ri += nwaves)
-            vll_fast_run<H, kKahan, kMode>(P, A, F, tb, fits, nodata, ri, lane);
+            vll_fast_run<H, kKahan, kMode, kCombined>(P, A, F, tb, fits, nodata, ri, lane);
         return;
     }
     // 2. Tall footprints (wide height bands, strong convergence): stage one
@@ -393,7 +395,7 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
         if (fits) box_stage(views, tb);
         __syncthreads();
         const bool nodata = tb.nodata != 0;
-        if (c0 + wave < r1) vll_fast_run<H, kKahan, kMode>(P, A, F, tb, fits, nodata, c0 + wave, lane);
+        if (c0 + wave < r1) vll_fast_run<H, kKahan, kMode, kCombined>(P, A, F, tb, fits, nodata, c0 + wave, lane);
     }
 }
 
@@ -435,7 +437,7 @@ __device__ __forceinline__ void vll_fast_persistent(const BatchArgs &B)
             int j = 0;
             while (j + 1 < B.njobs && B.jobs[j + 1].tile_begin <= t) j++;
             const BatchJob &J = B.jobs[j];
-            vll_fast_body<H, kKahan>(J.P, J.A, J.F, t - J.tile_begin);
+            vll_fast_body<H, kKahan, kSingle, false>(J.P, J.A, J.F, t - J.tile_begin);
         }
     }
 }
