@@ -44,11 +44,15 @@ extern "C" {
 /* Arithmetic of the VerticalLineLocus kernel. */
 #define SMX_PREC_FP64     0   /* plain FP64, same operation order as SETSM:
                                  INCC cube bit-identical to the CPU path      */
-#define SMX_PREC_FAST     1   /* FP16/uint16 texels staged in LDS, FP64 RPC,
-                                 FP32 pivot-shifted moments with compensated
-                                 summation; |rho - rho_fp64| <= 1e-5          */
+#define SMX_PREC_FAST     1   /* 11-bit texels staged in LDS as FP32 float2
+                                 (intensity, magnitude) about a pivot, FP64
+                                 RPC, FP32 bilinear taps and moments, each
+                                 window row's FP32 partials summed into FP64
+                                 window totals, FP64 Pearson finish;
+                                 |rho - rho_fp64| <= 1e-5                     */
 
 typedef struct smx_job smx_job;
+typedef struct smx_graph smx_graph;
 
 const char *smx_last_error(void);
 int  smx_version(void);
@@ -77,7 +81,8 @@ int  smx_job_set_view(smx_job *job, int img, int level,
 /* Build levels 0..num_levels-1 of image `img` on the GPU from its level-0
  * intensity (9x9 sigma-1.5 Gaussian decimation, Sobel magnitude, 18-bin
  * orientation: Template.hpp:82-132, SubFunctions.cpp:2418-2571).
- * start_x0/start_y0 are the level-0 subset start (Startpos_ori). */
+ * start_x0/start_y0 are the level-0 subset start (Startpos_ori). level0 may
+ * be host or device memory. */
 int  smx_job_build_pyramid(smx_job *job, int img, const uint16_t *level0,
                            uint32_t width, uint32_t height, double start_x0, double start_y0);
 
@@ -121,12 +126,37 @@ int  smx_initialize_voxel(smx_job *job, int iteration);
 long long smx_vertical_line_locus(smx_job *job, int iteration, int precision, int keep_rho);
 
 /* VerticalLineLocus over several jobs (tiles of different sizes, height
- * ranges and pyramids) at one window size. SMX_PREC_FAST runs ONE persistent
- * launch that takes the tiles of all jobs from a work queue;
- * SMX_PREC_FP64 launches the plain kernel once per job. accessible[i]
- * receives job i's Accessable_grid. Not for combined-level iterations. */
+ * ranges, pyramids and levels) at one window size. SMX_PREC_FAST runs ONE
+ * persistent launch that takes the tiles of all jobs from a work queue (jobs
+ * at a combined-level iteration, setsm_code.cpp:6087-6088, in two more: the
+ * level pass and the level-below pass over all such jobs); SMX_PREC_FP64
+ * launches the plain kernel once per job. accessible[i] receives job i's
+ * Accessable_grid. Every argument is checked before anything is queued. */
 int  smx_vertical_line_locus_batch(smx_job **jobs, int n, int iteration, int precision, int keep_rho,
                                    long long *accessible);
+
+/* Half_template_size of the job's current level (setsm_code.cpp:6057-6077):
+ * the window size a batch groups jobs by. */
+int  smx_job_half(const smx_job *job);
+
+/* ---------------------------------------------------- captured sequences */
+/* Capture one iteration's VerticalLineLocus (+ AWNCC when with_awncc) over
+ * jobs[0..n) as a hipGraph: per job the GNCC geometry, gate, run-list scan and
+ * fill; the NCC kernel (SMX_PREC_FAST with n > 1: the batched persistent
+ * launch); the FP64 hand-over pass; max_WNCC; then SGM and the peak scan.
+ * Nothing in it reads back to the host or allocates; buffers are sized from
+ * the height lists InitializeVoxel left. A graph stays valid until a job's
+ * level, search space, cube layout, buffers or launch shape change
+ * (smx_graph_launch then fails with SMX_E_STATE). Not for RA or
+ * matching-rate passes. No counterpart in the reference (its calls run
+ * eagerly on the CPU). */
+int  smx_graph_create(smx_graph **out, smx_job **jobs, int n, int iteration, int precision, int with_awncc);
+/* Replay; waits for it and fills accessible[i] (may be NULL) and each job's
+ * last_evaluated / redo statistics. */
+int  smx_graph_launch(smx_graph *graph, long long *accessible);
+/* Milliseconds of the last replay (HIP events around it on the stream). */
+double smx_graph_last_ms(const smx_graph *graph);
+int  smx_graph_destroy(smx_graph *graph);
 
 /* AWNCC (setsm_code.cpp:7511-7702, setsm_code.hpp:91): at levels
  * <= SGM_py the SGM 8-path aggregation first (setsm_code.cpp:7017-7509,

@@ -20,11 +20,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TUNING_PATH = os.path.join(HERE, "tuning.json")
 
 
-def tune_fast(job, iteration=1, reps=2, log=None, only_kahan=False):
+def tune_fast(job, iteration=1, reps=2, log=None, only_wide=False):
     from .smx import PREC_FAST, fast_configs
     rows = []
     for i, desc in fast_configs():
-        if only_kahan and not desc[5]:
+        if only_wide and not desc[5]:
             continue
         job.set_fast_config(i)
         job.vertical_line_locus(iteration, precision=PREC_FAST)   # warm-up
@@ -34,19 +34,20 @@ def tune_fast(job, iteration=1, reps=2, log=None, only_kahan=False):
             ms = job.last_kernel_ms
             best = ms if best is None else min(best, ms)
         st = job.last_stats
-        rows.append(dict(config=i, tx=desc[0], ty=desc[1], lds_texels=desc[2], occ=desc[3], stage=desc[4], kahan=desc[5],
+        rows.append(dict(config=i, tx=desc[0], ty=desc[1], lds_texels=desc[2], occ=desc[3], stage=desc[4], wide=desc[5],
                          kernel_ms=best, tiles_lds=st["tiles_lds"], waves_hbm=st["waves_hbm"]))
         if log:
             log(f"tune config {i} {desc}: {best:.2f} ms (tiles in LDS {st['tiles_lds']}, HBM waves {st['waves_hbm']})")
     rows.sort(key=lambda r: r["kernel_ms"])
-    pick = [r for r in rows if r["kahan"]] or rows   # compensated summation is the contract
+    pick = [r for r in rows if r["wide"]] or rows   # FP64 window totals are the precision contract
     job.set_fast_config(pick[0]["config"])
     return pick[0], rows
 
 
 def tune_batch(jobs, iteration=1, reps=2, log=None):
     """The same table for one persistent launch over several jobs (mixed tile
-    sizes): every compensated, LDS-staged shape, timed by the batched call."""
+    sizes): every LDS-staged shape with FP64 window totals, timed by the
+    batched call."""
     from .smx import PREC_FAST, fast_configs, vertical_line_locus_batch
     rows = []
     for i, desc in fast_configs():
@@ -60,7 +61,7 @@ def tune_batch(jobs, iteration=1, reps=2, log=None):
             vertical_line_locus_batch(jobs, iteration, precision=PREC_FAST)
             ms = jobs[0].last_kernel_ms
             best = ms if best is None else min(best, ms)
-        rows.append(dict(config=i, tx=desc[0], ty=desc[1], lds_texels=desc[2], occ=desc[3], stage=desc[4], kahan=desc[5],
+        rows.append(dict(config=i, tx=desc[0], ty=desc[1], lds_texels=desc[2], occ=desc[3], stage=desc[4], wide=desc[5],
                          kernel_ms=best))
         if log:
             log(f"batch tune config {i} {desc}: {best:.2f} ms")
@@ -82,7 +83,7 @@ def load(key):
         return None
     from .smx import fast_configs
     b = entry["best"]
-    want = (b["tx"], b["ty"], b["lds_texels"], b["occ"], b["stage"], b["kahan"])
+    want = (b["tx"], b["ty"], b["lds_texels"], b["occ"], b["stage"], b["wide"])
     for i, desc in fast_configs():
         if tuple(desc) == want:
             entry["best"] = dict(b, config=i)

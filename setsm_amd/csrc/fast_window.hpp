@@ -133,26 +133,27 @@ SMX_FN void mom_add(Mom &s, float li, float ri, float lm, float rm)
 }
 
 // Window totals: sums of the row partials; the squared / product terms in
-// FP64 when compensated (the row partials themselves are short FP32 sums),
-// FP32 otherwise.
-template <bool kKahan>
+// FP64 when kWide (the row partials themselves are short FP32 sums, at most
+// 2H+1 samples each), FP32 otherwise. No compensation term is carried: the
+// FP64 totals make the rounding of the additions negligible.
+template <bool kWide>
 struct Tot {
-    using Q = typename std::conditional<kKahan, double, float>::type;
+    using Q = typename std::conditional<kWide, double, float>::type;
     float a, b, c, d, n;
     Q aa, bb, ab, cc, dd, cd;
 };
 
-template <bool kKahan>
-SMX_FN void tot_zero(Tot<kKahan> &t)
+template <bool kWide>
+SMX_FN void tot_zero(Tot<kWide> &t)
 {
     t.a = t.b = t.c = t.d = t.n = 0.f;
     t.aa = t.bb = t.ab = t.cc = t.dd = t.cd = 0;
 }
 
-template <bool kKahan>
-SMX_FN void mom_fold(Tot<kKahan> &t, const Mom &r, float rn)
+template <bool kWide>
+SMX_FN void mom_fold(Tot<kWide> &t, const Mom &r, float rn)
 {
-    using Q = typename Tot<kKahan>::Q;
+    using Q = typename Tot<kWide>::Q;
     t.a += r.a;
     t.b += r.b;
     t.c += r.c;
@@ -178,7 +179,7 @@ SMX_FN void mom_fold(Tot<kKahan> &t, const Mom &r, float rn)
 // walk passes that window's half-width on the row (the windows are nested
 // and each is a symmetric column range on a row), so every sample updates
 // one fixed set of registers.
-template <int H, bool kCheck, bool kKahan, class SL, class SR>
+template <int H, bool kCheck, bool kWide, class SL, class SR>
 SMX_FN bool fast_window(const SL &L, const SR &R, float fxl, float fyl, float ftx, float fty,
                                          float cos0, float sin0, float piv_l, float piv_r, float lim_lx0,
                                          float lim_lx1, float lim_ly0, float lim_ly1, float lim_rx0,
@@ -186,7 +187,7 @@ SMX_FN bool fast_window(const SL &L, const SR &R, float fxl, float fyl, float ft
 {
     constexpr int O = H + 2;   // origin offset
     constexpr RowWidths<H> kW{};
-    Tot<kKahan> tot[3];
+    Tot<kWide> tot[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) tot_zero(tot[k]);
     // Per-hypothesis pivots: the window-centre samples. Moments are taken on
@@ -281,7 +282,7 @@ SMX_FN bool fast_window(const SL &L, const SR &R, float fxl, float fyl, float ft
             if (w0 >= 0) mom_fold(tot[0], run, rn);
         }
     }
-    // window totals (Kahan compensation folded back in FP64)
+    // window totals (FP64 sums of the FP32 row partials when kWide)
     double n[3], S[3][10];
 #pragma unroll
     for (int w = 0; w < 3; w++) {

@@ -90,12 +90,14 @@ int job_alloc(smx_job *j, void **p, size_t bytes)
     }
     j->device_bytes += (long long)cls;
     j->sizes.push_back({*p, cls});
+    j->generation++;   // captured launch sequences refer to the old buffers
     return 0;
 }
 
 void job_free(smx_job *j, void *p)
 {
     if (!p) return;
+    j->generation++;
     size_t cls = 0;
     for (size_t i = 0; i < j->sizes.size(); i++)
         if (j->sizes[i].first == p) {
@@ -130,42 +132,22 @@ static void jfree(smx_job *j, T *&p)
     p = nullptr;
 }
 
-int job_scan(smx_job *j, const int64_t *in, int64_t *out, long n)
+hipError_t job_scan(const VllScratch &S, const int64_t *in, int64_t *out, long n, hipStream_t s)
 {
-    size_t need = 0;
-    HIPCHK(scan_i64(in, out, n, nullptr, need, j->stream));
-    if (need > j->scan_tmp_bytes) {
-        jfree(j, j->scan_tmp);
-        int r = jalloc(j, (char **)&j->scan_tmp, need);
-        if (r) return r;
-        j->scan_tmp_bytes = need;
-    }
-    size_t have = j->scan_tmp_bytes;
-    HIPCHK(scan_i64(in, out, n, j->scan_tmp, have, j->stream));
-    return 0;
+    if (n > scan_max_n() || scan_partials(n) > S.scan_part_cap) return hipErrorInvalidValue;
+    return scan_i64(in, out, n, S.scan_part, s);
 }
 
-int fast_scratch(smx_job *j, int slot, long ntiles, long nruns)
+// Grow a job buffer to hold `need` elements (old contents are not kept).
+template <class T, class C>
+static int grow(smx_job *j, T *&p, C &cap, long long need)
 {
-    int64_t *&count = slot ? j->d_tile2_count : j->d_tile_count;
-    int64_t *&off = slot ? j->d_tile2_off : j->d_tile_off;
-    int2 *&runs = slot ? j->d_tile2_runs : j->d_tile_runs;
-    long &tcap = slot ? j->tiles2_cap : j->tiles_cap;
-    long &rcap = slot ? j->tile2_runs_cap : j->tile_runs_cap;
-    if (ntiles + 1 > tcap) {
-        jfree(j, count);
-        jfree(j, off);
-        int r = jalloc(j, &count, (size_t)ntiles + 1);
-        if (!r) r = jalloc(j, &off, (size_t)ntiles + 1);
-        if (r) return r;
-        tcap = ntiles + 1;
-    }
-    if (nruns > rcap) {
-        jfree(j, runs);
-        int r = jalloc(j, &runs, (size_t)nruns);
-        if (r) return r;
-        rcap = nruns;
-    }
+    if (p && (long long)cap >= need) return 0;
+    jfree(j, p);
+    cap = 0;
+    int r = jalloc(j, &p, (size_t)std::max(need, 1LL));
+    if (r) return r;
+    cap = (C)need;
     return 0;
 }
 
@@ -179,18 +161,25 @@ static void free_level_state(smx_job *j)
     jfree(j, j->d_rho);
     jfree(j, j->d_run_count);
     jfree(j, j->d_run_off);
-    jfree(j, j->d_runs);
     jfree(j, j->d_max_bits);
     jfree(j, j->d_action);
     jfree(j, j->d_new_len);
     jfree(j, j->d_new_off);
-    jfree(j, j->d_tile_count);
-    jfree(j, j->d_tile_off);
-    jfree(j, j->d_tile_runs);
-    jfree(j, j->d_tile2_count);
-    jfree(j, j->d_tile2_off);
-    jfree(j, j->d_tile2_runs);
-    j->tiles2_cap = j->tile2_runs_cap = 0;
+    VllScratch &S = j->js;
+    jfree(j, S.runs);
+    S.runs_cap = 0;
+    jfree(j, S.tile_count);
+    jfree(j, S.tile_off);
+    jfree(j, S.tile_runs);
+    S.tiles_cap = 0;
+    jfree(j, S.tile2_count);
+    jfree(j, S.tile2_off);
+    jfree(j, S.tile2_runs);
+    S.tiles2_cap = 0;
+    jfree(j, S.redo);
+    jfree(j, S.part);
+    S.part_cap = 0;
+    j->cap_tile_runs = j->cap_tile2_runs = j->cap_redo = 0;
     jfree(j, j->d_sum);
     j->sum_total = 0;
     jfree(j, j->d_grid_pts);
@@ -200,20 +189,17 @@ static void free_level_state(smx_job *j)
     jfree(j, j->d_gval);
     jfree(j, j->d_peak_rho);
     j->peak_cap = 0;
-    jfree(j, j->d_redo);
-    j->redo_cap = 0;
-    jfree(j, j->d_part);
-    j->part_cap = 0;
     j->cd_bytes = j->cdn_bytes = 0;
-    j->tiles_cap = j->tile_runs_cap = 0;
-    j->runs_cap = 0;
     j->vox_total = 0;
     j->level_set = false;
     j->searchspace_set = false;
+    j->stats_valid = false;
+    j->runs_bound = 0;
+    j->max_nh = 0;
 }
 
-// Fill the per-call VllParams and copy it to the device.
-static int upload_params(smx_job *j, int iteration)
+// The per-call VllParams (host copy), for iteration `iteration`.
+static void fill_params(smx_job *j, int iteration)
 {
     VllParams &P = j->hp;
     const smx_level &L = j->lv;
@@ -236,7 +222,15 @@ static int upload_params(smx_job *j, int iteration)
             P.next[ti] = DevView{nullptr, nullptr, 0, 0, 0.0, 0.0};
         }
     }
-    HIPCHK(hipMemcpyAsync(j->d_params, &P, sizeof(VllParams), hipMemcpyHostToDevice, j->stream));
+}
+
+// Copy the host VllParams to the device through the job's pinned staging
+// block (every call waits for its stream before returning, so the block is
+// free again by the next call).
+static int put_params(smx_job *j, hipStream_t s)
+{
+    *j->h_params = j->hp;
+    HIPCHK(hipMemcpyAsync(j->d_params, j->h_params, sizeof(VllParams), hipMemcpyHostToDevice, s));
     return 0;
 }
 
@@ -274,14 +268,348 @@ int vll_half_template(int level, int template_size, double resolution)
     return half;
 }
 
+// Statistics of the current height lists (the bounds the launch sequences
+// size their scratch by), recomputed after the lists changed outside
+// InitializeVoxel.
+static int refresh_stats(smx_job *j)
+{
+    unsigned long long *st = j->js.counters + 6;
+    HIPCHK(launch_fill_u64(st, 0, 2, j->stream));
+    HIPCHK(launch_nh_stats(j->d_ncc, j->ncells, st, j->stream));
+    HIPCHK(hipMemcpyAsync(j->h_small, st, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, j->stream));
+    HIPCHK(hipStreamSynchronize(j->stream));
+    j->runs_bound = j->h_small[0];
+    j->max_nh = (int)j->h_small[1];
+    j->stats_valid = true;
+    return 0;
+}
+
+// ---------------------------------------------------- VerticalLineLocus
+
+// Host-side decisions of one VerticalLineLocus call.
+struct VllPlan {
+    int precision = SMX_PREC_FP64;
+    bool keep_rho = false;
+    bool gncc_on = false;        // the GNCC term is computed this call
+    bool combined_wncc = false;  // check_combined_WNCC (setsm_code.cpp:6081-6088)
+    bool ra = false;             // RA / matching-rate pass (no cube; FindPeakNcc inline)
+    bool fast = false;           // SMX_PREC_FAST kernels
+    int half = 5;
+    int slot = 1;                // GridPT3.ortho_ncc index of the target image
+    double ortho_th = 0;
+};
+
+// Validates a call and fills j->hp for it; queues nothing.
+static int vll_plan(smx_job *j, int iteration, int precision, int keep_rho, VllPlan &p)
+{
+    if (!j || !j->level_set || !j->searchspace_set) return fail(SMX_E_STATE, "level and search space must be set");
+    if (iteration < 1) return fail(SMX_E_ARG, "iteration starts at 1");
+    if (precision != SMX_PREC_FP64 && precision != SMX_PREC_FAST) return fail(SMX_E_ARG, "unknown precision %d", precision);
+    const smx_level &L = j->lv;
+    p = VllPlan{};
+    p.precision = precision;
+    p.keep_rho = keep_rho != 0;
+    // setsm_code.cpp:6100-6112 check_ortho, 6240 the per-cell GNCC gate
+    bool check_ortho = !((L.level == 4 && iteration == 1) || L.is_ra);
+    if (L.pre_dem && !(L.level == 4 && iteration == 1)) check_ortho = true;
+    if (L.check_matchtag) check_ortho = true;
+    if (L.level <= L.sgm_py) check_ortho = false;
+    p.ortho_th = 0.7 - (4 - L.level) * 0.10;
+    p.slot = 1 - L.reference_id;
+    if (p.slot < 0 || p.slot > 1) return fail(SMX_E_ARG, "reference_id must be 0 or 1");
+    p.gncc_on = check_ortho && (double)j->max_ortho_ncc[p.slot] / 1000.0 > p.ortho_th;
+    p.combined_wncc = !L.is_ra && (L.level > L.py_combined_level);
+    if (p.gncc_on) {
+        if (!j->d_ortho_wgs || !j->d_grid_pts || !j->d_cd)
+            return fail(SMX_E_ARG, "the GNCC term is active (ortho_ncc > %.2f) but the level has no ortho_wgs / "
+                                   "grid_pts", p.ortho_th);
+        if (p.combined_wncc) {
+            for (int ti = 0; ti < 2; ti++)
+                if (!j->views[ti][L.level - 1].texel)
+                    return fail(SMX_E_STATE, "the combined GNCC term needs level %d", L.level - 1);
+            if (!j->d_cdn) return fail(SMX_E_STATE, "no level-below ortho coordinates");
+        }
+    }
+    p.half = j->hp.half;
+    if (p.half < 5 || p.half > 7) return fail(SMX_E_UNSUPPORTED, "Half_template_size %d not built (5..7 are)", p.half);
+    fill_params(j, iteration);
+    if (j->hp.combined_incc)
+        for (int ti = 0; ti < 2; ti++)
+            if (!j->views[ti][L.level - 1].texel) return fail(SMX_E_STATE, "combined INCC needs level %d", L.level - 1);
+    p.ra = L.is_ra || L.check_matching_rate;
+    p.fast = precision != SMX_PREC_FP64 && !p.ra;
+    return 0;
+}
+
+// Grows the job's own scratch for the call (allocations only; the one queued
+// kernel is the NaN fill of a newly allocated mean-NCC array).
+static int vll_reserve(smx_job *j, const VllPlan &p)
+{
+    int r = 0;
+    if (!j->stats_valid && (r = refresh_stats(j))) return r;
+    VllScratch &S = j->js;
+    if ((r = grow(j, S.runs, S.runs_cap, j->runs_bound))) return r;
+    if (p.fast) {
+        const long t1 = fast_tiles(j, 0);
+        if (t1 > S.tiles_cap || !S.tile_count) {
+            jfree(j, S.tile_count);
+            jfree(j, S.tile_off);
+            S.tiles_cap = 0;
+            if ((r = jalloc(j, &S.tile_count, (size_t)t1 + 1)) || (r = jalloc(j, &S.tile_off, (size_t)t1 + 1))) return r;
+            S.tiles_cap = t1;
+        }
+        if ((r = grow(j, S.tile_runs, j->cap_tile_runs, S.runs_cap))) return r;
+        if ((r = grow(j, S.redo, j->cap_redo, S.runs_cap))) return r;
+        if (j->hp.combined_incc) {
+            const long t2 = fast_tiles(j, 1);
+            if (t2 > S.tiles2_cap || !S.tile2_count) {
+                jfree(j, S.tile2_count);
+                jfree(j, S.tile2_off);
+                S.tiles2_cap = 0;
+                if ((r = jalloc(j, &S.tile2_count, (size_t)t2 + 1)) || (r = jalloc(j, &S.tile2_off, (size_t)t2 + 1)))
+                    return r;
+                S.tiles2_cap = t2;
+            }
+            if ((r = grow(j, S.tile2_runs, j->cap_tile2_runs, S.runs_cap))) return r;
+            if ((r = grow(j, S.part, S.part_cap, j->vox_total))) return r;
+        }
+    }
+    if (p.keep_rho && !j->d_rho && j->vox_total > 0) {
+        if ((r = jalloc(j, &j->d_rho, (size_t)j->vox_total))) return r;
+        HIPCHK(launch_fill_u32((uint32_t *)j->d_rho, 0xffffffffu, j->vox_total, j->stream));   // NaN
+    }
+    return 0;
+}
+
+// The kernel arguments of a call over scratch S (host only).
+static void vll_args(smx_job *j, const VllPlan &p, const VllScratch &S, CellArgs &A, OrthoArgs &O)
+{
+    const smx_level &L = j->lv;
+    A = CellArgs{};
+    A.ncells = j->ncells;
+    A.gridpt3 = j->d_gridpt3;
+    A.ncc = j->d_ncc;
+    A.grid_wgs = j->d_grid_wgs;
+    A.vox_off = j->d_vox_off;
+    A.vox = j->d_vox;
+    A.rho = p.keep_rho ? j->d_rho : nullptr;
+    A.run_count = j->d_run_count;
+    A.run_offset = j->d_run_off;
+    A.runs = S.runs;
+    A.runs_cap = S.runs_cap;
+    A.nruns_dev = j->d_run_off + j->ncells;
+    A.max_bits = j->d_max_bits;
+    A.accessible = S.counters;
+    A.evaluated = S.counters + 1;
+    A.ncc_alpha = j->hp.ncc_alpha;
+    A.redo_n = S.counters + 4;
+    if (p.fast) {
+        A.redo = S.redo;
+        if (j->hp.combined_incc) A.part = S.part;
+    }
+    if (p.ra) A.peak_rho = j->d_peak_rho;
+    O = OrthoArgs{};
+    if (!p.gncc_on) return;
+    // setsm_code.cpp:6121-6129: ortho image coordinates of this iteration's
+    // surface (GridPT3.Height)
+    const double im_res = L.resolution * (double)(1 << L.level);
+    const double im_res_next = p.combined_wncc ? L.resolution * (double)(1 << (L.level - 1)) : L.resolution;
+    O.W = L.grid_w;
+    O.H = L.grid_h;
+    O.sw = L.ortho_w;
+    O.sh = L.ortho_h;
+    if (p.combined_wncc) {
+        O.swn = (long)((L.boundary[2] - L.boundary[0]) / im_res_next) + 1;
+        O.shn = (long)((L.boundary[3] - L.boundary[1]) / im_res_next) + 1;
+    }
+    O.b0 = L.boundary[0];
+    O.b1 = L.boundary[1];
+    O.im_res = im_res;
+    O.im_res_next = im_res_next;
+    O.grid_res = L.grid_resolution;
+    O.combined = p.combined_wncc ? 1 : 0;
+    O.slot = p.slot;
+    O.ortho_th = p.ortho_th;
+    O.grid_pts = j->d_grid_pts;
+    O.ortho_wgs = j->d_ortho_wgs;
+    O.gridpt3 = j->d_gridpt3;
+    O.cd[0] = j->d_cd;
+    O.cd[1] = j->d_cd + (size_t)O.sw * O.sh * 2;
+    O.cdn[0] = O.swn ? j->d_cdn : nullptr;
+    O.cdn[1] = O.swn ? j->d_cdn + (size_t)O.swn * O.shn * 2 : nullptr;
+    A.acc_flag = j->d_action;
+    A.gval = j->d_gval;
+}
+
+// Queues everything up to the NCC kernel: GNCC geometry, counter clear,
+// gate, GNCC term, run-list scan and fill. An RA / matching-rate pass (the
+// gate recomputes NumOfHeight from the search space) reads the run count
+// back and grows the job's scratch: only where may_grow (eager calls on the
+// job's own scratch).
+static int vll_front(smx_job *j, const VllPlan &p, VllScratch &S, bool may_grow, hipStream_t s, CellArgs &A,
+                     OrthoArgs &O)
+{
+    if (p.gncc_on) {
+        // calloc'd in the reference: pixels it skips read as (0, 0)
+        HIPCHK(launch_fill_u64((unsigned long long *)j->d_cd, 0, (long)O.sw * O.sh * 4, s));
+        if (O.swn) HIPCHK(launch_fill_u64((unsigned long long *)j->d_cdn, 0, (long)O.swn * O.shn * 4, s));
+        HIPCHK(launch_ortho_coords(j->d_params, O, s));
+    }
+    HIPCHK(launch_fill_u64(S.counters, 0, kCounters, s));
+    HIPCHK(launch_vll_gate(j->d_params, A, s));
+    if (p.gncc_on) HIPCHK(launch_gncc(p.half, j->d_params, O, A, j->d_gval, s));
+    HIPCHK(job_scan(S, j->d_run_count, j->d_run_off, j->ncells + 1, s));
+    if (p.ra) {
+        if (!may_grow)
+            return fail(SMX_E_UNSUPPORTED, "RA and matching-rate passes size their run list from a count read back "
+                                           "mid-call: use smx_vertical_line_locus");
+        HIPCHK(hipMemcpyAsync(j->h_small, j->d_run_off + j->ncells, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        const long nruns = (long)j->h_small[0];
+        int r = grow(j, S.runs, S.runs_cap, nruns);
+        if (!r && nruns > j->peak_cap) {
+            jfree(j, j->d_peak_rho);
+            j->peak_cap = 0;
+            r = jalloc(j, &j->d_peak_rho, (size_t)std::max(nruns, 1L) * kRun);
+            if (!r) j->peak_cap = nruns;
+        }
+        if (r) return r;
+        A.runs = S.runs;
+        A.runs_cap = S.runs_cap;
+        A.peak_rho = j->d_peak_rho;
+        j->stats_valid = false;   // NumOfHeight now follows the search space
+    }
+    HIPCHK(launch_vll_fill_runs(A, s));
+    return 0;
+}
+
+// The NCC kernel(s) of a single-job call: FP64, or FAST plus the FP64 pass
+// over the runs it hands over. With `timed`, HIP events bracket the kernel.
+static int vll_kernels(smx_job *j, const VllPlan &p, const CellArgs &A, const VllScratch &S, hipStream_t s, bool timed)
+{
+    if (!p.fast) {
+        if (timed) HIPCHK(hipEventRecord(j->ev0, s));
+        HIPCHK(launch_vll_fp64(p.half, j->d_params, A, s));
+        if (timed) HIPCHK(hipEventRecord(j->ev1, s));
+        if (A.peak_rho && j->lv.check_matching_rate) HIPCHK(launch_find_peak(j->d_params, A, s));
+        return 0;
+    }
+    int r = launch_vll_fast(j, p.half, A, S, s, timed ? j->ev0 : nullptr, timed ? j->ev1 : nullptr);
+    if (r) return r;
+    HIPCHK(launch_vll_redo(p.half, j->d_params, A, s));
+    if (timed) HIPCHK(hipEventRecord(j->ev2, s));
+    return 0;
+}
+
+// Waits for stream s and reads the call's counters. Returns Accessable_grid.
+static long long vll_finish(smx_job *j, const unsigned long long *counters, hipStream_t s)
+{
+    HIPCHK(hipMemcpyAsync(j->h_small, counters, kCounters * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    j->last_evaluated = j->h_small[1] + j->h_small[5];
+    j->last_redo_hyps = j->h_small[5];
+    j->last_tiles_lds = j->h_small[2];
+    j->last_waves_hbm = j->h_small[3];
+    j->last_redo_runs = j->h_small[4];
+    return (long long)j->h_small[0];
+}
+
+static int read_times(smx_job *j, bool fast)
+{
+    float ms = 0, rms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, j->ev0, j->ev1));
+    if (fast) HIPCHK(hipEventElapsedTime(&rms, j->ev1, j->ev2));
+    j->last_ms = ms;
+    j->last_redo_ms = rms;
+    return 0;
+}
+
+// One call on one job over scratch S, queued on the job's stream.
+static long long vll_run(smx_job *j, const VllPlan &p, VllScratch &S, bool may_grow)
+{
+    int r = put_params(j, j->stream);
+    if (r) return r;
+    CellArgs A;
+    OrthoArgs O;
+    vll_args(j, p, S, A, O);
+    if ((r = vll_front(j, p, S, may_grow, j->stream, A, O))) return r;
+    if ((r = vll_kernels(j, p, A, S, j->stream, true))) return r;
+    HIPCHK(launch_vll_finalize(A, j->stream));
+    const long long acc = vll_finish(j, S.counters, j->stream);
+    if (acc < 0) return acc;
+    if ((r = read_times(j, p.fast))) return r;
+    return acc;
+}
+
+// -------------------------------------------------------------- AWNCC
+
+static bool awncc_sgm(const smx_job *j) { return j->lv.level <= j->lv.sgm_py; }
+
+static int sgm_cap(const smx_job *j) { return (int)((std::max(j->max_nh, 1) + 3) & ~3); }
+
+// Buffers of the SGM aggregation (levels <= SGM_py): SumCost and, when the
+// path buffers exceed LDS, their global scratch.
+static int awncc_reserve(smx_job *j)
+{
+    if (!awncc_sgm(j)) return 0;
+    int r = 0;
+    if (!j->stats_valid && (r = refresh_stats(j))) return r;
+    if ((r = grow(j, j->d_sum, j->sum_total, j->vox_total))) return r;
+    const int cap = sgm_cap(j);
+    if ((size_t)10 * cap > sgm_lds_limit()) {
+        const size_t need = (size_t)10 * cap * (size_t)std::max(j->lv.grid_w, j->lv.grid_h);
+        if ((r = grow(j, j->d_sgm_scratch, j->sgm_scratch_bytes, (long long)need))) return r;
+    }
+    return 0;
+}
+
+// setsm_code.cpp:7017-7702: SGM (levels <= SGM_py) and the peak scan.
+static int awncc_enqueue(smx_job *j, hipStream_t s)
+{
+    const bool sgm = awncc_sgm(j);
+    if (sgm) {
+        HIPCHK(launch_fill_u32((uint32_t *)j->d_sum, 0u, j->vox_total, s));
+        SgmLaunch L{};
+        L.W = (int)j->lv.grid_w;
+        L.H = (int)j->lv.grid_h;
+        L.step = j->lv.height_step;
+        L.ncc = j->d_ncc;
+        L.gridpt3 = j->d_gridpt3;
+        L.vox_off = j->d_vox_off;
+        L.vox = j->d_vox;
+        L.sum = j->d_sum;
+        L.cap = sgm_cap(j);
+        L.gscratch = (size_t)10 * L.cap > sgm_lds_limit() ? j->d_sgm_scratch : nullptr;
+        HIPCHK(launch_sgm(L, s));
+    }
+    PeakArgs A{j->ncells, j->d_ncc, j->d_gridpt3, j->d_vox_off, j->d_vox, sgm ? j->d_sum : nullptr};
+    HIPCHK(launch_awncc(j->d_params, A, s));
+    return 0;
+}
+
 }  // namespace smx
+
+// ------------------------------------------------------------ captured graphs
+
+struct smx_graph {
+    std::vector<smx_job *> jobs;
+    std::vector<unsigned long long> gens;     // each job's generation at capture
+    std::vector<smx::VllParams> params;       // each job's VllParams of the captured iteration
+    std::vector<int> fast;                    // whether the job's call is FAST
+    hipStream_t stream = nullptr;             // the first job's stream
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void *desc = nullptr;                     // batched-launch descriptors owned by the graph
+    double last_ms = 0;
+};
 
 using namespace smx;
 
 extern "C" {
 
 const char *smx_last_error(void) { return g_err.c_str(); }
-int smx_version(void) { return 10000; }
+int smx_version(void) { return 10100; }
 
 int smx_set_device(int device)
 {
@@ -303,12 +631,13 @@ int smx_job_create(smx_job **out, int num_levels)
     if (e == hipSuccess) e = hipEventCreate(&j->ev1);
     if (e == hipSuccess) e = hipEventCreate(&j->ev2);
     if (e == hipSuccess) e = hipHostMalloc((void **)&j->h_small, 64 * sizeof(int64_t), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&j->h_params, sizeof(VllParams), hipHostMallocDefault);
     if (e != hipSuccess) {
-        delete j;
+        smx_job_destroy(j);
         return fail(SMX_E_HIP, "job setup: %s", hipGetErrorString(e));
     }
     int r = jalloc(j, &j->d_params, 1);
-    if (!r) r = jalloc(j, &j->d_counters, 8);
+    if (!r) r = jalloc(j, &j->js.counters, kCounters);
     if (r) {
         smx_job_destroy(j);
         return r;
@@ -328,14 +657,16 @@ int smx_job_destroy(smx_job *j)
             jfree(j, v.texel);
             jfree(j, v.ori);
         }
-    jfree(j, j->scan_tmp);
+    jfree(j, j->js.scan_part);
+    j->js.scan_part_cap = 0;
     jfree(j, j->d_sgm_scratch);
     job_free(j, j->d_batch);
     j->d_batch = nullptr;
     jfree(j, j->d_params);
-    jfree(j, j->d_counters);
+    jfree(j, j->js.counters);
     for (auto &s : j->sizes) (void)hipFree(s.first);
     if (j->h_small) (void)hipHostFree(j->h_small);
+    if (j->h_params) (void)hipHostFree(j->h_params);
     if (j->ev0) (void)hipEventDestroy(j->ev0);
     if (j->ev1) (void)hipEventDestroy(j->ev1);
     if (j->ev2) (void)hipEventDestroy(j->ev2);
@@ -386,9 +717,9 @@ int smx_job_set_view(smx_job *j, int img, int level, const uint16_t *intensity, 
         jfree(j, d_img);
         return r;
     }
-    HIPCHK(hipMemcpyAsync(d_img, intensity, n * 2, hipMemcpyHostToDevice, j->stream));
-    HIPCHK(hipMemcpyAsync(d_mag, magnitude, n * 2, hipMemcpyHostToDevice, j->stream));
-    HIPCHK(hipMemcpyAsync(v.ori, orientation, n, hipMemcpyHostToDevice, j->stream));
+    HIPCHK(hipMemcpyAsync(d_img, intensity, n * 2, hipMemcpyDefault, j->stream));
+    HIPCHK(hipMemcpyAsync(d_mag, magnitude, n * 2, hipMemcpyDefault, j->stream));
+    HIPCHK(hipMemcpyAsync(v.ori, orientation, n, hipMemcpyDefault, j->stream));
     HIPCHK(launch_pack(d_img, d_mag, v.texel, (long)n, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
     jfree(j, d_img);
@@ -410,7 +741,8 @@ int smx_job_build_pyramid(smx_job *j, int img, const uint16_t *level0, uint32_t 
     if (!r) r = jalloc(j, &mag, n0);
     if (!r) r = jalloc(j, &ow, n0);
     if (r) goto done;
-    HIPCHK(hipMemcpyAsync(cur, level0, n0 * 2, hipMemcpyHostToDevice, j->stream));
+    // host or device memory (hipMemcpyDefault: unified addressing)
+    HIPCHK(hipMemcpyAsync(cur, level0, n0 * 2, hipMemcpyDefault, j->stream));
     {
         uint32_t w = width, h = height;
         for (int lv = 0; lv < j->num_levels; lv++) {
@@ -469,11 +801,12 @@ int smx_job_set_level(smx_job *j, const smx_level *p)
     for (int ti = 0; ti < 2; ti++)
         if (!j->views[ti][p->level].texel) return fail(SMX_E_STATE, "image %d level %d not uploaded", ti, p->level);
     if (p->grid_w == 0 || p->grid_h == 0 || !p->grid_wgs) return fail(SMX_E_ARG, "empty grid");
+    const long n = (long)p->grid_w * (long)p->grid_h;
+    if (n + 1 > scan_max_n()) return fail(SMX_E_ARG, "grid of %ld cells exceeds the scan limit", n);
     HIPCHK(hipSetDevice(j->device));
     HIPCHK(hipStreamSynchronize(j->stream));
     free_level_state(j);
     j->lv = *p;
-    const long n = (long)p->grid_w * (long)p->grid_h;
     j->ncells = n;
     int r = jalloc(j, &j->d_grid_wgs, (size_t)n * 2);
     if (!r) r = jalloc(j, &j->d_gridpt3, (size_t)n);
@@ -485,6 +818,7 @@ int smx_job_set_level(smx_job *j, const smx_level *p)
     if (!r) r = jalloc(j, &j->d_action, (size_t)n);
     if (!r) r = jalloc(j, &j->d_new_len, (size_t)n + 1);
     if (!r) r = jalloc(j, &j->d_new_off, (size_t)n + 1);
+    if (!r) r = grow(j, j->js.scan_part, j->js.scan_part_cap, scan_partials(n + 1));
     if (r) return r;
     HIPCHK(hipMemcpyAsync(j->d_grid_wgs, p->grid_wgs, (size_t)n * 16, hipMemcpyHostToDevice, j->stream));
     if (p->grid_pts) {
@@ -492,22 +826,6 @@ int smx_job_set_level(smx_job *j, const smx_level *p)
         if (r) return r;
         HIPCHK(hipMemcpyAsync(j->d_grid_pts, p->grid_pts, (size_t)n * 16, hipMemcpyHostToDevice, j->stream));
     }
-    j->lv.ortho_w = j->lv.ortho_h = 0;
-    j->lv.ortho_wgs = nullptr;
-    if (p->ortho_wgs) {
-        j->level_set = true;
-        r = smx_job_set_ortho_wgs(j, p->ortho_wgs, p->ortho_w, p->ortho_h);
-        if (r) {
-            j->level_set = false;
-            return r;
-        }
-    }
-    // SETSM callocs nccresult per level (setsm_code.cpp:3769): all zero.
-    HIPCHK(hipMemsetAsync(j->d_ncc, 0, (size_t)n * sizeof(smx_nccresult), j->stream));
-    HIPCHK(hipMemsetAsync(j->d_vox_off, 0, ((size_t)n + 1) * sizeof(int64_t), j->stream));
-    HIPCHK(hipMemsetAsync(j->d_run_count, 0, ((size_t)n + 1) * sizeof(int64_t), j->stream));
-    HIPCHK(hipMemsetAsync(j->d_new_len, 0, ((size_t)n + 1) * sizeof(int64_t), j->stream));
-
     VllParams &P = j->hp;
     std::memset(&P, 0, sizeof P);
     P.rpc[0] = p->rpc[0];
@@ -531,8 +849,27 @@ int smx_job_set_level(smx_job *j, const smx_level *p)
         P.rot_cos[k] = cos(-rot_theta);
         P.rot_sin[k] = sin(-rot_theta);
     }
+    j->lv.ortho_w = j->lv.ortho_h = 0;
+    j->lv.ortho_wgs = nullptr;
+    if (p->ortho_wgs) {
+        j->level_set = true;
+        r = smx_job_set_ortho_wgs(j, p->ortho_wgs, p->ortho_w, p->ortho_h);
+        if (r) {
+            j->level_set = false;
+            return r;
+        }
+    }
+    // SETSM callocs nccresult per level (setsm_code.cpp:3769): all zero.
+    HIPCHK(hipMemsetAsync(j->d_ncc, 0, (size_t)n * sizeof(smx_nccresult), j->stream));
+    HIPCHK(hipMemsetAsync(j->d_vox_off, 0, ((size_t)n + 1) * sizeof(int64_t), j->stream));
+    HIPCHK(hipMemsetAsync(j->d_run_count, 0, ((size_t)n + 1) * sizeof(int64_t), j->stream));
+    HIPCHK(hipMemsetAsync(j->d_new_len, 0, ((size_t)n + 1) * sizeof(int64_t), j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
+    j->stats_valid = true;   // no height lists yet
+    j->runs_bound = 0;
+    j->max_nh = 0;
     j->level_set = true;
+    j->generation++;
     return 0;
 }
 
@@ -551,10 +888,24 @@ int smx_job_set_ortho_wgs(smx_job *j, const double *lonlat, uint32_t w, uint32_t
     jfree(j, j->d_ortho_wgs);
     int r = jalloc(j, &j->d_ortho_wgs, (size_t)sw * sh * 2);
     if (r) return r;
+    // the GNCC term's buffers: image coordinates of every ortho pixel (and of
+    // the level-below grid when the level has a combined term), per-cell value
+    const bool combined_wncc = !p.is_ra && (p.level > p.py_combined_level) && p.level >= 1;
+    const size_t cdb = (size_t)sw * sh * 2 * 2;
+    size_t cdnb = 0;
+    if (combined_wncc) {
+        const double im_res_next = p.resolution * (double)(1 << (p.level - 1));
+        cdnb = (size_t)((long)((p.boundary[2] - p.boundary[0]) / im_res_next) + 1) *
+               (size_t)((long)((p.boundary[3] - p.boundary[1]) / im_res_next) + 1) * 2 * 2;
+    }
+    if ((r = grow(j, j->d_cd, j->cd_bytes, (long long)cdb))) return r;
+    if (cdnb && (r = grow(j, j->d_cdn, j->cdn_bytes, (long long)cdnb))) return r;
+    if (!j->d_gval && (r = jalloc(j, &j->d_gval, (size_t)j->ncells))) return r;
     HIPCHK(hipMemcpyAsync(j->d_ortho_wgs, lonlat, (size_t)sw * sh * 16, hipMemcpyHostToDevice, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
     j->lv.ortho_w = w;
     j->lv.ortho_h = h;
+    j->generation++;
     return 0;
 }
 
@@ -573,6 +924,7 @@ int smx_job_set_searchspace(smx_job *j, const smx_ugrid *g, const double minmax[
     HIPCHK(hipMemcpyAsync(j->d_gridpt3, g, (size_t)j->ncells * sizeof(smx_ugrid), hipMemcpyHostToDevice, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
     j->searchspace_set = true;
+    j->generation++;
     return 0;
 }
 
@@ -597,7 +949,15 @@ int smx_job_set_nccresult(smx_job *j, const smx_nccresult *in)
     if (!j || !in || !j->level_set) return fail(SMX_E_STATE, "no level set");
     HIPCHK(hipMemcpyAsync(j->d_ncc, in, (size_t)j->ncells * sizeof(smx_nccresult), hipMemcpyHostToDevice, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
+    j->stats_valid = false;   // the height lists may have changed
+    j->generation++;
     return 0;
+}
+
+int smx_job_half(const smx_job *j)
+{
+    if (!j || !j->level_set) return fail(SMX_E_STATE, "no level set");
+    return j->hp.half;
 }
 
 int smx_initialize_voxel(smx_job *j, int iteration)
@@ -605,7 +965,8 @@ int smx_initialize_voxel(smx_job *j, int iteration)
     if (!j || !j->level_set || !j->searchspace_set) return fail(SMX_E_STATE, "level and search space must be set");
     if (iteration < 1) return fail(SMX_E_ARG, "iteration starts at 1");
     HIPCHK(hipSetDevice(j->device));
-    int r = upload_params(j, iteration);
+    fill_params(j, iteration);
+    int r = put_params(j, j->stream);
     if (r) return r;
     InitArgs A{};
     A.ncells = j->ncells;
@@ -619,11 +980,18 @@ int smx_initialize_voxel(smx_job *j, int iteration)
     A.old_vox = j->d_vox;
     A.old_rho = j->d_rho;
     HIPCHK(launch_init_voxel(j->d_params, A, j->stream));
-    r = job_scan(j, j->d_new_len, j->d_new_off, j->ncells + 1);
-    if (r) return r;
+    HIPCHK(job_scan(j->js, j->d_new_len, j->d_new_off, j->ncells + 1, j->stream));
+    // the new height lists' bounds for the launch sequences
+    unsigned long long *st = j->js.counters + 6;
+    HIPCHK(launch_fill_u64(st, 0, 2, j->stream));
+    HIPCHK(launch_nh_stats(j->d_ncc, j->ncells, st, j->stream));
     HIPCHK(hipMemcpyAsync(j->h_small, j->d_new_off + j->ncells, sizeof(int64_t), hipMemcpyDeviceToHost, j->stream));
+    HIPCHK(hipMemcpyAsync(j->h_small + 1, st, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
     const long long total = j->h_small[0];
+    j->runs_bound = j->h_small[1];
+    j->max_nh = (int)j->h_small[2];
+    j->stats_valid = true;
     uint32_t *nv = nullptr;
     float *nr = nullptr;
     r = jalloc(j, &nv, (size_t)std::max(total, 1LL));
@@ -642,214 +1010,18 @@ int smx_initialize_voxel(smx_job *j, int iteration)
     j->d_rho = nr;
     std::swap(j->d_vox_off, j->d_new_off);
     j->vox_total = total;
+    j->generation++;
     return 0;
 }
-
-}  // extern "C"
-
-namespace smx {
-
-// VerticalLineLocus up to the NCC kernel: checks, parameters, gate, run list.
-static int vll_prepare(smx_job *j, int iteration, int precision, int keep_rho, CellArgs &A)
-{
-    if (!j || !j->level_set || !j->searchspace_set) return fail(SMX_E_STATE, "level and search space must be set");
-    if (precision != SMX_PREC_FP64 && precision != SMX_PREC_FAST) return fail(SMX_E_ARG, "unknown precision %d", precision);
-    const smx_level &L = j->lv;
-    // setsm_code.cpp:6100-6112 check_ortho, 6240 the per-cell GNCC gate
-    bool check_ortho = !((L.level == 4 && iteration == 1) || L.is_ra);
-    if (L.pre_dem && !(L.level == 4 && iteration == 1)) check_ortho = true;
-    if (L.check_matchtag) check_ortho = true;
-    if (L.level <= L.sgm_py) check_ortho = false;
-    const double ortho_th = 0.7 - (4 - L.level) * 0.10;
-    const int slot = 1 - L.reference_id;
-    if (slot < 0 || slot > 1) return fail(SMX_E_ARG, "reference_id must be 0 or 1");
-    const bool gncc_on = check_ortho && (double)j->max_ortho_ncc[slot] / 1000.0 > ortho_th;
-    const bool combined_wncc = !L.is_ra && (L.level > L.py_combined_level);
-    if (gncc_on) {
-        if (!j->d_ortho_wgs || !j->d_grid_pts)
-            return fail(SMX_E_ARG, "the GNCC term is active (ortho_ncc > %.2f) but the level has no ortho_wgs / "
-                                   "grid_pts", ortho_th);
-        if (combined_wncc)
-            for (int ti = 0; ti < 2; ti++)
-                if (!j->views[ti][L.level - 1].texel)
-                    return fail(SMX_E_STATE, "the combined GNCC term needs level %d", L.level - 1);
-    }
-    const int half = j->hp.half;
-    if (half < 5 || half > 7) return fail(SMX_E_UNSUPPORTED, "Half_template_size %d not built (5..7 are)", half);
-    HIPCHK(hipSetDevice(j->device));
-    int r = upload_params(j, iteration);
-    if (r) return r;
-    if (j->hp.combined_incc)
-        for (int ti = 0; ti < 2; ti++)
-            if (!j->views[ti][L.level - 1].texel) return fail(SMX_E_STATE, "combined INCC needs level %d", L.level - 1);
-    if (keep_rho && !j->d_rho && j->vox_total > 0) {
-        r = jalloc(j, &j->d_rho, (size_t)j->vox_total);
-        if (r) return r;
-        HIPCHK(hipMemsetAsync(j->d_rho, 0xff, (size_t)j->vox_total * sizeof(float), j->stream));   // NaN
-    }
-    A = CellArgs{};
-    A.ncells = j->ncells;
-    A.gridpt3 = j->d_gridpt3;
-    A.ncc = j->d_ncc;
-    A.grid_wgs = j->d_grid_wgs;
-    A.vox_off = j->d_vox_off;
-    A.vox = j->d_vox;
-    A.rho = keep_rho ? j->d_rho : nullptr;
-    A.run_count = j->d_run_count;
-    A.run_offset = j->d_run_off;
-    A.max_bits = j->d_max_bits;
-    A.accessible = j->d_counters;
-    A.evaluated = j->d_counters + 1;
-    A.ncc_alpha = j->hp.ncc_alpha;
-    OrthoArgs O{};
-    if (gncc_on) {
-        // setsm_code.cpp:6121-6129: ortho image coordinates of this iteration's
-        // surface (GridPT3.Height)
-        const double im_res = L.resolution * (double)(1 << L.level);
-        const double im_res_next = combined_wncc ? L.resolution * (double)(1 << (L.level - 1)) : L.resolution;
-        O.W = L.grid_w;
-        O.H = L.grid_h;
-        O.sw = L.ortho_w;
-        O.sh = L.ortho_h;
-        if (combined_wncc) {
-            O.swn = (long)((L.boundary[2] - L.boundary[0]) / im_res_next) + 1;
-            O.shn = (long)((L.boundary[3] - L.boundary[1]) / im_res_next) + 1;
-        }
-        O.b0 = L.boundary[0];
-        O.b1 = L.boundary[1];
-        O.im_res = im_res;
-        O.im_res_next = im_res_next;
-        O.grid_res = L.grid_resolution;
-        O.combined = combined_wncc ? 1 : 0;
-        O.slot = slot;
-        O.ortho_th = ortho_th;
-        O.grid_pts = j->d_grid_pts;
-        O.ortho_wgs = j->d_ortho_wgs;
-        O.gridpt3 = j->d_gridpt3;
-        const size_t cdb = (size_t)O.sw * O.sh * 2 * 2, cdnb = (size_t)O.swn * O.shn * 2 * 2;
-        if (cdb > j->cd_bytes) {
-            jfree(j, j->d_cd);
-            int r2 = jalloc(j, &j->d_cd, cdb);
-            if (r2) return r2;
-            j->cd_bytes = cdb;
-        }
-        if (cdnb > j->cdn_bytes) {
-            jfree(j, j->d_cdn);
-            int r2 = jalloc(j, &j->d_cdn, cdnb);
-            if (r2) return r2;
-            j->cdn_bytes = cdnb;
-        }
-        if (!j->d_gval) {
-            int r2 = jalloc(j, &j->d_gval, (size_t)j->ncells);
-            if (r2) return r2;
-        }
-        O.cd[0] = j->d_cd;
-        O.cd[1] = j->d_cd + (size_t)O.sw * O.sh * 2;
-        O.cdn[0] = cdnb ? j->d_cdn : nullptr;
-        O.cdn[1] = cdnb ? j->d_cdn + (size_t)O.swn * O.shn * 2 : nullptr;
-        // calloc'd in the reference: pixels it skips read as (0, 0)
-        HIPCHK(hipMemsetAsync(j->d_cd, 0, cdb * sizeof(double), j->stream));
-        if (cdnb) HIPCHK(hipMemsetAsync(j->d_cdn, 0, cdnb * sizeof(double), j->stream));
-        HIPCHK(launch_ortho_coords(j->d_params, O, j->stream));
-        A.acc_flag = j->d_action;
-    }
-    HIPCHK(hipMemsetAsync(j->d_counters, 0, 8 * sizeof(unsigned long long), j->stream));
-    HIPCHK(launch_vll_gate(j->d_params, A, j->stream));
-    if (gncc_on) {
-        HIPCHK(launch_gncc(half, j->d_params, O, A, j->d_gval, j->stream));
-        A.gval = j->d_gval;
-    }
-    r = job_scan(j, j->d_run_count, j->d_run_off, j->ncells + 1);
-    if (r) return r;
-    HIPCHK(hipMemcpyAsync(j->h_small, j->d_run_off + j->ncells, sizeof(int64_t), hipMemcpyDeviceToHost, j->stream));
-    HIPCHK(hipStreamSynchronize(j->stream));
-    const long nruns = (long)j->h_small[0];
-    if (nruns > j->runs_cap) {
-        jfree(j, j->d_runs);
-        r = jalloc(j, &j->d_runs, (size_t)nruns);
-        if (r) return r;
-        j->runs_cap = nruns;
-    }
-    A.runs = j->d_runs;
-    A.nruns = nruns;
-    HIPCHK(launch_vll_fill_runs(A, j->stream));
-    A.redo_n = j->d_counters + 4;
-    if (precision != SMX_PREC_FP64 && !L.is_ra && !L.check_matching_rate) {
-        // FAST: runs handed to the FP64 pass (each at most once)
-        if (nruns > j->redo_cap) {
-            jfree(j, j->d_redo);
-            r = jalloc(j, &j->d_redo, (size_t)std::max(nruns, 1L));
-            if (r) return r;
-            j->redo_cap = nruns;
-        }
-        A.redo = j->d_redo;
-        if (j->hp.combined_incc) {
-            // the level term per hypothesis between the two passes
-            if (j->vox_total > j->part_cap || !j->d_part) {
-                jfree(j, j->d_part);
-                r = jalloc(j, &j->d_part, (size_t)std::max(j->vox_total, 1LL));
-                if (r) return r;
-                j->part_cap = j->vox_total;
-            }
-            A.part = j->d_part;
-        }
-    }
-    if (L.is_ra || L.check_matching_rate) {
-        // RA / matching-rate pass: temp_rho per run lane for FindPeakNcc
-        if (nruns > j->peak_cap) {
-            jfree(j, j->d_peak_rho);
-            r = jalloc(j, &j->d_peak_rho, (size_t)std::max(nruns, 1L) * kRun);
-            if (r) return r;
-            j->peak_cap = nruns;
-        }
-        A.peak_rho = j->d_peak_rho;
-    }
-    return 0;
-}
-
-// After the NCC kernel: max_WNCC, counters. Returns Accessable_grid.
-static long long vll_finish(smx_job *j, const CellArgs &A)
-{
-    HIPCHK(launch_vll_finalize(A, j->stream));
-    HIPCHK(hipMemcpyAsync(j->h_small, j->d_counters, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, j->stream));
-    HIPCHK(hipStreamSynchronize(j->stream));
-    j->last_evaluated = j->h_small[1] + j->h_small[5];
-    j->last_redo_hyps = j->h_small[5];
-    j->last_tiles_lds = j->h_small[2];
-    j->last_waves_hbm = j->h_small[3];
-    j->last_redo_runs = j->h_small[4];
-    return (long long)j->h_small[0];
-}
-
-}  // namespace smx
-
-extern "C" {
 
 long long smx_vertical_line_locus(smx_job *j, int iteration, int precision, int keep_rho)
 {
-    CellArgs A;
-    int r = vll_prepare(j, iteration, precision, keep_rho, A);
+    VllPlan p;
+    int r = vll_plan(j, iteration, precision, keep_rho, p);
     if (r) return r;
-    if (precision == SMX_PREC_FP64 || A.peak_rho) {
-        // RA / matching-rate passes (no voxel cube) always take the FP64 kernel
-        HIPCHK(hipEventRecord(j->ev0, j->stream));
-        HIPCHK(launch_vll_fp64(j->hp.half, j->d_params, A, j->stream));
-        HIPCHK(hipEventRecord(j->ev1, j->stream));
-        if (A.peak_rho && j->lv.check_matching_rate) HIPCHK(launch_find_peak(j->d_params, A, j->stream));
-    } else {
-        r = launch_vll_fast(j, j->hp.half, A, j->ev0, j->ev1);
-        if (r) return r;
-        HIPCHK(launch_vll_redo(j->hp.half, j->d_params, A, j->stream));
-        HIPCHK(hipEventRecord(j->ev2, j->stream));
-    }
-    const long long acc = vll_finish(j, A);
-    if (acc < 0) return acc;
-    float ms = 0, rms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, j->ev0, j->ev1));
-    if (precision != SMX_PREC_FP64 && !A.peak_rho) HIPCHK(hipEventElapsedTime(&rms, j->ev1, j->ev2));
-    j->last_ms = ms;
-    j->last_redo_ms = rms;
-    return acc;
+    HIPCHK(hipSetDevice(j->device));
+    if ((r = vll_reserve(j, p))) return r;
+    return vll_run(j, p, j->js, true);
 }
 
 int smx_vertical_line_locus_batch(smx_job **jobs, int n, int iteration, int precision, int keep_rho,
@@ -859,9 +1031,17 @@ int smx_vertical_line_locus_batch(smx_job **jobs, int n, int iteration, int prec
     for (int i = 0; i < n; i++) {
         if (!jobs[i]) return fail(SMX_E_ARG, "null job %d", i);
         if (jobs[i]->device != jobs[0]->device) return fail(SMX_E_ARG, "jobs of one batch share a device");
+        for (int k = 0; k < i; k++)
+            if (jobs[k] == jobs[i]) return fail(SMX_E_ARG, "job %d appears twice in the batch", i);
+    }
+    // every check before anything is queued on any job
+    std::vector<VllPlan> P((size_t)n);
+    for (int i = 0; i < n; i++) {
+        int r = vll_plan(jobs[i], iteration, precision, keep_rho, P[i]);
+        if (r) return r;
     }
     bool ra = false;
-    for (int i = 0; i < n; i++) ra = ra || jobs[i]->lv.is_ra || jobs[i]->lv.check_matching_rate;
+    for (int i = 0; i < n; i++) ra = ra || P[i].ra;
     if (precision == SMX_PREC_FP64 || ra) {
         // the plain kernel: one tile per launch (RA / matching-rate passes included)
         for (int i = 0; i < n; i++) {
@@ -871,27 +1051,50 @@ int smx_vertical_line_locus_batch(smx_job **jobs, int n, int iteration, int prec
         }
         return 0;
     }
-    std::vector<CellArgs> As((size_t)n);
-    for (int i = 0; i < n; i++) {
-        int r = vll_prepare(jobs[i], iteration, precision, keep_rho, As[i]);
-        if (r) return r;
-        if (jobs[i]->hp.half != jobs[0]->hp.half)
-            return fail(SMX_E_ARG, "a batch holds one window size (job %d has half %d, job 0 %d)", i,
-                        jobs[i]->hp.half, jobs[0]->hp.half);
-        if (jobs[i]->hp.combined_incc)
-            return fail(SMX_E_UNSUPPORTED, "combined-level iterations are launched per job");
-    }
+    for (int i = 0; i < n; i++)
+        if (P[i].half != P[0].half)
+            return fail(SMX_E_ARG, "a batch holds one window size (job %d has half %d, job 0 %d)", i, P[i].half,
+                        P[0].half);
     smx_job *j0 = jobs[0];
-    int r = launch_vll_fast_batch(jobs, n, As.data(), j0->hp.half, j0->ev0, j0->ev1);
+    HIPCHK(hipSetDevice(j0->device));
+    for (int i = 0; i < n; i++) {
+        int r = vll_reserve(jobs[i], P[i]);
+        if (r) return r;
+    }
+    const size_t need = fast_batch_desc_bytes(n);
+    if (need > j0->batch_bytes) {
+        job_free(j0, j0->d_batch);
+        j0->d_batch = nullptr;
+        j0->batch_bytes = 0;
+        int r = job_alloc(j0, &j0->d_batch, need);
+        if (r) return r;
+        j0->batch_bytes = need;
+    }
+    // the jobs' own earlier work (set-up, fills) before the batch stream's
+    for (int i = 0; i < n; i++) HIPCHK(hipStreamSynchronize(jobs[i]->stream));
+    const hipStream_t s = j0->stream;
+    std::vector<CellArgs> As((size_t)n);
+    std::vector<OrthoArgs> Os((size_t)n);
+    std::vector<VllScratch> Ss((size_t)n);
+    for (int i = 0; i < n; i++) {
+        Ss[i] = jobs[i]->js;
+        vll_args(jobs[i], P[i], Ss[i], As[i], Os[i]);
+        int r = put_params(jobs[i], s);
+        if (!r) r = vll_front(jobs[i], P[i], Ss[i], false, s, As[i], Os[i]);
+        if (r) return r;
+    }
+    int r = fast_batch_descriptors(jobs, n, As.data(), Ss.data(), j0->d_batch, s);
+    if (!r) r = launch_vll_fast_batch(jobs, n, As.data(), Ss.data(), P[0].half, j0->d_batch, s, j0->ev0, j0->ev1);
     if (r) return r;
-    for (int i = 0; i < n; i++) HIPCHK(launch_vll_redo(j0->hp.half, jobs[i]->d_params, As[i], j0->stream));
-    HIPCHK(hipEventRecord(j0->ev2, j0->stream));
-    HIPCHK(hipStreamSynchronize(j0->stream));
+    for (int i = 0; i < n; i++) HIPCHK(launch_vll_redo(P[0].half, jobs[i]->d_params, As[i], s));
+    HIPCHK(hipEventRecord(j0->ev2, s));
+    for (int i = 0; i < n; i++) HIPCHK(launch_vll_finalize(As[i], s));
+    HIPCHK(hipStreamSynchronize(s));
     float ms = 0, rms = 0;
     HIPCHK(hipEventElapsedTime(&ms, j0->ev0, j0->ev1));
     HIPCHK(hipEventElapsedTime(&rms, j0->ev1, j0->ev2));
     for (int i = 0; i < n; i++) {
-        const long long a = vll_finish(jobs[i], As[i]);
+        const long long a = vll_finish(jobs[i], Ss[i].counters, s);
         if (a < 0) return (int)a;
         accessible[i] = a;
         jobs[i]->last_ms = ms;
@@ -900,61 +1103,16 @@ int smx_vertical_line_locus_batch(smx_job **jobs, int n, int iteration, int prec
     return 0;
 }
 
-// setsm_code.cpp:7017-7509: SumCost of the current cube by the reference's
-// 12 path loops (levels <= SGM_py).
-static int sgm_aggregate(smx_job *j)
-{
-    const long long total = j->vox_total;
-    if (total != j->sum_total || !j->d_sum) {
-        jfree(j, j->d_sum);
-        int r = jalloc(j, &j->d_sum, (size_t)std::max(total, 1LL));
-        if (r) return r;
-        j->sum_total = total;
-    }
-    if (total) HIPCHK(hipMemsetAsync(j->d_sum, 0, (size_t)total * sizeof(float), j->stream));
-    // the longest height list sizes the per-path buffers
-    HIPCHK(hipMemsetAsync(j->d_counters + 2, 0, sizeof(unsigned long long), j->stream));
-    HIPCHK(launch_max_nh(j->d_ncc, j->ncells, j->d_counters + 2, j->stream));
-    HIPCHK(hipMemcpyAsync(j->h_small, j->d_counters + 2, sizeof(int64_t), hipMemcpyDeviceToHost, j->stream));
-    HIPCHK(hipStreamSynchronize(j->stream));
-    const int cap = (int)((std::max<int64_t>(j->h_small[0], 1) + 3) & ~3LL);
-    SgmLaunch L{};
-    L.W = (int)j->lv.grid_w;
-    L.H = (int)j->lv.grid_h;
-    L.step = j->lv.height_step;
-    L.ncc = j->d_ncc;
-    L.gridpt3 = j->d_gridpt3;
-    L.vox_off = j->d_vox_off;
-    L.vox = j->d_vox;
-    L.sum = j->d_sum;
-    L.cap = cap;
-    if ((size_t)10 * cap > sgm_lds_limit()) {
-        const size_t need = (size_t)10 * cap * (size_t)std::max(L.W, L.H);
-        if (need > j->sgm_scratch_bytes) {
-            jfree(j, j->d_sgm_scratch);
-            int r = jalloc(j, &j->d_sgm_scratch, need);
-            if (r) return r;
-            j->sgm_scratch_bytes = need;
-        }
-        L.gscratch = j->d_sgm_scratch;
-    }
-    HIPCHK(launch_sgm(L, j->stream));
-    return 0;
-}
-
 int smx_awncc(smx_job *j, int iteration)
 {
     if (!j || !j->level_set || !j->searchspace_set) return fail(SMX_E_STATE, "level and search space must be set");
+    if (iteration < 1) return fail(SMX_E_ARG, "iteration starts at 1");
     HIPCHK(hipSetDevice(j->device));
-    int r = upload_params(j, iteration);
+    fill_params(j, iteration);
+    int r = awncc_reserve(j);
+    if (!r) r = put_params(j, j->stream);
+    if (!r) r = awncc_enqueue(j, j->stream);
     if (r) return r;
-    const bool sgm = j->lv.level <= j->lv.sgm_py;
-    if (sgm) {
-        r = sgm_aggregate(j);
-        if (r) return r;
-    }
-    PeakArgs A{j->ncells, j->d_ncc, j->d_gridpt3, j->d_vox_off, j->d_vox, sgm ? j->d_sum : nullptr};
-    HIPCHK(launch_awncc(j->d_params, A, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
     return 0;
 }
@@ -970,6 +1128,163 @@ int smx_job_get_sumcost(smx_job *j, float *out)
     HIPCHK(hipStreamSynchronize(j->stream));
     return 0;
 }
+
+// ------------------------------------------------------------ graphs
+
+int smx_graph_destroy(smx_graph *g)
+{
+    if (!g) return 0;
+    if (!g->jobs.empty()) (void)hipSetDevice(g->jobs[0]->device);
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->desc) (void)hipFree(g->desc);
+    if (g->ev0) (void)hipEventDestroy(g->ev0);
+    if (g->ev1) (void)hipEventDestroy(g->ev1);
+    delete g;
+    return 0;
+}
+
+// Queue the captured sequence on s (called between begin and end capture).
+static int graph_body(smx_graph *g, const std::vector<VllPlan> &P, std::vector<CellArgs> &As,
+                      std::vector<OrthoArgs> &Os, std::vector<VllScratch> &Ss, int with_awncc)
+{
+    const int n = (int)g->jobs.size();
+    smx_job *const *jobs = g->jobs.data();
+    const hipStream_t s = g->stream;
+    for (int i = 0; i < n; i++) {
+        int r = vll_front(jobs[i], P[i], Ss[i], false, s, As[i], Os[i]);
+        if (r) return r;
+    }
+    if (P[0].fast && n > 1) {
+        int r = launch_vll_fast_batch(jobs, n, As.data(), Ss.data(), P[0].half, g->desc, s, nullptr, nullptr);
+        if (r) return r;
+        for (int i = 0; i < n; i++) HIPCHK(launch_vll_redo(P[i].half, jobs[i]->d_params, As[i], s));
+    } else {
+        for (int i = 0; i < n; i++) {
+            int r = vll_kernels(jobs[i], P[i], As[i], Ss[i], s, false);
+            if (r) return r;
+        }
+    }
+    for (int i = 0; i < n; i++) HIPCHK(launch_vll_finalize(As[i], s));
+    if (with_awncc)
+        for (int i = 0; i < n; i++) {
+            int r = awncc_enqueue(jobs[i], s);
+            if (r) return r;
+        }
+    return 0;
+}
+
+int smx_graph_create(smx_graph **out, smx_job **jobs, int n, int iteration, int precision, int with_awncc)
+{
+    if (!out || !jobs || n < 1) return fail(SMX_E_ARG, "smx_graph_create: bad argument");
+    *out = nullptr;
+    for (int i = 0; i < n; i++) {
+        if (!jobs[i]) return fail(SMX_E_ARG, "null job %d", i);
+        if (jobs[i]->device != jobs[0]->device) return fail(SMX_E_ARG, "jobs of one graph share a device");
+        for (int k = 0; k < i; k++)
+            if (jobs[k] == jobs[i]) return fail(SMX_E_ARG, "job %d appears twice", i);
+    }
+    std::vector<VllPlan> P((size_t)n);
+    for (int i = 0; i < n; i++) {
+        int r = vll_plan(jobs[i], iteration, precision, 0, P[i]);
+        if (r) return r;
+        if (P[i].ra)
+            return fail(SMX_E_UNSUPPORTED, "RA and matching-rate passes read a count back mid-call and cannot be "
+                                           "captured");
+        if (P[i].fast && n > 1 && P[i].half != P[0].half)
+            return fail(SMX_E_ARG, "a batched graph holds one window size (job %d has half %d, job 0 %d)", i,
+                        P[i].half, P[0].half);
+    }
+    HIPCHK(hipSetDevice(jobs[0]->device));
+    for (int i = 0; i < n; i++) {
+        int r = vll_reserve(jobs[i], P[i]);
+        if (!r && with_awncc) r = awncc_reserve(jobs[i]);
+        if (r) return r;
+    }
+    smx_graph *g = new smx_graph();
+    g->jobs.assign(jobs, jobs + n);
+    g->stream = jobs[0]->stream;
+    int r = 0;
+    hipError_t e = hipEventCreate(&g->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&g->ev1);
+    if (e == hipSuccess && P[0].fast && n > 1) e = hipMalloc(&g->desc, fast_batch_desc_bytes(n));
+    for (int i = 0; i < n && e == hipSuccess; i++) e = hipStreamSynchronize(jobs[i]->stream);
+    if (e != hipSuccess) {
+        smx_graph_destroy(g);
+        return fail(SMX_E_HIP, "smx_graph_create: %s", hipGetErrorString(e));
+    }
+    std::vector<CellArgs> As((size_t)n);
+    std::vector<OrthoArgs> Os((size_t)n);
+    std::vector<VllScratch> Ss((size_t)n);
+    for (int i = 0; i < n && !r; i++) {
+        Ss[i] = jobs[i]->js;
+        vll_args(jobs[i], P[i], Ss[i], As[i], Os[i]);
+        r = put_params(jobs[i], g->stream);
+        g->params.push_back(jobs[i]->hp);
+        g->fast.push_back(P[i].fast ? 1 : 0);
+    }
+    // descriptors of a batched launch live outside the graph (uploaded here)
+    if (!r && P[0].fast && n > 1) r = fast_batch_descriptors(jobs, n, As.data(), Ss.data(), g->desc, g->stream);
+    if (!r && (e = hipStreamSynchronize(g->stream)) != hipSuccess) r = fail(SMX_E_HIP, "%s", hipGetErrorString(e));
+    if (r) {
+        smx_graph_destroy(g);
+        return r;
+    }
+    e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        smx_graph_destroy(g);
+        return fail(SMX_E_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+    }
+    r = graph_body(g, P, As, Os, Ss, with_awncc);
+    const std::string body_err = r ? g_err : std::string();
+    e = hipStreamEndCapture(g->stream, &g->graph);
+    if (!r && e != hipSuccess) r = fail(SMX_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    if (!r && (e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0)) != hipSuccess)
+        r = fail(SMX_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    if (r) {
+        if (!body_err.empty()) g_err = body_err;
+        smx_graph_destroy(g);
+        return r;
+    }
+    for (int i = 0; i < n; i++) g->gens.push_back(jobs[i]->generation);
+    *out = g;
+    return 0;
+}
+
+int smx_graph_launch(smx_graph *g, long long *accessible)
+{
+    if (!g || !g->exec) return fail(SMX_E_ARG, "smx_graph_launch: no graph");
+    const int n = (int)g->jobs.size();
+    for (int i = 0; i < n; i++)
+        if (g->jobs[i]->generation != g->gens[i])
+            return fail(SMX_E_STATE, "job %d changed since the capture (level, search space, cube, buffers or launch "
+                                     "shape): capture again", i);
+    HIPCHK(hipSetDevice(g->jobs[0]->device));
+    for (int i = 0; i < n; i++) {
+        g->jobs[i]->hp = g->params[i];
+        int r = put_params(g->jobs[i], g->stream);
+        if (r) return r;
+    }
+    HIPCHK(hipEventRecord(g->ev0, g->stream));
+    HIPCHK(hipGraphLaunch(g->exec, g->stream));
+    HIPCHK(hipEventRecord(g->ev1, g->stream));
+    for (int i = 0; i < n; i++) {
+        const long long a = vll_finish(g->jobs[i], g->jobs[i]->js.counters, g->stream);
+        if (a < 0) return (int)a;
+        if (accessible) accessible[i] = a;
+    }
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+    g->last_ms = ms;
+    for (int i = 0; i < n; i++) {
+        g->jobs[i]->last_ms = ms;
+        g->jobs[i]->last_redo_ms = 0;
+    }
+    return 0;
+}
+
+double smx_graph_last_ms(const smx_graph *g) { return g ? g->last_ms : 0.0; }
 
 long long smx_job_num_hypotheses(const smx_job *j) { return j ? j->vox_total : 0; }
 
@@ -1023,7 +1338,13 @@ int smx_job_set_voxels(smx_job *j, const uint32_t *words)
     HIPCHK(hipMemcpyAsync(ncc.data(), j->d_ncc, ncc.size() * sizeof(smx_nccresult), hipMemcpyDeviceToHost, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
     off[0] = 0;
-    for (long c = 0; c < j->ncells; c++) off[c + 1] = off[c] + ncc[c].NumOfHeight;
+    long long runs = 0;
+    int max_nh = 0;
+    for (long c = 0; c < j->ncells; c++) {
+        off[c + 1] = off[c] + ncc[c].NumOfHeight;
+        runs += (ncc[c].NumOfHeight + kRun - 1) / kRun;
+        max_nh = std::max(max_nh, (int)ncc[c].NumOfHeight);
+    }
     const long long total = off[j->ncells];
     if (total != j->vox_total || !j->d_vox) {
         jfree(j, j->d_vox);
@@ -1036,6 +1357,10 @@ int smx_job_set_voxels(smx_job *j, const uint32_t *words)
     if (total)
         HIPCHK(hipMemcpyAsync(j->d_vox, words, (size_t)total * 4, hipMemcpyHostToDevice, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
+    j->runs_bound = runs;
+    j->max_nh = max_nh;
+    j->stats_valid = true;
+    j->generation++;
     return 0;
 }
 
@@ -1044,6 +1369,7 @@ int smx_fast_num_configs(void) { return fast_num_configs(); }
 int smx_job_set_fast_config(smx_job *j, int config)
 {
     if (!j || config < -1 || config >= fast_num_configs()) return fail(SMX_E_ARG, "fast config %d out of range", config);
+    if (j->fast_cfg != config) j->generation++;
     j->fast_cfg = config;
     return 0;
 }

@@ -20,6 +20,34 @@ struct LevelView {
     double sx = 0, sy = 0;
 };
 
+// Scratch of one VerticalLineLocus call: the height-run list, the FAST
+// path's per-tile run lists (two sets: a combined iteration's level-below
+// pass has its own tiles), the queue of runs handed to the FP64 pass, the
+// level term of a combined iteration, the scan partials and the call's
+// counters. A job owns one set, grown before any launch is queued; the
+// no-alloc entry point carves one out of the library's static device array.
+struct VllScratch {
+    int2 *runs = nullptr;                                   // runs_cap
+    long runs_cap = 0;
+    int64_t *tile_count = nullptr, *tile_off = nullptr;     // tiles_cap + 1 each
+    int2 *tile_runs = nullptr;                              // runs_cap
+    long tiles_cap = 0;
+    int64_t *tile2_count = nullptr, *tile2_off = nullptr;
+    int2 *tile2_runs = nullptr;
+    long tiles2_cap = 0;
+    int2 *redo = nullptr;                                   // runs_cap
+    double *part = nullptr;                                 // part_cap
+    long long part_cap = 0;
+    int64_t *scan_part = nullptr;                           // scan_part_cap
+    long scan_part_cap = 0;
+    unsigned long long *counters = nullptr;                 // kCounters
+};
+
+// Counter slots of a call: accessible cells, evaluated hypotheses, FAST tiles
+// staged in LDS, FAST waves reading taps from HBM, runs handed to the FP64
+// pass, hypotheses that pass evaluated, and two scratch slots.
+constexpr int kCounters = 8;
+
 }  // namespace smx
 
 struct smx_job {
@@ -51,10 +79,6 @@ struct smx_job {
     double *d_gval = nullptr;                     // per-cell GNCC value (NaN: none)
     double *d_peak_rho = nullptr;                 // RA / matching-rate pass: temp_rho per run lane
     long peak_cap = 0;
-    int2 *d_redo = nullptr;                      // FAST: runs handed to the FP64 pass
-    long redo_cap = 0;
-    double *d_part = nullptr;                    // FAST, combined iterations: level term per hypothesis
-    long long part_cap = 0;
     smx_ugrid *d_gridpt3 = nullptr;
     smx_nccresult *d_ncc = nullptr;
     int64_t *d_vox_off = nullptr;
@@ -66,28 +90,32 @@ struct smx_job {
     unsigned char *d_sgm_scratch = nullptr;      // SGM path buffers when they exceed LDS
     size_t sgm_scratch_bytes = 0;
 
-    // scratch of the launch sequences
+    // per-cell arrays of the launch sequences (allocated with the level)
     int64_t *d_run_count = nullptr, *d_run_off = nullptr;
-    int2 *d_runs = nullptr;
-    long runs_cap = 0;
     unsigned long long *d_max_bits = nullptr;
-    unsigned long long *d_counters = nullptr;
     uint8_t *d_action = nullptr;
     int64_t *d_new_len = nullptr, *d_new_off = nullptr;
-    void *scan_tmp = nullptr;
-    size_t scan_tmp_bytes = 0;
 
-    // fast-kernel launch shape (autotuned) and its tile run lists
-    int fast_cfg = -1;
-    int64_t *d_tile_count = nullptr, *d_tile_off = nullptr;
-    int2 *d_tile_runs = nullptr;
-    long tiles_cap = 0, tile_runs_cap = 0;
-    // second tile lists: the level-below pass of a combined iteration
-    int64_t *d_tile2_count = nullptr, *d_tile2_off = nullptr;
-    int2 *d_tile2_runs = nullptr;
-    long tiles2_cap = 0, tile2_runs_cap = 0;
-    void *d_batch = nullptr;                     // batched-launch descriptors
+    // the job's own call scratch (grown before a call queues anything;
+    // js.counters is allocated with the job)
+    smx::VllScratch js;
+    long cap_tile_runs = 0, cap_tile2_runs = 0, cap_redo = 0;   // capacities beside js.runs_cap
+
+    int fast_cfg = -1;                           // FAST launch shape (autotuned), -1 = default
+    void *d_batch = nullptr;                     // batched-launch descriptors (first job of a batch)
     size_t batch_bytes = 0;
+    smx::VllParams *h_params = nullptr;          // pinned staging of hp for its upload
+
+    // statistics of the cube layout, taken when it is (re)built: an upper bound
+    // on the runs a (non-RA) VerticalLineLocus call can queue, and the longest
+    // height list (SGM path buffers). Launch sequences size their scratch from
+    // them instead of reading counts back mid-sequence.
+    bool stats_valid = false;
+    long long runs_bound = 0;
+    int max_nh = 0;
+    // bumped by every call that changes what a captured launch sequence
+    // refers to (level, search space, cube layout, launch shape)
+    unsigned long long generation = 1;
 
     double last_ms = 0;
     long long last_evaluated = 0;
@@ -106,7 +134,12 @@ void job_free(smx_job *j, void *p);
 double ncc_alpha(int Pyramid_step, int iteration, int IsRA);
 int vll_half_template(int level, int template_size, double resolution);
 
-hipError_t scan_i64(const int64_t *in, int64_t *out, long n, void *tmp, size_t &tmp_bytes, hipStream_t s);
+hipError_t scan_i64(const int64_t *in, int64_t *out, long n, int64_t *part, hipStream_t s);
+long scan_partials(long n);
+long scan_max_n();
+hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, long n, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, long n, hipStream_t s);
+hipError_t launch_nh_stats(const smx_nccresult *ncc, long n, unsigned long long *stats, hipStream_t s);
 hipError_t launch_gauss_decimate(const uint16_t *in, uint32_t w, uint32_t h, uint16_t *out, hipStream_t s);
 hipError_t launch_sobel(const uint16_t *in, uint32_t w, uint32_t h, uint16_t *mag, uint32_t *ow, hipStream_t s);
 hipError_t launch_orientation(const uint32_t *ow, uint32_t w, uint32_t h, uint8_t *ori, hipStream_t s);
@@ -139,11 +172,26 @@ struct SgmLaunch {
 hipError_t launch_sgm(const SgmLaunch &L, hipStream_t s);
 hipError_t launch_max_nh(const smx_nccresult *ncc, long n, unsigned long long *out, hipStream_t s);
 size_t sgm_lds_limit();
-int launch_vll_fast(smx_job *j, int half, const CellArgs &A, hipEvent_t ev0, hipEvent_t ev1);
-int launch_vll_fast_batch(smx_job **jobs, int n, const CellArgs *As, int half, hipEvent_t ev0, hipEvent_t ev1);
+// FAST launch sequences (vll_fast.hip). Scratch buffers come from S; the
+// tile counts they need are fast_tiles().
+int launch_vll_fast(smx_job *j, int half, const CellArgs &A, const VllScratch &S, hipStream_t s, hipEvent_t ev0,
+                    hipEvent_t ev1);
+// One batched launch over jobs[0..n) (same half-window size): single-pass
+// jobs in one persistent launch, combined-iteration jobs in two (the level
+// pass, then the level-below pass). `desc` is device memory for
+// fast_batch_desc_bytes(n) bytes; the descriptors are copied there on `s`.
+size_t fast_batch_desc_bytes(int n);
+// Computes the descriptors of a batch and copies them to `desc` (waits on s).
+int fast_batch_descriptors(smx_job *const *jobs, int n, const CellArgs *As, const VllScratch *Ss, void *desc,
+                           hipStream_t s);
+int launch_vll_fast_batch(smx_job *const *jobs, int n, const CellArgs *As, const VllScratch *Ss, int half,
+                          void *desc, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1);
 int fast_num_configs();
 int fast_config_desc(int i, char *buf, int n);
-int fast_scratch(smx_job *j, int slot, long ntiles, long nruns);
-int job_scan(smx_job *j, const int64_t *in, int64_t *out, long n);
+int fast_config_index(const smx_job *j);
+// tiles of the job's current level for its launch shape: slot 0 the level
+// pass, slot 1 the level-below pass of a combined iteration
+long fast_tiles(const smx_job *j, int slot);
+hipError_t job_scan(const VllScratch &S, const int64_t *in, int64_t *out, long n, hipStream_t s);
 
 }  // namespace smx

@@ -23,7 +23,8 @@ struct CellArgs {
     int64_t *run_count;            // per cell (ncells + 1)
     const int64_t *run_offset;     // exclusive prefix sum of run_count
     int2 *runs;                    // (cell, first height)
-    long nruns;
+    long runs_cap;                 // capacity of runs (host-known bound on the count)
+    const int64_t *nruns_dev;      // the count itself, on the device (run_offset[ncells])
     unsigned long long *max_bits;  // per cell, order_key of max_WNCC
     unsigned long long *accessible;
     unsigned long long *evaluated;
@@ -32,7 +33,7 @@ struct CellArgs {
     double ncc_alpha;              // SetNCC_alpha of the call (GNCC blend)
     double *peak_rho;              // RA / matching-rate pass: temp_rho per run lane (NaN: not evaluated);
                                    // the cube is not touched
-    int2 *redo;                    // FAST: runs handed to the FP64 pass (capacity nruns)
+    int2 *redo;                    // FAST: runs handed to the FP64 pass (capacity runs_cap)
     double *part;                  // FAST, combined iterations: the level term per hypothesis
     unsigned long long *redo_n;    // FAST: their count
 };

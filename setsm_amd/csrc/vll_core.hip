@@ -86,8 +86,39 @@ __global__ void __launch_bounds__(256) k_vll_fill_runs(CellArgs A)
     if (c >= A.ncells) return;
     const int n = A.run_count[c];
     const long base = A.run_offset[c];
-    for (int i = 0; i < n; i++)
+    // the capacity is an upper bound on the count; never write past it
+    for (int i = 0; i < n && base + i < A.runs_cap; i++)
         A.runs[base + i] = make_int2((int)c, i * kRun);
+}
+
+// Per-cell height lists -> {sum of runs, longest list}: the bounds the launch
+// sequences size their scratch by (stats[0] += runs, stats[1] = max).
+__global__ void __launch_bounds__(256) k_nh_stats(const smx_nccresult *__restrict__ ncc, long n,
+                                                  unsigned long long *stats)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned nh = c < n ? ncc[c].NumOfHeight : 0u;
+    unsigned long long runs = (nh + kRun - 1) / kRun;
+    for (int o = 32; o > 0; o >>= 1) {
+        runs += (unsigned long long)__shfl_xor((long long)runs, o);
+        nh = max(nh, (unsigned)__shfl_xor((int)nh, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (runs) atomicAdd(&stats[0], runs);
+        if (nh) atomicMax(&stats[1], (unsigned long long)nh);
+    }
+}
+
+// Fills (the launch sequences clear their buffers with kernels, not runtime
+// memsets, so that they can be captured in a hipGraph).
+__global__ void __launch_bounds__(256) k_fill_u64(unsigned long long *p, unsigned long long v, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t *p, uint32_t v, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
 }
 
 // setsm_code.cpp:6509-6510: max_WNCC of cells whose heights were evaluated.
@@ -361,11 +392,13 @@ __device__ __forceinline__ void vll_fp64_run(const VllParams *__restrict__ P, co
     wave_max_commit(temp_rho, evaluated, &A.max_bits[cell], A.evaluated);
 }
 
+// One wave per run; the grid covers the capacity and the waves past the
+// count on the device return at once.
 template <int H>
 __global__ void __launch_bounds__(256) k_vll_fp64(const VllParams *__restrict__ P, CellArgs A)
 {
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (wave >= A.nruns) return;
+    if (wave >= *A.nruns_dev || wave >= A.runs_cap) return;
     vll_fp64_run<H>(P, A, A.runs[wave], wave, threadIdx.x & 63);
 }
 
@@ -806,8 +839,8 @@ hipError_t launch_vll_fill_runs(const CellArgs &A, hipStream_t s)
 
 hipError_t launch_vll_fp64(int half, const VllParams *dP, const CellArgs &A, hipStream_t s)
 {
-    if (A.nruns == 0) return hipSuccess;
-    const dim3 grid(nblk(A.nruns, 4)), block(256);   // 4 waves of 64 lanes per block
+    if (A.runs_cap == 0) return hipSuccess;
+    const dim3 grid(nblk(A.runs_cap, 4)), block(256);   // 4 waves of 64 lanes per block
     switch (half) {
     case 5: hipLaunchKernelGGL(k_vll_fp64<5>, grid, block, 0, s, dP, A); break;
     case 6: hipLaunchKernelGGL(k_vll_fp64<6>, grid, block, 0, s, dP, A); break;
@@ -819,18 +852,38 @@ hipError_t launch_vll_fp64(int half, const VllParams *dP, const CellArgs &A, hip
 
 hipError_t launch_vll_redo(int half, const VllParams *dP, const CellArgs &A, hipStream_t s)
 {
-    if (A.nruns == 0 || !A.redo) return hipSuccess;
+    if (A.runs_cap == 0 || !A.redo) return hipSuccess;
     // up to 32 waves per CU of the FP64 body over the queue; its hypotheses
     // are counted in the slot after the queue length
     CellArgs R = A;
     R.evaluated = A.redo_n + 1;
-    const dim3 grid((unsigned)std::min<long>(nblk(A.nruns, 4), 2048)), block(256);
+    const dim3 grid((unsigned)std::min<long>(nblk(A.runs_cap, 4), 2048)), block(256);
     switch (half) {
     case 5: hipLaunchKernelGGL(k_vll_redo<5>, grid, block, 0, s, dP, R); break;
     case 6: hipLaunchKernelGGL(k_vll_redo<6>, grid, block, 0, s, dP, R); break;
     case 7: hipLaunchKernelGGL(k_vll_redo<7>, grid, block, 0, s, dP, R); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_nh_stats(const smx_nccresult *ncc, long n, unsigned long long *stats, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_nh_stats, dim3(nblk(n)), dim3(256), 0, s, ncc, n, stats);
+    return hipGetLastError();
+}
+
+static unsigned fill_blocks(long n) { return (unsigned)std::max<long>(1, std::min<long>(nblk(n), 8192)); }
+
+hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, long n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_fill_u64, dim3(fill_blocks(n)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, long n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks(n)), dim3(256), 0, s, p, v, n);
     return hipGetLastError();
 }
 

@@ -13,10 +13,11 @@
 //    numbers (NCC is shift invariant);
 //  * one lane evaluates one height hypothesis; taps are LDS reads at
 //    compile-time offsets (left window) or at rotated offsets (right window);
-//  * bilinear taps and the 5 moments per channel pair are FP32; each window
-//    row's partial sums are added into per-ring totals with Kahan
-//    compensation; rings (outer / middle / inner window) are combined and
-//    the six correlations finished in FP64;
+//  * bilinear taps and the 5 moments per channel pair are FP32, about
+//    per-hypothesis pivots; each window row's partial sums (at most 2H+1
+//    samples) are added into FP64 window totals (kWide) for the nested
+//    outer / middle / inner windows, and the six correlations are finished
+//    in FP64;
 //  * a tile whose box does not fit the LDS budget reads texels straight from
 //    HBM with the same arithmetic.
 //
@@ -94,7 +95,7 @@ struct TileBox {
 // Window of one hypothesis with taps straight from HBM (tiles whose box
 // does not fit LDS, and the level-1 term of the combined INCC). Pivots are
 // the centre texels; every sample is tested like SetVecKernelValue.
-template <int H, bool kKahan>
+template <int H, bool kWide>
 __device__ __forceinline__ bool global_window(const DevView &V0, const DevView &V1, double rpx, double rpy, double tpx,
                                               double tpy, float cos0, float sin0, double &rho)
 {
@@ -108,7 +109,7 @@ __device__ __forceinline__ bool global_window(const DevView &V0, const DevView &
                        (float)(pl >> 16)};
     const GlobalImg Rg{gptr(V1.texel), (int)V1.width, (int)V1.height, itx - O, ity - O, (float)(pr & 0xffffu),
                        (float)(pr >> 16)};
-    return fast_window<H, true, kKahan>(Lg, Rg, (float)(rpx - flx), (float)(rpy - fly), (float)(tpx - ftx),
+    return fast_window<H, true, kWide>(Lg, Rg, (float)(rpx - flx), (float)(rpy - fly), (float)(tpx - ftx),
                                         (float)(tpy - fty), cos0, sin0, Lg.piv_i, Rg.piv_i, -(float)rpx,
                                         (float)((double)V0.width - 1.0 - rpx), -(float)rpy,
                                         (float)((double)V0.height - 1.0 - rpy), -(float)itx,
@@ -198,7 +199,7 @@ __device__ __forceinline__ void box_stage(const DevView *views, TileBox &tb)
 // One run (64 consecutive heights of one cell) on one wavefront. kCombined:
 // a single-pass launch may meet a combined-level iteration (the persistent
 // kernel never does: batches hold no such iteration).
-template <int H, bool kKahan, int kMode, bool kCombined = true>
+template <int H, bool kWide, int kMode, bool kCombined = true>
 __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs &A, const FastArgs &F,
                                              const TileBox &tb, const bool fits, const bool nodata, const long ri,
                                              const int lane)
@@ -258,16 +259,16 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
                 const LdsImg Lw{(ily - O - tb.y0[0]) * tb.p[0] + (ilx - O - tb.x0[0]), tb.p[0]};
                 const LdsImg Rw{tb.p[0] * tb.h[0] + (ity - O - tb.y0[1]) * tb.p[1] + (itx - O - tb.x0[1]), tb.p[1]};
                 if (fast_ok)
-                    all3 = fast_window<H, false, kKahan>(Lw, Rw, fxl, fyl, fxt, fyt, cos0, sin0, tb.piv_i[0],
+                    all3 = fast_window<H, false, kWide>(Lw, Rw, fxl, fyl, fxt, fyt, cos0, sin0, tb.piv_i[0],
                                                          tb.piv_i[1], 0, 0, 0, 0, 0, 0, 0, 0, rho);
                 else
-                    all3 = fast_window<H, true, kKahan>(
+                    all3 = fast_window<H, true, kWide>(
                         Lw, Rw, fxl, fyl, fxt, fyt, cos0, sin0, tb.piv_i[0], tb.piv_i[1], -(float)rpx,
                         (float)((double)V0.width - 1.0 - rpx), -(float)rpy,
                         (float)((double)V0.height - 1.0 - rpy), -(float)itx, (float)((int)V1.width - 1 - itx),
                         -(float)ity, (float)((int)V1.height - 1 - ity), rho);
             } else {
-                all3 = global_window<H, kKahan>(V0, V1, rpx, rpy, tpx, tpy, cos0, sin0, rho);
+                all3 = global_window<H, kWide>(V0, V1, rpx, rpy, tpx, tpy, cos0, sin0, rho);
             }
         }
         if (lane == 0 && !use_lds && __any(do_ncc)) atomicAdd(A.accessible + 3, 1ull);
@@ -275,7 +276,7 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
         bool all3n = false;
         if constexpr (kMode == kSingle && kCombined) {
             if (do_ncc && P.combined_incc)
-                all3n = global_window<H, kKahan>(P.next[0], P.next[1], hy.rnx, hy.rny, hy.tnx, hy.tny, cos0, sin0,
+                all3n = global_window<H, kWide>(P.next[0], P.next[1], hy.rnx, hy.rny, hy.tnx, hy.tny, cos0, sin0,
                                                  rho_n);
         }
         // a window this path cannot decide: the FP64 pass redoes the whole
@@ -326,7 +327,7 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
         wave_max_commit(temp_rho, evaluated, &A.max_bits[cell], A.evaluated);
     }
 
-template <int H, bool kKahan, int kMode = kSingle, bool kCombined = true>
+template <int H, bool kWide, int kMode = kSingle, bool kCombined = true>
 __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, const CellArgs &A, const FastArgs &F,
                                               const long tile)
 {
@@ -364,7 +365,7 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
         __syncthreads();
         const bool nodata = tb.nodata != 0;
         for (long ri = r0 + wave; ri < r1; ri += nwaves)
-            vll_fast_run<H, kKahan, kMode, kCombined>(P, A, F, tb, fits, nodata, ri, lane);
+            vll_fast_run<H, kWide, kMode, kCombined>(P, A, F, tb, fits, nodata, ri, lane);
         return;
     }
     // 2. Tall footprints (wide height bands, strong convergence): stage one
@@ -395,7 +396,7 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
         if (fits) box_stage(views, tb);
         __syncthreads();
         const bool nodata = tb.nodata != 0;
-        if (c0 + wave < r1) vll_fast_run<H, kKahan, kMode, kCombined>(P, A, F, tb, fits, nodata, c0 + wave, lane);
+        if (c0 + wave < r1) vll_fast_run<H, kWide, kMode, kCombined>(P, A, F, tb, fits, nodata, c0 + wave, lane);
     }
 }
 
@@ -416,8 +417,10 @@ __device__ __forceinline__ long xcd_tile(long b, long n)
 // Persistent form over the tiles of many jobs (tiles of different sizes,
 // height ranges and pyramids): each workgroup takes the next tile of its
 // XCD's contiguous range from an atomic counter and, when that range is
-// drained, steals from the other XCDs' ranges.
-template <int H, bool kKahan>
+// drained, steals from the other XCDs' ranges. kMode: kSingle for jobs
+// without a combined-level term, or one of the two passes of a combined
+// iteration over every such job's tiles.
+template <int H, int kMode>
 __device__ __forceinline__ void vll_fast_persistent(const BatchArgs &B)
 {
     __shared__ long cur;
@@ -437,54 +440,54 @@ __device__ __forceinline__ void vll_fast_persistent(const BatchArgs &B)
             int j = 0;
             while (j + 1 < B.njobs && B.jobs[j + 1].tile_begin <= t) j++;
             const BatchJob &J = B.jobs[j];
-            vll_fast_body<H, kKahan, kSingle, false>(J.P, J.A, J.F, t - J.tile_begin);
+            vll_fast_body<H, true, kMode, false>(J.P, J.A, J.F, t - J.tile_begin);
         }
     }
 }
 
-template <int H, bool kKahan>
+template <int H, int kMode>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) k_vll_fast_batch(BatchArgs B)
 {
-    vll_fast_persistent<H, kKahan>(B);
+    vll_fast_persistent<H, kMode>(B);
 }
 
 
 // Two register budgets of the same body: the compiler's own choice, and one
 // capped so that four waves fit per SIMD. The autotuner picks per shape.
-template <int H, bool kKahan>
+template <int H, bool kWide>
 __global__ void __launch_bounds__(256) k_vll_fast(const VllParams *__restrict__ Pp, CellArgs A, FastArgs F)
 {
-    vll_fast_body<H, kKahan>(Pp, A, F, xcd_tile(blockIdx.x, gridDim.x));
+    vll_fast_body<H, kWide>(Pp, A, F, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // The two passes of a combined iteration (kLevelPass, kBelowPass).
-template <int H, bool kKahan, int kMode>
+template <int H, bool kWide, int kMode>
 __global__ void __launch_bounds__(256) k_vll_fast_pass(const VllParams *__restrict__ Pp, CellArgs A, FastArgs F)
 {
-    vll_fast_body<H, kKahan, kMode>(Pp, A, F, xcd_tile(blockIdx.x, gridDim.x));
+    vll_fast_body<H, kWide, kMode>(Pp, A, F, xcd_tile(blockIdx.x, gridDim.x));
 }
 
-template <int H, bool kKahan>
+template <int H, bool kWide>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_vll_fast_o4(const VllParams *__restrict__ Pp, CellArgs A, FastArgs F)
 {
-    vll_fast_body<H, kKahan>(Pp, A, F, xcd_tile(blockIdx.x, gridDim.x));
+    vll_fast_body<H, kWide>(Pp, A, F, xcd_tile(blockIdx.x, gridDim.x));
 }
 
-template <int H, bool kKahan>
+template <int H, bool kWide>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
 k_vll_fast_o3(const VllParams *__restrict__ Pp, CellArgs A, FastArgs F)
 {
-    vll_fast_body<H, kKahan>(Pp, A, F, xcd_tile(blockIdx.x, gridDim.x));
+    vll_fast_body<H, kWide>(Pp, A, F, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 
 // Launch entry points (declared in vll_fast_types.hpp).
 template <int H>
-hipError_t fast_launch(int occ, bool kahan, dim3 grid, size_t lds, hipStream_t s, const VllParams *dP,
+hipError_t fast_launch(int occ, bool wide, dim3 grid, size_t lds, hipStream_t s, const VllParams *dP,
                        const CellArgs &A, const FastArgs &F)
 {
-    if (kahan) {
+    if (wide) {
         if (occ == 4) hipLaunchKernelGGL((k_vll_fast_o4<H, true>), grid, dim3(256), lds, s, dP, A, F);
         else if (occ == 3) hipLaunchKernelGGL((k_vll_fast_o3<H, true>), grid, dim3(256), lds, s, dP, A, F);
         else hipLaunchKernelGGL((k_vll_fast<H, true>), grid, dim3(256), lds, s, dP, A, F);
@@ -497,10 +500,10 @@ hipError_t fast_launch(int occ, bool kahan, dim3 grid, size_t lds, hipStream_t s
 }
 
 template <int H>
-hipError_t fast_launch_passes(bool kahan, dim3 g1, dim3 g2, size_t lds, hipStream_t s, const VllParams *dP,
+hipError_t fast_launch_passes(bool wide, dim3 g1, dim3 g2, size_t lds, hipStream_t s, const VllParams *dP,
                               const CellArgs &A, const FastArgs &F1, const FastArgs &F2)
 {
-    if (kahan) {
+    if (wide) {
         hipLaunchKernelGGL((k_vll_fast_pass<H, true, kLevelPass>), g1, dim3(256), lds, s, dP, A, F1);
         hipLaunchKernelGGL((k_vll_fast_pass<H, true, kBelowPass>), g2, dim3(256), lds, s, dP, A, F2);
     } else {
@@ -511,20 +514,27 @@ hipError_t fast_launch_passes(bool kahan, dim3 g1, dim3 g2, size_t lds, hipStrea
 }
 
 template <int H>
-hipError_t fast_launch_batch(bool kahan, int grid, size_t lds, hipStream_t s, const BatchArgs &B)
+hipError_t fast_launch_batch(int mode, int grid, size_t lds, hipStream_t s, const BatchArgs &B)
 {
-    if (kahan)
-        hipLaunchKernelGGL((k_vll_fast_batch<H, true>), dim3(grid), dim3(256), lds, s, B);
+    if (mode == kLevelPass)
+        hipLaunchKernelGGL((k_vll_fast_batch<H, kLevelPass>), dim3(grid), dim3(256), lds, s, B);
+    else if (mode == kBelowPass)
+        hipLaunchKernelGGL((k_vll_fast_batch<H, kBelowPass>), dim3(grid), dim3(256), lds, s, B);
     else
-        hipLaunchKernelGGL((k_vll_fast_batch<H, false>), dim3(grid), dim3(256), lds, s, B);
+        hipLaunchKernelGGL((k_vll_fast_batch<H, kSingle>), dim3(grid), dim3(256), lds, s, B);
     return hipGetLastError();
 }
 
 template <int H>
-int fast_batch_blocks_per_cu(size_t lds)
+int fast_batch_blocks_per_cu(int mode, size_t lds)
 {
     int per_cu = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vll_fast_batch<H, true>, 256, lds);
+    if (mode == kLevelPass)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vll_fast_batch<H, kLevelPass>, 256, lds);
+    else if (mode == kBelowPass)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vll_fast_batch<H, kBelowPass>, 256, lds);
+    else
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vll_fast_batch<H, kSingle>, 256, lds);
     return per_cu;
 }
 

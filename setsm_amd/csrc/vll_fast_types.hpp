@@ -18,6 +18,7 @@ struct FastArgs {
     uint32_t grid_w, grid_h;
     int lds_texels;                // float2 texels of LDS for both images' boxes
     int stage;                     // 0: read every tap from HBM
+    long runs_cap;                 // capacity of tile_runs
 };
 
 // Launch modes of the FAST kernel. kSingle: the whole hypothesis in one pass
@@ -46,29 +47,34 @@ struct BatchArgs {
     unsigned long long *next;   // 8 per-XCD work counters
 };
 
+// A launch shape: cells per tile, LDS texels for both boxes, waves per SIMD
+// the register budget is capped for (0: the compiler's choice), LDS staging
+// on/off, and whether window totals are FP64 sums of the FP32 row partials
+// (wide) or FP32 sums.
 struct FastCfg {
-    int tx, ty, lds_texels, occ, stage, kahan;
+    int tx, ty, lds_texels, occ, stage, wide;
 };
 
-// Launch entry points, one set per half-window size H (5, 6, 7).
+// Launch entry points, one set per half-window size H (5, 6, 7). The
+// persistent batch kernel always keeps FP64 window totals.
 template <int H>
-hipError_t fast_launch(int occ, bool kahan, dim3 grid, size_t lds, hipStream_t s, const VllParams *dP,
+hipError_t fast_launch(int occ, bool wide, dim3 grid, size_t lds, hipStream_t s, const VllParams *dP,
                        const CellArgs &A, const FastArgs &F);
 template <int H>
-hipError_t fast_launch_passes(bool kahan, dim3 g1, dim3 g2, size_t lds, hipStream_t s, const VllParams *dP,
+hipError_t fast_launch_passes(bool wide, dim3 g1, dim3 g2, size_t lds, hipStream_t s, const VllParams *dP,
                               const CellArgs &A, const FastArgs &F1, const FastArgs &F2);
 template <int H>
-hipError_t fast_launch_batch(bool kahan, int grid, size_t lds, hipStream_t s, const BatchArgs &B);
+hipError_t fast_launch_batch(int mode, int grid, size_t lds, hipStream_t s, const BatchArgs &B);
 template <int H>
-int fast_batch_blocks_per_cu(size_t lds);
+int fast_batch_blocks_per_cu(int mode, size_t lds);
 
 #define SMX_FAST_EXTERN(H)                                                                                         \
     extern template hipError_t fast_launch<H>(int, bool, dim3, size_t, hipStream_t, const VllParams *,              \
                                               const CellArgs &, const FastArgs &);                                \
     extern template hipError_t fast_launch_passes<H>(bool, dim3, dim3, size_t, hipStream_t, const VllParams *,     \
                                                      const CellArgs &, const FastArgs &, const FastArgs &);       \
-    extern template hipError_t fast_launch_batch<H>(bool, int, size_t, hipStream_t, const BatchArgs &);           \
-    extern template int fast_batch_blocks_per_cu<H>(size_t);
+    extern template hipError_t fast_launch_batch<H>(int, int, size_t, hipStream_t, const BatchArgs &);            \
+    extern template int fast_batch_blocks_per_cu<H>(int, size_t);
 SMX_FAST_EXTERN(5)
 SMX_FAST_EXTERN(6)
 SMX_FAST_EXTERN(7)

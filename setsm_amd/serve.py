@@ -190,8 +190,7 @@ class Server:
             groups = {}
             tk = time.perf_counter()
             for a, lv in zip(step, ex.map(self._begin_level, step)):
-                half = 7 if lv >= 4 else (6 if lv == 3 else 5)
-                groups.setdefault(half, []).append(a)
+                groups.setdefault(a.job.half, []).append(a)   # the library's Half_template_size
             phase["begin_s"] += time.perf_counter() - tk
             for half, grp in groups.items():
                 tk = time.perf_counter()

@@ -36,6 +36,7 @@ EXPORTS = [
     "smx_job_get_sumcost", "smx_job_set_ortho_wgs",
     "smx_job_last_kernel_ms", "smx_job_last_evaluated", "smx_job_last_redo", "smx_job_sync", "smx_fast_num_configs",
     "smx_job_set_fast_config", "smx_fast_config_desc", "smx_job_last_stats", "smx_vertical_line_locus_batch",
+    "smx_job_half", "smx_graph_create", "smx_graph_launch", "smx_graph_last_ms", "smx_graph_destroy",
 ]
 
 
@@ -92,6 +93,11 @@ def lib():
             "smx_fast_config_desc": (i, [i, C.c_char_p, i]),
             "smx_job_last_stats": (i, [vp, vp]),
             "smx_vertical_line_locus_batch": (i, [vp, i, i, i, i, vp]),
+            "smx_job_half": (i, [vp]),
+            "smx_graph_create": (i, [C.POINTER(vp), vp, i, i, i, i]),
+            "smx_graph_launch": (i, [vp, vp]),
+            "smx_graph_last_ms": (C.c_double, [vp]),
+            "smx_graph_destroy": (i, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -150,9 +156,18 @@ class Job:
         _check(lib().smx_job_set_view(self.h, img, level, _p(a), _p(m), _p(o), w, h, float(start[0]), float(start[1])))
 
     def build_pyramid(self, img, level0, start0=(0.0, 0.0)):
-        a = np.ascontiguousarray(level0, np.uint16)
-        h, w = a.shape
-        _check(lib().smx_job_build_pyramid(self.h, img, _p(a), w, h, float(start0[0]), float(start0[1])))
+        """level0: uint16 (H, W) numpy array, or a contiguous uint16 torch
+        tensor already on the GPU (copied device to device)."""
+        if hasattr(level0, "data_ptr"):
+            if not level0.is_contiguous() or level0.element_size() != 2 or level0.dim() != 2:
+                raise SmxError("build_pyramid: a 2-D contiguous 16-bit tensor is required")
+            h, w = level0.shape
+            ptr = level0.data_ptr()
+        else:
+            a = np.ascontiguousarray(level0, np.uint16)
+            h, w = a.shape
+            ptr = _p(a)
+        _check(lib().smx_job_build_pyramid(self.h, img, ptr, w, h, float(start0[0]), float(start0[1])))
 
     def get_view(self, img, level, shape):
         h, w = shape
@@ -252,6 +267,11 @@ class Job:
         return dict(evaluated=int(out[0]), tiles_lds=int(out[1]), waves_hbm=int(out[2]), fast_config=int(out[3]),
                     redo_runs=runs.value, redo_hyps=hyps.value, redo_ms=ms.value)
 
+    @property
+    def half(self) -> int:
+        """Half_template_size of the current level (the window size batches group by)."""
+        return _check(lib().smx_job_half(self.h))
+
     def set_fast_config(self, config: int):
         _check(lib().smx_job_set_fast_config(self.h, int(config)))
 
@@ -260,7 +280,9 @@ class Job:
 
 
 def fast_configs():
-    """[(index, (tx, ty, lds_rows, o4))] of the SMX_PREC_FAST launch shapes."""
+    """[(index, (tx, ty, lds_texels, occ, stage, wide))] of the SMX_PREC_FAST
+    launch shapes: cells per tile, LDS texels of both boxes, register cap
+    (waves per SIMD, 0 = compiler's), LDS staging, FP64 window totals."""
     out = []
     buf = C.create_string_buffer(64)
     for i in range(lib().smx_fast_num_configs()):
@@ -276,3 +298,36 @@ def vertical_line_locus_batch(jobs, iteration: int, precision: int = PREC_FAST, 
     _check(lib().smx_vertical_line_locus_batch(arr, len(jobs), int(iteration), int(precision),
                                                1 if keep_rho else 0, _p(acc)))
     return acc
+
+
+class Graph:
+    """One iteration's VerticalLineLocus (+ AWNCC) over `jobs`, captured as a
+    hipGraph (smx_graph_create) and replayed by launch()."""
+
+    def __init__(self, jobs, iteration: int, precision: int = PREC_FAST, with_awncc: bool = True):
+        self.jobs = list(jobs)
+        arr = (C.c_void_p * len(self.jobs))(*[j.h.value for j in self.jobs])
+        h = C.c_void_p()
+        _check(lib().smx_graph_create(C.byref(h), arr, len(self.jobs), int(iteration), int(precision),
+                                      1 if with_awncc else 0))
+        self.h = h
+
+    def launch(self):
+        acc = np.zeros(len(self.jobs), np.int64)
+        _check(lib().smx_graph_launch(self.h, _p(acc)))
+        return acc
+
+    @property
+    def last_ms(self) -> float:
+        return float(lib().smx_graph_last_ms(self.h))
+
+    def close(self):
+        if self.h:
+            lib().smx_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
