@@ -135,6 +135,20 @@ long long smx_vertical_line_locus(smx_job *job, int iteration, int precision, in
 int  smx_vertical_line_locus_batch(smx_job **jobs, int n, int iteration, int precision, int keep_rho,
                                    long long *accessible);
 
+/* smx_vertical_line_locus without any device allocation: the call's scratch
+ * (height-run list, per-tile run lists, FP64 hand-over queue, level term of a
+ * combined iteration, scan partials, counters) comes from the library's
+ * static 64 MiB device array instead of the job. No counterpart in the
+ * reference: for callers that cannot allocate (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The job must have
+ * been set up by the allocating calls (set_level, set_searchspace,
+ * initialize_voxel). Levels needing more scratch than the array holds (about
+ * 2.7 M height runs, ~170 M hypotheses) fail with SMX_E_NOMEM; RA and
+ * matching-rate passes with SMX_E_UNSUPPORTED. The mean NCC is not kept.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU race. */
+long long smx_vertical_line_locus_noalloc(smx_job *job, int iteration, int precision);
+
 /* Half_template_size of the job's current level (setsm_code.cpp:6057-6077):
  * the window size a batch groups jobs by. */
 int  smx_job_half(const smx_job *job);

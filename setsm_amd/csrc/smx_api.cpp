@@ -284,6 +284,48 @@ static int refresh_stats(smx_job *j)
     return 0;
 }
 
+}  // namespace smx
+
+// ------------------------------------------------------------ static scratch
+//
+// Scratch of smx_vertical_line_locus_noalloc: one array per GPU, held from
+// the library's load until the process ends. Host code never passes the
+// array's name to a kernel or a copy (on the host it is only a stand-in for
+// the symbol); it uses the device address hipGetSymbolAddress returns, taken
+// once per device at job set-up with that device current.
+static constexpr size_t kStaticScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace smx {
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, unsigned char *> g_scratch_ptr;   // device -> address of `scratch` on it
+std::map<int, int> g_scratch_err;               // device -> hipGetSymbolAddress error
+}  // namespace
+
+// Once per device, with that device current.
+static void scratch_setup(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr.count(device) || g_scratch_err.count(device)) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p)
+        g_scratch_ptr[device] = (unsigned char *)p;
+    else
+        g_scratch_err[device] = e == hipSuccess ? (int)hipErrorInvalidSymbol : (int)e;
+}
+
+static unsigned char *scratch_of(int device, int &err)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it != g_scratch_ptr.end()) return it->second;
+    auto e = g_scratch_err.find(device);
+    err = e != g_scratch_err.end() ? e->second : (int)hipErrorNotInitialized;
+    return nullptr;
+}
+
 // ---------------------------------------------------- VerticalLineLocus
 
 // Host-side decisions of one VerticalLineLocus call.
@@ -377,6 +419,68 @@ static int vll_reserve(smx_job *j, const VllPlan &p)
     if (p.keep_rho && !j->d_rho && j->vox_total > 0) {
         if ((r = jalloc(j, &j->d_rho, (size_t)j->vox_total))) return r;
         HIPCHK(launch_fill_u32((uint32_t *)j->d_rho, 0xffffffffu, j->vox_total, j->stream));   // NaN
+    }
+    return 0;
+}
+
+// Carves the call's scratch out of the static array (no-alloc entry point).
+static int carve_scratch(smx_job *j, const VllPlan &p, VllScratch &S)
+{
+    int err = 0;
+    unsigned char *base = scratch_of(j->device, err);
+    if (!base)
+        return fail(SMX_E_HIP, "static scratch unavailable on device %d: hipGetSymbolAddress: %s", j->device,
+                    hipGetErrorString((hipError_t)err));
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        off = (off + 255) & ~(size_t)255;
+        const size_t at = off;
+        off += std::max<size_t>(bytes, 8);
+        return at;
+    };
+    const long runs = (long)j->runs_bound;
+    const long nscan = scan_partials(j->ncells + 1);
+    const size_t o_cnt = take(kCounters * 8), o_scan = take((size_t)nscan * 8), o_runs = take((size_t)runs * 8);
+    size_t o_tc = 0, o_to = 0, o_tr = 0, o_redo = 0, o_t2c = 0, o_t2o = 0, o_t2r = 0, o_part = 0;
+    long t1 = 0, t2 = 0;
+    if (p.fast) {
+        t1 = fast_tiles(j, 0);
+        o_tc = take(((size_t)t1 + 1) * 8);
+        o_to = take(((size_t)t1 + 1) * 8);
+        o_tr = take((size_t)runs * 8);
+        o_redo = take((size_t)runs * 8);
+        if (j->hp.combined_incc) {
+            t2 = fast_tiles(j, 1);
+            o_t2c = take(((size_t)t2 + 1) * 8);
+            o_t2o = take(((size_t)t2 + 1) * 8);
+            o_t2r = take((size_t)runs * 8);
+            o_part = take((size_t)j->vox_total * 8);
+        }
+    }
+    if (off > kStaticScratchBytes)
+        return fail(SMX_E_NOMEM, "smx_vertical_line_locus_noalloc: this level needs %zu bytes of scratch (%ld height "
+                                 "runs), the static array holds %zu; use smx_vertical_line_locus",
+                    off, runs, kStaticScratchBytes);
+    S = VllScratch{};
+    S.counters = (unsigned long long *)(base + o_cnt);
+    S.scan_part = (int64_t *)(base + o_scan);
+    S.scan_part_cap = nscan;
+    S.runs = (int2 *)(base + o_runs);
+    S.runs_cap = runs;
+    if (p.fast) {
+        S.tile_count = (int64_t *)(base + o_tc);
+        S.tile_off = (int64_t *)(base + o_to);
+        S.tile_runs = (int2 *)(base + o_tr);
+        S.tiles_cap = t1;
+        S.redo = (int2 *)(base + o_redo);
+        if (j->hp.combined_incc) {
+            S.tile2_count = (int64_t *)(base + o_t2c);
+            S.tile2_off = (int64_t *)(base + o_t2o);
+            S.tile2_runs = (int2 *)(base + o_t2r);
+            S.tiles2_cap = t2;
+            S.part = (double *)(base + o_part);
+            S.part_cap = j->vox_total;
+        }
     }
     return 0;
 }
@@ -636,6 +740,8 @@ int smx_job_create(smx_job **out, int num_levels)
         smx_job_destroy(j);
         return fail(SMX_E_HIP, "job setup: %s", hipGetErrorString(e));
     }
+    // the no-alloc entry point's scratch address on this device
+    scratch_setup(j->device);
     int r = jalloc(j, &j->d_params, 1);
     if (!r) r = jalloc(j, &j->js.counters, kCounters);
     if (r) {
@@ -1022,6 +1128,23 @@ long long smx_vertical_line_locus(smx_job *j, int iteration, int precision, int 
     HIPCHK(hipSetDevice(j->device));
     if ((r = vll_reserve(j, p))) return r;
     return vll_run(j, p, j->js, true);
+}
+
+long long smx_vertical_line_locus_noalloc(smx_job *j, int iteration, int precision)
+{
+    VllPlan p;
+    int r = vll_plan(j, iteration, precision, 0, p);
+    if (r) return r;
+    if (p.ra)
+        return fail(SMX_E_UNSUPPORTED, "smx_vertical_line_locus_noalloc: RA and matching-rate passes grow their run "
+                                       "list mid-call; use smx_vertical_line_locus");
+    if (!j->stats_valid)
+        return fail(SMX_E_STATE, "smx_vertical_line_locus_noalloc: the height lists changed since the last "
+                                 "smx_initialize_voxel / smx_job_set_voxels; run one first");
+    HIPCHK(hipSetDevice(j->device));
+    VllScratch S;
+    if ((r = carve_scratch(j, p, S))) return r;
+    return vll_run(j, p, S, false);
 }
 
 int smx_vertical_line_locus_batch(smx_job **jobs, int n, int iteration, int precision, int keep_rho,

@@ -36,7 +36,7 @@ EXPORTS = [
     "smx_job_get_sumcost", "smx_job_set_ortho_wgs",
     "smx_job_last_kernel_ms", "smx_job_last_evaluated", "smx_job_last_redo", "smx_job_sync", "smx_fast_num_configs",
     "smx_job_set_fast_config", "smx_fast_config_desc", "smx_job_last_stats", "smx_vertical_line_locus_batch",
-    "smx_job_half", "smx_graph_create", "smx_graph_launch", "smx_graph_last_ms", "smx_graph_destroy",
+    "smx_vertical_line_locus_noalloc", "smx_job_half", "smx_graph_create", "smx_graph_launch", "smx_graph_last_ms", "smx_graph_destroy",
 ]
 
 
@@ -93,6 +93,7 @@ def lib():
             "smx_fast_config_desc": (i, [i, C.c_char_p, i]),
             "smx_job_last_stats": (i, [vp, vp]),
             "smx_vertical_line_locus_batch": (i, [vp, i, i, i, i, vp]),
+            "smx_vertical_line_locus_noalloc": (ll, [vp, i, i]),
             "smx_job_half": (i, [vp]),
             "smx_graph_create": (i, [C.POINTER(vp), vp, i, i, i, i]),
             "smx_graph_launch": (i, [vp, vp]),
@@ -213,6 +214,11 @@ class Job:
 
     def vertical_line_locus(self, iteration: int, precision: int = PREC_FP64, keep_rho: bool = False) -> int:
         return _check(lib().smx_vertical_line_locus(self.h, int(iteration), int(precision), 1 if keep_rho else 0))
+
+    def vertical_line_locus_noalloc(self, iteration: int, precision: int = PREC_FP64) -> int:
+        """smx_vertical_line_locus without device allocation (the library's
+        static scratch array; no mean NCC kept)."""
+        return _check(lib().smx_vertical_line_locus_noalloc(self.h, int(iteration), int(precision)))
 
     def awncc(self, iteration: int):
         _check(lib().smx_awncc(self.h, int(iteration)))
