@@ -8,15 +8,17 @@
  * operand order, same casts, no FMA contraction: build with
  * -ffp-contract=off), so the FP64 GPU kernel can be compared bit for bit.
  *
- * Pins (checks that do not just retype these formulas) live in
- * tests/test_oracle.py: closed-form RPCs, window counts, numpy corrcoef,
- * brute-force bilinear, hand peak sequences, and the real reference
- * functions built into oracle/_ref where they compile from their own sources.
+ * Pins (checks that do not just retype these formulas) live in tests/:
+ * closed-form RPCs and the real reference header code built into oracle/_ref
+ * (test_oracle.py), window counts, numpy corrcoef, brute-force bilinear,
+ * hand-derived tables for SetNCC_alpha, check_image_boundary, the
+ * VerticalLineLocus gate, InitializeVoxel and FindPeakNcc
+ * (test_oracle_pins.py), a brute-force GNCC (test_oracle_gncc.py) and
+ * closed-form SGM cases (test_oracle_sgm.py).
  *
- * Scope limits (parity unpinned / unsupported here and in the product):
- *  - two images only (reference_id 0, ti 1) and the SB (RPC) sensor;
- *  - the ortho GNCC term (setsm_code.cpp:6240-6305) is not computed: a call
- *    where it would be active returns -2 (see orc_gncc_active()).
+ * Scope: two images (reference_id 0 or 1, ti the other) and the SB (RPC)
+ * sensor. The ortho GNCC term (setsm_code.cpp:6240-6305) is computed, from
+ * the ortho-grid lon/lat the caller supplies (smx_level.ortho_wgs).
  */
 #include "smx_oracle.h"
 
@@ -649,6 +651,60 @@ static double gncc_cell(const smx_level *Lv, const smx_ugrid *GridPT3, long pt_i
     return count_GNCC;
 }
 
+
+/* setsm_code.cpp:6672-6752 FindPeakNcc: one call per evaluated height, in
+ * height order, with the caller's check_rho / pre_rho / pre_height /
+ * direction / max_WNCC kept in st. (The Pyramid_step / iteration branches of
+ * the reference all apply the same peak test.) */
+void orc_peak_state_init(orc_peak_state *st)
+{
+    st->check_rho = 0;
+    st->pre_rho = -1.0;
+    st->pre_height = 0.0f;
+    st->direction = 0;
+    st->max_wncc = -100.0;
+}
+
+void orc_find_peak_ncc(double temp_rho, float iter_height, orc_peak_state *st, smx_nccresult *nr)
+{
+    double diff_rho;
+    int t_direction;
+    if (temp_rho >= 0 && st->pre_rho >= 0) diff_rho = temp_rho - st->pre_rho;
+    else if (temp_rho < 0 && st->pre_rho < 0) diff_rho = -(fabs(temp_rho) - fabs(st->pre_rho));
+    else if (temp_rho >= 0 && st->pre_rho < 0) diff_rho = 1;
+    else diff_rho = -1;
+    if (diff_rho > 0) t_direction = 1;
+    else if (diff_rho < 0) t_direction = -1;
+    else t_direction = 0;
+    if (!st->check_rho) {
+        st->check_rho = 1;
+        t_direction = -1;
+    } else if (st->pre_rho != -1) {
+        if (st->direction > 0 && t_direction < 0) {
+            if (nr->result4 < 10) nr->result4 += 1;
+            if (nr->result0 < q_result(st->pre_rho)) {
+                double temp_1 = dq(nr->result0);
+                nr->result0 = q_result(st->pre_rho);
+                float temp_2 = nr->result2;
+                nr->result2 = st->pre_height;
+                if (nr->result1 < q_result(temp_1)) {
+                    nr->result1 = q_result(temp_1);
+                    nr->result3 = temp_2;
+                }
+            } else {
+                if (nr->result1 < q_result(st->pre_rho)) {
+                    nr->result1 = q_result(st->pre_rho);
+                    nr->result3 = st->pre_height;
+                }
+            }
+        }
+    }
+    st->pre_rho = temp_rho;
+    st->pre_height = iter_height;
+    st->direction = t_direction;
+    if (st->max_wncc < temp_rho) st->max_wncc = temp_rho;
+}
+
 /* --------------------------------------------------- VerticalLineLocus */
 
 long orc_vertical_line_locus(const smx_level *Lv, orc_voxels *vox, smx_nccresult *nccresult,
@@ -782,10 +838,8 @@ long orc_vertical_line_locus(const smx_level *Lv, orc_voxels *vox, smx_nccresult
                 nr->result4 = 0;
 
                 /* FindPeakNcc state (check_matching_rate mode only) */
-                double pre_rho = -1.0;
-                float pre_height = 0.0f;
-                int direction = 0;
-                int check_rho = 0;
+                orc_peak_state peak;
+                orc_peak_state_init(&peak);
 
                 for (int hidx = 0; hidx < nr->NumOfHeight; hidx++) {
                     float iter_height;
@@ -892,45 +946,7 @@ long orc_vertical_line_locus(const smx_level *Lv, orc_voxels *vox, smx_nccresult
                         temp_rho = -1.0;
                     if (max_WNCC < temp_rho) max_WNCC = temp_rho;
 
-                    if (check_matching_rate) {
-                        /* setsm_code.cpp:6672-6752 FindPeakNcc */
-                        double diff_rho;
-                        int t_direction;
-                        if (temp_rho >= 0 && pre_rho >= 0) diff_rho = temp_rho - pre_rho;
-                        else if (temp_rho < 0 && pre_rho < 0) diff_rho = -(fabs(temp_rho) - fabs(pre_rho));
-                        else if (temp_rho >= 0 && pre_rho < 0) diff_rho = 1;
-                        else diff_rho = -1;
-                        if (diff_rho > 0) t_direction = 1;
-                        else if (diff_rho < 0) t_direction = -1;
-                        else t_direction = 0;
-                        if (!check_rho) {
-                            check_rho = 1;
-                            t_direction = -1;
-                        } else if (pre_rho != -1) {
-                            if (direction > 0 && t_direction < 0) {
-                                if (nr->result4 < 10) nr->result4 += 1;
-                                if (nr->result0 < q_result(pre_rho)) {
-                                    double temp_1 = dq(nr->result0);
-                                    nr->result0 = q_result(pre_rho);
-                                    float temp_2 = nr->result2;
-                                    nr->result2 = pre_height;
-                                    if (nr->result1 < q_result(temp_1)) {
-                                        nr->result1 = q_result(temp_1);
-                                        nr->result3 = temp_2;
-                                    }
-                                } else {
-                                    if (nr->result1 < q_result(pre_rho)) {
-                                        nr->result1 = q_result(pre_rho);
-                                        nr->result3 = pre_height;
-                                    }
-                                }
-                            }
-                        }
-                        pre_rho = temp_rho;
-                        pre_height = iter_height;
-                        direction = t_direction;
-                        if (max_WNCC < temp_rho) max_WNCC = temp_rho;
-                    }
+                    if (check_matching_rate) orc_find_peak_ncc(temp_rho, iter_height, &peak, nr);
                 }
                 if (nr->check_height_change)
                     nr->max_WNCC = q_result(max_WNCC);

@@ -52,6 +52,20 @@ int    orc_check_image_boundary(const smx_level *L, long cell, double minH, doub
 /* setsm_code.cpp:6014-6047 SetNCC_alpha */
 double orc_ncc_alpha(int level, int iteration, int is_ra);
 
+/* setsm_code.cpp:6672-6752 FindPeakNcc, one evaluated height per call in
+ * height order; st carries check_rho, pre_rho, pre_height, direction and
+ * max_WNCC between calls (orc_peak_state_init gives the reference's initial
+ * values, setsm_code.cpp:6343-6347). */
+typedef struct orc_peak_state {
+    int    check_rho;
+    double pre_rho;
+    float  pre_height;
+    int    direction;
+    double max_wncc;
+} orc_peak_state;
+void orc_peak_state_init(orc_peak_state *st);
+void orc_find_peak_ncc(double temp_rho, float iter_height, orc_peak_state *st, smx_nccresult *nr);
+
 /* setsm_code.cpp:5759-6012 InitializeVoxel */
 void orc_initialize_voxel(const smx_level *L, orc_voxels *vox, smx_ugrid *gridpt3,
                           smx_nccresult *ncc, int iteration, const double minmax[2]);
