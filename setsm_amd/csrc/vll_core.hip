@@ -392,14 +392,16 @@ __device__ __forceinline__ void vll_fp64_run(const VllParams *__restrict__ P, co
     wave_max_commit(temp_rho, evaluated, &A.max_bits[cell], A.evaluated);
 }
 
-// One wave per run; the grid covers the capacity and the waves past the
-// count on the device return at once.
+// One wave per run, grid-stride over the count on the device (the launch
+// covers the capacity up to a cap: a grid of one wave per run would pass
+// 2^32 work-items at a few 10^9 hypotheses).
 template <int H>
 __global__ void __launch_bounds__(256) k_vll_fp64(const VllParams *__restrict__ P, CellArgs A)
 {
-    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (wave >= *A.nruns_dev || wave >= A.runs_cap) return;
-    vll_fp64_run<H>(P, A, A.runs[wave], wave, threadIdx.x & 63);
+    const long nw = ((long)gridDim.x * blockDim.x) >> 6;
+    const long n = min((long)*A.nruns_dev, A.runs_cap);
+    for (long q = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < n; q += nw)
+        vll_fp64_run<H>(P, A, A.runs[q], q, threadIdx.x & 63);
 }
 
 // The runs the FAST kernel handed over (a window whose spread it cannot tell
@@ -670,23 +672,24 @@ __global__ void __launch_bounds__(256) k_init_voxel(const VllParams *__restrict_
 // initialises its row in 64-word strides.
 __global__ void __launch_bounds__(256) k_relayout(InitArgs A)
 {
-    const long c = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    if (c >= A.ncells) return;
-    const int act = A.action[c];
-    const int n = A.new_len[c];
-    if (act == 0 || n == 0) return;
-    const long dst = A.new_off[c];
-    const long src = A.old_off[c];
-    for (int h = lane; h < n; h += 64) {
-        uint32_t w = SMX_VOXEL_WORD(1, -1000);
-        float r = __builtin_nanf("");
-        if (act == 1) {
-            w = A.old_vox[src + h];
-            if (A.old_rho) r = A.old_rho[src + h];
+    const long nw = ((long)gridDim.x * blockDim.x) >> 6;
+    for (long c = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; c < A.ncells; c += nw) {
+        const int act = A.action[c];
+        const int n = A.new_len[c];
+        if (act == 0 || n == 0) continue;
+        const long dst = A.new_off[c];
+        const long src = A.old_off[c];
+        for (int h = lane; h < n; h += 64) {
+            uint32_t w = SMX_VOXEL_WORD(1, -1000);
+            float r = __builtin_nanf("");
+            if (act == 1) {
+                w = A.old_vox[src + h];
+                if (A.old_rho) r = A.old_rho[src + h];
+            }
+            A.new_vox[dst + h] = w;
+            if (A.new_rho) A.new_rho[dst + h] = r;
         }
-        A.new_vox[dst + h] = w;
-        if (A.new_rho) A.new_rho[dst + h] = r;
     }
 }
 
@@ -840,7 +843,8 @@ hipError_t launch_vll_fill_runs(const CellArgs &A, hipStream_t s)
 hipError_t launch_vll_fp64(int half, const VllParams *dP, const CellArgs &A, hipStream_t s)
 {
     if (A.runs_cap == 0) return hipSuccess;
-    const dim3 grid(nblk(A.runs_cap, 4)), block(256);   // 4 waves of 64 lanes per block
+    // 4 waves of 64 lanes per block, at most 2^20 blocks
+    const dim3 grid((unsigned)std::min<long>(nblk(A.runs_cap, 4), 1L << 20)), block(256);
     switch (half) {
     case 5: hipLaunchKernelGGL(k_vll_fp64<5>, grid, block, 0, s, dP, A); break;
     case 6: hipLaunchKernelGGL(k_vll_fp64<6>, grid, block, 0, s, dP, A); break;
@@ -901,7 +905,8 @@ hipError_t launch_init_voxel(const VllParams *dP, const InitArgs &A, hipStream_t
 
 hipError_t launch_relayout(const InitArgs &A, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_relayout, dim3(nblk(A.ncells, 4)), dim3(256), 0, s, A);
+    // one wave per cell, grid-stride beyond 2^20 blocks
+    hipLaunchKernelGGL(k_relayout, dim3((unsigned)std::min<long>(nblk(A.ncells, 4), 1L << 20)), dim3(256), 0, s, A);
     return hipGetLastError();
 }
 

@@ -336,3 +336,45 @@ def ortho_wgs(scene: Scene, boundary, level: int, resolution: float = 0.5, x_off
     XX, YY = np.meshgrid(X, Y)
     lon, lat = scene.to_wgs(XX - x_offset, YY - y_offset)
     return np.stack([lon, lat], -1)
+
+
+@dataclass
+class TileDraw:
+    """One tile of the mixed-size workload: cells per side (2 m cells),
+    hypotheses per cell, terrain relief, view convergence and scene seed."""
+    side: int
+    nheights: int
+    relief: float
+    convergence: float
+    seed: int
+
+
+def draw_tiles(n, seed, sides=(512, 4096), heights=(32, 256), relief=(50.0, 1500.0), convergence=(15.0, 40.0)):
+    """Seeded draws: side log-uniform in `sides`, heights uniform integers in
+    `heights`, relief and convergence uniform, an independent scene seed each."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        side = int(round(math.exp(rng.uniform(math.log(sides[0]), math.log(sides[1])))))
+        out.append(TileDraw(side, int(rng.integers(heights[0], heights[1] + 1)), float(rng.uniform(*relief)),
+                            float(rng.uniform(*convergence)), int(rng.integers(1, 1 << 30))))
+    return out
+
+
+def tile_image_px(d: TileDraw, gsd=0.5, grid_res=2.0, step=0.25):
+    """Square image side that holds the tile's cell grid plus the parallax of
+    every height of its bands (relief + band, each view at convergence / 2)
+    and the window margins."""
+    band = d.nheights * step
+    tan = math.tan(math.radians(d.convergence / 2.0))
+    margin = int(math.ceil((d.relief + band + 40.0) * tan / gsd)) + 32
+    return int(round(d.side * grid_res / gsd)) + 2 * margin
+
+
+def render_tile(d: TileDraw, device="cuda", base=200.0, gsd=0.5):
+    """Scene and level-0 cell grid (2 m cells, d.nheights heights at the
+    0.25 m step around the true surface) of one drawn tile."""
+    spec = SceneSpec(image_px=tile_image_px(d, gsd), gsd=gsd, relief=d.relief, base=base,
+                     view_deg=(d.convergence / 2.0, -d.convergence / 2.0), seed=d.seed)
+    sc = make_scene(spec, device=device)
+    return sc, level_grid(sc, 0, d.side, d.nheights, seed=d.seed)
