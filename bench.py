@@ -1,19 +1,35 @@
 #!/usr/bin/env python3
 """Benchmark of the VerticalLineLocus hot path on one MI355X.
 
-Workload (one pyramid level at full scale): a 16384 x 16384 uint16 11-bit
-stereo pair rendered from a seeded fBm terrain over 8 km (Hurst 0.8, 300 m
-relief, 0.5 m GSD, along-track views +12 / -12 degrees, 80-coefficient RPCs),
-a 4000 x 4000 cell grid at 2 m, 128 height hypotheses per cell at the level-0
-step of 0.25 m (a 32 m band around the true height), 5x... windows of the
-reference geometry (11x11 disc / 7x7 / 5x5, intensity + Sobel magnitude).
+Headline line (`value`): one pyramid level at full scale — a 16384 x 16384
+uint16 11-bit stereo pair rendered from a seeded fBm terrain over 8 km (Hurst
+0.8, 300 m relief, 0.5 m GSD, along-track views +12 / -12 degrees,
+80-coefficient RPCs), a 4000 x 4000 cell grid at 2 m, 128 height hypotheses
+per cell at the level-0 step of 0.25 m (a 32 m band around the true height),
+windows of the reference geometry (11x11 disc / 7x7 disc / 5x5, intensity +
+Sobel magnitude), in the precision form (FP32 moments, FP64 RPC and window
+totals). One step = one VerticalLineLocus call over the whole level (gate,
+run lists, NCC kernel, FP64 hand-over, max_WNCC) on inputs resident in HBM.
+The metric is evaluated height hypotheses per second.
 
-One step = one VerticalLineLocus call over the whole level (gate + run list +
-NCC kernel + max_WNCC) on inputs already resident in HBM. The metric is
-evaluated height hypotheses per second. The same run times the plain FP64
-kernel (bit-identical to the CPU path) and, on a bounded stripe of cells, the
-CPU FP64 restatement of the reference's OpenMP loop (the reported CPU
-baseline), and checks the GPU cube against it.
+The same run reports beside it:
+  * the plain FP64 kernel (bit-identical to the CPU path) on that level, and
+    its roofline;
+  * the CPU FP64 restatement of the reference's OpenMP loop on a 64-row
+    stripe of the level (median of 3 runs, the reported CPU baseline),
+    checked word for word against the GPU cube;
+  * SGM aggregation + peak scan (AWNCC at a level <= SGM_py) on the level;
+  * `batched`: the mixed-size tile workload — 64 tiles with sides log-uniform
+    over 512..4096 cells, 32..256 heights, 50..1500 m relief, 15..40 degree
+    convergence, each its own seeded scene — in ONE persistent FAST launch
+    replayed from a captured hipGraph, against one FAST launch per tile and
+    the plain FP64 kernel per tile, with the roofline of the persistent
+    kernel and the FAST-vs-FP64 error on a stripe of every tile;
+  * `served`: a Poisson stream at 20 requests/s of those tiles as requests,
+    4 pyramid levels x 2 iterations each, 200 GB HBM admission budget,
+    continuous batching against the plain FP64 server (one request at a
+    time); the arrival window is shortened so the run takes minutes and is
+    stated in the line.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -36,6 +52,10 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def window_counts(half):
     from setsm_amd.records import window_bits
     n = [0, 0, 0]
@@ -56,88 +76,38 @@ def flops_per_hypothesis(half):
     return 24 * n0 + 20 * (n0 + n1 + n2)
 
 
-def save_scene(d, sc):
-    os.makedirs(d, exist_ok=True)
-    np.savez(os.path.join(d, "scene.npz"), img0=sc.images[0], img1=sc.images[1], rpc0=sc.rpcs[0], rpc1=sc.rpcs[1],
-             terrain=sc.terrain, origin=np.array(sc.terrain_origin), extent=np.array(sc.extent),
-             resid=np.array(sc.fit_residual_px))
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
-def load_scene(d, spec):
-    from setsm_amd.synth import Scene
-    path = os.path.join(d, "scene.npz")
-    if not os.path.exists(path):
-        return None
-    z = np.load(path)
-    return Scene(spec, [z["img0"], z["img1"]], [z["rpc0"], z["rpc1"]], z["terrain"], tuple(z["origin"]),
-                 tuple(z["extent"]), list(z["resid"]))
-
-
-def batched_run(pool, ntiles, seed, cfg, tune_key=None, tuning_out=None):
-    """Tiles of mixed size (log-uniform 128..1024 cells per side) and height
-    range (32..256 per cell) from independently seeded scenes, one level:
-    ONE persistent FAST launch over all tiles vs one FAST launch per tile vs
-    the plain FP64 kernel one tile per launch. With tune_key, the launch shape
-    is first tuned on these tiles and stored under that key."""
-    import math
-    from setsm_amd import smx
-    from setsm_amd.records import LevelInputs
-    from setsm_amd.synth import level_grid
-    rng = np.random.default_rng(seed + 5)
-    jobs = []
-    for k in range(ntiles):
-        sc = pool[k % len(pool)]
-        cells = int(round(math.exp(rng.uniform(math.log(128), math.log(1024)))))
-        side = sc.spec.image_px * sc.spec.gsd
-        cells = min(cells, int((side - 100.0) / 2.0))
-        nh = int(rng.integers(32, 257))
-        g = level_grid(sc, 0, cells, nh, seed=int(rng.integers(0, 1 << 20)))
-        j = smx.Job(1)
-        if cfg is not None:
-            j.set_fast_config(cfg)
-        for ti in range(2):
-            j.build_pyramid(ti, sc.images[ti])
-        stub = [[(sc.images[ti], sc.images[ti], np.zeros((1, 1), np.uint8))] for ti in range(2)]
-        li = LevelInputs(stub, 0, sc.rpcs, g, grid_resolution=g["grid_res"])
-        j.set_level(li.level)
-        j.set_searchspace(g["ugrid"], g["minmax"])
-        j.initialize_voxel(1)
-        jobs.append(j)
-
-    def timed(fn, reps=3):
-        fn()
-        best = None
-        for _ in range(reps):
-            t = time.perf_counter()
-            fn()
-            dt = time.perf_counter() - t
-            best = dt if best is None else min(best, dt)
-        return best
-
-    if tune_key:
-        from setsm_amd import autotune
-        best, rows = autotune.tune_batch(jobs, log=log)
-        autotune.save(tune_key, best, rows, path=tuning_out or autotune.TUNING_PATH)
-        cfg = best["config"]
-    t_batch = timed(lambda: smx.vertical_line_locus_batch(jobs, 1, precision=smx.PREC_FAST))
-    hyps = sum(j.last_evaluated for j in jobs)
-    t_each = timed(lambda: [j.vertical_line_locus(1, precision=smx.PREC_FAST) for j in jobs])
-    t_fp64 = timed(lambda: [j.vertical_line_locus(1, precision=smx.PREC_FP64) for j in jobs], reps=1)
-    for j in jobs:
-        j.close()
-    return dict(tiles=ntiles, hypotheses=hyps, batched_hyp_per_s=hyps / t_batch, per_tile_fast_hyp_per_s=hyps / t_each,
-                plain_fp64_hyp_per_s=hyps / t_fp64, speedup_vs_plain=t_fp64 / t_batch, fast_config=cfg)
+def measured_traffic():
+    """HBM bytes per FAST launch at the headline workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated): the
+    counters cannot be read from inside the timed run."""
+    path = os.path.join(ROOT, "profiles", "vll_c1_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def peak_heights(rho64, rhof, lens, step, tol=1e-5):
-    """Per-cell peak height (argmax of the mean NCC, first maximum) of the
-    FAST cube against the FP64 cube: RMS height difference in metres and the
-    cells whose peak moved, split into near-ties (the FP64 top two closer
-    than tol) and real flips."""
+    """Per-cell peak height (first argmax of the mean NCC) of the FAST cube
+    against the FP64 cube: RMS height difference in metres and the cells
+    whose peak moved, split into near-ties (the FP64 top two closer than
+    tol) and real flips."""
     lens = lens.astype(np.int64)
     keep = lens > 0
     off = np.concatenate([[0], np.cumsum(lens)])[:-1][keep]
-    seg = np.repeat(np.arange(len(lens)), lens)
     ro = np.where(np.isfinite(rho64), rho64.astype(np.float64), -np.inf)
     rf = np.where(np.isfinite(rhof), rhof.astype(np.float64), -np.inf)
     idx = np.arange(len(ro)) - np.repeat(np.concatenate([[0], np.cumsum(lens)])[:-1], lens)
@@ -147,6 +117,8 @@ def peak_heights(rho64, rhof, lens, step, tol=1e-5):
         hit = v == np.repeat(m, lens[keep])
         return m, np.minimum.reduceat(np.where(hit, idx, np.iinfo(np.int64).max), off)
 
+    if not keep.any():
+        return dict(peak_cells=0, peak_height_rms_m=0.0, peak_near_ties=0, peak_flips=0)
     mo, io = first_argmax(ro)
     _, jf = first_argmax(rf)
     ok = np.isfinite(mo)
@@ -160,40 +132,361 @@ def peak_heights(rho64, rhof, lens, step, tol=1e-5):
                 peak_near_ties=ties, peak_flips=int(moved.sum()) - ties)
 
 
-def served_run(pool, nreq, rate, seed, cfg):
-    """Poisson streams of 4-level tile requests: continuous batching on the
-    FAST kernel vs the plain FP64 kernel serving one request at a time.
-    Run twice: at `rate` (latency under the stated load) and with every
-    request arriving within 0.25 s (saturation: throughput = capacity)."""
+def rho_error(rho64, rhof, w64, wf):
+    ev = np.isfinite(rho64)
+    d = np.abs(rhof[ev].astype(np.float64) - rho64[ev].astype(np.float64))
+    return dict(max_abs=float(d.max()) if d.size else 0.0, rms=float(np.sqrt(np.mean(d * d))) if d.size else 0.0,
+                incc_diff=int(np.sum((wf >> 16) != (w64 >> 16))), n=int(ev.sum()))
+
+
+def level_job(sc, g, levels=1, grid_pts=True):
+    """A job holding one tile's level-0 pyramid, grid and search space, after
+    InitializeVoxel of iteration 1."""
     from setsm_amd import smx
-    from setsm_amd.serve import Server, make_stream
-    out = dict(requests=nreq, levels=4, budget_gib=200, sizes_cells=[96, 640], heights=[32, 256])
-    for tag, r in (("load", rate), ("saturated", nreq / 0.25)):
-        stream = make_stream(nreq, r, sizes=(96, 640), heights=(32, 256), n_scenes=len(pool), seed=seed + 9)
-        Server(pool, levels=4).prepare(stream)          # request payloads, built before the clock starts
-        _, fast = Server(pool, levels=4, fast_config=cfg).run(stream)
-        _, plain = Server(pool, levels=4, precision=smx.PREC_FP64, batching=False).run(stream)
-        out[tag] = dict(rate_per_s=r, fast=fast, plain_fp64=plain,
-                        speedup_vs_plain=fast["hyp_per_s"] / plain["hyp_per_s"])
-    return out
+    from setsm_amd.records import LevelInputs
+    job = smx.Job(levels)
+    for ti in range(2):
+        job.build_pyramid(ti, sc.images[ti])
+    stub = [[(sc.images[ti], sc.images[ti], np.zeros((1, 1), np.uint8))] for ti in range(2)]
+    li = LevelInputs(stub, 0, sc.rpcs, g, grid_resolution=g["grid_res"])
+    if not grid_pts:
+        li.level.grid_pts = None    # GridPts feed only the GNCC term, off at level 0
+    job.set_level(li.level)
+    job.set_searchspace(g["ugrid"], g["minmax"])
+    job.initialize_voxel(1)
+    return job
 
 
-def log(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+def sub_grid(g, r0, rows):
+    cols = g["shape"][1]
+    c0, c1 = r0 * cols, (r0 + rows) * cols
+    sub = {k: g[k][c0:c1].copy() for k in ("grid_pts", "grid_wgs", "ugrid")}
+    for k in ("boundary", "minmax", "grid_res"):
+        sub[k] = g[k]
+    sub["shape"] = (rows, cols)
+    return sub
 
 
-def measured_traffic():
-    """HBM bytes per FAST launch at this workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated); the
-    counters cannot be read from inside the timed run."""
-    path = os.path.join(ROOT, "profiles", "vll_c1_hbm_traffic.md")
-    try:
-        for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":")[1])
-    except OSError:
-        pass
-    return None
+# ------------------------------------------------------------ headline level
+def headline(args, out):
+    import torch
+    from setsm_amd import smx
+    from setsm_amd.records import LevelInputs
+    from setsm_amd.synth import GridSpec, SceneSpec, make_grid, make_scene
+
+    t0 = time.time()
+    spec = SceneSpec(image_px=args.image_px, gsd=0.5, relief=300.0, base=200.0, seed=args.seed)
+    sc = make_scene(spec, device="cuda")
+    torch.cuda.empty_cache()
+    log(f"scene {args.image_px}^2 x2 ready in {time.time() - t0:.1f}s, RPC fit residual {max(sc.fit_residual_px):.2e} px")
+    side = args.image_px * spec.gsd
+    cells = args.cells
+    inset = (side - cells * 2.0) / 2.0
+    g = make_grid(sc, GridSpec(cells=cells, grid_res=2.0, band=32.0, inset=inset, x_offset=0.0, y_offset=0.0,
+                               seed=args.seed + 1))
+    g["boundary"] = np.array([inset, inset, inset + cells * 2.0, inset + cells * 2.0])
+    g["shape"] = (cells, cells)
+    g["grid_res"] = 2.0
+    job = level_job(sc, g)
+    log(f"job resident: {job.device_bytes / 2**30:.1f} GiB, {job.num_hypotheses} hypotheses")
+    fph = flops_per_hypothesis(5)
+
+    def run(prec, steps, warmup, keep_rho=False):
+        for _ in range(warmup):
+            job.vertical_line_locus(1, precision=prec, keep_rho=keep_rho)
+        job.sync()
+        torch.cuda.synchronize()
+        kms = []
+        t = time.perf_counter()
+        for _ in range(steps):
+            acc = job.vertical_line_locus(1, precision=prec, keep_rho=keep_rho)
+            kms.append(job.last_kernel_ms)
+        job.sync()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t) / steps
+        return dict(acc=acc, evaluated=job.last_evaluated, wall_s=wall, kernel_ms=float(np.mean(kms)),
+                    stats=job.last_stats)
+
+    log("timing plain FP64 kernel")
+    p64 = run(smx.PREC_FP64, args.fp64_steps, 1, keep_rho=True)
+    log(f"fp64: {p64['evaluated']} hyps, {p64['wall_s'] * 1e3:.1f} ms/step, kernel {p64['kernel_ms']:.1f} ms")
+    stripe_rows = max(1, args.cpu_rows)
+    r0 = cells // 2
+    c0, c1 = r0 * cells, (r0 + stripe_rows) * cells
+    lens = job.get_nccresult()["NumOfHeight"].astype(np.int64)
+    nw = int(lens[c0:c1].sum())
+    w64, rho64 = job.get_voxel_rows(c0, c1, nw, with_rho=True)
+
+    from setsm_amd import autotune
+    key = f"vll_L0_{args.image_px}px_{cells}cells_h128"
+    tuned = None
+    if args.autotune:
+        best, rows = autotune.tune_fast(job, 1, log=log)
+        autotune.save(key, best, rows, path=args.tuning_out or autotune.TUNING_PATH)
+        tuned = best
+    else:
+        entry = autotune.load(key)
+        if entry:
+            job.set_fast_config(entry["best"]["config"])
+            tuned = entry["best"]
+    log(f"FAST launch shape: {tuned}")
+    log("timing FAST kernel")
+    fast = run(smx.PREC_FAST, args.steps, args.warmup)
+    job.vertical_line_locus(1, precision=smx.PREC_FAST, keep_rho=True)
+    wf, rhof = job.get_voxel_rows(c0, c1, nw, with_rho=True)
+    fast_err = rho_error(rho64, rhof, w64, wf)
+    fast_err.update(peak_heights(rho64, rhof, lens[c0:c1], 0.25))
+    log(f"fast: {fast['wall_s'] * 1e3:.1f} ms/step, kernel {fast['kernel_ms']:.1f} ms, err {fast_err}")
+
+    # SGM aggregation + peak scan (AWNCC at a level <= SGM_py) on this cube
+    sgm = None
+    if args.sgm:
+        stub = [[(sc.images[ti], sc.images[ti], np.zeros((1, 1), np.uint8))] for ti in range(2)]
+        li = LevelInputs(stub, 0, sc.rpcs, g, grid_resolution=2.0, sgm_py=1)
+        job.set_level(li.level)
+        job.set_searchspace(g["ugrid"], g["minmax"])
+        job.initialize_voxel(1)
+        job.vertical_line_locus(1, precision=smx.PREC_FAST)
+        job.awncc(1)   # warm-up
+        ts = []
+        for _ in range(2):
+            job.sync()
+            t = time.perf_counter()
+            job.awncc(1)
+            ts.append(time.perf_counter() - t)
+        sgm = dict(ms=1e3 * min(ts), hypotheses=job.num_hypotheses, paths=12,
+                   hyp_per_s=job.num_hypotheses / min(ts), cells=cells * cells)
+        log(f"SGM + peak scan: {sgm}")
+
+    cpu = parity = None
+    if not args.no_cpu:
+        from oracle import oracle_py as O
+        imgs = [job.get_view(ti, 0, sc.images[ti].shape) for ti in range(2)]
+        sub = sub_grid(g, r0, stripe_rows)
+        lio = LevelInputs([[imgs[0]], [imgs[1]]], 0, sc.rpcs, sub, grid_resolution=2.0)
+        times, wo, acc_cpu, hyps_cpu = [], None, 0, 0
+        threads = O.threads()
+        for _ in range(3):
+            st = O.OracleState(lio, keep_rho=False)
+            ug = sub["ugrid"].copy()
+            st.initialize_voxel(ug, 1, g["minmax"])
+            t = time.perf_counter()
+            acc_cpu = st.vertical_line_locus(ug, 1, g["minmax"])
+            times.append(time.perf_counter() - t)
+            wo, _ = st.export()
+            hyps_cpu = int(st.ncc["NumOfHeight"].astype(np.int64).sum())
+            del st
+        med = float(np.median(times))
+        cpu = dict(value=hyps_cpu / med, unit="height-hypotheses/s", cores=threads, kind="port",
+                   cpu_model=cpu_model(), runs_s=[round(x, 3) for x in times],
+                   sample=f"{stripe_rows} x {cells} cell stripe (rows {r0}..{r0 + stripe_rows - 1}) of the headline "
+                          f"level, {hyps_cpu} hypotheses, oracle FP64 OpenMP dynamic,1 over cells, median of 3 "
+                          f"runs ({med:.1f} s)")
+        parity = dict(cells=c1 - c0, words=int(len(wo)), fp64_words_equal=bool(np.array_equal(wo, w64)),
+                      accessible_cpu=int(acc_cpu))
+        log(f"cpu: {cpu['value']:.3e} hyp/s on {threads} threads ({cpu['cpu_model']}); fp64 parity {parity}")
+
+    job.close()
+    del job
+    torch.cuda.empty_cache()
+
+    value = fast["evaluated"] / fast["wall_s"]
+    redo = fast["stats"]
+    log(f"FP64 pass over handed-over runs: {redo['redo_runs']} runs, {redo['redo_hyps']} hypotheses, "
+        f"{redo['redo_ms']:.2f} ms")
+    achieved = (fast["evaluated"] - redo["redo_hyps"]) * fph / (fast["kernel_ms"] / 1e3) / 1e12
+    traffic = measured_traffic() if (args.image_px, cells) == (16384, 4000) else None
+    p64_rate = p64["evaluated"] / p64["wall_s"]
+    out.update({
+        "metric": "height-hypotheses/s (VerticalLineLocus, one level, 16384^2 pair, 4000^2 cells x 128 heights)",
+        "value": value,
+        "unit": "height-hypotheses/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": fast["wall_s"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 moments, f64 window totals + rpc",
+        "data": "synthetic (seeded fBm terrain, rendered stereo pair, fitted RPCs)",
+        "config": {"workload": f"VLL level 0: {args.image_px}^2 uint16 pair, {cells}^2 cells at 2 m, 128 heights/cell",
+                   "precision": "fast", "hypotheses_per_step": fast["evaluated"], "flops_per_hypothesis": fph},
+        "roofline": dict(bound="valu", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                         frac=achieved / PEAK_FP32_TFLOPS, traffic=traffic,
+                         traffic_from="profiles/vll_c1_hbm_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE of this "
+                                      "kernel at this workload, copy-calibrated); not read in this run"),
+        "cpu_baseline": cpu,
+        "plain_fp64": {"value": p64_rate, "ms_per_step": p64["wall_s"] * 1e3, "kernel_ms": p64["kernel_ms"],
+                       "roofline_frac_fp64_peak":
+                           p64["evaluated"] * fph / (p64["kernel_ms"] / 1e3) / 1e12 / PEAK_FP64_TFLOPS},
+        "speedup_vs_plain_fp64": value / p64_rate,
+        "fast_vs_fp64_error": fast_err,
+        "parity_stripe": parity,
+        "fast_launch": dict(tuned or {}, **fast["stats"]),
+        "kernel_ms": fast["kernel_ms"],
+        "sgm": sgm,
+    })
+
+
+# ------------------------------------------------------------ mixed tiles
+def render_pool(draws):
+    import torch
+    from setsm_amd.synth import SceneSpec, make_scene, tile_image_px
+    scenes = []
+    t = time.time()
+    for d in draws:
+        spec = SceneSpec(image_px=tile_image_px(d), gsd=0.5, relief=d.relief, base=200.0,
+                         view_deg=(d.convergence / 2.0, -d.convergence / 2.0), seed=d.seed)
+        scenes.append(make_scene(spec, device="cuda"))
+    torch.cuda.empty_cache()
+    px = sum(2 * s.images[0].size for s in scenes)
+    log(f"{len(draws)} tile scenes rendered ({px / 1e9:.2f} G pixels) in {time.time() - t:.1f}s")
+    return scenes
+
+
+def batched_run(args, draws, scenes):
+    """all tiles in one persistent FAST launch (captured with the
+    peak scan as a hipGraph and replayed) vs one FAST launch per tile vs the
+    plain FP64 kernel per tile."""
+    from setsm_amd import autotune, smx
+    from setsm_amd.synth import level_grid
+    t = time.time()
+    jobs, stripes = [], []
+    for d, sc in zip(draws, scenes):
+        g = level_grid(sc, 0, d.side, d.nheights, seed=d.seed)
+        rows = max(1, min(8, d.side // 64))
+        stripes.append(sub_grid(g, d.side // 2, rows))
+        jobs.append(level_job(sc, g, grid_pts=False))
+        del g
+    hyps_total = sum(j.num_hypotheses for j in jobs)
+    resident = sum(j.device_bytes for j in jobs)
+    log(f"{len(jobs)} tile jobs resident: {resident / 2**30:.1f} GiB, {hyps_total / 1e9:.2f} G hypotheses "
+        f"({time.time() - t:.1f}s)")
+    key = "vll_mixed_tiles"
+    cfg = None
+    if args.autotune:
+        sub = sorted(range(len(jobs)), key=lambda k: jobs[k].num_hypotheses)[: max(1, len(jobs) // 4)]
+        best, rows = autotune.tune_batch([jobs[k] for k in sub], log=log)
+        autotune.save(key, best, rows, path=args.tuning_out or autotune.TUNING_PATH)
+        cfg = best["config"]
+    else:
+        entry = autotune.load(key)
+        cfg = entry["best"]["config"] if entry else None
+    if cfg is not None:
+        for j in jobs:
+            j.set_fast_config(cfg)
+    # eager batched call: the persistent kernel's own time
+    smx.vertical_line_locus_batch(jobs, 1, precision=smx.PREC_FAST)
+    t = time.perf_counter()
+    smx.vertical_line_locus_batch(jobs, 1, precision=smx.PREC_FAST)
+    eager_s = time.perf_counter() - t
+    kernel_ms = jobs[0].last_kernel_ms
+    evaluated = sum(j.last_evaluated for j in jobs)
+    redo_hyps = sum(j.last_stats["redo_hyps"] for j in jobs)
+    # the per-level sequence (fronts, persistent launch, hand-over, max_WNCC,
+    # peak scan) captured as one hipGraph
+    graph = smx.Graph(jobs, 1, precision=smx.PREC_FAST, with_awncc=True)
+    graph.launch()
+    g_ms = []
+    for _ in range(max(1, args.batch_reps)):
+        graph.launch()
+        g_ms.append(graph.last_ms)
+    graph.close()
+    graph_s = min(g_ms) / 1e3
+    log(f"batched: graph {graph_s * 1e3:.1f} ms, eager {eager_s * 1e3:.1f} ms, kernel {kernel_ms:.1f} ms, "
+        f"{evaluated} hypotheses")
+    t = time.perf_counter()
+    for j in jobs:
+        j.vertical_line_locus(1, precision=smx.PREC_FAST)
+    each_s = time.perf_counter() - t
+    t = time.perf_counter()
+    k64 = 0.0
+    ev64 = 0
+    for j in jobs:
+        j.vertical_line_locus(1, precision=smx.PREC_FP64)
+        k64 += j.last_kernel_ms
+        ev64 += j.last_evaluated
+    fp64_s = time.perf_counter() - t
+    log(f"per-tile FAST {each_s:.2f} s, plain FP64 per tile {fp64_s:.2f} s")
+    for j in jobs:
+        j.close()
+    jobs.clear()
+    smx.release_cached()
+    # FAST vs FP64 on a stripe of every tile
+    errs, worst = [], dict(max_abs=0.0)
+    d_all, lens_all, r64_all, rf_all = [], [], [], []
+    for d, sc, sub in zip(draws, scenes, stripes):
+        j = level_job(sc, sub)
+        j.vertical_line_locus(1, precision=smx.PREC_FP64, keep_rho=True)
+        w64, rho64, _ = j.get_voxels(with_rho=True)
+        j.vertical_line_locus(1, precision=smx.PREC_FAST, keep_rho=True)
+        wf, rhof, _ = j.get_voxels(with_rho=True)
+        lens_all.append(j.get_nccresult()["NumOfHeight"].astype(np.int64))
+        j.close()
+        e = rho_error(rho64, rhof, w64, wf)
+        errs.append(e)
+        r64_all.append(rho64)
+        rf_all.append(rhof)
+        if e["max_abs"] >= worst["max_abs"]:
+            worst = dict(e, side=d.side, relief=round(d.relief, 1), convergence=round(d.convergence, 1))
+    n = sum(e["n"] for e in errs)
+    err = dict(max_abs=max(e["max_abs"] for e in errs), rms=float(np.sqrt(sum(e["rms"] ** 2 * e["n"] for e in errs) / n)),
+               incc_diff=sum(e["incc_diff"] for e in errs), n=n, worst_tile=worst)
+    err.update(peak_heights(np.concatenate(r64_all), np.concatenate(rf_all), np.concatenate(lens_all), 0.25))
+    smx.release_cached()
+    fph = flops_per_hypothesis(5)
+    achieved = (evaluated - redo_hyps) * fph / (kernel_ms / 1e3) / 1e12
+    plain = ev64 / fp64_s
+    return dict(
+        config=dict(workload="configs[3]: mixed-size tiles, one persistent launch", tiles=len(draws),
+                    sides_cells=[min(d.side for d in draws), max(d.side for d in draws)],
+                    heights=[min(d.nheights for d in draws), max(d.nheights for d in draws)],
+                    relief_m=[round(min(d.relief for d in draws)), round(max(d.relief for d in draws))],
+                    convergence_deg=[round(min(d.convergence for d in draws), 1),
+                                     round(max(d.convergence for d in draws), 1)],
+                    resident_gib=round(resident / 2**30, 1), fast_config=cfg, tile_seed=args.tile_seed),
+        hypotheses=evaluated,
+        batched_hyp_per_s=evaluated / graph_s,
+        graph_ms=graph_s * 1e3,
+        eager_batched_ms=eager_s * 1e3,
+        persistent_kernel_ms=kernel_ms,
+        per_tile_fast_hyp_per_s=evaluated / each_s,
+        plain_fp64_hyp_per_s=plain,
+        plain_fp64_roofline_frac_fp64_peak=ev64 * fph / (k64 / 1e3) / 1e12 / PEAK_FP64_TFLOPS,
+        speedup_vs_plain=(evaluated / graph_s) / plain,
+        roofline=dict(bound="valu", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                      frac=achieved / PEAK_FP32_TFLOPS),
+        fast_vs_fp64_error=err,
+    )
+
+
+def served_run(args, draws, scenes, cfg):
+    """Poisson requests (the tiles above, drawn without
+    replacement) at `rate`/s, 4 levels x 2 iterations each, 200 GB budget."""
+    from setsm_amd import smx
+    from setsm_amd.serve import Request, Server
+    rng = np.random.default_rng(args.tile_seed + 7)
+    n = min(args.serve_requests, len(draws))
+    pick = rng.permutation(len(draws))[:n]
+    t = np.cumsum(rng.exponential(1.0 / args.serve_rate, n))
+    stream = [Request(i, float(t[i]), int(k), draws[k].side, draws[k].nheights, int(draws[k].seed))
+              for i, k in enumerate(pick)]
+    tp = time.time()
+    Server(scenes, levels=4, iterations=2).prepare(stream)   # request payloads, built before the clock starts
+    log(f"{n} request payloads prepared in {time.time() - tp:.1f}s; arrival window {t[-1]:.2f} s")
+    _, fast = Server(scenes, levels=4, iterations=2, fast_config=cfg).run(stream)
+    smx.release_cached()
+    log(f"served FAST: {fast}")
+    _, plain = Server(scenes, levels=4, iterations=2, precision=smx.PREC_FP64, batching=False).run(stream)
+    smx.release_cached()
+    log(f"served plain FP64: {plain}")
+    return dict(config=dict(workload="configs[4]: Poisson tile requests, continuous batching", rate_per_s=args.serve_rate,
+                            requests=n, arrival_window_s=round(float(t[-1]), 3), levels=4, iterations_per_level=2,
+                            budget_gib=200, sizes_cells=[int(min(draws[k].side for k in pick)),
+                                                         int(max(draws[k].side for k in pick))],
+                            note="arrival window shortened from 300 s so the run takes minutes"),
+                fast=fast, plain_fp64=plain, speedup_vs_plain=fast["hyp_per_s"] / plain["hyp_per_s"])
 
 
 def main():
@@ -206,13 +499,15 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--cpu-rows", type=int, default=64, help="cell rows of the CPU baseline stripe")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sgm", dest="sgm", action="store_false")
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--autotune", action="store_true", help="time every FAST launch shape and store the best")
     ap.add_argument("--tuning-out", default=None,
                     help="file the --autotune table is written to (default: the package's tuning.json)")
-    ap.add_argument("--scene-cache", default="", help="directory to load / save the rendered scene")
-    ap.add_argument("--batch-tiles", type=int, default=16, help="tiles of the batched run (0 = skip)")
-    ap.add_argument("--serve-requests", type=int, default=80, help="requests of the served run (0 = skip)")
+    ap.add_argument("--tiles", type=int, default=64, help="tiles of the mixed-size workload (0 = skip)")
+    ap.add_argument("--tile-seed", type=int, default=2026)
+    ap.add_argument("--batch-reps", type=int, default=2)
+    ap.add_argument("--serve-requests", type=int, default=24, help="requests of the served run (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="first run one device-to-device copy of this size (HBM counter calibration)")
@@ -224,10 +519,6 @@ def main():
         sys.exit(2)
 
     import torch
-    from setsm_amd import smx
-    from setsm_amd.records import LevelInputs
-    from setsm_amd.synth import GridSpec, SceneSpec, make_grid, make_scene
-
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.copy_calib_gib > 0:
         nbytes = int(args.copy_calib_gib * 2**30)
@@ -239,202 +530,22 @@ def main():
         log(f"calibration copy: {nbytes} bytes read, {nbytes} bytes written")
         del src, dst
         torch.cuda.empty_cache()
-    t0 = time.time()
-    spec = SceneSpec(image_px=args.image_px, gsd=0.5, relief=300.0, base=200.0, seed=args.seed)
-    sc = load_scene(args.scene_cache, spec) if args.scene_cache else None
-    if sc is None:
-        sc = make_scene(spec, device="cuda")
-        torch.cuda.empty_cache()
-        if args.scene_cache:
-            save_scene(args.scene_cache, sc)
-    log(f"scene {args.image_px}^2 x2 ready in {time.time() - t0:.1f}s, RPC fit residual {max(sc.fit_residual_px):.2e} px")
 
-    side = args.image_px * spec.gsd
-    grid_res = 2.0
-    cells = args.cells
-    inset = (side - cells * grid_res) / 2.0
-    g = make_grid(sc, GridSpec(cells=cells, grid_res=grid_res, band=32.0, inset=inset, x_offset=0.0,
-                               y_offset=0.0, seed=args.seed + 1))
-    g["boundary"] = np.array([inset, inset, inset + cells * grid_res, inset + cells * grid_res])
-    g["shape"] = (cells, cells)
-    log(f"grid {cells}^2 cells ready at {time.time() - t0:.1f}s")
-
-    job = smx.Job(1)
-    for ti in range(2):
-        job.build_pyramid(ti, sc.images[ti])
-    pyr_stub = [[(sc.images[ti], sc.images[ti], np.zeros((1, 1), np.uint8))] for ti in range(2)]
-    li = LevelInputs(pyr_stub, 0, sc.rpcs, g, grid_resolution=grid_res)
-    job.set_level(li.level)
-    job.set_searchspace(g["ugrid"], g["minmax"])
-    job.initialize_voxel(1)
-    nh = job.num_hypotheses
-    log(f"job resident: {job.device_bytes / 2**30:.1f} GiB, {nh} hypotheses")
-
-    half = 5
-    fph = flops_per_hypothesis(half)
-
-    def run(prec, steps, warmup, keep_rho=False):
-        for _ in range(warmup):
-            job.vertical_line_locus(1, precision=prec, keep_rho=keep_rho)
-        torch.cuda.synchronize()
-        job.sync()
-        kms = []
-        t = time.perf_counter()
-        for _ in range(steps):
-            acc = job.vertical_line_locus(1, precision=prec, keep_rho=keep_rho)
-            kms.append(job.last_kernel_ms)
-        job.sync()
-        torch.cuda.synchronize()
-        wall = (time.perf_counter() - t) / steps
-        return dict(acc=acc, evaluated=job.last_evaluated, wall_s=wall, kernel_ms=float(np.mean(kms)))
-
-    # plain FP64 kernel (bit-identical to the CPU path)
-    if args.fp64_steps > 0:
-        log("timing plain FP64 kernel")
-        p64 = run(smx.PREC_FP64, args.fp64_steps, 1, keep_rho=True)
-        log(f"fp64: {p64['evaluated']} hyps, {p64['wall_s'] * 1e3:.1f} ms/step, kernel {p64['kernel_ms']:.1f} ms")
-    else:
-        job.vertical_line_locus(1, precision=smx.PREC_FP64, keep_rho=True)   # reference values for the error check
-        p64 = None
-
-    # stripe of cells checked against the CPU restatement and timed there
-    stripe_rows = max(1, args.cpu_rows)
-    r0 = cells // 2
-    c0, c1 = r0 * cells, (r0 + stripe_rows) * cells
-    off = np.zeros(cells * cells + 1, np.int64)
-    ncc_all = job.get_nccresult()
-    lens = ncc_all["NumOfHeight"].astype(np.int64)
-    nw = int(lens[c0:c1].sum())
-    w64, rho64 = job.get_voxel_rows(c0, c1, nw, with_rho=True)
-
-    fast = None
-    fast_err = None
-    tuned = None
-    try:
+    out = {}
+    headline(args, out)
+    if args.tiles > 0 or args.serve_requests > 0:
         from setsm_amd import autotune
-        key = f"vll_L0_{args.image_px}px_{cells}cells_h128"
-        if args.autotune:
-            best, rows = autotune.tune_fast(job, 1, log=log)
-            out_path = args.tuning_out or autotune.TUNING_PATH
-            os.makedirs(os.path.dirname(os.path.abspath(out_path)), exist_ok=True)
-            autotune.save(key, best, rows, path=out_path)
-            tuned = best
-        else:
-            entry = autotune.load(key)
-            if entry:
-                job.set_fast_config(entry["best"]["config"])
-                tuned = entry["best"]
-        log(f"FAST launch shape: {tuned}")
-        log("timing FAST kernel")
-        fast = run(smx.PREC_FAST, args.steps, args.warmup, keep_rho=False)
-        fast["stats"] = job.last_stats
-        job.vertical_line_locus(1, precision=smx.PREC_FAST, keep_rho=True)
-        wf, rhof = job.get_voxel_rows(c0, c1, nw, with_rho=True)
-        ev = np.isfinite(rho64)
-        d = np.abs(rhof[ev].astype(np.float64) - rho64[ev].astype(np.float64))
-        fast_err = dict(max_abs=float(d.max()) if d.size else 0.0, rms=float(np.sqrt(np.mean(d * d))) if d.size else 0.0,
-                        incc_diff=int(np.sum((wf >> 16) != (w64 >> 16))), n=int(ev.sum()))
-        fast_err.update(peak_heights(rho64, rhof, lens[c0:c1], li.level.height_step))
-        log(f"fast: {fast['wall_s'] * 1e3:.1f} ms/step, kernel {fast['kernel_ms']:.1f} ms, err {fast_err}")
-    except smx.SmxError as e:
-        log(f"FAST path unavailable: {e}")
-
-    cpu = None
-    parity = None
-    if not args.no_cpu:
-        from oracle import oracle_py as O
-        img0 = job.get_view(0, 0, sc.images[0].shape)
-        img1 = job.get_view(1, 0, sc.images[1].shape)
-        sub = {k: g[k][c0:c1] for k in ("grid_pts", "grid_wgs", "ugrid", "truth")}
-        sub["boundary"] = g["boundary"]
-        sub["shape"] = (stripe_rows, cells)
-        lio = LevelInputs([[img0], [img1]], 0, sc.rpcs, sub, grid_resolution=grid_res)
-        st = O.OracleState(lio, keep_rho=False)
-        ug = sub["ugrid"].copy()
-        st.initialize_voxel(ug, 1, g["minmax"])
-        threads = O.threads()
-        t = time.perf_counter()
-        acc_cpu = st.vertical_line_locus(ug, 1, g["minmax"])
-        cpu_s = time.perf_counter() - t
-        wo, _ = st.export()
-        hyps_cpu = int(st.ncc["NumOfHeight"].astype(np.int64).sum())
-        cpu = dict(value=hyps_cpu / cpu_s, unit="height-hypotheses/s", cores=threads, kind="port",
-                   sample=f"{stripe_rows} x {cells} cell stripe (rows {r0}..{r0 + stripe_rows - 1}), {hyps_cpu} hypotheses, "
-                          f"oracle FP64 OpenMP dynamic,1 over cells, {cpu_s:.1f}s")
-        parity = dict(cells=c1 - c0, words=int(len(wo)), fp64_words_equal=bool(np.array_equal(wo, w64)),
-                      accessible_cpu=int(acc_cpu))
-        log(f"cpu: {cpu['value']:.3e} hyp/s on {threads} threads; fp64 parity {parity}")
-
-    if fast is None and p64 is None:
-        raise SystemExit("no timed path (FAST failed and --fp64-steps 0)")
-    best = fast if fast is not None else p64
-    best_prec = "fast" if fast is not None else "fp64"
-    value = best["evaluated"] / best["wall_s"]
-    kernel_s = best["kernel_ms"] / 1e3
-    if best_prec == "fast":
-        # the FAST window kernel alone: the hypotheses of the runs it handed to
-        # the FP64 pass, and that pass's time, are not its own
-        redo = best.get("stats", {})
-        log(f"FP64 pass over handed-over runs: {redo.get('redo_runs')} runs, {redo.get('redo_hyps')} hypotheses, "
-            f"{redo.get('redo_ms', 0.0):.2f} ms")
-        achieved = (best["evaluated"] - redo.get("redo_hyps", 0)) * fph / kernel_s / 1e12
-        roof = dict(bound="valu", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / PEAK_FP32_TFLOPS,
-                    traffic=measured_traffic() if (args.image_px, cells) == (16384, 4000) else None)
-    else:
-        achieved = best["evaluated"] * fph / kernel_s / 1e12
-        roof = dict(bound="valu", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / PEAK_FP64_TFLOPS, traffic=None)
-    out = {
-        "metric": "height-hypotheses/s (VerticalLineLocus, one level, 16384^2 pair, 4000^2 cells x 128 heights)",
-        "value": value,
-        "unit": "height-hypotheses/s",
-        "n_gpus": 1,
-        "steps": args.steps if fast is not None else args.fp64_steps,
-        "warmup": args.warmup,
-        "ms_per_step": best["wall_s"] * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32-compensated+f64-rpc" if best_prec == "fast" else "f64",
-        "data": "synthetic (seeded fBm terrain, rendered stereo pair, fitted RPCs)",
-        "config": {"workload": f"VLL level 0: {args.image_px}^2 uint16 pair, {cells}^2 cells at 2 m, 128 heights/cell",
-                   "precision": best_prec, "hypotheses_per_step": best["evaluated"], "flops_per_hypothesis": fph},
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "plain_fp64": ({"value": p64["evaluated"] / p64["wall_s"], "ms_per_step": p64["wall_s"] * 1e3,
-                        "kernel_ms": p64["kernel_ms"],
-                        "frac_fp64_peak": p64["evaluated"] * fph / (p64["kernel_ms"] / 1e3) / 1e12 / PEAK_FP64_TFLOPS}
-                       if p64 else None),
-        "speedup_vs_plain_fp64": ((value / (p64["evaluated"] / p64["wall_s"])) if fast is not None else 1.0)
-                                 if p64 else None,
-        "fast_vs_fp64_error": fast_err,
-        "parity_stripe": parity,
-        "fast_launch": (dict(tuned or {}, **fast.get("stats", {})) if fast else None),
-        "kernel_ms": best["kernel_ms"],
-    }
-    job.close()
-    del job
-    torch.cuda.empty_cache()
-    if args.batch_tiles > 0 or args.serve_requests > 0:
-        from setsm_amd.serve import make_pool
-        pool = make_pool(n_scenes=4, image_px=3072, device="cuda", seed=100 + args.seed)
-        torch.cuda.empty_cache()
-        # mixed small tiles (96..1024 cells per side) have their own tuned shape:
-        # the level-0 one is sized for 4000^2 cells and leaves these launches
-        # with too few, too large tiles
-        from setsm_amd import autotune
-        mixed_key = "vll_mixed_tiles_3072px"
-        entry = autotune.load(mixed_key)
-        mixed_cfg = entry["best"]["config"] if entry else None
-        if args.batch_tiles > 0:
-            out["batched"] = batched_run(pool, args.batch_tiles, args.seed, mixed_cfg,
-                                         tune_key=mixed_key if args.autotune else None, tuning_out=args.tuning_out)
-            mixed_cfg = out["batched"]["fast_config"]
-            log(f"batched: {out['batched']}")
+        from setsm_amd.synth import draw_tiles
+        draws = draw_tiles(max(args.tiles, args.serve_requests), args.tile_seed)
+        scenes = render_pool(draws)
+        entry = autotune.load("vll_mixed_tiles")
+        cfg = entry["best"]["config"] if entry else None
+        if args.tiles > 0:
+            out["batched"] = batched_run(args, draws[: args.tiles], scenes[: args.tiles])
+            cfg = out["batched"]["config"]["fast_config"]
+            log(f"batched: {json.dumps(out['batched'])}")
         if args.serve_requests > 0:
-            out["served"] = served_run(pool, args.serve_requests, args.serve_rate, args.seed, mixed_cfg)
-            log(f"served: {out['served']}")
+            out["served"] = served_run(args, draws, scenes, cfg)
     print(json.dumps(out), flush=True)
 
 

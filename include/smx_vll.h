@@ -69,6 +69,10 @@ long long smx_job_device_bytes(const smx_job *job);
 /* Cap the HBM the job may allocate (0 = no cap); allocations beyond it fail
  * with SMX_E_NOMEM. */
 int  smx_job_set_budget(smx_job *job, long long bytes);
+/* Device blocks freed by jobs are kept (up to 64 GiB per process) for later
+ * jobs and levels; this returns them to the HIP runtime and reports their
+ * bytes. No counterpart in the reference (library memory management). */
+long long smx_release_cached(void);
 
 /* Upload one pyramid level of image `img` (0 = reference, 1 = target):
  * intensity, Sobel magnitude and orientation, as SetPyramidImages

@@ -783,6 +783,24 @@ int smx_job_destroy(smx_job *j)
 
 long long smx_job_device_bytes(const smx_job *j) { return j ? j->device_bytes : 0; }
 
+long long smx_release_cached(void)
+{
+    std::map<std::pair<int, size_t>, std::vector<void *>> blocks;
+    long long bytes = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        blocks.swap(g_cache);
+        bytes = (long long)g_cached;
+        g_cached = 0;
+    }
+    for (auto &kv : blocks) {
+        (void)hipSetDevice(kv.first.first);
+        for (void *p : kv.second) (void)hipFree(p);
+    }
+    (void)hipSetDevice(g_device);
+    return bytes;
+}
+
 int smx_job_set_budget(smx_job *j, long long bytes)
 {
     if (!j) return fail(SMX_E_ARG, "null job");

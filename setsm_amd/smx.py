@@ -36,7 +36,7 @@ EXPORTS = [
     "smx_job_get_sumcost", "smx_job_set_ortho_wgs",
     "smx_job_last_kernel_ms", "smx_job_last_evaluated", "smx_job_last_redo", "smx_job_sync", "smx_fast_num_configs",
     "smx_job_set_fast_config", "smx_fast_config_desc", "smx_job_last_stats", "smx_vertical_line_locus_batch",
-    "smx_vertical_line_locus_noalloc", "smx_job_half", "smx_graph_create", "smx_graph_launch", "smx_graph_last_ms", "smx_graph_destroy",
+    "smx_vertical_line_locus_noalloc", "smx_release_cached", "smx_job_half", "smx_graph_create", "smx_graph_launch", "smx_graph_last_ms", "smx_graph_destroy",
 ]
 
 
@@ -94,6 +94,7 @@ def lib():
             "smx_job_last_stats": (i, [vp, vp]),
             "smx_vertical_line_locus_batch": (i, [vp, i, i, i, i, vp]),
             "smx_vertical_line_locus_noalloc": (ll, [vp, i, i]),
+            "smx_release_cached": (ll, []),
             "smx_job_half": (i, [vp]),
             "smx_graph_create": (i, [C.POINTER(vp), vp, i, i, i, i]),
             "smx_graph_launch": (i, [vp, vp]),
@@ -283,6 +284,11 @@ class Job:
 
     def sync(self):
         _check(lib().smx_job_sync(self.h))
+
+
+def release_cached() -> int:
+    """Return the library's cached device blocks to the runtime (bytes)."""
+    return int(lib().smx_release_cached())
 
 
 def fast_configs():

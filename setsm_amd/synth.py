@@ -210,7 +210,7 @@ def make_scene(spec: SceneSpec, device="cpu") -> Scene:
     images = []
     rows = torch.arange(W, device=device, dtype=torch.float64)
     c0 = (W - 1) / 2.0
-    chunk = max(1, (1 << 22) // W)
+    chunk = max(1, (1 << 24) // W)   # rows per pass: ~16 M pixels
     for vd in spec.view_deg:
         out = torch.empty((W, W), dtype=torch.int32, device=device)
         tv = math.tan(math.radians(vd))
@@ -231,7 +231,7 @@ def make_scene(spec: SceneSpec, device="cpu") -> Scene:
             sh = _sample(shade, origin, post, E.float(), N.float())
             dn = 1.0 + torch.round(2046.0 * (0.08 + 0.92 * a * (0.25 + 0.75 * sh)).clamp(0.0, 1.0))
             out[r0:r0 + chunk] = dn.int()
-        images.append(out.clamp(1, 2047).to(torch.int32).cpu().numpy().astype(np.uint16))
+        images.append(out.clamp(1, 2047).to(torch.int16).cpu().numpy().view(np.uint16))
 
     terrain = z.cpu().numpy()
     scene = Scene(spec, images, [], terrain, origin, (0.0, 0.0, side, side))
