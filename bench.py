@@ -366,7 +366,7 @@ def batched_run(args, draws, scenes):
     key = "vll_mixed_tiles"
     cfg = None
     if args.autotune:
-        sub = sorted(range(len(jobs)), key=lambda k: jobs[k].num_hypotheses)[: max(1, len(jobs) // 4)]
+        sub = list(range(0, len(jobs), 8))   # every 8th tile: the spread of sizes, heights and footprints
         best, rows = autotune.tune_batch([jobs[k] for k in sub], log=log)
         autotune.save(key, best, rows, path=args.tuning_out or autotune.TUNING_PATH)
         cfg = best["config"]

@@ -1,11 +1,12 @@
 """Autotuner for the SMX_PREC_FAST launch shape.
 
 The FAST kernel's shape is the cell tile a workgroup owns (which sets the LDS
-footprint, the texel reuse and the number of workgroups), the LDS
-reserved for the two images' boxes, and the register budget (compiler's choice, or capped
-for four waves per SIMD). heights-per-wavefront is fixed at 64: one run of
-consecutive heights of one cell per wavefront, so lanes of a wave read
-neighbouring texels along the epipolar line.
+footprint, the texel reuse and the number of workgroups), the LDS reserved
+for the two images' boxes, the register budget (compiler's choice, or capped
+for three or four waves per SIMD) and the run length: a wavefront evaluates
+64 / run_len runs of run_len consecutive heights (64, 32 or 16), so short
+height lists (32..256 heights per cell, not multiples of 64) leave fewer
+lanes idle at shorter runs, while longer runs keep a wave on one cell.
 
 `tune_fast` times every shape on the job's current level and keeps the
 fastest; results are stored per workload in setsm_amd/tuning.json so the
@@ -35,7 +36,7 @@ def tune_fast(job, iteration=1, reps=2, log=None, only_wide=False):
             best = ms if best is None else min(best, ms)
         st = job.last_stats
         rows.append(dict(config=i, tx=desc[0], ty=desc[1], lds_texels=desc[2], occ=desc[3], stage=desc[4], wide=desc[5],
-                         kernel_ms=best, tiles_lds=st["tiles_lds"], waves_hbm=st["waves_hbm"]))
+                         run_len=desc[6], kernel_ms=best, tiles_lds=st["tiles_lds"], waves_hbm=st["waves_hbm"]))
         if log:
             log(f"tune config {i} {desc}: {best:.2f} ms (tiles in LDS {st['tiles_lds']}, HBM waves {st['waves_hbm']})")
     rows.sort(key=lambda r: r["kernel_ms"])
@@ -62,7 +63,7 @@ def tune_batch(jobs, iteration=1, reps=2, log=None):
             ms = jobs[0].last_kernel_ms
             best = ms if best is None else min(best, ms)
         rows.append(dict(config=i, tx=desc[0], ty=desc[1], lds_texels=desc[2], occ=desc[3], stage=desc[4], wide=desc[5],
-                         kernel_ms=best))
+                         run_len=desc[6], kernel_ms=best))
         if log:
             log(f"batch tune config {i} {desc}: {best:.2f} ms")
     rows.sort(key=lambda r: r["kernel_ms"])
@@ -83,7 +84,7 @@ def load(key):
         return None
     from .smx import fast_configs
     b = entry["best"]
-    want = (b["tx"], b["ty"], b["lds_texels"], b["occ"], b["stage"], b["wide"])
+    want = (b["tx"], b["ty"], b["lds_texels"], b["occ"], b["stage"], b["wide"], b.get("run_len", 64))
     for i, desc in fast_configs():
         if tuple(desc) == want:
             entry["best"] = dict(b, config=i)

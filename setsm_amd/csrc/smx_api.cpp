@@ -194,7 +194,7 @@ static void free_level_state(smx_job *j)
     j->level_set = false;
     j->searchspace_set = false;
     j->stats_valid = false;
-    j->runs_bound = 0;
+    j->runs_bound[0] = j->runs_bound[1] = j->runs_bound[2] = 0;
     j->max_nh = 0;
 }
 
@@ -273,13 +273,13 @@ int vll_half_template(int level, int template_size, double resolution)
 // InitializeVoxel.
 static int refresh_stats(smx_job *j)
 {
-    unsigned long long *st = j->js.counters + 6;
-    HIPCHK(launch_fill_u64(st, 0, 2, j->stream));
+    unsigned long long *st = j->js.counters + 4;   // four slots between calls
+    HIPCHK(launch_fill_u64(st, 0, 4, j->stream));
     HIPCHK(launch_nh_stats(j->d_ncc, j->ncells, st, j->stream));
-    HIPCHK(hipMemcpyAsync(j->h_small, st, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, j->stream));
+    HIPCHK(hipMemcpyAsync(j->h_small, st, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
-    j->runs_bound = j->h_small[0];
-    j->max_nh = (int)j->h_small[1];
+    for (int k = 0; k < 3; k++) j->runs_bound[k] = j->h_small[k];
+    j->max_nh = (int)j->h_small[3];
     j->stats_valid = true;
     return 0;
 }
@@ -336,6 +336,7 @@ struct VllPlan {
     bool combined_wncc = false;  // check_combined_WNCC (setsm_code.cpp:6081-6088)
     bool ra = false;             // RA / matching-rate pass (no cube; FindPeakNcc inline)
     bool fast = false;           // SMX_PREC_FAST kernels
+    int run_len = kRun;          // heights per run of the call's run list
     int half = 5;
     int slot = 1;                // GridPT3.ortho_ncc index of the target image
     double ortho_th = 0;
@@ -380,6 +381,7 @@ static int vll_plan(smx_job *j, int iteration, int precision, int keep_rho, VllP
             if (!j->views[ti][L.level - 1].texel) return fail(SMX_E_STATE, "combined INCC needs level %d", L.level - 1);
     p.ra = L.is_ra || L.check_matching_rate;
     p.fast = precision != SMX_PREC_FP64 && !p.ra;
+    p.run_len = p.fast ? fast_run_len(j) : kRun;
     return 0;
 }
 
@@ -390,7 +392,7 @@ static int vll_reserve(smx_job *j, const VllPlan &p)
     int r = 0;
     if (!j->stats_valid && (r = refresh_stats(j))) return r;
     VllScratch &S = j->js;
-    if ((r = grow(j, S.runs, S.runs_cap, j->runs_bound))) return r;
+    if ((r = grow(j, S.runs, S.runs_cap, j->runs_bound[run_len_slot(p.run_len)]))) return r;
     if (p.fast) {
         const long t1 = fast_tiles(j, 0);
         if (t1 > S.tiles_cap || !S.tile_count) {
@@ -438,7 +440,7 @@ static int carve_scratch(smx_job *j, const VllPlan &p, VllScratch &S)
         off += std::max<size_t>(bytes, 8);
         return at;
     };
-    const long runs = (long)j->runs_bound;
+    const long runs = (long)j->runs_bound[run_len_slot(p.run_len)];
     const long nscan = scan_partials(j->ncells + 1);
     const size_t o_cnt = take(kCounters * 8), o_scan = take((size_t)nscan * 8), o_runs = take((size_t)runs * 8);
     size_t o_tc = 0, o_to = 0, o_tr = 0, o_redo = 0, o_t2c = 0, o_t2o = 0, o_t2r = 0, o_part = 0;
@@ -501,6 +503,7 @@ static void vll_args(smx_job *j, const VllPlan &p, const VllScratch &S, CellArgs
     A.run_offset = j->d_run_off;
     A.runs = S.runs;
     A.runs_cap = S.runs_cap;
+    A.run_len = p.run_len;
     A.nruns_dev = j->d_run_off + j->ncells;
     A.max_bits = j->d_max_bits;
     A.accessible = S.counters;
@@ -990,7 +993,7 @@ int smx_job_set_level(smx_job *j, const smx_level *p)
     HIPCHK(hipMemsetAsync(j->d_new_len, 0, ((size_t)n + 1) * sizeof(int64_t), j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
     j->stats_valid = true;   // no height lists yet
-    j->runs_bound = 0;
+    j->runs_bound[0] = j->runs_bound[1] = j->runs_bound[2] = 0;
     j->max_nh = 0;
     j->level_set = true;
     j->generation++;
@@ -1106,15 +1109,15 @@ int smx_initialize_voxel(smx_job *j, int iteration)
     HIPCHK(launch_init_voxel(j->d_params, A, j->stream));
     HIPCHK(job_scan(j->js, j->d_new_len, j->d_new_off, j->ncells + 1, j->stream));
     // the new height lists' bounds for the launch sequences
-    unsigned long long *st = j->js.counters + 6;
-    HIPCHK(launch_fill_u64(st, 0, 2, j->stream));
+    unsigned long long *st = j->js.counters + 4;   // four slots between calls
+    HIPCHK(launch_fill_u64(st, 0, 4, j->stream));
     HIPCHK(launch_nh_stats(j->d_ncc, j->ncells, st, j->stream));
     HIPCHK(hipMemcpyAsync(j->h_small, j->d_new_off + j->ncells, sizeof(int64_t), hipMemcpyDeviceToHost, j->stream));
-    HIPCHK(hipMemcpyAsync(j->h_small + 1, st, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, j->stream));
+    HIPCHK(hipMemcpyAsync(j->h_small + 1, st, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
     const long long total = j->h_small[0];
-    j->runs_bound = j->h_small[1];
-    j->max_nh = (int)j->h_small[2];
+    for (int k = 0; k < 3; k++) j->runs_bound[k] = j->h_small[1 + k];
+    j->max_nh = (int)j->h_small[4];
     j->stats_valid = true;
     uint32_t *nv = nullptr;
     float *nr = nullptr;
@@ -1479,11 +1482,11 @@ int smx_job_set_voxels(smx_job *j, const uint32_t *words)
     HIPCHK(hipMemcpyAsync(ncc.data(), j->d_ncc, ncc.size() * sizeof(smx_nccresult), hipMemcpyDeviceToHost, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
     off[0] = 0;
-    long long runs = 0;
+    long long runs[3] = {0, 0, 0};
     int max_nh = 0;
     for (long c = 0; c < j->ncells; c++) {
         off[c + 1] = off[c] + ncc[c].NumOfHeight;
-        runs += (ncc[c].NumOfHeight + kRun - 1) / kRun;
+        for (int k = 0; k < 3; k++) runs[k] += (ncc[c].NumOfHeight + kRunLens[k] - 1) / kRunLens[k];
         max_nh = std::max(max_nh, (int)ncc[c].NumOfHeight);
     }
     const long long total = off[j->ncells];
@@ -1498,7 +1501,7 @@ int smx_job_set_voxels(smx_job *j, const uint32_t *words)
     if (total)
         HIPCHK(hipMemcpyAsync(j->d_vox, words, (size_t)total * 4, hipMemcpyHostToDevice, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
-    j->runs_bound = runs;
+    for (int k = 0; k < 3; k++) j->runs_bound[k] = runs[k];
     j->max_nh = max_nh;
     j->stats_valid = true;
     j->generation++;

@@ -111,7 +111,7 @@ struct smx_job {
     // height list (SGM path buffers). Launch sequences size their scratch from
     // them instead of reading counts back mid-sequence.
     bool stats_valid = false;
-    long long runs_bound = 0;
+    long long runs_bound[3] = {0, 0, 0};   // runs of 16, 32, 64 heights
     int max_nh = 0;
     // bumped by every call that changes what a captured launch sequence
     // refers to (level, search space, cube layout, launch shape)
@@ -189,6 +189,7 @@ int launch_vll_fast_batch(smx_job *const *jobs, int n, const CellArgs *As, const
 int fast_num_configs();
 int fast_config_desc(int i, char *buf, int n);
 int fast_config_index(const smx_job *j);
+int fast_run_len(const smx_job *j);   // heights per run of the job's launch shape
 // tiles of the job's current level for its launch shape: slot 0 the level
 // pass, slot 1 the level-below pass of a combined iteration
 long fast_tiles(const smx_job *j, int slot);

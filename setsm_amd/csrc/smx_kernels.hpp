@@ -7,7 +7,11 @@
 
 namespace smx {
 
-constexpr int kRun = 64;   // heights per run = one wavefront of hypotheses
+constexpr int kRun = 64;   // lanes of a wavefront: the longest run of heights one wave evaluates
+// Run lengths (heights per run) a launch may use; a wavefront evaluates 64 / R
+// runs side by side. The plain FP64 and RA / matching-rate passes use 64.
+constexpr int kRunLens[3] = {16, 32, 64};
+__host__ __device__ constexpr int run_len_slot(int R) { return R == 16 ? 0 : R == 32 ? 1 : 2; }
 // RA / matching-rate passes: a NaN payload marking "no NCC at this height"
 // (FindPeakNcc is not called there), distinct from an evaluated NaN.
 constexpr unsigned long long kNotEvaluated = 0x7ff80000000bad01ull;
@@ -24,6 +28,7 @@ struct CellArgs {
     const int64_t *run_offset;     // exclusive prefix sum of run_count
     int2 *runs;                    // (cell, first height)
     long runs_cap;                 // capacity of runs (host-known bound on the count)
+    int run_len;                   // heights per run (16, 32 or 64)
     const int64_t *nruns_dev;      // the count itself, on the device (run_offset[ncells])
     unsigned long long *max_bits;  // per cell, order_key of max_WNCC
     unsigned long long *accessible;
@@ -161,20 +166,23 @@ __device__ __forceinline__ bool project_hypothesis(const VllParams &P, const Cel
     return true;
 }
 
-// Wave-wide max of temp_rho into the cell's max_WNCC key and a count of the
-// hypotheses that computed an NCC. Every lane of the wave must call it.
+// Max of temp_rho over each run of a wavefront (runs of R lanes) into its
+// cell's max_WNCC key, and a count of the hypotheses that computed an NCC.
+// Every lane of the wave must call it; cell_key is the key of the lane's run.
 __device__ __forceinline__ void wave_max_commit(double v, unsigned evaluated, unsigned long long *cell_key,
-                                                unsigned long long *counter)
+                                                unsigned long long *counter, int R = kRun)
 {
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
         const double o = __shfl_xor(v, m, 64);
-        v = o > v ? o : v;
+        if (m < R) v = o > v ? o : v;
     }
     const unsigned long long any = __ballot(evaluated != 0);
-    if ((threadIdx.x & 63) == 0 && any) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long mine = R == 64 ? any : (any >> (lane & ~(R - 1))) & ((1ull << R) - 1);
+    if ((lane & (R - 1)) == 0 && mine) {
         atomicMax(cell_key, order_key(v));
-        atomicAdd(counter, (unsigned long long)__popcll(any));
+        atomicAdd(counter, (unsigned long long)__popcll(mine));
     }
 }
 

@@ -71,7 +71,7 @@ __device__ __forceinline__ bool gate_cell(const VllParams *__restrict__ P, CellA
                 nr->result3 = -1000;
                 nr->result4 = 0;
                 A.max_bits[c] = order_key(-100.0);
-                runs = (nr->NumOfHeight + kRun - 1) / kRun;
+                runs = (nr->NumOfHeight + A.run_len - 1) / A.run_len;
             }
         }
     }
@@ -88,24 +88,30 @@ __global__ void __launch_bounds__(256) k_vll_fill_runs(CellArgs A)
     const long base = A.run_offset[c];
     // the capacity is an upper bound on the count; never write past it
     for (int i = 0; i < n && base + i < A.runs_cap; i++)
-        A.runs[base + i] = make_int2((int)c, i * kRun);
+        A.runs[base + i] = make_int2((int)c, i * A.run_len);
 }
 
-// Per-cell height lists -> {sum of runs, longest list}: the bounds the launch
-// sequences size their scratch by (stats[0] += runs, stats[1] = max).
+// Per-cell height lists -> the bounds the launch sequences size their
+// scratch by: stats[k] += runs of kRunLens[k] heights (k = 0, 1, 2) and
+// stats[3] = the longest list.
 __global__ void __launch_bounds__(256) k_nh_stats(const smx_nccresult *__restrict__ ncc, long n,
                                                   unsigned long long *stats)
 {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned nh = c < n ? ncc[c].NumOfHeight : 0u;
-    unsigned long long runs = (nh + kRun - 1) / kRun;
+    unsigned long long runs[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) runs[k] = (nh + kRunLens[k] - 1) / kRunLens[k];
     for (int o = 32; o > 0; o >>= 1) {
-        runs += (unsigned long long)__shfl_xor((long long)runs, o);
+#pragma unroll
+        for (int k = 0; k < 3; k++) runs[k] += (unsigned long long)__shfl_xor((long long)runs[k], o);
         nh = max(nh, (unsigned)__shfl_xor((int)nh, o));
     }
     if ((threadIdx.x & 63) == 0) {
-        if (runs) atomicAdd(&stats[0], runs);
-        if (nh) atomicMax(&stats[1], (unsigned long long)nh);
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            if (runs[k]) atomicAdd(&stats[k], runs[k]);
+        if (nh) atomicMax(&stats[3], (unsigned long long)nh);
     }
 }
 
@@ -329,7 +335,7 @@ __device__ __forceinline__ bool multi_ncc_fp64_at(const DevView &Lv, const DevVi
     return true;
 }
 
-// One run (64 consecutive heights of one cell) on one wavefront, FP64.
+// One run (up to 64 consecutive heights of one cell) on one wavefront, FP64.
 template <int H>
 __device__ __forceinline__ void vll_fp64_run(const VllParams *__restrict__ P, const CellArgs &A, const int2 run,
                                              const long wave, const int lane)
@@ -340,8 +346,10 @@ __device__ __forceinline__ void vll_fp64_run(const VllParams *__restrict__ P, co
     const smx_ugrid g = A.gridpt3[cell];
     const int start_H = g.minHeight, end_H = g.maxHeight;
 
+    // lanes past the run's length (runs of 16 or 32 handed over by the FAST
+    // path) stay idle
     Hyp hy;
-    bool act = project_hypothesis(*P, A, cell, hidx, nr, start_H, end_H, hy);
+    bool act = lane < A.run_len && project_hypothesis(*P, A, cell, hidx, nr, start_H, end_H, hy);
     double temp_rho = -100.0;
     unsigned evaluated = 0;
     if (A.peak_rho) {

@@ -46,7 +46,7 @@ __global__ void __launch_bounds__(256) k_tile_fill(CellArgs A, FastArgs F, int2 
             if (col >= F.grid_w) break;
             const long c = row * F.grid_w + col;
             const int n = (int)A.run_count[c];
-            for (int i = 0; i < n && k < F.runs_cap; i++) out[k++] = make_int2((int)c, i * kRun);
+            for (int i = 0; i < n && k < F.runs_cap; i++) out[k++] = make_int2((int)c, i * A.run_len);
         }
     }
 }
@@ -57,18 +57,24 @@ __global__ void k_zero_u64(unsigned long long *p, int n)
 }
 
 // Launch shapes the autotuner chooses from: {tx, ty, lds_texels, occ, stage,
-// wide} (cells per tile, LDS texels for both boxes, register cap for 3 / 4
-// waves per SIMD or 0, LDS staging, FP64 window totals).
+// wide, run_len} (cells per tile, LDS texels for both boxes, register cap for
+// 3 / 4 waves per SIMD or 0, LDS staging, FP64 window totals, heights per run).
+// Boxes stay at or below 7680 texels (60 KB): with 80 KB only one
+// workgroup fits a CU and the kernel runs at half speed.
 static const FastCfg kFastCfgs[] = {
-    {1, 4, 3072, 0, 1, 1}, {2, 4, 3584, 0, 1, 1}, {2, 2, 3072, 0, 1, 1}, {4, 4, 4608, 0, 1, 1},
-    {1, 8, 4096, 0, 1, 1}, {2, 8, 4608, 0, 1, 1}, {4, 8, 6144, 0, 1, 1}, {1, 16, 6144, 0, 1, 1},
-    {1, 4, 3072, 3, 1, 1}, {2, 4, 3584, 3, 1, 1}, {4, 4, 4608, 3, 1, 1}, {1, 4, 3072, 4, 1, 1},
-    {2, 4, 3584, 0, 0, 1}, {1, 4, 3072, 0, 1, 0}, {2, 4, 3584, 3, 1, 0}, {2, 8, 4608, 3, 1, 1},
-    {1, 8, 4096, 3, 1, 1}, {4, 2, 3584, 3, 1, 1}, {2, 2, 3072, 3, 1, 1}, {2, 4, 3584, 4, 1, 1},
-    {8, 4, 6144, 0, 1, 1}, {4, 16, 7680, 0, 1, 1}, {8, 8, 7680, 0, 1, 1}, {4, 8, 6144, 3, 1, 1},
+    {1, 4, 3072, 0, 1, 1, 64}, {2, 4, 3584, 0, 1, 1, 64}, {2, 2, 3072, 0, 1, 1, 64}, {4, 4, 4608, 0, 1, 1, 64},
+    {1, 8, 4096, 0, 1, 1, 64}, {2, 8, 4608, 0, 1, 1, 64}, {4, 8, 6144, 0, 1, 1, 64}, {1, 16, 6144, 0, 1, 1, 64},
+    {1, 4, 3072, 3, 1, 1, 64}, {2, 4, 3584, 3, 1, 1, 64}, {4, 4, 4608, 3, 1, 1, 64}, {1, 4, 3072, 4, 1, 1, 64},
+    {2, 4, 3584, 0, 0, 1, 64}, {1, 4, 3072, 0, 1, 0, 64}, {2, 4, 3584, 3, 1, 0, 64}, {2, 8, 4608, 3, 1, 1, 64},
+    {1, 8, 4096, 3, 1, 1, 64}, {4, 2, 3584, 3, 1, 1, 64}, {2, 2, 3072, 3, 1, 1, 64}, {2, 4, 3584, 4, 1, 1, 64},
+    {8, 4, 6144, 0, 1, 1, 64}, {4, 16, 7680, 0, 1, 1, 64}, {8, 8, 7680, 0, 1, 1, 64}, {4, 8, 6144, 3, 1, 1, 64},
+    // runs of 32 and 16 heights
+    {1, 16, 6144, 0, 1, 1, 32}, {2, 8, 4608, 0, 1, 1, 32}, {4, 8, 6144, 0, 1, 1, 32}, {4, 16, 7680, 0, 1, 1, 32},
+    {1, 16, 6144, 0, 1, 1, 16}, {2, 8, 4608, 0, 1, 1, 16}, {4, 8, 6144, 0, 1, 1, 16}, {4, 16, 7680, 0, 1, 1, 16},
+    {1, 8, 4096, 0, 1, 1, 32}, {2, 16, 7680, 0, 1, 1, 32}, {1, 32, 7680, 0, 1, 1, 32}, {1, 16, 6144, 3, 1, 1, 32},
 };
 constexpr int kNumFastCfgs = sizeof(kFastCfgs) / sizeof(kFastCfgs[0]);
-constexpr int kDefaultFastCfg = 6;   // 4 x 8 cells, compiler's register budget, FP64 totals
+constexpr int kDefaultFastCfg = 6;   // 4 x 8 cells, compiler's register budget, FP64 totals, runs of 64
 
 int fast_config_index(const smx_job *j)
 {
@@ -83,6 +89,8 @@ static void tile_shape(const FastCfg &cfg, int slot, int &tx, int &ty)
     tx = slot ? std::max(1, cfg.tx / 2) : cfg.tx;
     ty = slot ? std::max(1, cfg.ty / 2) : cfg.ty;
 }
+
+int fast_run_len(const smx_job *j) { return kFastCfgs[fast_config_index(j)].run_len; }
 
 long fast_tiles(const smx_job *j, int slot)
 {
@@ -266,7 +274,10 @@ int launch_vll_fast_batch(smx_job *const *jobs, int n, const CellArgs *As, const
     BatchJob *dj = (BatchJob *)desc;
     unsigned long long *dnext = (unsigned long long *)(dj + 2 * (size_t)n);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, dnext, 24);
-    const size_t lds = (size_t)kFastCfgs[fast_config_index(jobs[0])].lds_texels * sizeof(float2);
+    // every job's box-fit test uses its own budget: allocate the largest
+    int lds_texels = 0;
+    for (int i = 0; i < n; i++) lds_texels = std::max(lds_texels, kFastCfgs[fast_config_index(jobs[i])].lds_texels);
+    const size_t lds = (size_t)lds_texels * sizeof(float2);
     int cus = 0;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, jobs[0]->device);
     auto launch = [&](int mode, const BatchJob *jl, int nj, long total, unsigned long long *next) -> int {
@@ -303,8 +314,8 @@ int fast_num_configs() { return kNumFastCfgs; }
 int fast_config_desc(int i, char *buf, int n)
 {
     if (i < 0 || i >= kNumFastCfgs) return -1;
-    return snprintf(buf, (size_t)n, "%d %d %d %d %d %d", kFastCfgs[i].tx, kFastCfgs[i].ty, kFastCfgs[i].lds_texels,
-                    kFastCfgs[i].occ, kFastCfgs[i].stage, kFastCfgs[i].wide);
+    return snprintf(buf, (size_t)n, "%d %d %d %d %d %d %d", kFastCfgs[i].tx, kFastCfgs[i].ty, kFastCfgs[i].lds_texels,
+                    kFastCfgs[i].occ, kFastCfgs[i].stage, kFastCfgs[i].wide, kFastCfgs[i].run_len);
 }
 
 }  // namespace smx

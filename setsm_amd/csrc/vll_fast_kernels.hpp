@@ -196,24 +196,42 @@ __device__ __forceinline__ void box_stage(const DevView *views, TileBox &tb)
     if (nod) tb.nodata = 1;   // benign race: every writer stores 1
 }
 
-// One run (64 consecutive heights of one cell) on one wavefront. kCombined:
-// a single-pass launch may meet a combined-level iteration (the persistent
-// kernel never does: batches hold no such iteration).
+// log2 of the run length (16, 32 or 64 heights)
+__device__ __forceinline__ int run_shift(int R) { return R == 16 ? 4 : R == 32 ? 5 : 6; }
+
+// Whether x holds on any lane of the calling lane's run (R lanes). Every lane
+// of the wave must call it.
+__device__ __forceinline__ bool group_any(bool x, int R)
+{
+    const unsigned long long b = __ballot(x);
+    if (R == 64) return b != 0;
+    const int lane = threadIdx.x & 63;
+    return ((b >> (lane & ~(R - 1))) & ((1ull << R) - 1)) != 0;
+}
+
+// The runs ri .. ri + 64/R - 1 of the tile's list (those below r1) on one
+// wavefront, R = A.run_len lanes each. kCombined: a single-pass launch may
+// meet a combined-level iteration (the single-pass persistent kernel never
+// does: batches launch such jobs in two passes).
 template <int H, bool kWide, int kMode, bool kCombined = true>
 __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs &A, const FastArgs &F,
                                              const TileBox &tb, const bool fits, const bool nodata, const long ri,
-                                             const int lane)
+                                             const long r1, const int lane)
 {
     constexpr int O = H + 2;
     constexpr bool kBelow = kMode == kBelowPass;   // window on the level below
 
-        const int2 run = F.tile_runs[ri];
+        const int R = A.run_len;
+        const int lr = lane & (R - 1);
+        const long rl = ri + (lane >> run_shift(R));
+        const bool have = rl < r1;
+        const int2 run = have ? F.tile_runs[rl] : make_int2(0, 0);
         const long cell = run.x;
-        const int hidx = run.y + lane;
+        const int hidx = run.y + lr;
         const smx_nccresult nr = A.ncc[cell];
         const smx_ugrid g = A.gridpt3[cell];
         Hyp hy{};
-        const bool act = project_hypothesis(P, A, cell, hidx, nr, g.minHeight, g.maxHeight, hy);
+        const bool act = have && project_hypothesis(P, A, cell, hidx, nr, g.minHeight, g.maxHeight, hy);
         double temp_rho = -100.0;
         unsigned evaluated = 0;
         bool do_ncc = false;
@@ -226,10 +244,15 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
             else do_ncc = true;
         }
         double part = 0;
+        bool queued = false;   // the level pass already handed this lane's run to the FP64 pass
         if constexpr (kBelow) {
-            // the level pass handed this run to the FP64 pass: nothing to add
+            // a run the level pass handed over (it marked every evaluated lane
+            // of it) is the FP64 pass's: its lanes add nothing here. The
+            // level-below tiles group runs into waves differently, so this is
+            // decided per lane, not per wave.
             if (do_ncc) part = A.part[wi];
-            if (__any(do_ncc && part == kRedoRho)) return;
+            queued = do_ncc && part == kRedoRho;
+            if (queued) do_ncc = false;
         }
         const double rpx = kBelow ? hy.rnx : hy.rpx, rpy = kBelow ? hy.rny : hy.rpy;
         const double tpx = kBelow ? hy.tnx : hy.tpx, tpy = kBelow ? hy.tny : hy.tpy;
@@ -284,7 +307,12 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
         // orientation-rejected lanes are rewritten identically there)
         const bool redo = (do_ncc && all3 && rho == kRedoRho) || (all3n && rho_n == kRedoRho);
         if (__any(redo)) {
-            if (lane == 0) A.redo[atomicAdd(A.redo_n, 1ull)] = run;
+            // every run of the wave with something to evaluate goes to the
+            // FP64 pass, each once: the level pass marks every evaluated lane
+            // of the runs it queues, and the level-below pass queues no marked
+            // run (so the queue never holds more than the run count)
+            const bool run_queued = group_any(queued, R), run_has = group_any(do_ncc, R);   // all lanes vote
+            if (lr == 0 && have && run_has && !run_queued) A.redo[atomicAdd(A.redo_n, 1ull)] = run;
             if constexpr (kMode == kLevelPass) {
                 if (do_ncc) A.part[wi] = kRedoRho;   // the level-below pass skips the run
             }
@@ -324,7 +352,7 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
             temp_rho = wncc_of(A, cell, sum, cnt, rr);
             evaluated = 1;
         }
-        wave_max_commit(temp_rho, evaluated, &A.max_bits[cell], A.evaluated);
+        wave_max_commit(temp_rho, evaluated, &A.max_bits[cell], A.evaluated, R);
     }
 
 template <int H, bool kWide, int kMode = kSingle, bool kCombined = true>
@@ -340,6 +368,7 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
     const long r0 = F.tile_run_off[tile], r1 = F.tile_run_off[tile + 1];
     if (r0 == r1) return;
     const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6, lane = threadIdx.x & 63;
+    const int per = kRun >> run_shift(A.run_len);   // runs per wavefront
 
     // 1. One box for the whole tile: project each active cell's band ends.
     box_reset(sbox);
@@ -364,25 +393,26 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
         if (fits) box_stage(views, tb);
         __syncthreads();
         const bool nodata = tb.nodata != 0;
-        for (long ri = r0 + wave; ri < r1; ri += nwaves)
-            vll_fast_run<H, kWide, kMode, kCombined>(P, A, F, tb, fits, nodata, ri, lane);
+        for (long ri = r0 + (long)wave * per; ri < r1; ri += (long)nwaves * per)
+            vll_fast_run<H, kWide, kMode, kCombined>(P, A, F, tb, fits, nodata, ri, r1, lane);
         return;
     }
     // 2. Tall footprints (wide height bands, strong convergence): stage one
-    // box per chunk of runs the waves evaluate together; each run spans 64
-    // heights, so a chunk's box stays small.
-    for (long c0 = r0; c0 < r1; c0 += nwaves) {
+    // box per chunk of runs the waves evaluate together; each run spans at
+    // most 64 heights, so a chunk's box stays small.
+    const long cw = (long)nwaves * per;
+    for (long c0 = r0; c0 < r1; c0 += cw) {
         __syncthreads();
         box_reset(sbox);
         __syncthreads();
-        if ((int)threadIdx.x < 2 * nwaves) {
+        if ((long)threadIdx.x < 2 * cw) {
             const long ri = c0 + threadIdx.x / 2;
             if (ri < r1) {
                 const int2 run = F.tile_runs[ri];
                 const long cell = run.x;
                 const smx_nccresult nr = A.ncc[cell];
                 const smx_ugrid g = A.gridpt3[cell];
-                const int hl = (threadIdx.x & 1) ? min(run.y + kRun - 1, (int)nr.NumOfHeight - 1) : run.y;
+                const int hl = (threadIdx.x & 1) ? min(run.y + A.run_len - 1, (int)nr.NumOfHeight - 1) : run.y;
                 double hh = (double)(float)(nr.minHeight + hl * P.height_step);
                 hh = fmin(fmax(hh, (double)g.minHeight), (double)g.maxHeight);
                 box_add(P, views, level, sbox, A.grid_wgs[2 * cell], A.grid_wgs[2 * cell + 1], hh);
@@ -396,7 +426,8 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
         if (fits) box_stage(views, tb);
         __syncthreads();
         const bool nodata = tb.nodata != 0;
-        if (c0 + wave < r1) vll_fast_run<H, kWide, kMode, kCombined>(P, A, F, tb, fits, nodata, c0 + wave, lane);
+        if (c0 + (long)wave * per < r1)
+            vll_fast_run<H, kWide, kMode, kCombined>(P, A, F, tb, fits, nodata, c0 + (long)wave * per, r1, lane);
     }
 }
 

@@ -49,10 +49,11 @@ struct BatchArgs {
 
 // A launch shape: cells per tile, LDS texels for both boxes, waves per SIMD
 // the register budget is capped for (0: the compiler's choice), LDS staging
-// on/off, and whether window totals are FP64 sums of the FP32 row partials
-// (wide) or FP32 sums.
+// on/off, whether window totals are FP64 sums of the FP32 row partials
+// (wide) or FP32 sums, and the run length (heights per run: a wavefront
+// evaluates 64 / run_len runs, possibly of different cells, side by side).
 struct FastCfg {
-    int tx, ty, lds_texels, occ, stage, wide;
+    int tx, ty, lds_texels, occ, stage, wide, run_len;
 };
 
 // Launch entry points, one set per half-window size H (5, 6, 7). The

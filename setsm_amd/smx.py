@@ -292,9 +292,10 @@ def release_cached() -> int:
 
 
 def fast_configs():
-    """[(index, (tx, ty, lds_texels, occ, stage, wide))] of the SMX_PREC_FAST
-    launch shapes: cells per tile, LDS texels of both boxes, register cap
-    (waves per SIMD, 0 = compiler's), LDS staging, FP64 window totals."""
+    """[(index, (tx, ty, lds_texels, occ, stage, wide, run_len))] of the
+    SMX_PREC_FAST launch shapes: cells per tile, LDS texels of both boxes,
+    register cap (waves per SIMD, 0 = compiler's), LDS staging, FP64 window
+    totals, heights per run (64 / run_len runs per wavefront)."""
     out = []
     buf = C.create_string_buffer(64)
     for i in range(lib().smx_fast_num_configs()):

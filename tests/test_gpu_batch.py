@@ -59,14 +59,31 @@ def _oracle_check(m, ug_seq, minmax, rho, acc):
     assert acc == ro["acc"]
 
 
-@pytest.mark.parametrize("level,specs", [
-    (0, [(60, 3, 512, 60.0, 32), (24, 4, 512, 80.0, 16), (44, 5, 512, 40.0, 64)]),
-    (3, [(20, 8, 1024, 120.0, 96), (12, 9, 1024, 60.0, 48)]),      # half window 6
-    (4, [(10, 10, 1024, 150.0, 120), (8, 11, 1024, 90.0, 64)]),     # half window 7
+def _config(run_len, tx=1, ty=16, lds=6144):
+    """Index of the launch shape (tx, ty, lds, compiler's registers, staged,
+    FP64 totals, run_len)."""
+    from setsm_amd.smx import fast_configs
+    for i, desc in fast_configs():
+        if desc == (tx, ty, lds, 0, 1, 1, run_len):
+            return i
+    raise LookupError((tx, ty, lds, run_len))
+
+
+@pytest.mark.parametrize("level,specs,run_len", [
+    (0, [(60, 3, 512, 60.0, 32), (24, 4, 512, 80.0, 16), (44, 5, 512, 40.0, 64)], None),
+    (0, [(60, 3, 512, 60.0, 33), (24, 4, 512, 80.0, 17), (44, 5, 512, 40.0, 9)], 32),   # runs of 32 heights
+    (0, [(60, 3, 512, 60.0, 33), (24, 4, 512, 80.0, 17), (44, 5, 512, 40.0, 9)], 16),   # runs of 16 heights
+    (3, [(20, 8, 1024, 120.0, 96), (12, 9, 1024, 60.0, 48)], None),      # half window 6
+    (4, [(10, 10, 1024, 150.0, 120), (8, 11, 1024, 90.0, 64)], None),     # half window 7
 ])
-def test_batch_equals_alone(require_gpu, level, specs):
+def test_batch_equals_alone(require_gpu, level, specs, run_len):
     from setsm_amd import smx
     made = [_make(level, *s[:4], band=s[4]) for s in specs]
+    if run_len:
+        cfg = _config(run_len)
+        for m in made:
+            m["alone"].set_fast_config(cfg)
+            m["batch"].set_fast_config(cfg)
     alone = []
     for m in made:
         j = m["alone"]

@@ -191,13 +191,17 @@ def test_batch_with_degenerate_windows(require_gpu):
         job.close()
 
 
-def test_nodata_combined_iteration(require_gpu):
+@pytest.mark.parametrize("run_len", [None, 32, 16])
+def test_nodata_combined_iteration(require_gpu, run_len):
     """Level 1, iterations 1 and 2 (the second adds the level-0 term: two FAST
     passes, the level-0 window staged on its own tiles) over pyramids with
     nodata rectangles and flat patches: the FAST contract against the oracle,
-    the degenerate windows of either pass decided by the FP64 pass."""
+    the degenerate windows of either pass decided by the FP64 pass. With runs
+    of 32 or 16 heights a wave of the level-below pass holds runs the level
+    pass handed over beside runs it did not: each is finished exactly once."""
     from setsm_amd.smx import PREC_FAST
     from tests.helpers import perturb_searchspace
+    from tests.test_gpu_batch import _config
     sc, _ = scene(image_px=512, levels=2)
     imgs, _ = _nodata_pyramid(sc, 2, seed=7)
     rng = np.random.default_rng(9)
@@ -209,6 +213,8 @@ def test_nodata_combined_iteration(require_gpu):
     g = level_grid(sc, 1, 48)
     o = OracleRun(pyr, sc.rpcs, 1, g)
     job = _job_with(pyr)
+    if run_len:
+        job.set_fast_config(_config(run_len))
     gr = GpuRun(job, 1, pyr, sc.rpcs, g)
     ug = g["ugrid"]
     for it in (1, 2):
@@ -216,6 +222,7 @@ def test_nodata_combined_iteration(require_gpu):
             ug = perturb_searchspace(ug, 200 + it, 1)
         ro = o.step(ug, it, g["minmax"])
         rf = gr.step(ug, it, g["minmax"], precision=PREC_FAST)
-        print(f"[FAST nodata combined] it{it}: runs redone in FP64 {job.last_stats['redo_runs']}", flush=True)
+        print(f"[FAST nodata combined R{run_len}] it{it}: runs redone in FP64 {job.last_stats['redo_runs']}, "
+              f"evaluated {rf['evaluated']}", flush=True)
         _fast_vs_fp64(ro, rf, len(ug), f"FAST nodata L1 it{it}")
     job.close()
