@@ -19,6 +19,9 @@ The same run reports beside it:
     stripe of the level (median of 3 runs, the reported CPU baseline),
     checked word for word against the GPU cube;
   * SGM aggregation + peak scan (AWNCC at a level <= SGM_py) on the level;
+  * `configs0`: the reference's CPU-runnable case (2048^2 pair, levels
+    L4..L0, two iterations each) on the CPU restatement, timed beside the
+    GPU FP64 and FAST paths;
   * `batched`: the mixed-size tile workload — 64 tiles with sides log-uniform
     over 512..4096 cells, 32..256 heights, 50..1500 m relief, 15..40 degree
     convergence, each its own seeded scene — in ONE persistent FAST launch
@@ -329,6 +332,95 @@ def headline(args, out):
     })
 
 
+# ------------------------------------------------------------ configs[0]
+def c0_run(args):
+    """configs[0], the reference's CPU-runnable case: a 2048^2 pair over 1 km
+    (300 m relief, views +-12 degrees), matching levels L4..L0 at SETSM's grid
+    spacings (8/8/6/3/2 m), 64 heights per cell, two iterations per level
+    (the second adds the level-below term above level 0; its search space
+    from the seeded host-step stand-in). CPU FP64 restatement (median of 3
+    passes) against the GPU FP64 and FAST paths, InitializeVoxel +
+    VerticalLineLocus + AWNCC per step, with word parity per step."""
+    import math
+    from oracle import oracle_py as O
+    from setsm_amd import smx
+    from setsm_amd.records import LevelInputs, height_step
+    from setsm_amd.serve import next_searchspace, setgrids_spacing
+    from setsm_amd.synth import GridSpec, SceneSpec, make_grid, make_scene
+    sc = make_scene(SceneSpec(image_px=2048, relief=300.0, seed=21, view_deg=(12.0, -12.0)), device="cuda")
+    pyr = [O.build_pyramid(im, 5) for im in sc.images]   # the same views on both sides
+    side = sc.spec.image_px * sc.spec.gsd
+    steps = []
+    for lv in (4, 3, 2, 1, 0):
+        res = setgrids_spacing(lv)
+        cells = int(math.ceil(1000.0 / res))
+        inset = (side - cells * res) / 2.0
+        band = float(max(1, int(round(64 * height_step(lv, 0.5)))))
+        g = make_grid(sc, GridSpec(cells=cells, grid_res=res, band=band, inset=inset, x_offset=0.0, y_offset=0.0,
+                                   seed=40 + lv))
+        g["boundary"] = np.array([inset, inset, inset + cells * res, inset + cells * res])
+        g["shape"] = (cells, cells)
+        g["grid_res"] = res
+        li = LevelInputs(pyr, lv, sc.rpcs, g, grid_resolution=res)
+        ug2 = next_searchspace(g["ugrid"], 500 + lv, lv)
+        steps.append((lv, g, li, [g["ugrid"], ug2]))
+
+    def cpu_pass():
+        words, t, hyps = [], 0.0, 0
+        for lv, g, li, ugs in steps:
+            st = O.OracleState(li, keep_rho=False)
+            for it, ug0 in enumerate(ugs, start=1):
+                ug = ug0.copy()
+                t0 = time.perf_counter()
+                st.initialize_voxel(ug, it, g["minmax"])
+                st.vertical_line_locus(ug, it, g["minmax"])
+                st.awncc(ug, it)
+                t += time.perf_counter() - t0
+                words.append(st.export()[0])
+                hyps += int(st.ncc["NumOfHeight"].astype(np.int64).sum())
+        return t, words, hyps
+
+    def gpu_pass(prec):
+        job = smx.Job(5)
+        for ti in range(2):
+            for lv, (img, mag, ori) in enumerate(pyr[ti]):
+                job.set_view(ti, lv, img, mag, ori, start=(0.0, 0.0))
+        words, t, ev = [], 0.0, 0
+        for lv, g, li, ugs in steps:
+            job.set_level(li.level)
+            for it, ug in enumerate(ugs, start=1):
+                job.sync()
+                t0 = time.perf_counter()
+                job.set_searchspace(ug, g["minmax"])
+                job.initialize_voxel(it)
+                job.vertical_line_locus(it, precision=prec)
+                job.awncc(it)
+                t += time.perf_counter() - t0
+                ev += job.last_evaluated
+                words.append(job.get_voxels()[0])
+        job.close()
+        return t, words, ev
+
+    gpu_pass(smx.PREC_FAST)   # warm-up
+    t64, w64, ev64 = gpu_pass(smx.PREC_FP64)
+    tf, wf, evf = gpu_pass(smx.PREC_FAST)
+    cpu = [cpu_pass() for _ in range(3)]
+    tc = float(np.median([c[0] for c in cpu]))
+    wc = cpu[0][1]
+    parity = all(np.array_equal(a, b) for a, b in zip(w64, wc))
+    dq = max(int(np.abs((a >> 16).astype(np.uint16).view(np.int16).astype(int)
+                        - (b >> 16).astype(np.uint16).view(np.int16).astype(int)).max(initial=0))
+             for a, b in zip(wf, wc))
+    out = dict(config=dict(workload="configs[0]: 2048^2 pair over 1 km, L4..L0 at 8/8/6/3/2 m, 64 heights, "
+                                    "2 iterations per level", cells=[int(s[1]["shape"][0]) for s in steps]),
+               hypotheses_evaluated=evf, cpu_s=tc, cpu_cores=O.threads(), cpu_runs_s=[round(c[0], 3) for c in cpu],
+               cpu_hyp_per_s=ev64 / tc, gpu_fp64_s=t64, gpu_fast_s=tf, speedup_fp64_vs_cpu=tc / t64,
+               speedup_fast_vs_cpu=tc / tf, fp64_words_equal_cpu=bool(parity), fast_incc_max_quanta=dq,
+               evaluated_equal=bool(ev64 == evf))
+    log(f"configs[0]: {out}")
+    return out
+
+
 # ------------------------------------------------------------ mixed tiles
 def render_pool(draws):
     import torch
@@ -504,6 +596,7 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="time every FAST launch shape and store the best")
     ap.add_argument("--tuning-out", default=None,
                     help="file the --autotune table is written to (default: the package's tuning.json)")
+    ap.add_argument("--no-c0", dest="c0", action="store_false", help="skip the configs[0] levels L4..L0 leg")
     ap.add_argument("--tiles", type=int, default=64, help="tiles of the mixed-size workload (0 = skip)")
     ap.add_argument("--tile-seed", type=int, default=2026)
     ap.add_argument("--batch-reps", type=int, default=2)
@@ -533,6 +626,8 @@ def main():
 
     out = {}
     headline(args, out)
+    if args.c0:
+        out["configs0"] = c0_run(args)
     if args.tiles > 0 or args.serve_requests > 0:
         from setsm_amd import autotune
         from setsm_amd.synth import draw_tiles
