@@ -59,6 +59,25 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def save_scene(d, sc):
+    os.makedirs(d, exist_ok=True)
+    np.savez(os.path.join(d, "scene.npz"), img0=sc.images[0], img1=sc.images[1], rpc0=sc.rpcs[0], rpc1=sc.rpcs[1],
+             terrain=sc.terrain, origin=np.array(sc.terrain_origin), extent=np.array(sc.extent),
+             resid=np.array(sc.fit_residual_px))
+
+
+def load_scene(d, spec):
+    """The headline scene saved by an earlier run with the same --scene-cache
+    (profiled runs load it instead of rendering on the GPU)."""
+    from setsm_amd.synth import Scene
+    path = os.path.join(d, "scene.npz")
+    if not os.path.exists(path):
+        return None
+    z = np.load(path)
+    return Scene(spec, [z["img0"], z["img1"]], [z["rpc0"], z["rpc1"]], z["terrain"], tuple(z["origin"]),
+                 tuple(z["extent"]), list(z["resid"]))
+
+
 def window_counts(half):
     from setsm_amd.records import window_bits
     n = [0, 0, 0]
@@ -179,7 +198,11 @@ def headline(args, out):
 
     t0 = time.time()
     spec = SceneSpec(image_px=args.image_px, gsd=0.5, relief=300.0, base=200.0, seed=args.seed)
-    sc = make_scene(spec, device="cuda")
+    sc = load_scene(args.scene_cache, spec) if args.scene_cache else None
+    if sc is None:
+        sc = make_scene(spec, device="cuda")
+        if args.scene_cache:
+            save_scene(args.scene_cache, sc)
     torch.cuda.empty_cache()
     log(f"scene {args.image_px}^2 x2 ready in {time.time() - t0:.1f}s, RPC fit residual {max(sc.fit_residual_px):.2e} px")
     side = args.image_px * spec.gsd
@@ -211,7 +234,7 @@ def headline(args, out):
                     stats=job.last_stats)
 
     log("timing plain FP64 kernel")
-    p64 = run(smx.PREC_FP64, args.fp64_steps, 1, keep_rho=True)
+    p64 = run(smx.PREC_FP64, max(1, args.fp64_steps), 1 if args.fp64_steps > 1 else 0, keep_rho=True)
     log(f"fp64: {p64['evaluated']} hyps, {p64['wall_s'] * 1e3:.1f} ms/step, kernel {p64['kernel_ms']:.1f} ms")
     stripe_rows = max(1, args.cpu_rows)
     r0 = cells // 2
@@ -588,11 +611,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--image-px", type=int, default=16384)
     ap.add_argument("--cells", type=int, default=4000)
-    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--fp64-steps", type=int, default=2, help="timed plain FP64 steps (at least one runs; "
+                    "below 2, without a warm-up)")
     ap.add_argument("--cpu-rows", type=int, default=64, help="cell rows of the CPU baseline stripe")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sgm", dest="sgm", action="store_false")
     ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--scene-cache", default="", help="directory to load / save the headline scene")
     ap.add_argument("--autotune", action="store_true", help="time every FAST launch shape and store the best")
     ap.add_argument("--tuning-out", default=None,
                     help="file the --autotune table is written to (default: the package's tuning.json)")

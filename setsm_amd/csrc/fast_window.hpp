@@ -7,6 +7,7 @@
 #pragma once
 
 #include <type_traits>
+#include <utility>
 
 #include "smx_window.hpp"
 
@@ -167,18 +168,73 @@ SMX_FN void mom_fold(Tot<kWide> &t, const Mom &r, float rn)
     t.cd += (Q)r.cd;
 }
 
+// Fractional part of a positive coordinate (v_fract_f32 on the GPU).
+SMX_FN float fract_pos(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_fractf(x);
+#else
+    return x - floorf(x);
+#endif
+}
+
+// a * b + c for |a|, |b| < 2^23 (one v_mad_i32_i24 on the GPU).
+SMX_FN int imad24(int a, int b, int c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __mul24(a, b) + c;
+#else
+    return a * b + c;
+#endif
+}
+
+// f(integral_constant<int, I>) for I = Lo .. Hi - 1, unrolled at compile time.
+template <int Lo, class F, int... I>
+SMX_FN void static_for_impl(F &&f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, Lo + I>{}), ...);
+}
+template <int Lo, int Hi, class F>
+SMX_FN void static_for(F &&f)
+{
+    if constexpr (Hi > Lo) static_for_impl<Lo>(f, std::make_integer_sequence<int, Hi - Lo>{});
+}
+
+// Marks the window totals as used here. The product sums are read only
+// inside the Pearson finish's conditional branch, and without this the
+// compiler sinks their whole accumulation chain there, holding every
+// sample of the window in registers until then.
+template <bool kWide>
+SMX_FN void pin_totals(const Tot<kWide> &t)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" ::"v"(t.a), "v"(t.b), "v"(t.c), "v"(t.d), "v"(t.aa), "v"(t.bb), "v"(t.ab), "v"(t.cc),
+                 "v"(t.dd), "v"(t.cd));
+#else
+    (void)t;
+#endif
+}
+
 // One hypothesis' window loop. L/R are samplers positioned so that at(0,0)
-// is the texel (center - (H+2)) in both axes. (fxl, fyl) is the fractional
-// part of the left centre, (ftx, fty) of the right one. Check = per-sample
-// image-edge and nodata tests (SubFunctions.cpp:3252-3257); otherwise every
-// sample is known to be valid. Returns false if a window count is zero;
-// rho = kRedoRho when a window's spread is not clear of FP32 rounding.
+// is the texel (center - (H+2)) in both axes; R.quad(iy, ix) returns the
+// taps (iy, ix) .. (iy + 1, ix + 1) about that origin. (fxl, fyl) is the
+// fractional part of the left centre, (ftx, fty) of the right one.
+// Check = per-sample image-edge and nodata tests (SubFunctions.cpp:3252-3257);
+// otherwise every sample is known to be valid. Returns false if a window
+// count is zero; rho = kRedoRho when a window's spread is not clear of FP32
+// rounding.
 //
 // Each row is walked from its centre column outwards into one running sum;
 // the sum is folded into the inner / middle / outer window totals when the
 // walk passes that window's half-width on the row (the windows are nested
 // and each is a symmetric column range on a row), so every sample updates
-// one fixed set of registers.
+// one fixed set of registers. Without the per-sample test every row is
+// straight-line code with its column range fixed at compile time.
+//
+// The right window's sample positions are taken about the window origin:
+// every sample of the outer window lies within H + 1 of the centre, so its
+// position is at least 1 in both axes (origin offset H + 2, centre fraction
+// >= 0) — truncation is the floor and v_fract the bilinear weight.
 template <int H, bool kCheck, bool kWide, class SL, class SR>
 SMX_FN bool fast_window(const SL &L, const SR &R, float fxl, float fyl, float ftx, float fty,
                                          float cos0, float sin0, float piv_l, float piv_r, float lim_lx0,
@@ -197,84 +253,87 @@ SMX_FN bool fast_window(const SL &L, const SR &R, float fxl, float fyl, float ft
     const float2 pvr = bilerp(R.at(O, O), R.at(O, O + 1), R.at(O + 1, O), R.at(O + 1, O + 1), ftx, fty);
     // the left patch shares the centre's fractional offset: one set of weights
     const float lw11 = fxl * fyl, lw10 = fyl - lw11, lw01 = fxl - lw11, lw00 = (1.f - fxl) - lw10;
-#pragma unroll 1
-    for (int row = -H; row <= H; row++) {
-        const int w0 = kW.w[row + H][0], w1 = kW.w[row + H][1], w2 = kW.w[row + H][2];
-        const float frow = (float)row;
+    // right centre about the window origin
+    const float ftxo = ftx + (float)O, ftyo = fty + (float)O;
+    Mom run = Mom{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    float rn = 0.f;
+    // One sample at (row, col) of the window; xr / yr: the row's rotated
+    // start about the origin. first: the row's first sample (no test).
+    auto sample = [&](const int row, const int col, const float xr, const float yr, const bool first) {
+        // left: integer offsets, the hypothesis' bilinear weights; the
+        // pivot enters as the first term of the weighted sum
+        const float2 a = L.at(O + row, O + col), b = L.at(O + row, O + col + 1);
+        const float2 c = L.at(O + row + 1, O + col), d = L.at(O + row + 1, O + col + 1);
+        const float li = fmaf(lw11, d.x, fmaf(lw10, c.x, fmaf(lw01, b.x, fmaf(lw00, a.x, -pvl.x))));
+        const float lm = fmaf(lw11, d.y, fmaf(lw10, c.y, fmaf(lw01, b.y, fmaf(lw00, a.y, -pvl.y))));
+        // right: rotated offsets (setsm_code.cpp:6437-6439),
         // x = ftx + col*cos - row*sin, y = fty + col*sin + row*cos
-        const float xr = fmaf(-frow, sin0, ftx), yr = fmaf(frow, cos0, fty);
-        Mom run = Mom{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        float rn = 0.f;
-        auto sample = [&](const int col, const bool first) {
-            // left: integer offsets, the hypothesis' bilinear weights; the
-            // pivot enters as the first term of the weighted sum
-            const float2 a = L.at(O + row, O + col), b = L.at(O + row, O + col + 1);
-            const float2 c = L.at(O + row + 1, O + col), d = L.at(O + row + 1, O + col + 1);
-            const float li = fmaf(lw11, d.x, fmaf(lw10, c.x, fmaf(lw01, b.x, fmaf(lw00, a.x, -pvl.x))));
-            const float lm = fmaf(lw11, d.y, fmaf(lw10, c.y, fmaf(lw01, b.y, fmaf(lw00, a.y, -pvl.y))));
-            // right: rotated offsets (setsm_code.cpp:6437-6439)
-            const float x = fmaf((float)col, cos0, xr);
-            const float y = fmaf((float)col, sin0, yr);
-            const float flx = floorf(x), fly = floorf(y);
-            const float wx = x - flx, wy = y - fly;
-            float2 r00, r01, r10, r11;
-            R.quad(fly, flx, O, r00, r01, r10, r11);
-            const float w11 = wx * wy, w10 = wy - w11, w01 = wx - w11, w00 = (1.f - wx) - w10;
-            const float ri = fmaf(w11, r11.x, fmaf(w10, r10.x, fmaf(w01, r01.x, fmaf(w00, r00.x, -pvr.x))));
-            const float rm = fmaf(w11, r11.y, fmaf(w10, r10.y, fmaf(w01, r01.y, fmaf(w00, r00.y, -pvr.y))));
-            bool ok = true;
-            if (kCheck) {
-                // pos >= 0 && pos + 1 < size on both images, then L > 0 && R > 0
-                // (SubFunctions.cpp:3252-3257). The interpolation of
-                // non-negative DNs is positive exactly when a tap with a
-                // non-zero weight is positive; tested on the taps, which hold
-                // DN - pivot exactly, so rounding in the weighted sum cannot
-                // turn a nodata sample into a valid one.
-                const bool lpos = a.x > -piv_l || (fxl > 0.f && b.x > -piv_l) || (fyl > 0.f && c.x > -piv_l) ||
-                                  (fxl > 0.f && fyl > 0.f && d.x > -piv_l);
-                const bool rpos = r00.x > -piv_r || (wx > 0.f && r01.x > -piv_r) || (wy > 0.f && r10.x > -piv_r) ||
-                                  (wx > 0.f && wy > 0.f && r11.x > -piv_r);
-                ok = (float)col >= lim_lx0 && (float)col < lim_lx1 && frow >= lim_ly0 && frow < lim_ly1 &&
-                     x >= lim_rx0 && x < lim_rx1 && y >= lim_ry0 && y < lim_ry1 && lpos && rpos;
+        const float x = col == 0 ? xr : fmaf((float)col, cos0, xr);
+        const float y = col == 0 ? yr : fmaf((float)col, sin0, yr);
+        const float wx = fract_pos(x), wy = fract_pos(y);
+        float2 r00, r01, r10, r11;
+        R.quad((int)y, (int)x, r00, r01, r10, r11);
+        const float w11 = wx * wy, w10 = wy - w11, w01 = wx - w11, w00 = (1.f - wx) - w10;
+        const float ri = fmaf(w11, r11.x, fmaf(w10, r10.x, fmaf(w01, r01.x, fmaf(w00, r00.x, -pvr.x))));
+        const float rm = fmaf(w11, r11.y, fmaf(w10, r10.y, fmaf(w01, r01.y, fmaf(w00, r00.y, -pvr.y))));
+        bool ok = true;
+        if (kCheck) {
+            // pos >= 0 && pos + 1 < size on both images, then L > 0 && R > 0
+            // (SubFunctions.cpp:3252-3257). The interpolation of
+            // non-negative DNs is positive exactly when a tap with a
+            // non-zero weight is positive; tested on the taps, which hold
+            // DN - pivot exactly, so rounding in the weighted sum cannot
+            // turn a nodata sample into a valid one.
+            const float frow = (float)row;
+            const bool lpos = a.x > -piv_l || (fxl > 0.f && b.x > -piv_l) || (fyl > 0.f && c.x > -piv_l) ||
+                              (fxl > 0.f && fyl > 0.f && d.x > -piv_l);
+            const bool rpos = r00.x > -piv_r || (wx > 0.f && r01.x > -piv_r) || (wy > 0.f && r10.x > -piv_r) ||
+                              (wx > 0.f && wy > 0.f && r11.x > -piv_r);
+            ok = (float)col >= lim_lx0 && (float)col < lim_lx1 && frow >= lim_ly0 && frow < lim_ly1 &&
+                 x >= lim_rx0 + (float)O && x < lim_rx1 + (float)O && y >= lim_ry0 + (float)O &&
+                 y < lim_ry1 + (float)O && lpos && rpos;
+        }
+        if (!kCheck && first) {
+            run = Mom{li, ri, li * li, ri * ri, li * ri, lm, rm, lm * lm, rm * rm, lm * rm};
+        } else if (ok) {
+            mom_add(run, li, ri, lm, rm);
+            if (kCheck) rn += 1.f;
+        }
+    };
+    if constexpr (!kCheck) {
+        // every sample valid: straight-line code, rows and columns unrolled
+        static_for<-H, H + 1>([&](auto RowC) {
+            constexpr int row = decltype(RowC)::value;
+            constexpr int W0 = kW.w[row + H][0], W1 = kW.w[row + H][1], W2 = kW.w[row + H][2];
+            if constexpr (W0 >= 0) {
+                const float xr = row == 0 ? ftxo : fmaf(-(float)row, sin0, ftxo);
+                const float yr = row == 0 ? ftyo : fmaf((float)row, cos0, ftyo);
+                static_for<0, W0 + 1>([&](auto KC) {
+                    constexpr int k = decltype(KC)::value;
+                    sample(row, k, xr, yr, k == 0);
+                    if constexpr (k > 0) sample(row, -k, xr, yr, false);
+                    if constexpr (k == W2) mom_fold(tot[2], run, rn);
+                    if constexpr (k == W1) mom_fold(tot[1], run, rn);
+                });
+                mom_fold(tot[0], run, rn);
+                pin_totals(tot[0]);
+                if constexpr (W1 >= 0) pin_totals(tot[1]);
+                if constexpr (W2 >= 0) pin_totals(tot[2]);
             }
-            if (!kCheck && first) {
-                run = Mom{li, ri, li * li, ri * ri, li * ri, lm, rm, lm * lm, rm * rm, lm * rm};
-            } else if (ok) {
-                mom_add(run, li, ri, lm, rm);
-                if (kCheck) rn += 1.f;
-            }
-        };
-        if constexpr (!kCheck) {
-            // every sample valid: straight-line code per half-width triple
-            auto walk = [&](auto W0c, auto W1c, auto W2c) {
-                constexpr int W0 = decltype(W0c)::value, W1 = decltype(W1c)::value, W2 = decltype(W2c)::value;
-#pragma unroll
-                for (int k = 0; k <= W0; k++) {
-                    sample(k, k == 0);
-                    if (k > 0) sample(-k, false);
-                    if (k == W2) mom_fold(tot[2], run, rn);
-                    if (k == W1) mom_fold(tot[1], run, rn);
-                }
-                if (W0 >= 0) mom_fold(tot[0], run, rn);
-            };
-            constexpr RowSigs<H> kS{};
-#define SMX_SIG(I)                                                                                                  \
-    case I:                                                                                                         \
-        if constexpr (I < kS.n)                                                                                     \
-            walk(std::integral_constant<int, kS.w[I][0]>{}, std::integral_constant<int, kS.w[I][1]>{},             \
-                 std::integral_constant<int, kS.w[I][2]>{});                                                        \
-        break;
-            switch (kS.of_row[row + H]) {
-                SMX_SIG(0) SMX_SIG(1) SMX_SIG(2) SMX_SIG(3) SMX_SIG(4) SMX_SIG(5)
-            default: break;
-            }
-#undef SMX_SIG
-        } else {
+        });
+    } else {
+#pragma unroll 1
+        for (int row = -H; row <= H; row++) {
+            const int w0 = kW.w[row + H][0], w1 = kW.w[row + H][1], w2 = kW.w[row + H][2];
+            const float frow = (float)row;
+            const float xr = fmaf(-frow, sin0, ftxo), yr = fmaf(frow, cos0, ftyo);
+            run = Mom{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            rn = 0.f;
 #pragma unroll
             for (int k = 0; k <= H; k++) {
                 if (k <= w0) {
-                    sample(k, false);
-                    if (k > 0) sample(-k, false);
+                    sample(row, k, xr, yr, false);
+                    if (k > 0) sample(row, -k, xr, yr, false);
                 }
                 if (k == w2) mom_fold(tot[2], run, rn);
                 if (k == w1) mom_fold(tot[1], run, rn);

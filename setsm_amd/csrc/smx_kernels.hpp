@@ -167,10 +167,17 @@ __device__ __forceinline__ bool project_hypothesis(const VllParams &P, const Cel
 }
 
 // Max of temp_rho over each run of a wavefront (runs of R lanes) into its
-// cell's max_WNCC key, and a count of the hypotheses that computed an NCC.
-// Every lane of the wave must call it; cell_key is the key of the lane's run.
-__device__ __forceinline__ void wave_max_commit(double v, unsigned evaluated, unsigned long long *cell_key,
-                                                unsigned long long *counter, int R = kRun)
+// cell's max_WNCC key. Returns the wave's count of hypotheses that computed
+// an NCC (the same value on every lane). Every lane of the wave must call it;
+// cell_key is the key of the lane's run.
+//
+// The count is not added to the launch's counter here: callers sum it over
+// the runs a wave evaluates and add it once (wave_count_commit). One global
+// counter updated by every run would serialise tens of millions of
+// device-scope atomics on one address per launch — they, not the windows,
+// would set the kernel's duration.
+__device__ __forceinline__ unsigned wave_max_commit(double v, unsigned evaluated, unsigned long long *cell_key,
+                                                    int R = kRun)
 {
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
@@ -180,10 +187,14 @@ __device__ __forceinline__ void wave_max_commit(double v, unsigned evaluated, un
     const unsigned long long any = __ballot(evaluated != 0);
     const int lane = threadIdx.x & 63;
     const unsigned long long mine = R == 64 ? any : (any >> (lane & ~(R - 1))) & ((1ull << R) - 1);
-    if ((lane & (R - 1)) == 0 && mine) {
-        atomicMax(cell_key, order_key(v));
-        atomicAdd(counter, (unsigned long long)__popcll(mine));
-    }
+    if ((lane & (R - 1)) == 0 && mine) atomicMax(cell_key, order_key(v));
+    return (unsigned)__popcll(any);
+}
+
+// Adds a wave's count (uniform over the wave) to a launch counter, once.
+__device__ __forceinline__ void wave_count_commit(unsigned long long *counter, unsigned long long n)
+{
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(counter, n);
 }
 
 }  // namespace smx

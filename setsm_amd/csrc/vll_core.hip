@@ -336,8 +336,9 @@ __device__ __forceinline__ bool multi_ncc_fp64_at(const DevView &Lv, const DevVi
 }
 
 // One run (up to 64 consecutive heights of one cell) on one wavefront, FP64.
+// Returns the run's count of hypotheses that computed an NCC (uniform).
 template <int H>
-__device__ __forceinline__ void vll_fp64_run(const VllParams *__restrict__ P, const CellArgs &A, const int2 run,
+__device__ __forceinline__ unsigned vll_fp64_run(const VllParams *__restrict__ P, const CellArgs &A, const int2 run,
                                              const long wave, const int lane)
 {
     const long cell = run.x;
@@ -373,8 +374,7 @@ __device__ __forceinline__ void vll_fp64_run(const VllParams *__restrict__ P, co
             evaluated = 1;
         }
         A.peak_rho[wave * kRun + lane] = pr;
-        wave_max_commit(temp_rho, evaluated, &A.max_bits[cell], A.evaluated);
-        return;
+        return wave_max_commit(temp_rho, evaluated, &A.max_bits[cell]);
     }
     if (act) {
         const long wi = A.vox_off[cell] + hidx;
@@ -397,7 +397,7 @@ __device__ __forceinline__ void vll_fp64_run(const VllParams *__restrict__ P, co
             evaluated = 1;
         }
     }
-    wave_max_commit(temp_rho, evaluated, &A.max_bits[cell], A.evaluated);
+    return wave_max_commit(temp_rho, evaluated, &A.max_bits[cell]);
 }
 
 // One wave per run, grid-stride over the count on the device (the launch
@@ -408,8 +408,10 @@ __global__ void __launch_bounds__(256) k_vll_fp64(const VllParams *__restrict__ 
 {
     const long nw = ((long)gridDim.x * blockDim.x) >> 6;
     const long n = min((long)*A.nruns_dev, A.runs_cap);
+    unsigned long long cnt = 0;
     for (long q = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < n; q += nw)
-        vll_fp64_run<H>(P, A, A.runs[q], q, threadIdx.x & 63);
+        cnt += vll_fp64_run<H>(P, A, A.runs[q], q, threadIdx.x & 63);
+    wave_count_commit(A.evaluated, cnt);
 }
 
 // The runs the FAST kernel handed over (a window whose spread it cannot tell
@@ -421,8 +423,10 @@ __global__ void __launch_bounds__(256) k_vll_redo(const VllParams *__restrict__ 
 {
     const long nw = ((long)gridDim.x * blockDim.x) >> 6;
     const long n = (long)*A.redo_n;
+    unsigned long long cnt = 0;
     for (long q = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < n; q += nw)
-        vll_fp64_run<H>(P, A, A.redo[q], q, threadIdx.x & 63);
+        cnt += vll_fp64_run<H>(P, A, A.redo[q], q, threadIdx.x & 63);
+    wave_count_commit(A.evaluated, cnt);
 }
 
 template __global__ void k_vll_fp64<5>(const VllParams *, CellArgs);

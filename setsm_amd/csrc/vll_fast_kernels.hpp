@@ -35,6 +35,14 @@ namespace smx {
 // ------------------------------------------------------------- samplers
 
 extern __shared__ float2 smx_lds[];
+// A float pair in LDS addressed by its 32-bit LDS offset (address space 3).
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(3))) f32x2 lds_f2;
+__device__ __forceinline__ float2 lds_at(uint32_t byte_offset)
+{
+    const f32x2 v = *(lds_f2 *)(uintptr_t)byte_offset;
+    return make_float2(v.x, v.y);
+}
 
 // Texels of one image staged in LDS: float2 per texel, rows of the box
 // packed back to back (pitch = box width). Indexing the __shared__ array
@@ -43,24 +51,25 @@ struct LdsImg {
     int base;   // index of texel (0, 0) of the window origin
     int pitch;
     __device__ __forceinline__ float2 at(int row, int col) const { return smx_lds[base + row * pitch + col]; }
-    // The 2x2 taps at (row + off, col + off) .. (row + off + 1, col + off + 1)
-    // for integer-valued float coordinates: the offset is formed with one
-    // exact FP32 fma (offsets stay far below 2^24); `off` (a constant) folds
-    // into the per-hypothesis base.
-    __device__ __forceinline__ void quad(float row, float col, int off, float2 &t00, float2 &t01, float2 &t10,
-                                         float2 &t11) const
+    // The 2x2 taps (iy, ix) .. (iy + 1, ix + 1) about the window origin
+    // (0 <= iy, ix, small): one 24-bit multiply-add and the scaled base add
+    // form the address, the second row is one pitch further.
+    __device__ __forceinline__ void quad(int iy, int ix, float2 &t00, float2 &t01, float2 &t10, float2 &t11) const
     {
-        const int o = (base + off * pitch + off) + (int)fmaf(row, (float)pitch, col);
-        t00 = smx_lds[o];
-        t01 = smx_lds[o + 1];
-        t10 = smx_lds[o + pitch];
-        t11 = smx_lds[o + pitch + 1];
+        const int pitch8 = pitch * (int)sizeof(float2);
+        // LDS byte address of (0, ix), made opaque to the compiler, which
+        // would otherwise split the multiply-add below into a multiply and a
+        // three-way add
+        int col = (ix << 3) + (int)(uint32_t)(uintptr_t)(lds_f2 *)(smx_lds + base);
+        asm("" : "+v"(col));
+        const uint32_t o = (uint32_t)imad24(iy, pitch8, col);
+        t00 = lds_at(o);
+        t01 = lds_at(o + sizeof(float2));
+        t10 = lds_at(o + pitch8);
+        t11 = lds_at(o + pitch8 + sizeof(float2));
     }
 };
 
-// Texels read from HBM (packed u16 pair), shifted by a pivot on the fly.
-// Coordinates are clamped into the image so that edge taps stay in bounds;
-// samples that need such taps are rejected by the per-sample test.
 struct GlobalImg {
     gptr_t<uint32_t> texel;
     int width, height;
@@ -73,14 +82,12 @@ struct GlobalImg {
         const uint32_t t = texel[(long)y * width + x];
         return make_float2((float)(t & 0xffffu) - piv_i, (float)(t >> 16) - piv_m);
     }
-    __device__ __forceinline__ void quad(float row, float col, int off, float2 &t00, float2 &t01, float2 &t10,
-                                         float2 &t11) const
+    __device__ __forceinline__ void quad(int iy, int ix, float2 &t00, float2 &t01, float2 &t10, float2 &t11) const
     {
-        const int r = (int)row + off, c = (int)col + off;
-        t00 = at(r, c);
-        t01 = at(r, c + 1);
-        t10 = at(r + 1, c);
-        t11 = at(r + 1, c + 1);
+        t00 = at(iy, ix);
+        t01 = at(iy, ix + 1);
+        t10 = at(iy + 1, ix);
+        t11 = at(iy + 1, ix + 1);
     }
 };
 
@@ -210,11 +217,13 @@ __device__ __forceinline__ bool group_any(bool x, int R)
 }
 
 // The runs ri .. ri + 64/R - 1 of the tile's list (those below r1) on one
-// wavefront, R = A.run_len lanes each. kCombined: a single-pass launch may
+// wavefront, R = A.run_len lanes each. Returns the wave's count of
+// hypotheses that computed an NCC here (uniform; runs handed to the FP64
+// pass are counted there). kCombined: a single-pass launch may
 // meet a combined-level iteration (the single-pass persistent kernel never
 // does: batches launch such jobs in two passes).
 template <int H, bool kWide, int kMode, bool kCombined = true>
-__device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs &A, const FastArgs &F,
+__device__ __forceinline__ unsigned vll_fast_run(const VllParams &P, const CellArgs &A, const FastArgs &F,
                                              const TileBox &tb, const bool fits, const bool nodata, const long ri,
                                              const long r1, const int lane)
 {
@@ -316,12 +325,12 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
             if constexpr (kMode == kLevelPass) {
                 if (do_ncc) A.part[wi] = kRedoRho;   // the level-below pass skips the run
             }
-            return;
+            return 0;
         }
         if constexpr (kMode == kLevelPass) {
             // the level term, kept for the level-below pass
             if (do_ncc) A.part[wi] = !all3 ? kPartNoWindow : rho == 1e300 ? kPartNoCorr : rho;
-            return;
+            return 0;
         }
         // setsm_code.cpp:6456-6493: average of the level term and, on even
         // iterations above level 0, the level-1 term (in that order).
@@ -352,7 +361,7 @@ __device__ __forceinline__ void vll_fast_run(const VllParams &P, const CellArgs 
             temp_rho = wncc_of(A, cell, sum, cnt, rr);
             evaluated = 1;
         }
-        wave_max_commit(temp_rho, evaluated, &A.max_bits[cell], A.evaluated, R);
+        return wave_max_commit(temp_rho, evaluated, &A.max_bits[cell], R);
     }
 
 template <int H, bool kWide, int kMode = kSingle, bool kCombined = true>
@@ -393,14 +402,17 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
         if (fits) box_stage(views, tb);
         __syncthreads();
         const bool nodata = tb.nodata != 0;
+        unsigned long long cnt = 0;
         for (long ri = r0 + (long)wave * per; ri < r1; ri += (long)nwaves * per)
-            vll_fast_run<H, kWide, kMode, kCombined>(P, A, F, tb, fits, nodata, ri, r1, lane);
+            cnt += vll_fast_run<H, kWide, kMode, kCombined>(P, A, F, tb, fits, nodata, ri, r1, lane);
+        wave_count_commit(A.evaluated, cnt);
         return;
     }
     // 2. Tall footprints (wide height bands, strong convergence): stage one
     // box per chunk of runs the waves evaluate together; each run spans at
     // most 64 heights, so a chunk's box stays small.
     const long cw = (long)nwaves * per;
+    unsigned long long cnt = 0;
     for (long c0 = r0; c0 < r1; c0 += cw) {
         __syncthreads();
         box_reset(sbox);
@@ -427,8 +439,10 @@ __device__ __forceinline__ void vll_fast_body(const VllParams *__restrict__ Pp, 
         __syncthreads();
         const bool nodata = tb.nodata != 0;
         if (c0 + (long)wave * per < r1)
-            vll_fast_run<H, kWide, kMode, kCombined>(P, A, F, tb, fits, nodata, c0 + (long)wave * per, r1, lane);
+            cnt += vll_fast_run<H, kWide, kMode, kCombined>(P, A, F, tb, fits, nodata, c0 + (long)wave * per, r1,
+                                                            lane);
     }
+    wave_count_commit(A.evaluated, cnt);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8), each

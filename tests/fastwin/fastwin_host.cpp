@@ -24,13 +24,12 @@ struct HostImg {
         const uint32_t t = texel[(long)y * width + x];
         return make_float2((float)(t & 0xffffu) - piv_i, (float)(t >> 16) - piv_m);
     }
-    void quad(float row, float col, int off, float2 &t00, float2 &t01, float2 &t10, float2 &t11) const
+    void quad(int iy, int ix, float2 &t00, float2 &t01, float2 &t10, float2 &t11) const
     {
-        const int r = (int)row + off, c = (int)col + off;
-        t00 = at(r, c);
-        t01 = at(r, c + 1);
-        t10 = at(r + 1, c);
-        t11 = at(r + 1, c + 1);
+        t00 = at(iy, ix);
+        t01 = at(iy, ix + 1);
+        t10 = at(iy + 1, ix);
+        t11 = at(iy + 1, ix + 1);
     }
 };
 
