@@ -485,6 +485,10 @@ def batched_run(args, draws, scenes):
         best, rows = autotune.tune_batch([jobs[k] for k in sub], log=log)
         autotune.save(key, best, rows, path=args.tuning_out or autotune.TUNING_PATH)
         cfg = best["config"]
+        # the sweep's small-tile and short-run shapes grew these jobs' lists
+        for k in sub:
+            jobs[k].trim()
+        smx.release_cached()
     else:
         entry = autotune.load(key)
         cfg = entry["best"]["config"] if entry else None

@@ -73,6 +73,13 @@ int  smx_job_set_budget(smx_job *job, long long bytes);
  * jobs and levels; this returns them to the HIP runtime and reports their
  * bytes. No counterpart in the reference (library memory management). */
 long long smx_release_cached(void);
+/* Frees the job's per-call working lists (run and tile lists, the FP64
+ * hand-over queue, level-pass partials, batch descriptors), which the next
+ * VerticalLineLocus call reserves again at the size it needs: after a launch
+ * shape with many small tiles or short runs, they can hold far more than the
+ * job's usual shape. The cube, search space and views are kept. No
+ * counterpart in the reference (library memory management). */
+int  smx_job_trim(smx_job *job);
 
 /* Upload one pyramid level of image `img` (0 = reference, 1 = target):
  * intensity, Sobel magnitude and orientation, as SetPyramidImages

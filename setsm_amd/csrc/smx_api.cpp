@@ -786,6 +786,31 @@ int smx_job_destroy(smx_job *j)
 
 long long smx_job_device_bytes(const smx_job *j) { return j ? j->device_bytes : 0; }
 
+int smx_job_trim(smx_job *j)
+{
+    if (!j) return fail(SMX_E_ARG, "null job");
+    (void)hipSetDevice(j->device);
+    VllScratch &S = j->js;
+    jfree(j, S.runs);
+    S.runs_cap = 0;
+    jfree(j, S.tile_count);
+    jfree(j, S.tile_off);
+    jfree(j, S.tile_runs);
+    S.tiles_cap = 0;
+    jfree(j, S.tile2_count);
+    jfree(j, S.tile2_off);
+    jfree(j, S.tile2_runs);
+    S.tiles2_cap = 0;
+    jfree(j, S.redo);
+    jfree(j, S.part);
+    S.part_cap = 0;
+    j->cap_tile_runs = j->cap_tile2_runs = j->cap_redo = 0;
+    job_free(j, j->d_batch);
+    j->d_batch = nullptr;
+    j->batch_bytes = 0;
+    return 0;
+}
+
 long long smx_release_cached(void)
 {
     std::map<std::pair<int, size_t>, std::vector<void *>> blocks;

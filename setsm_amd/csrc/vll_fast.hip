@@ -278,15 +278,17 @@ int launch_vll_fast_batch(smx_job *const *jobs, int n, const CellArgs *As, const
     int lds_texels = 0;
     for (int i = 0; i < n; i++) lds_texels = std::max(lds_texels, kFastCfgs[fast_config_index(jobs[i])].lds_texels);
     const size_t lds = (size_t)lds_texels * sizeof(float2);
+    // register budget: capped when the first job's shape asks for it
+    const bool capped = kFastCfgs[fast_config_index(jobs[0])].occ != 0;
     int cus = 0;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, jobs[0]->device);
     auto launch = [&](int mode, const BatchJob *jl, int nj, long total, unsigned long long *next) -> int {
         if (nj == 0 || total == 0) return 0;
         int per_cu = 0;
         switch (half) {
-        case 5: per_cu = fast_batch_blocks_per_cu<5>(mode, lds); break;
-        case 6: per_cu = fast_batch_blocks_per_cu<6>(mode, lds); break;
-        case 7: per_cu = fast_batch_blocks_per_cu<7>(mode, lds); break;
+        case 5: per_cu = fast_batch_blocks_per_cu<5>(mode, capped, lds); break;
+        case 6: per_cu = fast_batch_blocks_per_cu<6>(mode, capped, lds); break;
+        case 7: per_cu = fast_batch_blocks_per_cu<7>(mode, capped, lds); break;
         default: return fail(SMX_E_UNSUPPORTED, "half %d", half);
         }
         int grid = std::max(1, cus) * std::max(1, per_cu);
@@ -294,9 +296,9 @@ int launch_vll_fast_batch(smx_job *const *jobs, int n, const CellArgs *As, const
         const BatchArgs B{jl, nj, total, next};
         hipError_t e = hipSuccess;
         switch (half) {
-        case 5: e = fast_launch_batch<5>(mode, grid, lds, s, B); break;
-        case 6: e = fast_launch_batch<6>(mode, grid, lds, s, B); break;
-        case 7: e = fast_launch_batch<7>(mode, grid, lds, s, B); break;
+        case 5: e = fast_launch_batch<5>(mode, capped, grid, lds, s, B); break;
+        case 6: e = fast_launch_batch<6>(mode, capped, grid, lds, s, B); break;
+        case 7: e = fast_launch_batch<7>(mode, capped, grid, lds, s, B); break;
         }
         if (e != hipSuccess) return fail(SMX_E_HIP, "k_vll_fast_batch: %s", hipGetErrorString(e));
         return 0;

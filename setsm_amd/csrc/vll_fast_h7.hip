@@ -8,7 +8,7 @@ template hipError_t fast_launch<7>(int, bool, dim3, size_t, hipStream_t, const V
                                    const FastArgs &);
 template hipError_t fast_launch_passes<7>(bool, dim3, dim3, size_t, hipStream_t, const VllParams *,
                                           const CellArgs &, const FastArgs &, const FastArgs &);
-template hipError_t fast_launch_batch<7>(int, int, size_t, hipStream_t, const BatchArgs &);
-template int fast_batch_blocks_per_cu<7>(int, size_t);
+template hipError_t fast_launch_batch<7>(int, bool, int, size_t, hipStream_t, const BatchArgs &);
+template int fast_batch_blocks_per_cu<7>(int, bool, size_t);
 
 }  // namespace smx

@@ -490,8 +490,17 @@ __device__ __forceinline__ void vll_fast_persistent(const BatchArgs &B)
     }
 }
 
+// The persistent kernel in the same two register budgets as the per-tile
+// kernels: the compiler's own choice, and one capped for three waves per
+// SIMD. The batch's launch shape picks (its occupancy field).
 template <int H, int kMode>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) k_vll_fast_batch(BatchArgs B)
+__global__ void __launch_bounds__(256) k_vll_fast_batch(BatchArgs B)
+{
+    vll_fast_persistent<H, kMode>(B);
+}
+
+template <int H, int kMode>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) k_vll_fast_batch_o3(BatchArgs B)
 {
     vll_fast_persistent<H, kMode>(B);
 }
@@ -558,28 +567,34 @@ hipError_t fast_launch_passes(bool wide, dim3 g1, dim3 g2, size_t lds, hipStream
     return hipGetLastError();
 }
 
+template <int H, int kMode>
+static auto batch_kernel(bool capped)
+{
+    return capped ? k_vll_fast_batch_o3<H, kMode> : k_vll_fast_batch<H, kMode>;
+}
+
 template <int H>
-hipError_t fast_launch_batch(int mode, int grid, size_t lds, hipStream_t s, const BatchArgs &B)
+hipError_t fast_launch_batch(int mode, bool capped, int grid, size_t lds, hipStream_t s, const BatchArgs &B)
 {
     if (mode == kLevelPass)
-        hipLaunchKernelGGL((k_vll_fast_batch<H, kLevelPass>), dim3(grid), dim3(256), lds, s, B);
+        hipLaunchKernelGGL((batch_kernel<H, kLevelPass>(capped)), dim3(grid), dim3(256), lds, s, B);
     else if (mode == kBelowPass)
-        hipLaunchKernelGGL((k_vll_fast_batch<H, kBelowPass>), dim3(grid), dim3(256), lds, s, B);
+        hipLaunchKernelGGL((batch_kernel<H, kBelowPass>(capped)), dim3(grid), dim3(256), lds, s, B);
     else
-        hipLaunchKernelGGL((k_vll_fast_batch<H, kSingle>), dim3(grid), dim3(256), lds, s, B);
+        hipLaunchKernelGGL((batch_kernel<H, kSingle>(capped)), dim3(grid), dim3(256), lds, s, B);
     return hipGetLastError();
 }
 
 template <int H>
-int fast_batch_blocks_per_cu(int mode, size_t lds)
+int fast_batch_blocks_per_cu(int mode, bool capped, size_t lds)
 {
     int per_cu = 0;
     if (mode == kLevelPass)
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vll_fast_batch<H, kLevelPass>, 256, lds);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<H, kLevelPass>(capped), 256, lds);
     else if (mode == kBelowPass)
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vll_fast_batch<H, kBelowPass>, 256, lds);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<H, kBelowPass>(capped), 256, lds);
     else
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vll_fast_batch<H, kSingle>, 256, lds);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<H, kSingle>(capped), 256, lds);
     return per_cu;
 }
 

@@ -65,17 +65,17 @@ template <int H>
 hipError_t fast_launch_passes(bool wide, dim3 g1, dim3 g2, size_t lds, hipStream_t s, const VllParams *dP,
                               const CellArgs &A, const FastArgs &F1, const FastArgs &F2);
 template <int H>
-hipError_t fast_launch_batch(int mode, int grid, size_t lds, hipStream_t s, const BatchArgs &B);
+hipError_t fast_launch_batch(int mode, bool capped, int grid, size_t lds, hipStream_t s, const BatchArgs &B);
 template <int H>
-int fast_batch_blocks_per_cu(int mode, size_t lds);
+int fast_batch_blocks_per_cu(int mode, bool capped, size_t lds);
 
 #define SMX_FAST_EXTERN(H)                                                                                         \
     extern template hipError_t fast_launch<H>(int, bool, dim3, size_t, hipStream_t, const VllParams *,              \
                                               const CellArgs &, const FastArgs &);                                \
     extern template hipError_t fast_launch_passes<H>(bool, dim3, dim3, size_t, hipStream_t, const VllParams *,     \
                                                      const CellArgs &, const FastArgs &, const FastArgs &);       \
-    extern template hipError_t fast_launch_batch<H>(int, int, size_t, hipStream_t, const BatchArgs &);            \
-    extern template int fast_batch_blocks_per_cu<H>(int, size_t);
+    extern template hipError_t fast_launch_batch<H>(int, bool, int, size_t, hipStream_t, const BatchArgs &);            \
+    extern template int fast_batch_blocks_per_cu<H>(int, bool, size_t);
 SMX_FAST_EXTERN(5)
 SMX_FAST_EXTERN(6)
 SMX_FAST_EXTERN(7)

@@ -36,7 +36,7 @@ EXPORTS = [
     "smx_job_get_sumcost", "smx_job_set_ortho_wgs",
     "smx_job_last_kernel_ms", "smx_job_last_evaluated", "smx_job_last_redo", "smx_job_sync", "smx_fast_num_configs",
     "smx_job_set_fast_config", "smx_fast_config_desc", "smx_job_last_stats", "smx_vertical_line_locus_batch",
-    "smx_vertical_line_locus_noalloc", "smx_release_cached", "smx_job_half", "smx_graph_create", "smx_graph_launch", "smx_graph_last_ms", "smx_graph_destroy",
+    "smx_vertical_line_locus_noalloc", "smx_release_cached", "smx_job_trim", "smx_job_half", "smx_graph_create", "smx_graph_launch", "smx_graph_last_ms", "smx_graph_destroy",
 ]
 
 
@@ -95,6 +95,7 @@ def lib():
             "smx_vertical_line_locus_batch": (i, [vp, i, i, i, i, vp]),
             "smx_vertical_line_locus_noalloc": (ll, [vp, i, i]),
             "smx_release_cached": (ll, []),
+            "smx_job_trim": (i, [vp]),
             "smx_job_half": (i, [vp]),
             "smx_graph_create": (i, [C.POINTER(vp), vp, i, i, i, i]),
             "smx_graph_launch": (i, [vp, vp]),
@@ -136,6 +137,11 @@ class Job:
         if self.h:
             lib().smx_job_destroy(self.h)
             self.h = None
+
+    def trim(self):
+        """Free the per-call working lists (run / tile lists, hand-over queue,
+        batch descriptors); the next call reserves them at the size it needs."""
+        _check(lib().smx_job_trim(self.h))
 
     def __del__(self):
         try:

@@ -178,3 +178,33 @@ def test_batch_fp64_is_per_job(require_gpu):
         assert np.array_equal(w, ro["words"])
         m["alone"].close()
         m["batch"].close()
+
+
+def test_trim_then_rerun(require_gpu):
+    """smx_job_trim frees the per-call lists; the next calls (alone and
+    batched, with a shape of short runs in between) reserve them again and
+    give the same cube and counts."""
+    from setsm_amd import smx
+    m = _make(0, 40, 41)
+    j = m["alone"]
+
+    def fresh():
+        j.set_searchspace(m["g"]["ugrid"], m["g"]["minmax"])
+        j.initialize_voxel(1)
+
+    states = []
+    for cfg in (_config(64), _config(16), _config(64)):
+        j.set_fast_config(cfg)
+        fresh()
+        j.vertical_line_locus(1, precision=smx.PREC_FAST, keep_rho=True)
+        states.append(_state(j))
+        before = j.device_bytes
+        j.trim()
+        assert j.device_bytes < before
+    _assert_same(states[0], states[2], "rerun after trim")
+    assert states[1]["evaluated"] == states[0]["evaluated"]
+    fresh()
+    smx.vertical_line_locus_batch([j], 1, precision=smx.PREC_FAST, keep_rho=True)
+    _assert_same(states[0], _state(j), "batched after trim")
+    m["alone"].close()
+    m["batch"].close()
