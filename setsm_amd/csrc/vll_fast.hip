@@ -197,9 +197,19 @@ int launch_vll_fast(smx_job *j, int half, const CellArgs &A, const VllScratch &S
 //
 // Descriptor memory of a batch of n jobs: n single-pass or level-pass
 // descriptors (slot-0 tiles), then n level-below-pass descriptors (slot-1
-// tiles of the combined jobs), then 8 work counters per persistent launch.
+// tiles of the combined jobs), then 8 work counters per persistent launch,
+// then the VllParams of each descriptor (same index).
 
-size_t fast_batch_desc_bytes(int n) { return sizeof(BatchJob) * 2 * (size_t)n + 3 * 8 * sizeof(unsigned long long); }
+size_t fast_batch_desc_bytes(int n)
+{
+    return (sizeof(BatchJob) + sizeof(VllParams)) * 2 * (size_t)n + 3 * 8 * sizeof(unsigned long long);
+}
+
+namespace {
+BatchJob *desc_jobs(void *desc) { return (BatchJob *)desc; }
+unsigned long long *desc_next(void *desc, int n) { return (unsigned long long *)(desc_jobs(desc) + 2 * (size_t)n); }
+VllParams *desc_params(void *desc, int n) { return (VllParams *)(desc_next(desc, n) + 24); }
+}  // namespace
 
 namespace {
 struct BatchLayout {
@@ -224,7 +234,7 @@ int batch_layout(smx_job *const *jobs, int n, const CellArgs *As, const VllScrat
         if (r) return r;
         const long nt = (long)F.tiles_x * F.tiles_y;
         L.who[k] = i;
-        L.desc[k++] = BatchJob{jobs[i]->d_params, As[i], F, L.tiles_single, nt};
+        L.desc[k++] = BatchJob{As[i], F, L.tiles_single, nt};
         L.tiles_single += nt;
     }
     int k2 = n;
@@ -235,8 +245,8 @@ int batch_layout(smx_job *const *jobs, int n, const CellArgs *As, const VllScrat
         if (r) return r;
         const long n1 = (long)F1.tiles_x * F1.tiles_y, n2 = (long)F2.tiles_x * F2.tiles_y;
         L.who[k] = L.who[k2] = i;
-        L.desc[k++] = BatchJob{jobs[i]->d_params, As[i], F1, L.tiles_level, n1};
-        L.desc[k2++] = BatchJob{jobs[i]->d_params, As[i], F2, L.tiles_below, n2};
+        L.desc[k++] = BatchJob{As[i], F1, L.tiles_level, n1};
+        L.desc[k2++] = BatchJob{As[i], F2, L.tiles_below, n2};
         L.tiles_level += n1;
         L.tiles_below += n2;
     }
@@ -250,7 +260,14 @@ int fast_batch_descriptors(smx_job *const *jobs, int n, const CellArgs *As, cons
     BatchLayout L;
     int r = batch_layout(jobs, n, As, Ss, L);
     if (r) return r;
-    hipError_t e = hipMemcpyAsync(desc, L.desc.data(), sizeof(BatchJob) * L.desc.size(), hipMemcpyHostToDevice, s);
+    // each descriptor's VllParams, copied from its job's current parameters
+    std::vector<VllParams> params(L.desc.size());
+    for (size_t k = 0; k < L.desc.size(); k++) params[k] = jobs[L.who[k]]->hp;
+    hipError_t e = hipMemcpyAsync(desc_jobs(desc), L.desc.data(), sizeof(BatchJob) * L.desc.size(),
+                                  hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(desc_params(desc, n), params.data(), sizeof(VllParams) * params.size(),
+                           hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return fail(SMX_E_HIP, "batch descriptors: %s", hipGetErrorString(e));
     return 0;
@@ -271,8 +288,9 @@ int launch_vll_fast_batch(smx_job *const *jobs, int n, const CellArgs *As, const
         r = fast_lists(L.desc[k].A, L.desc[k].F, Ss[L.who[k]], s);
         if (r) return r;
     }
-    BatchJob *dj = (BatchJob *)desc;
-    unsigned long long *dnext = (unsigned long long *)(dj + 2 * (size_t)n);
+    BatchJob *dj = desc_jobs(desc);
+    unsigned long long *dnext = desc_next(desc, n);
+    const VllParams *dp = desc_params(desc, n);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, dnext, 24);
     // every job's box-fit test uses its own budget: allocate the largest
     int lds_texels = 0;
@@ -282,7 +300,8 @@ int launch_vll_fast_batch(smx_job *const *jobs, int n, const CellArgs *As, const
     const bool capped = kFastCfgs[fast_config_index(jobs[0])].occ != 0;
     int cus = 0;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, jobs[0]->device);
-    auto launch = [&](int mode, const BatchJob *jl, int nj, long total, unsigned long long *next) -> int {
+    auto launch = [&](int mode, const BatchJob *jl, const VllParams *pl, int nj, long total,
+                      unsigned long long *next) -> int {
         if (nj == 0 || total == 0) return 0;
         int per_cu = 0;
         switch (half) {
@@ -293,7 +312,7 @@ int launch_vll_fast_batch(smx_job *const *jobs, int n, const CellArgs *As, const
         }
         int grid = std::max(1, cus) * std::max(1, per_cu);
         grid = (grid + 7) / 8 * 8;
-        const BatchArgs B{jl, nj, total, next};
+        const BatchArgs B{jl, pl, nj, total, next};
         hipError_t e = hipSuccess;
         switch (half) {
         case 5: e = fast_launch_batch<5>(mode, capped, grid, lds, s, B); break;
@@ -304,9 +323,9 @@ int launch_vll_fast_batch(smx_job *const *jobs, int n, const CellArgs *As, const
         return 0;
     };
     if (ev0) (void)hipEventRecord(ev0, s);
-    r = launch(kSingle, dj, L.nsingle, L.tiles_single, dnext);
-    if (!r) r = launch(kLevelPass, dj + L.nsingle, L.ncomb, L.tiles_level, dnext + 8);
-    if (!r) r = launch(kBelowPass, dj + n, L.ncomb, L.tiles_below, dnext + 16);
+    r = launch(kSingle, dj, dp, L.nsingle, L.tiles_single, dnext);
+    if (!r) r = launch(kLevelPass, dj + L.nsingle, dp + L.nsingle, L.ncomb, L.tiles_level, dnext + 8);
+    if (!r) r = launch(kBelowPass, dj + n, dp + n, L.ncomb, L.tiles_below, dnext + 16);
     if (ev1) (void)hipEventRecord(ev1, s);
     return r;
 }

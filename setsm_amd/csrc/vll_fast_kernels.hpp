@@ -459,14 +459,33 @@ __device__ __forceinline__ long xcd_tile(long b, long n)
     return start + i;
 }
 
+// The same 64-bit value on every lane, held in scalar registers: v is
+// uniform over the wave, but the compiler cannot prove it for a value read
+// from LDS.
+__device__ __forceinline__ long uniform_long(long v)
+{
+    const unsigned long long u = (unsigned long long)v;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return (long)(((unsigned long long)hi << 32) | lo);
+}
+
 // Persistent form over the tiles of many jobs (tiles of different sizes,
 // height ranges and pyramids): each workgroup takes the next tile of its
 // XCD's contiguous range from an atomic counter and, when that range is
 // drained, steals from the other XCDs' ranges. kMode: kSingle for jobs
 // without a combined-level term, or one of the two passes of a combined
 // iteration over every such job's tiles.
+//
+// jobs[j] and params[j] (job j's VllParams) are separate no-alias kernel
+// arguments, indexed by a tile number made uniform: the compiler then reads
+// the job's argument blocks and its RPC coefficients with scalar loads, as
+// the per-tile kernels read theirs from the kernel arguments. (Reached
+// through a pointer inside the batch descriptor, every such field was a
+// per-lane vector load.)
 template <int H, int kMode>
-__device__ __forceinline__ void vll_fast_persistent(const BatchArgs &B)
+__device__ __forceinline__ void vll_fast_persistent(const BatchArgs &B, const BatchJob *__restrict__ jobs,
+                                                    const VllParams *__restrict__ params)
 {
     __shared__ long cur;
     constexpr long kXcd = 8;
@@ -480,12 +499,11 @@ __device__ __forceinline__ void vll_fast_persistent(const BatchArgs &B)
             __syncthreads();
             if (threadIdx.x == 0) cur = lo + (long)atomicAdd(&B.next[x], 1ull);
             __syncthreads();
-            const long t = cur;
+            const long t = uniform_long(cur);
             if (t >= hi) break;
             int j = 0;
-            while (j + 1 < B.njobs && B.jobs[j + 1].tile_begin <= t) j++;
-            const BatchJob &J = B.jobs[j];
-            vll_fast_body<H, true, kMode, false>(J.P, J.A, J.F, t - J.tile_begin);
+            while (j + 1 < B.njobs && jobs[j + 1].tile_begin <= t) j++;
+            vll_fast_body<H, true, kMode, false>(params + j, jobs[j].A, jobs[j].F, t - jobs[j].tile_begin);
         }
     }
 }
@@ -494,15 +512,17 @@ __device__ __forceinline__ void vll_fast_persistent(const BatchArgs &B)
 // kernels: the compiler's own choice, and one capped for three waves per
 // SIMD. The batch's launch shape picks (its occupancy field).
 template <int H, int kMode>
-__global__ void __launch_bounds__(256) k_vll_fast_batch(BatchArgs B)
+__global__ void __launch_bounds__(256) k_vll_fast_batch(BatchArgs B, const BatchJob *__restrict__ jobs,
+                                                        const VllParams *__restrict__ params)
 {
-    vll_fast_persistent<H, kMode>(B);
+    vll_fast_persistent<H, kMode>(B, jobs, params);
 }
 
 template <int H, int kMode>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) k_vll_fast_batch_o3(BatchArgs B)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
+k_vll_fast_batch_o3(BatchArgs B, const BatchJob *__restrict__ jobs, const VllParams *__restrict__ params)
 {
-    vll_fast_persistent<H, kMode>(B);
+    vll_fast_persistent<H, kMode>(B, jobs, params);
 }
 
 
@@ -577,11 +597,11 @@ template <int H>
 hipError_t fast_launch_batch(int mode, bool capped, int grid, size_t lds, hipStream_t s, const BatchArgs &B)
 {
     if (mode == kLevelPass)
-        hipLaunchKernelGGL((batch_kernel<H, kLevelPass>(capped)), dim3(grid), dim3(256), lds, s, B);
+        hipLaunchKernelGGL((batch_kernel<H, kLevelPass>(capped)), dim3(grid), dim3(256), lds, s, B, B.jobs, B.params);
     else if (mode == kBelowPass)
-        hipLaunchKernelGGL((batch_kernel<H, kBelowPass>(capped)), dim3(grid), dim3(256), lds, s, B);
+        hipLaunchKernelGGL((batch_kernel<H, kBelowPass>(capped)), dim3(grid), dim3(256), lds, s, B, B.jobs, B.params);
     else
-        hipLaunchKernelGGL((batch_kernel<H, kSingle>(capped)), dim3(grid), dim3(256), lds, s, B);
+        hipLaunchKernelGGL((batch_kernel<H, kSingle>(capped)), dim3(grid), dim3(256), lds, s, B, B.jobs, B.params);
     return hipGetLastError();
 }
 

@@ -31,9 +31,9 @@ constexpr int kSingle = 0, kLevelPass = 1, kBelowPass = 2;
 constexpr double kPartNoWindow = -2e300;   // a window count was zero
 constexpr double kPartNoCorr = 1e300;      // windows counted, no valid correlation
 
-// One job of a batched launch.
+// One job of a batched launch. Its VllParams is the entry of the same index
+// in BatchArgs::params (a copy made for the launch).
 struct BatchJob {
-    const VllParams *P;
     CellArgs A;
     FastArgs F;
     long tile_begin;   // first tile of this job in the flattened tile space
@@ -42,6 +42,7 @@ struct BatchJob {
 
 struct BatchArgs {
     const BatchJob *jobs;
+    const VllParams *params;    // params[j]: job j's VllParams
     int njobs;
     long total_tiles;
     unsigned long long *next;   // 8 per-XCD work counters
