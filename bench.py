@@ -78,24 +78,9 @@ def load_scene(d, spec):
                  tuple(z["extent"]), list(z["resid"]))
 
 
-def window_counts(half):
-    from setsm_amd.records import window_bits
-    n = [0, 0, 0]
-    for r in range(-half, half + 1):
-        for c in range(-half, half + 1):
-            b = window_bits(half, r, c)
-            for k in range(3):
-                if b & (1 << k):
-                    n[k] += 1
-    return n
-
-
 def flops_per_hypothesis(half):
-    """Algorithmic flops of one hypothesis: bilinear taps (4 channels x 3 FMA
-    per outer-window sample) + moments (5 FMA per sample per window per
-    channel pair). RPC FP64 work (~400 flops) is not counted."""
-    n0, n1, n2 = window_counts(half)
-    return 24 * n0 + 20 * (n0 + n1 + n2)
+    from setsm_amd.records import flops_per_hypothesis as f
+    return f(half)
 
 
 def cpu_model():
@@ -363,14 +348,34 @@ def c0_run(args):
     (the second adds the level-below term above level 0; its search space
     from the seeded host-step stand-in). CPU FP64 restatement (median of 3
     passes) against the GPU FP64 and FAST paths, InitializeVoxel +
-    VerticalLineLocus + AWNCC per step, with word parity per step."""
+    VerticalLineLocus + AWNCC per step, with word parity per step.
+
+    The pair goes through the reference's formats first: each image is
+    written as a GeoTIFF (WriteGeotiff, uint16) with its RPCs as a
+    DigitalGlobe .XML, and the run reads them back (Readtiff_T, OpenXMLFile).
+    The level-0 peak heights of the last step (AWNCC result2, before the
+    host's blunder filter and TIN) are written as a float GeoTIFF DEM per
+    path; the line reports their RMS against the FP64 run and the terrain."""
     import math
     from oracle import oracle_py as O
     from setsm_amd import smx
     from setsm_amd.records import LevelInputs, height_step
     from setsm_amd.serve import next_searchspace, setgrids_spacing
     from setsm_amd.synth import GridSpec, SceneSpec, make_grid, make_scene
+    import tempfile
+    from setsm_amd import io as sio
     sc = make_scene(SceneSpec(image_px=2048, relief=300.0, seed=21, view_deg=(12.0, -12.0)), device="cuda")
+    fdir = tempfile.mkdtemp(prefix="smx_c0_")
+    files = dict(dir=fdir)
+    for t in range(2):
+        tif, xml = os.path.join(fdir, f"img{t}.tif"), os.path.join(fdir, f"img{t}.xml")
+        sio.write_geotiff(tif, sc.images[t], sc.spec.gsd, sc.extent[0], sc.extent[3], projection=0, zone=14)
+        sio.write_dg_xml(xml, sc.rpcs[t], gsd=(sc.spec.gsd,) * 3)
+        img, _, _ = sio.read_tiff(tif)
+        rpc, _, _ = sio.read_dg_xml(xml)
+        assert img.tobytes() == sc.images[t].tobytes() and rpc.tobytes() == np.asarray(sc.rpcs[t]).tobytes()
+        sc.images[t], sc.rpcs[t] = img, rpc
+        files[f"img{t}_tif_bytes"] = os.path.getsize(tif)
     pyr = [O.build_pyramid(im, 5) for im in sc.images]   # the same views on both sides
     side = sc.spec.image_px * sc.spec.gsd
     steps = []
@@ -421,12 +426,31 @@ def c0_run(args):
                 t += time.perf_counter() - t0
                 ev += job.last_evaluated
                 words.append(job.get_voxels()[0])
+        nr = job.get_nccresult()
         job.close()
-        return t, words, ev
+        # the level-0 peak heights of the last step as a DEM (no peak: -9999)
+        n0 = steps[-1][1]["shape"][0]
+        dem = np.where(nr["result4"] > 0, nr["result2"], -9999.0).astype(np.float32).reshape(n0, n0)
+        return t, words, ev, dem
 
     gpu_pass(smx.PREC_FAST)   # warm-up
-    t64, w64, ev64 = gpu_pass(smx.PREC_FP64)
-    tf, wf, evf = gpu_pass(smx.PREC_FAST)
+    t64, w64, ev64, dem64 = gpu_pass(smx.PREC_FP64)
+    tf, wf, evf, demf = gpu_pass(smx.PREC_FAST)
+    g0 = steps[-1][1]
+    res0 = g0["grid_res"]
+    b0 = g0["boundary"]
+    for name, dem in (("dem_fp64", dem64), ("dem_fast", demf)):
+        # north-up raster: row 0 is the northernmost cell row
+        path = os.path.join(fdir, name + ".tif")
+        sio.write_geotiff(path, dem[::-1].copy(), res0, b0[0], b0[1] + dem.shape[0] * res0, projection=0, zone=14)
+        back, _, _ = sio.read_tiff(path)
+        assert back[::-1].tobytes() == dem.tobytes()
+    both = (dem64 > -9999) & (demf > -9999)
+    truth = g0["truth"].reshape(dem64.shape)
+    dem_rms = dict(cells=int(both.sum()), fast_vs_fp64_m=float(np.sqrt(np.mean((demf[both] - dem64[both]) ** 2))),
+                   fp64_vs_terrain_m=float(np.sqrt(np.mean((dem64[both] - truth[both]) ** 2))),
+                   fast_vs_terrain_m=float(np.sqrt(np.mean((demf[both] - truth[both]) ** 2))),
+                   note="level-0 AWNCC peak heights of iteration 2 (before blunder filtering and TIN)")
     cpu = [cpu_pass() for _ in range(3)]
     tc = float(np.median([c[0] for c in cpu]))
     wc = cpu[0][1]
@@ -439,7 +463,10 @@ def c0_run(args):
                hypotheses_evaluated=evf, cpu_s=tc, cpu_cores=O.threads(), cpu_runs_s=[round(c[0], 3) for c in cpu],
                cpu_hyp_per_s=ev64 / tc, gpu_fp64_s=t64, gpu_fast_s=tf, speedup_fp64_vs_cpu=tc / t64,
                speedup_fast_vs_cpu=tc / tf, fp64_words_equal_cpu=bool(parity), fast_incc_max_quanta=dq,
-               evaluated_equal=bool(ev64 == evf))
+               evaluated_equal=bool(ev64 == evf), dem=dem_rms,
+               files=dict(files, note="pair read back from GeoTIFF + DigitalGlobe XML; DEMs written as GeoTIFF"))
+    import shutil
+    shutil.rmtree(fdir, ignore_errors=True)
     log(f"configs[0]: {out}")
     return out
 
@@ -594,18 +621,47 @@ def served_run(args, draws, scenes, cfg):
     tp = time.time()
     Server(scenes, levels=4, iterations=2).prepare(stream)   # request payloads, built before the clock starts
     log(f"{n} request payloads prepared in {time.time() - tp:.1f}s; arrival window {t[-1]:.2f} s")
-    _, fast = Server(scenes, levels=4, iterations=2, fast_config=cfg).run(stream)
+    done_f, fast = Server(scenes, levels=4, iterations=2, fast_config=cfg).run(stream)
     smx.release_cached()
     log(f"served FAST: {fast}")
-    _, plain = Server(scenes, levels=4, iterations=2, precision=smx.PREC_FP64, batching=False).run(stream)
+    done_p, plain = Server(scenes, levels=4, iterations=2, precision=smx.PREC_FP64, batching=False).run(stream)
     smx.release_cached()
     log(f"served plain FP64: {plain}")
+    # the level-0 result of every request, FAST against the plain FP64 server:
+    # peak heights (result2) and peak NCC (result0, x1000 rounded)
+    by = {d["rid"]: d["ncc"] for d in done_p}
+    dh, dr, cells, moved = [], [], 0, 0
+    for d in done_f:
+        if d["rid"] not in by:
+            continue
+        a, b = d["ncc"], by[d["rid"]]
+        both = (a["result4"] > 0) & (b["result4"] > 0)
+        cells += int(both.sum())
+        dh.append((a["result2"][both] - b["result2"][both]).astype(np.float64))
+        dr.append(np.abs(a["result0"][both].astype(np.int64) - b["result0"][both].astype(np.int64)))
+        moved += int(np.sum(a["result2"][both] != b["result2"][both]))
+    dh = np.concatenate(dh) if dh else np.zeros(1)
+    dr = np.concatenate(dr) if dr else np.zeros(1, np.int64)
+    err = dict(cells=cells, peak_height_rms_m=float(np.sqrt(np.mean(dh * dh))), peak_moved_cells=moved,
+               peak_rho_max_quanta=int(dr.max()),
+               note="level-0 AWNCC peaks of every request, FAST server vs plain FP64 server (result0 is the "
+                    "peak NCC x1000 rounded: a difference of 1 quantum is 1e-3)")
+    # VALU roofline of the served run: algorithmic flops of every evaluated
+    # hypothesis (per level's window size) over the time in VerticalLineLocus calls
+    for r in (fast, plain):
+        r["achieved_tflops_in_vll_calls"] = r["flops"] / r["gpu_call_s"] / 1e12
+    fast["roofline"] = dict(bound="valu", achieved=fast["achieved_tflops_in_vll_calls"], peak=PEAK_FP32_TFLOPS,
+                            unit="TFLOP/s", frac=fast["achieved_tflops_in_vll_calls"] / PEAK_FP32_TFLOPS,
+                            note="wall time of the batched VerticalLineLocus calls (launch + hand-over + "
+                                 "max_WNCC included)")
+    plain["roofline_frac_fp64_peak"] = plain["achieved_tflops_in_vll_calls"] / PEAK_FP64_TFLOPS
     return dict(config=dict(workload="configs[4]: Poisson tile requests, continuous batching", rate_per_s=args.serve_rate,
                             requests=n, arrival_window_s=round(float(t[-1]), 3), levels=4, iterations_per_level=2,
                             budget_gib=200, sizes_cells=[int(min(draws[k].side for k in pick)),
                                                          int(max(draws[k].side for k in pick))],
                             note="arrival window shortened from 300 s so the run takes minutes"),
-                fast=fast, plain_fp64=plain, speedup_vs_plain=fast["hyp_per_s"] / plain["hyp_per_s"])
+                fast=fast, plain_fp64=plain, speedup_vs_plain=fast["hyp_per_s"] / plain["hyp_per_s"],
+                fast_vs_fp64=err)
 
 
 def main():

@@ -144,3 +144,23 @@ def window_bits(half: int, row: int, col: int) -> int:
     if r2 <= (half - s2 + 1) ** 2 and -half + s2 <= row <= half - s2 and -half + s2 <= col <= half - s2:
         bits |= 4
     return bits
+
+
+def window_counts(half: int):
+    """Samples of the outer, middle and inner windows at half-size `half`."""
+    n = [0, 0, 0]
+    for r in range(-half, half + 1):
+        for c in range(-half, half + 1):
+            b = window_bits(half, r, c)
+            for k in range(3):
+                if b & (1 << k):
+                    n[k] += 1
+    return n
+
+
+def flops_per_hypothesis(half: int) -> int:
+    """Algorithmic FP32 flops of one hypothesis: bilinear taps (4 channels x
+    3 FMA per outer-window sample) + moments (5 FMA per sample per window per
+    channel pair); the FP64 RPC work (~400 flops) is not counted."""
+    n0, n1, n2 = window_counts(half)
+    return 24 * n0 + 20 * (n0 + n1 + n2)

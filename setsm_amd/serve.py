@@ -49,7 +49,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import smx
-from .records import LevelInputs, height_step
+from .records import LevelInputs, flops_per_hypothesis, height_step
 from .synth import GridSpec, Scene, SceneSpec, make_grid, make_scene
 
 
@@ -114,6 +114,7 @@ class _Active:
     stage: int = 0
     admitted: float = 0.0
     evaluated: int = 0
+    flops: float = 0.0    # algorithmic flops of the evaluated hypotheses
     reserved: int = 0
     results: list = field(default_factory=list)
 
@@ -238,6 +239,7 @@ class Server:
         _, it, _, _, _ = a.req.plan[a.stage]
         a.job.awncc(it)
         a.evaluated += a.job.last_evaluated
+        a.flops += a.job.last_evaluated * flops_per_hypothesis(a.job.half)
         a.stage += 1
 
     def run(self, stream: list, log=None, workers=8):
@@ -339,7 +341,7 @@ class Server:
                 a.results = a.job.get_nccresult()
                 done.append(dict(rid=a.req.rid, arrival=a.req.arrival, admitted=a.admitted, done=t,
                                  latency=t - a.req.arrival, queue_s=a.admitted - a.req.arrival,
-                                 service_s=t - a.admitted, evaluated=a.evaluated, cells=a.req.cells,
+                                 service_s=t - a.admitted, evaluated=a.evaluated, flops=a.flops, cells=a.req.cells,
                                  nheights=a.req.nheights, ncc=a.results))
                 a.job.close()
                 self.in_use -= a.reserved
@@ -352,8 +354,9 @@ class Server:
         qw = np.array([d["queue_s"] for d in done]) if done else np.zeros(1)
         sv = np.array([d["service_s"] for d in done]) if done else np.zeros(1)
         hyps = sum(d["evaluated"] for d in done)
+        flops = sum(d["flops"] for d in done)
         return done, dict(requests=len(done), rejected=len(rejected), rejections=rejected[:8], wall_s=wall,
-                          hyps=hyps, hyp_per_s=hyps / wall,
+                          hyps=hyps, hyp_per_s=hyps / wall, flops=flops,
                           p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
                           queue_p50_ms=float(np.percentile(qw, 50) * 1e3),
                           queue_p99_ms=float(np.percentile(qw, 99) * 1e3),
