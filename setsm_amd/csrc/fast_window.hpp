@@ -1,7 +1,8 @@
 // fast_window.hpp — the FAST path's window arithmetic for one hypothesis:
 // bilinear taps about per-hypothesis pivots, FP32 running sums per window
 // row folded into the nested inner / middle / outer window totals, and the
-// FP64 Pearson finish. Written against a sampler interface (at(), quad()),
+// FP64 Pearson finish. Written against a sampler interface (at(), quad(),
+// hlerp()),
 // so the same code runs in the kernels (LDS and HBM samplers) and on the
 // host, where the CPU tests check it against the oracle.
 #pragma once
@@ -217,8 +218,11 @@ SMX_FN void pin_totals(const Tot<kWide> &t)
 
 // One hypothesis' window loop. L/R are samplers positioned so that at(0,0)
 // is the texel (center - (H+2)) in both axes; R.quad(iy, ix) returns the
-// taps (iy, ix) .. (iy + 1, ix + 1) about that origin. (fxl, fyl) is the
-// fractional part of the left centre, (ftx, fty) of the right one.
+// taps (iy, ix) .. (iy + 1, ix + 1) about that origin and R.hlerp(iy, ix,
+// wx) the interpolations at wx along its two rows (the LDS sampler keeps
+// each texel with its difference to the next one, so that is one FMA per
+// row and channel). (fxl, fyl) is the fractional part of the left centre,
+// (ftx, fty) of the right one.
 // Check = per-sample image-edge and nodata tests (SubFunctions.cpp:3252-3257);
 // otherwise every sample is known to be valid. Returns false if a window
 // count is zero; rho = kRedoRho when a window's spread is not clear of FP32
@@ -267,17 +271,20 @@ SMX_FN bool fast_window(const SL &L, const SR &R, float fxl, float fyl, float ft
         const float li = fmaf(lw11, d.x, fmaf(lw10, c.x, fmaf(lw01, b.x, fmaf(lw00, a.x, -pvl.x))));
         const float lm = fmaf(lw11, d.y, fmaf(lw10, c.y, fmaf(lw01, b.y, fmaf(lw00, a.y, -pvl.y))));
         // right: rotated offsets (setsm_code.cpp:6437-6439),
-        // x = ftx + col*cos - row*sin, y = fty + col*sin + row*cos
+        // x = ftx + col*cos - row*sin, y = fty + col*sin + row*cos; the
+        // sampler interpolates along both tap rows at wx, then the rows are
+        // interpolated at wy about the pivot
         const float x = col == 0 ? xr : fmaf((float)col, cos0, xr);
         const float y = col == 0 ? yr : fmaf((float)col, sin0, yr);
         const float wx = fract_pos(x), wy = fract_pos(y);
-        float2 r00, r01, r10, r11;
-        R.quad((int)y, (int)x, r00, r01, r10, r11);
-        const float w11 = wx * wy, w10 = wy - w11, w01 = wx - w11, w00 = (1.f - wx) - w10;
-        const float ri = fmaf(w11, r11.x, fmaf(w10, r10.x, fmaf(w01, r01.x, fmaf(w00, r00.x, -pvr.x))));
-        const float rm = fmaf(w11, r11.y, fmaf(w10, r10.y, fmaf(w01, r01.y, fmaf(w00, r00.y, -pvr.y))));
+        float2 top, bot;
+        R.hlerp((int)y, (int)x, wx, top, bot);
+        const float ri = fmaf(wy, bot.x - top.x, top.x - pvr.x);
+        const float rm = fmaf(wy, bot.y - top.y, top.y - pvr.y);
         bool ok = true;
         if (kCheck) {
+            float2 r00, r01, r10, r11;
+            R.quad((int)y, (int)x, r00, r01, r10, r11);
             // pos >= 0 && pos + 1 < size on both images, then L > 0 && R > 0
             // (SubFunctions.cpp:3252-3257). The interpolation of
             // non-negative DNs is positive exactly when a tap with a

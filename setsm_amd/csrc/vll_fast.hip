@@ -57,24 +57,25 @@ __global__ void k_zero_u64(unsigned long long *p, int n)
 }
 
 // Launch shapes the autotuner chooses from: {tx, ty, lds_texels, occ, stage,
-// wide, run_len} (cells per tile, LDS texels for both boxes, register cap for
-// 3 / 4 waves per SIMD or 0, LDS staging, FP64 window totals, heights per run).
-// Boxes stay at or below 7680 texels (60 KB): with 80 KB only one
-// workgroup fits a CU and the kernel runs at half speed.
+// wide, run_len} (cells per tile, LDS for both boxes in float2 units — the
+// left box takes one per texel, the right box two —, register cap for 3 / 4
+// waves per SIMD or 0, LDS staging, FP64 window totals, heights per run).
+// Budgets stay at or below 7680 units (60 KB, two workgroups per CU), and at
+// or below 6656 (52 KB) where the register cap asks for three.
 static const FastCfg kFastCfgs[] = {
-    {1, 4, 3072, 0, 1, 1, 64}, {2, 4, 3584, 0, 1, 1, 64}, {2, 2, 3072, 0, 1, 1, 64}, {4, 4, 4608, 0, 1, 1, 64},
-    {1, 8, 4096, 0, 1, 1, 64}, {2, 8, 4608, 0, 1, 1, 64}, {4, 8, 6144, 0, 1, 1, 64}, {1, 16, 6144, 0, 1, 1, 64},
-    {1, 4, 3072, 3, 1, 1, 64}, {2, 4, 3584, 3, 1, 1, 64}, {4, 4, 4608, 3, 1, 1, 64}, {1, 4, 3072, 4, 1, 1, 64},
-    {2, 4, 3584, 0, 0, 1, 64}, {1, 4, 3072, 0, 1, 0, 64}, {2, 4, 3584, 3, 1, 0, 64}, {2, 8, 4608, 3, 1, 1, 64},
-    {1, 8, 4096, 3, 1, 1, 64}, {4, 2, 3584, 3, 1, 1, 64}, {2, 2, 3072, 3, 1, 1, 64}, {2, 4, 3584, 4, 1, 1, 64},
-    {8, 4, 6144, 0, 1, 1, 64}, {4, 16, 7680, 0, 1, 1, 64}, {8, 8, 7680, 0, 1, 1, 64}, {4, 8, 6144, 3, 1, 1, 64},
+    {1, 4, 4608, 0, 1, 1, 64}, {2, 4, 5120, 0, 1, 1, 64}, {2, 2, 4096, 0, 1, 1, 64}, {4, 4, 6144, 0, 1, 1, 64},
+    {1, 8, 6144, 0, 1, 1, 64}, {2, 8, 7680, 0, 1, 1, 64}, {4, 8, 7680, 0, 1, 1, 64}, {1, 16, 7680, 0, 1, 1, 64},
+    {1, 4, 4608, 3, 1, 1, 64}, {2, 4, 5120, 3, 1, 1, 64}, {4, 4, 6656, 3, 1, 1, 64}, {1, 4, 4608, 4, 1, 1, 64},
+    {2, 4, 5120, 0, 0, 1, 64}, {1, 4, 4608, 0, 1, 0, 64}, {2, 4, 5120, 3, 1, 0, 64}, {2, 8, 6656, 3, 1, 1, 64},
+    {1, 8, 6144, 3, 1, 1, 64}, {4, 2, 5120, 3, 1, 1, 64}, {2, 2, 4096, 3, 1, 1, 64}, {2, 4, 5120, 4, 1, 1, 64},
+    {8, 4, 7680, 0, 1, 1, 64}, {4, 8, 6656, 3, 1, 1, 64}, {8, 2, 6656, 3, 1, 1, 64}, {4, 4, 5120, 4, 1, 1, 64},
     // runs of 32 and 16 heights
-    {1, 16, 6144, 0, 1, 1, 32}, {2, 8, 4608, 0, 1, 1, 32}, {4, 8, 6144, 0, 1, 1, 32}, {4, 16, 7680, 0, 1, 1, 32},
-    {1, 16, 6144, 0, 1, 1, 16}, {2, 8, 4608, 0, 1, 1, 16}, {4, 8, 6144, 0, 1, 1, 16}, {4, 16, 7680, 0, 1, 1, 16},
-    {1, 8, 4096, 0, 1, 1, 32}, {2, 16, 7680, 0, 1, 1, 32}, {1, 32, 7680, 0, 1, 1, 32}, {1, 16, 6144, 3, 1, 1, 32},
+    {1, 16, 7680, 0, 1, 1, 32}, {2, 8, 7680, 0, 1, 1, 32}, {4, 4, 6656, 3, 1, 1, 32}, {2, 8, 6656, 3, 1, 1, 32},
+    {1, 16, 7680, 0, 1, 1, 16}, {2, 8, 7680, 0, 1, 1, 16}, {4, 4, 6656, 3, 1, 1, 16}, {1, 8, 6144, 3, 1, 1, 16},
+    {1, 8, 6144, 0, 1, 1, 32}, {2, 4, 5120, 3, 1, 1, 32}, {1, 8, 6144, 3, 1, 1, 32}, {4, 8, 7680, 0, 1, 1, 32},
 };
 constexpr int kNumFastCfgs = sizeof(kFastCfgs) / sizeof(kFastCfgs[0]);
-constexpr int kDefaultFastCfg = 6;   // 4 x 8 cells, compiler's register budget, FP64 totals, runs of 64
+constexpr int kDefaultFastCfg = 3;   // 4 x 4 cells, compiler's register budget, FP64 totals, runs of 64
 
 int fast_config_index(const smx_job *j)
 {

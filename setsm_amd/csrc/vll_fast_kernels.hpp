@@ -7,10 +7,11 @@
 //
 //  * a workgroup owns a tile of TX x TY cells; it projects the tile's height
 //    bands through both RPCs, takes the bounding box of every texel the
-//    tile's windows can touch, and stages it in LDS once as float2
-//    (intensity - pivot, magnitude - pivot). The pivot is one texel value of
-//    the box, so moments are accumulated on small, exactly representable
-//    numbers (NCC is shift invariant);
+//    tile's windows can touch, and stages it in LDS once as (intensity -
+//    pivot, magnitude - pivot): float2 for the left image, float4 with each
+//    texel's difference to its right neighbour for the right image. The
+//    pivot is one texel value of the box, so the staged values are small,
+//    exactly representable numbers (NCC is shift invariant);
 //  * one lane evaluates one height hypothesis; taps are LDS reads at
 //    compile-time offsets (left window) or at rotated offsets (right window);
 //  * bilinear taps and the 5 moments per channel pair are FP32, about
@@ -34,39 +35,70 @@ namespace smx {
 
 // ------------------------------------------------------------- samplers
 
-extern __shared__ float2 smx_lds[];
-// A float pair in LDS addressed by its 32-bit LDS offset (address space 3).
+// 16-byte aligned: the right image's texels are read as float4
+extern __shared__ __attribute__((aligned(16))) float2 smx_lds[];
+// A float pair / quad in LDS addressed by its 32-bit LDS offset (address space 3).
 typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(3))) f32x2 lds_f2;
-__device__ __forceinline__ float2 lds_at(uint32_t byte_offset)
-{
-    const f32x2 v = *(lds_f2 *)(uintptr_t)byte_offset;
-    return make_float2(v.x, v.y);
-}
+typedef const __attribute__((address_space(3))) f32x4 lds_f4;
+__device__ __forceinline__ f32x4 lds_at4(uint32_t byte_offset) { return *(lds_f4 *)(uintptr_t)byte_offset; }
 
-// Texels of one image staged in LDS: float2 per texel, rows of the box
-// packed back to back (pitch = box width). Indexing the __shared__ array
-// itself keeps every tap a ds_read.
+// Left image texels staged in LDS: float2 (intensity, magnitude) per texel,
+// rows of the box packed back to back (pitch = box width, odd). Its window
+// samples sit at integer offsets: compile-time LDS offsets from one base.
 struct LdsImg {
-    int base;   // index of texel (0, 0) of the window origin
+    int base;   // float2 index of texel (0, 0) of the window origin
     int pitch;
     __device__ __forceinline__ float2 at(int row, int col) const { return smx_lds[base + row * pitch + col]; }
-    // The 2x2 taps (iy, ix) .. (iy + 1, ix + 1) about the window origin
-    // (0 <= iy, ix, small): one 24-bit multiply-add and the scaled base add
-    // form the address, the second row is one pitch further.
+};
+
+// Right image texels staged in LDS as float4 (intensity, magnitude, and
+// their differences to the next texel of the row): a bilinear tap pair
+// along a row is one FMA per channel. pitch and base in float4 units.
+struct LdsImgD {
+    int base;   // float4 index of texel (0, 0) of the window origin
+    int pitch;
+    __device__ __forceinline__ f32x4 texel(int row, int col) const
+    {
+        return *(lds_f4 *)((const float4 *)smx_lds + base + row * pitch + col);
+    }
+    __device__ __forceinline__ float2 at(int row, int col) const
+    {
+        const f32x4 v = texel(row, col);
+        return make_float2(v.x, v.y);
+    }
+    // LDS byte address of tap (iy, ix) about the window origin (0 <= iy, ix,
+    // small): one 24-bit multiply-add and the scaled base add; the next row
+    // is one pitch further
+    __device__ __forceinline__ uint32_t addr(int iy, int ix, int &pitch16) const
+    {
+        pitch16 = pitch * 16;
+        // made opaque to the compiler, which would otherwise split the
+        // multiply-add below into a multiply and a three-way add
+        int col = (ix << 4) + (int)(uint32_t)(uintptr_t)(lds_f4 *)((const float4 *)smx_lds + base);
+        asm("" : "+v"(col));
+        return (uint32_t)imad24(iy, pitch16, col);
+    }
+    __device__ __forceinline__ void hlerp(int iy, int ix, float wx, float2 &top, float2 &bot) const
+    {
+        int p16;
+        const uint32_t o = addr(iy, ix, p16);
+        const f32x4 a = lds_at4(o), b = lds_at4(o + p16);
+        top = make_float2(fmaf(wx, a.z, a.x), fmaf(wx, a.w, a.y));
+        bot = make_float2(fmaf(wx, b.z, b.x), fmaf(wx, b.w, b.y));
+    }
+    // The 2x2 taps (iy, ix) .. (iy + 1, ix + 1): a texel plus its
+    // difference is the next texel exactly (both are small integers)
     __device__ __forceinline__ void quad(int iy, int ix, float2 &t00, float2 &t01, float2 &t10, float2 &t11) const
     {
-        const int pitch8 = pitch * (int)sizeof(float2);
-        // LDS byte address of (0, ix), made opaque to the compiler, which
-        // would otherwise split the multiply-add below into a multiply and a
-        // three-way add
-        int col = (ix << 3) + (int)(uint32_t)(uintptr_t)(lds_f2 *)(smx_lds + base);
-        asm("" : "+v"(col));
-        const uint32_t o = (uint32_t)imad24(iy, pitch8, col);
-        t00 = lds_at(o);
-        t01 = lds_at(o + sizeof(float2));
-        t10 = lds_at(o + pitch8);
-        t11 = lds_at(o + pitch8 + sizeof(float2));
+        int p16;
+        const uint32_t o = addr(iy, ix, p16);
+        const f32x4 a = lds_at4(o), b = lds_at4(o + p16);
+        t00 = make_float2(a.x, a.y);
+        t01 = make_float2(a.x + a.z, a.y + a.w);
+        t10 = make_float2(b.x, b.y);
+        t11 = make_float2(b.x + b.z, b.y + b.w);
     }
 };
 
@@ -89,12 +121,20 @@ struct GlobalImg {
         t10 = at(iy + 1, ix);
         t11 = at(iy + 1, ix + 1);
     }
+    __device__ __forceinline__ void hlerp(int iy, int ix, float wx, float2 &top, float2 &bot) const
+    {
+        float2 t00, t01, t10, t11;
+        quad(iy, ix, t00, t01, t10, t11);
+        top = make_float2(fmaf(wx, t01.x - t00.x, t00.x), fmaf(wx, t01.y - t00.y, t00.y));
+        bot = make_float2(fmaf(wx, t11.x - t10.x, t10.x), fmaf(wx, t11.y - t10.y, t10.y));
+    }
 };
 
 struct TileBox {
     int x0[2], y0[2], w[2], h[2];
     int p[2];   // LDS row pitch: the width rounded up to odd, so that the rows of a
-                // float2 box fall on different LDS banks
+                // box fall on different LDS banks (float2 texels left, float4 right)
+    int rbase;  // float4 index of the right box (after the left one, 16-byte aligned)
     float piv_i[2], piv_m[2];
     int fits, nodata;
 };
@@ -179,25 +219,37 @@ __device__ __forceinline__ void box_finalize(const DevView *views, const FastArg
         tb.piv_i[ti] = (float)(t & 0xffffu);
         tb.piv_m[ti] = (float)(t >> 16);
     }
-    if (tb.p[0] * tb.h[0] + tb.p[1] * tb.h[1] > F.lds_texels) fits = 0;
+    // budget in float2 units: the right box holds two per texel
+    tb.rbase = (tb.p[0] * tb.h[0] + 1) / 2;
+    if (2 * tb.rbase + 2 * tb.p[1] * tb.h[1] > F.lds_texels) fits = 0;
     tb.fits = fits;
     tb.nodata = 0;
 }
 
-// All threads: copy the box into LDS as pivot-shifted float2 texels.
+// All threads: copy the box into LDS as pivot-shifted texels: float2 for
+// the left image, float4 with the difference to the next texel of the row
+// for the right one (0 in the box's last column, which no tap pair starts
+// from: the in-box test keeps ix + 1 inside).
 __device__ __forceinline__ void box_stage(const DevView *views, TileBox &tb)
 {
     int nod = 0;
     for (int ti = 0; ti < 2; ti++) {
         const DevView &v = views[ti];
-        const int dst = ti ? tb.p[0] * tb.h[0] : 0;
         const int w = tb.w[ti], h = tb.h[ti], pitch = tb.p[ti];
         const float pi = tb.piv_i[ti], pm = tb.piv_m[ti];
         for (int k = threadIdx.x; k < w * h; k += blockDim.x) {
             const int rr = k / w, cc = k % w;
-            const uint32_t t = gptr(v.texel)[(long)(tb.y0[ti] + rr) * v.width + tb.x0[ti] + cc];
+            const long src = (long)(tb.y0[ti] + rr) * v.width + tb.x0[ti] + cc;
+            const uint32_t t = gptr(v.texel)[src];
             nod |= (t & 0xffffu) == 0;
-            smx_lds[dst + rr * pitch + cc] = make_float2((float)(t & 0xffffu) - pi, (float)(t >> 16) - pm);
+            const float ti_ = (float)(t & 0xffffu) - pi, tm = (float)(t >> 16) - pm;
+            if (ti == 0) {
+                smx_lds[rr * pitch + cc] = make_float2(ti_, tm);
+            } else {
+                const uint32_t n = cc + 1 < w ? gptr(v.texel)[src + 1] : t;
+                const float di = (float)(n & 0xffffu) - (float)(t & 0xffffu), dm = (float)(n >> 16) - (float)(t >> 16);
+                ((float4 *)smx_lds)[tb.rbase + rr * pitch + cc] = make_float4(ti_, tm, di, dm);
+            }
         }
     }
     if (nod) tb.nodata = 1;   // benign race: every writer stores 1
@@ -289,7 +341,7 @@ __device__ __forceinline__ unsigned vll_fast_run(const VllParams &P, const CellA
                 const float fxl = (float)(rpx - flx), fyl = (float)(rpy - fly);
                 const float fxt = (float)(tpx - ftx), fyt = (float)(tpy - fty);
                 const LdsImg Lw{(ily - O - tb.y0[0]) * tb.p[0] + (ilx - O - tb.x0[0]), tb.p[0]};
-                const LdsImg Rw{tb.p[0] * tb.h[0] + (ity - O - tb.y0[1]) * tb.p[1] + (itx - O - tb.x0[1]), tb.p[1]};
+                const LdsImgD Rw{tb.rbase + (ity - O - tb.y0[1]) * tb.p[1] + (itx - O - tb.x0[1]), tb.p[1]};
                 if (fast_ok)
                     all3 = fast_window<H, false, kWide>(Lw, Rw, fxl, fyl, fxt, fyt, cos0, sin0, tb.piv_i[0],
                                                          tb.piv_i[1], 0, 0, 0, 0, 0, 0, 0, 0, rho);

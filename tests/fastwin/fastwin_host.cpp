@@ -31,6 +31,14 @@ struct HostImg {
         t10 = at(iy + 1, ix);
         t11 = at(iy + 1, ix + 1);
     }
+    // as the LDS sampler: a texel plus its difference to the next one
+    void hlerp(int iy, int ix, float wx, float2 &top, float2 &bot) const
+    {
+        float2 t00, t01, t10, t11;
+        quad(iy, ix, t00, t01, t10, t11);
+        top = make_float2(fmaf(wx, t01.x - t00.x, t00.x), fmaf(wx, t01.y - t00.y, t00.y));
+        bot = make_float2(fmaf(wx, t11.x - t10.x, t10.x), fmaf(wx, t11.y - t10.y, t10.y));
+    }
 };
 
 // stage > 0: pivots from the texel `stage` pixels right of and below the

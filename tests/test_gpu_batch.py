@@ -59,7 +59,7 @@ def _oracle_check(m, ug_seq, minmax, rho, acc):
     assert acc == ro["acc"]
 
 
-def _config(run_len, tx=1, ty=16, lds=6144):
+def _config(run_len, tx=1, ty=16, lds=7680):
     """Index of the launch shape (tx, ty, lds, compiler's registers, staged,
     FP64 totals, run_len)."""
     from setsm_amd.smx import fast_configs
