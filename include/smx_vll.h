@@ -44,9 +44,11 @@ extern "C" {
 /* Arithmetic of the VerticalLineLocus kernel. */
 #define SMX_PREC_FP64     0   /* plain FP64, same operation order as SETSM:
                                  INCC cube bit-identical to the CPU path      */
-#define SMX_PREC_FAST     1   /* 11-bit texels staged in LDS as FP32 float2
-                                 (intensity, magnitude) about a pivot, FP64
-                                 RPC, FP32 bilinear taps and moments, each
+#define SMX_PREC_FAST     1   /* 11-bit texels staged in LDS as FP32
+                                 (intensity, magnitude) about a pivot (the
+                                 right image with each texel's difference to
+                                 its row neighbour), FP64 RPC, FP32 bilinear
+                                 taps and moments, each
                                  window row's FP32 partials summed into FP64
                                  window totals, FP64 Pearson finish;
                                  |rho - rho_fp64| <= 1e-5                     */
@@ -154,7 +156,8 @@ int  smx_vertical_line_locus_batch(smx_job **jobs, int n, int iteration, int pre
  * tight serving loops, harnesses that forbid hipMalloc). The job must have
  * been set up by the allocating calls (set_level, set_searchspace,
  * initialize_voxel). Levels needing more scratch than the array holds (about
- * 2.7 M height runs, ~170 M hypotheses) fail with SMX_E_NOMEM; RA and
+ * 4 M height runs in the precision form, ~260 M hypotheses at runs of 64)
+ * fail with SMX_E_NOMEM; RA and
  * matching-rate passes with SMX_E_UNSUPPORTED. The mean NCC is not kept.
  * Every call on one GPU shares that GPU's array: calls that overlap on one
  * GPU race. */

@@ -402,8 +402,8 @@ static int vll_reserve(smx_job *j, const VllPlan &p)
             if ((r = jalloc(j, &S.tile_count, (size_t)t1 + 1)) || (r = jalloc(j, &S.tile_off, (size_t)t1 + 1))) return r;
             S.tiles_cap = t1;
         }
+        // the hand-over queue of the FP64 pass lives in S.runs (vll_args)
         if ((r = grow(j, S.tile_runs, j->cap_tile_runs, S.runs_cap))) return r;
-        if ((r = grow(j, S.redo, j->cap_redo, S.runs_cap))) return r;
         if (j->hp.combined_incc) {
             const long t2 = fast_tiles(j, 1);
             if (t2 > S.tiles2_cap || !S.tile2_count) {
@@ -443,14 +443,13 @@ static int carve_scratch(smx_job *j, const VllPlan &p, VllScratch &S)
     const long runs = (long)j->runs_bound[run_len_slot(p.run_len)];
     const long nscan = scan_partials(j->ncells + 1);
     const size_t o_cnt = take(kCounters * 8), o_scan = take((size_t)nscan * 8), o_runs = take((size_t)runs * 8);
-    size_t o_tc = 0, o_to = 0, o_tr = 0, o_redo = 0, o_t2c = 0, o_t2o = 0, o_t2r = 0, o_part = 0;
+    size_t o_tc = 0, o_to = 0, o_tr = 0, o_t2c = 0, o_t2o = 0, o_t2r = 0, o_part = 0;
     long t1 = 0, t2 = 0;
-    if (p.fast) {
+    if (p.fast) {   // the FP64 hand-over queue shares o_runs (vll_args)
         t1 = fast_tiles(j, 0);
         o_tc = take(((size_t)t1 + 1) * 8);
         o_to = take(((size_t)t1 + 1) * 8);
         o_tr = take((size_t)runs * 8);
-        o_redo = take((size_t)runs * 8);
         if (j->hp.combined_incc) {
             t2 = fast_tiles(j, 1);
             o_t2c = take(((size_t)t2 + 1) * 8);
@@ -474,7 +473,6 @@ static int carve_scratch(smx_job *j, const VllPlan &p, VllScratch &S)
         S.tile_off = (int64_t *)(base + o_to);
         S.tile_runs = (int2 *)(base + o_tr);
         S.tiles_cap = t1;
-        S.redo = (int2 *)(base + o_redo);
         if (j->hp.combined_incc) {
             S.tile2_count = (int64_t *)(base + o_t2c);
             S.tile2_off = (int64_t *)(base + o_t2o);
@@ -511,7 +509,11 @@ static void vll_args(smx_job *j, const VllPlan &p, const VllScratch &S, CellArgs
     A.ncc_alpha = j->hp.ncc_alpha;
     A.redo_n = S.counters + 4;
     if (p.fast) {
-        A.redo = S.redo;
+        // The FAST kernels take their runs from the per-tile lists, built
+        // from the per-cell run counts: the cell-order list is not filled
+        // (vll_front), and its memory (capacity = the run count) holds the
+        // queue of runs handed to the FP64 pass instead.
+        A.redo = S.runs;
         if (j->hp.combined_incc) A.part = S.part;
     }
     if (p.ra) A.peak_rho = j->d_peak_rho;
@@ -586,7 +588,7 @@ static int vll_front(smx_job *j, const VllPlan &p, VllScratch &S, bool may_grow,
         A.peak_rho = j->d_peak_rho;
         j->stats_valid = false;   // NumOfHeight now follows the search space
     }
-    HIPCHK(launch_vll_fill_runs(A, s));
+    if (!p.fast) HIPCHK(launch_vll_fill_runs(A, s));   // the FAST kernels read per-tile lists
     return 0;
 }
 

@@ -64,9 +64,12 @@ def test_noalloc_equals_allocating_call(require_gpu, precision, level, iteration
 
 
 def test_noalloc_rejects_beyond_its_scratch(require_gpu):
-    """500^2 cells x 992 heights: ~4 M height runs, beyond the static array."""
+    """500^2 cells x 992 heights in runs of 32: ~7.8 M height runs (16 B each
+    in the precision form), beyond the static array."""
     from setsm_amd import smx
     g, (ref, na) = _tile(0, 500, 77, 2304, 248)
+    runs32 = [i for i, d in smx.fast_configs() if d == (1, 16, 7680, 0, 1, 1, 32)]
+    na.set_fast_config(runs32[0])
     _bring(na, g, 0, 1, smx.PREC_FAST)
     before = na.get_nccresult()
     with pytest.raises(smx.SmxError, match="static array holds"):
