@@ -742,60 +742,77 @@ __global__ void __launch_bounds__(256) k_awncc(const VllParams *__restrict__ P, 
     // row there)
     const long nh = min((long)nr.NumOfHeight, (long)(A.vox_off[c + 1] - base));
     const double gncc = dq(nr.GNCC);
-    for (long hs = 0; hs < nh; hs++) {
-        const float iter_height = (float)(nr.minHeight + hs * P->height_step);
-        if (!(iter_height >= g.minHeight && iter_height <= g.maxHeight)) continue;
-        const double incc = dq(SMX_VOXEL_INCC(A.vox[base + hs]));
-        double WNCC_sum;
-        if (check_ortho && gncc > -1.0) WNCC_sum = incc * ncc_alpha + gncc * ncc_beta;
-        else WNCC_sum = incc;
-        const double WNCC_temp_rho = WNCC_sum;
-        const double temp_rho = A.sum ? (double)A.sum[base + hs] : WNCC_sum;
-        double diff_rho;
-        int t_direction;
-        if (temp_rho >= 0 && pre_rho >= 0) diff_rho = temp_rho - pre_rho;
-        else if (temp_rho < 0 && pre_rho < 0) diff_rho = -(fabs(temp_rho) - fabs(pre_rho));
-        else if (temp_rho >= 0 && pre_rho < 0) diff_rho = 1;
-        else diff_rho = -1;
-        if (diff_rho > 0) t_direction = 1;
-        else if (diff_rho < 0) t_direction = -1;
-        else t_direction = 0;
-        if (!check_rho) {
-            check_rho = true;
-            t_direction = -1;
-        } else if (pre_rho != -1) {
-            const bool check_peak = direction > 0 && t_direction < 0;
-            if (max_mode) {
-                if (temp_nccresult < temp_rho) {
-                    if (nr.result4 < 10) nr.result4 += 1;
-                    temp_nccresult = temp_rho;
-                    nr.result2 = iter_height;
-                    temp_nccresult_sec = -1.0;
-                    nr.result3 = -9999;
-                    max_roh = WNCC_temp_rho;
-                }
-            } else if (check_peak) {
-                if (nr.result4 < 10) nr.result4 += 1;
-                if (temp_nccresult < pre_rho) {
-                    const double temp_1 = temp_nccresult;
-                    temp_nccresult = pre_rho;
-                    const float temp_2 = nr.result2;
-                    nr.result2 = pre_height;
-                    max_roh = pre_rho_WNCC;
-                    if (temp_nccresult_sec < temp_1) {
-                        temp_nccresult_sec = temp_1;
-                        nr.result3 = temp_2;
+    // The row is walked in blocks of kAhead heights whose words (and SGM
+    // sums) are loaded before any of them is used: each lane walks its own
+    // row, so without that every step waits out one memory latency.
+    constexpr int kAhead = 8;
+    for (long h0 = 0; h0 < nh; h0 += kAhead) {
+        uint32_t wbuf[kAhead];
+        float sbuf[kAhead];
+#pragma unroll
+        for (int q = 0; q < kAhead; q++) {
+            const bool in = h0 + q < nh;
+            wbuf[q] = in ? A.vox[base + h0 + q] : 0u;
+            sbuf[q] = (in && A.sum) ? A.sum[base + h0 + q] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < kAhead; k++) {
+            const long hs = h0 + k;
+            if (hs >= nh) break;
+            const float iter_height = (float)(nr.minHeight + hs * P->height_step);
+            if (!(iter_height >= g.minHeight && iter_height <= g.maxHeight)) continue;
+            const double incc = dq(SMX_VOXEL_INCC(wbuf[k]));
+            double WNCC_sum;
+            if (check_ortho && gncc > -1.0) WNCC_sum = incc * ncc_alpha + gncc * ncc_beta;
+            else WNCC_sum = incc;
+            const double WNCC_temp_rho = WNCC_sum;
+            const double temp_rho = A.sum ? (double)sbuf[k] : WNCC_sum;
+            double diff_rho;
+            int t_direction;
+            if (temp_rho >= 0 && pre_rho >= 0) diff_rho = temp_rho - pre_rho;
+            else if (temp_rho < 0 && pre_rho < 0) diff_rho = -(fabs(temp_rho) - fabs(pre_rho));
+            else if (temp_rho >= 0 && pre_rho < 0) diff_rho = 1;
+            else diff_rho = -1;
+            if (diff_rho > 0) t_direction = 1;
+            else if (diff_rho < 0) t_direction = -1;
+            else t_direction = 0;
+            if (!check_rho) {
+                check_rho = true;
+                t_direction = -1;
+            } else if (pre_rho != -1) {
+                const bool check_peak = direction > 0 && t_direction < 0;
+                if (max_mode) {
+                    if (temp_nccresult < temp_rho) {
+                        if (nr.result4 < 10) nr.result4 += 1;
+                        temp_nccresult = temp_rho;
+                        nr.result2 = iter_height;
+                        temp_nccresult_sec = -1.0;
+                        nr.result3 = -9999;
+                        max_roh = WNCC_temp_rho;
                     }
-                } else if (temp_nccresult_sec < pre_rho) {
-                    temp_nccresult_sec = pre_rho;
-                    nr.result3 = pre_height;
+                } else if (check_peak) {
+                    if (nr.result4 < 10) nr.result4 += 1;
+                    if (temp_nccresult < pre_rho) {
+                        const double temp_1 = temp_nccresult;
+                        temp_nccresult = pre_rho;
+                        const float temp_2 = nr.result2;
+                        nr.result2 = pre_height;
+                        max_roh = pre_rho_WNCC;
+                        if (temp_nccresult_sec < temp_1) {
+                            temp_nccresult_sec = temp_1;
+                            nr.result3 = temp_2;
+                        }
+                    } else if (temp_nccresult_sec < pre_rho) {
+                        temp_nccresult_sec = pre_rho;
+                        nr.result3 = pre_height;
+                    }
                 }
             }
+            pre_rho = temp_rho;
+            pre_rho_WNCC = WNCC_temp_rho;
+            pre_height = iter_height;
+            direction = t_direction;
         }
-        pre_rho = temp_rho;
-        pre_rho_WNCC = WNCC_temp_rho;
-        pre_height = iter_height;
-        direction = t_direction;
     }
     if (A.sum && Pyramid_step == 0 && iteration >= 2) {
         nr.result0 = q_result(max_roh);
