@@ -88,16 +88,39 @@ __global__ void __launch_bounds__(64) k_sgm_path(SgmArgs A)
 
     int t_nh = nh, t_minH = minH;
     float t_maxW = (float)dq(A.ncc[c].max_WNCC);
-    for (row += A.dr, col += A.dc; row >= 0 && row < A.H && col >= 0 && col < A.W; row += A.dr, col += A.dc) {
+    // The next cell's record, search band and row offset are loaded one step
+    // ahead: a step then waits for its cube words and sums only.
+    struct Meta {
+        smx_nccresult nr;
+        short gmin, gmax;
+        int64_t off, end;
+    };
+    auto load_meta = [&](long cc) {
+        Meta m;
+        m.nr = A.ncc[cc];
+        m.gmin = A.gridpt3[cc].minHeight;
+        m.gmax = A.gridpt3[cc].maxHeight;
+        m.off = A.vox_off[cc];
+        m.end = A.vox_off[cc + 1];
+        return m;
+    };
+    auto inside = [&](int r, int q) { return r >= 0 && r < A.H && q >= 0 && q < A.W; };
+    row += A.dr;
+    col += A.dc;
+    Meta next{};
+    if (inside(row, col)) next = load_meta((long)row * A.W + col);
+    for (; inside(row, col); row += A.dr, col += A.dc) {
         // SGM_con_pos for cell (row, col); its predecessor on the path,
         // (row + v_row[d], col + u_col[d]) in the reference's tables
         // (setsm_code.cpp:7036-7037), is the previous cell.
         c = (long)row * A.W + col;
-        nh = row_len(c);
-        minH = A.ncc[c].minHeight;
-        gmin = A.gridpt3[c].minHeight;
-        gmax = A.gridpt3[c].maxHeight;
-        off = A.vox_off[c];
+        const Meta m = next;
+        if (inside(row + A.dr, col + A.dc)) next = load_meta((long)(row + A.dr) * A.W + col + A.dc);
+        nh = (int)min((long)m.nr.NumOfHeight, (long)(m.end - m.off));
+        minH = m.nr.minHeight;
+        gmin = m.gmin;
+        gmax = m.gmax;
+        off = m.off;
         const double V4 = (double)t_maxW - P2;
         for (int h = lane; h < nh; h += 64) {
             const uint32_t w = A.vox[off + h];
@@ -143,7 +166,7 @@ __global__ void __launch_bounds__(64) k_sgm_path(SgmArgs A)
         fcur = tb;
         t_nh = nh;
         t_minH = minH;
-        t_maxW = (float)dq(A.ncc[c].max_WNCC);
+        t_maxW = (float)dq(m.nr.max_WNCC);
     }
 }
 
