@@ -39,14 +39,27 @@ SMX_FN float2 bilerp(float2 t00, float2 t01, float2 t10, float2 t11, float wx, f
     return make_float2(lerp2(i0, i1, wy), lerp2(m0, m1, wy));
 }
 
-// Correlation of one window / channel from pivot-shifted moments (FP64
-// finish). Returns -99 for an undefined correlation, like Correlate.
+// FP32 reciprocal square root (v_rsq_f32 on the GPU).
+SMX_FN float rsqrt32(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.0f / sqrtf(x);
+#endif
+}
+
+// Correlation of one window / channel from pivot-shifted moments. The
+// variances and the covariance are formed in FP64 (the differences cancel);
+// the ratio cv / sqrt(va vb) only needs a few units in the 7th digit and is
+// taken in FP32 (|va vb| < 2^60 for 11-bit data). Returns -99 for an
+// undefined correlation, like Correlate.
 SMX_FN double corr_moments(double inv_n, double a, double b, double aa, double bb, double ab)
 {
     const double va = aa - a * a * inv_n;
     const double vb = bb - b * b * inv_n;
     const double cv = ab - a * b * inv_n;
-    if (va > 1e-8 && vb > 1e-8) return cv / sqrt(va * vb);
+    if (va > 1e-8 && vb > 1e-8) return (double)((float)cv * rsqrt32((float)(va * vb)));
     return -99.0;
 }
 

@@ -176,10 +176,9 @@ static void free_level_state(smx_job *j)
     jfree(j, S.tile2_off);
     jfree(j, S.tile2_runs);
     S.tiles2_cap = 0;
-    jfree(j, S.redo);
     jfree(j, S.part);
     S.part_cap = 0;
-    j->cap_tile_runs = j->cap_tile2_runs = j->cap_redo = 0;
+    j->cap_tile_runs = j->cap_tile2_runs = 0;
     jfree(j, j->d_sum);
     j->sum_total = 0;
     jfree(j, j->d_grid_pts);
@@ -803,10 +802,9 @@ int smx_job_trim(smx_job *j)
     jfree(j, S.tile2_off);
     jfree(j, S.tile2_runs);
     S.tiles2_cap = 0;
-    jfree(j, S.redo);
     jfree(j, S.part);
     S.part_cap = 0;
-    j->cap_tile_runs = j->cap_tile2_runs = j->cap_redo = 0;
+    j->cap_tile_runs = j->cap_tile2_runs = 0;
     job_free(j, j->d_batch);
     j->d_batch = nullptr;
     j->batch_bytes = 0;

@@ -20,10 +20,11 @@ struct LevelView {
     double sx = 0, sy = 0;
 };
 
-// Scratch of one VerticalLineLocus call: the height-run list, the FAST
-// path's per-tile run lists (two sets: a combined iteration's level-below
-// pass has its own tiles), the queue of runs handed to the FP64 pass, the
-// level term of a combined iteration, the scan partials and the call's
+// Scratch of one VerticalLineLocus call: the height-run list (in the FAST
+// path, which reads per-tile lists instead, the queue of runs handed to the
+// FP64 pass), the FAST path's per-tile run lists (two sets: a combined
+// iteration's level-below pass has its own tiles), the level term of a
+// combined iteration, the scan partials and the call's
 // counters. A job owns one set, grown before any launch is queued; the
 // no-alloc entry point carves one out of the library's static device array.
 struct VllScratch {
@@ -35,7 +36,6 @@ struct VllScratch {
     int64_t *tile2_count = nullptr, *tile2_off = nullptr;
     int2 *tile2_runs = nullptr;
     long tiles2_cap = 0;
-    int2 *redo = nullptr;                                   // runs_cap
     double *part = nullptr;                                 // part_cap
     long long part_cap = 0;
     int64_t *scan_part = nullptr;                           // scan_part_cap
@@ -99,7 +99,7 @@ struct smx_job {
     // the job's own call scratch (grown before a call queues anything;
     // js.counters is allocated with the job)
     smx::VllScratch js;
-    long cap_tile_runs = 0, cap_tile2_runs = 0, cap_redo = 0;   // capacities beside js.runs_cap
+    long cap_tile_runs = 0, cap_tile2_runs = 0;   // capacities beside js.runs_cap
 
     int fast_cfg = -1;                           // FAST launch shape (autotuned), -1 = default
     void *d_batch = nullptr;                     // batched-launch descriptors (first job of a batch)
