@@ -19,7 +19,8 @@ Scheduling:
     arrays, run lists, cube, the cube InitializeVoxel builds beside it and
     the level term of a combined iteration); each job is capped at its
     reservation (smx_job_set_budget), so a request that would exceed it
-    fails alone;
+    fails alone; an admitted request's pyramid is built by admission workers
+    on its own stream while the requests already admitted keep stepping;
   * every scheduling step takes all admitted requests, groups them by the
     window size (smx_job_half) and iteration of their next step, and runs ONE
     batched VerticalLineLocus per group (persistent kernels over all their
@@ -43,7 +44,7 @@ from __future__ import annotations
 
 import math
 import time
-from concurrent.futures import ThreadPoolExecutor
+from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -221,10 +222,14 @@ class Server:
         a = _Active(req, None, admitted=now, reserved=self._projected_bytes(req))
         job = smx.Job(self.levels, budget_bytes=a.reserved)
         a.job = job
-        if self.fast_config is not None:
-            job.set_fast_config(self.fast_config)
-        for ti in range(2):
-            job.build_pyramid(ti, sc.images[ti])
+        try:
+            if self.fast_config is not None:
+                job.set_fast_config(self.fast_config)
+            for ti in range(2):
+                job.build_pyramid(ti, sc.images[ti])
+        except smx.SmxError:
+            job.close()
+            raise
         return a
 
     def _begin(self, a: _Active):
@@ -245,8 +250,14 @@ class Server:
     def run(self, stream: list, log=None, workers=8):
         """Serve `stream`; returns per-request records and aggregate stats."""
         self.prepare(stream)
-        with ThreadPoolExecutor(max_workers=workers) as ex:
-            return self._run(stream, ex, log)
+        with ThreadPoolExecutor(max_workers=workers) as ex, ThreadPoolExecutor(max_workers=2) as ax:
+            return self._run(stream, ex, ax, log)
+
+    def _admit_guarded(self, req: Request, now: float):
+        try:
+            return True, self._admit(req, now)
+        except smx.SmxError as e:
+            return False, str(e)
 
     def _try(self, ex, fn, items):
         """fn over items in the pool; returns (results, failed items)."""
@@ -258,9 +269,10 @@ class Server:
         out = list(ex.map(guard, items))
         return [(x, r) for x, (ok, r) in zip(items, out) if ok], [(x, r) for x, (ok, r) in zip(items, out) if not ok]
 
-    def _run(self, stream, ex, log):
+    def _run(self, stream, ex, ax, log):
         pending = sorted(stream, key=lambda r: r.arrival)
         active: list = []
+        admitting: list = []   # (future, request, reserved bytes): pyramid builds in flight
         done, rejected = [], []
         t0 = time.perf_counter()
         gpu_busy = 0.0
@@ -276,11 +288,12 @@ class Server:
             if log:
                 log(f"request {a.req.rid} rejected: {why}")
 
-        while pending or active:
+        while pending or active or admitting:
             now = time.perf_counter() - t0
-            admit = []
+            # admissions: the request's pyramid is built on its own stream by
+            # the admission workers while the admitted requests keep stepping
             while pending and pending[0].arrival <= now:
-                if not self.batching and (active or admit):
+                if not self.batching and (active or admitting):
                     break        # the plain baseline serves one request at a time
                 need = self._projected_bytes(pending[0])
                 if need > self.budget:
@@ -291,16 +304,26 @@ class Server:
                 if self.in_use + need > self.budget:
                     break
                 self.in_use += need
-                admit.append(pending.pop(0))
+                req = pending.pop(0)
+                admitting.append((ax.submit(self._admit_guarded, req, now), req, need))
             tk = time.perf_counter()
-            ok, bad = self._try(ex, lambda r: self._admit(r, now), admit)
-            active += [a for _, a in ok]
-            for req, why in bad:
-                self.in_use -= self._projected_bytes(req)
-                rejected.append(dict(rid=req.rid, arrival=req.arrival, at=now, error=why))
+            if admitting and not active:
+                wait([f for f, _, _ in admitting], return_when=FIRST_COMPLETED)
+            still = []
+            for f, req, need in admitting:
+                if not f.done():
+                    still.append((f, req, need))
+                    continue
+                ok, a = f.result()
+                if ok:
+                    active.append(a)
+                else:
+                    self.in_use -= need
+                    rejected.append(dict(rid=req.rid, arrival=req.arrival, at=now, error=a))
+            admitting = still
             phase["admit_s"] += time.perf_counter() - tk
             if not active:
-                if pending:
+                if pending and not admitting:
                     time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
                 continue
             step = active if self.batching else active[:1]
