@@ -131,7 +131,49 @@ typedef struct smx_voxel {
 #define SMX_VOXEL_FLAG(w) ((int)((w) & 0xffu))
 #define SMX_VOXEL_INCC(w) ((int16_t)(uint16_t)((w) >> 16))
 
+/* D3DPOINT (Typedefine.hpp:140-194): a matched point (ground x, y in metres,
+ * height z) with its blunder flag; 32 bytes like the reference's struct. */
+typedef struct smx_point3 {
+    double  x;
+    double  y;
+    double  z;
+    uint8_t flag;
+    uint8_t pad_[7];
+} smx_point3;
+
+/* UI3DPOINT (Typedefine.hpp:73-97): a TIN triangle as three indices into
+ * the matched-point list. */
+typedef struct smx_tri {
+    uint32_t a;
+    uint32_t b;
+    uint32_t c;
+} smx_tri;
+
+/* SelectMPs' per-call arguments (setsm_code.cpp:8535); the level, SGM_py and
+ * IsRA come from the job's smx_level. */
+typedef struct smx_mps_args {
+    double  th_roh;                /* Th_roh                                   */
+    double  th_roh_min;            /* Th_roh_min                               */
+    double  th_roh_start;          /* Th_roh_start                             */
+    double  th_roh_next;           /* Th_roh_next                              */
+    double  mpp;                   /* MPP                                      */
+    double  mpp_stereo_angle;      /* MPP_stereo_angle                         */
+    int32_t iteration;
+    int32_t final_level_iteration;
+} smx_mps_args;
+
+/* SetHeightRange's per-call arguments (setsm_code.cpp:9851); the level,
+ * grid, DEM_resolution, resolution and pre_DEMtif come from smx_level. */
+typedef struct smx_hr_args {
+    double  mpp;                   /* MPP                                      */
+    double  seed_dem_sigma;        /* ProInfo.seedDEMsigma (read if pre_dem)   */
+    int32_t iteration;
+    int32_t check_matching_rate;   /* level_check_matching_rate                */
+} smx_hr_args;
+
 #if defined(__cplusplus)
+static_assert(sizeof(smx_point3) == 32 && offsetof(smx_point3, flag) == 24, "D3DPOINT layout");
+static_assert(sizeof(smx_tri) == 12, "UI3DPOINT layout");
 static_assert(sizeof(smx_rpc) == 90 * sizeof(double), "smx_rpc layout");
 static_assert(sizeof(smx_ugrid) == 20, "UGRID is 20 bytes");
 static_assert(offsetof(smx_ugrid, Matched_flag) == 16, "UGRID.Matched_flag");

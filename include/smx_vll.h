@@ -195,6 +195,29 @@ int  smx_awncc(smx_job *job, int iteration);
  * order (SETSM keeps it in AWNCC's local float **SumCost). */
 int  smx_job_get_sumcost(smx_job *job, float *sumcost);
 
+/* ------------------------------------------------- between iterations */
+/* SelectMPs (setsm_code.cpp:8535-8889, setsm_code.hpp:99): after AWNCC, the
+ * level's peak threshold from the first-peak histogram, the matched points
+ * and the search-space grid's update (roh, widened ranges of ambiguous
+ * matched cells, Matched_flag), on the job's NCCresult and GridPT3. The
+ * points go to pts in cell order, as the reference's vector receives them;
+ * *count gets their number even when it exceeds cap (then SMX_E_ARG after
+ * the grid update; a cap of the cell count always suffices). Needs the
+ * level's grid_pts. *minimum_th (may be NULL) gets the threshold. SMX_E_ARG
+ * where the reference leaves the threshold undefined (level > 4 above
+ * SGM_py), with the grid unchanged. */
+int  smx_select_mps(smx_job *job, const smx_mps_args *args, smx_point3 *pts, long cap, long *count,
+                    double *minimum_th);
+/* SetHeightRange (setsm_code.cpp:9851-10261, setsm_code.hpp:122): the
+ * triangles of the TIN over the matched points (built by the caller's host
+ * code, as SETSM's TINCreate_list does) give every cell its next height
+ * range, height and Matched_flag; the job's GridPT3 becomes the grid the
+ * reference returns. pts and tris are host arrays (a triangle indexing past
+ * npts is skipped, as in the reference). minH_grid / maxH_grid (may be NULL)
+ * get the reference's range bounds of the grid. */
+int  smx_set_height_range(smx_job *job, const smx_point3 *pts, long npts, const smx_tri *tris, long ntris,
+                          const smx_hr_args *args, double *minH_grid, double *maxH_grid);
+
 /* Number of hypotheses in the cube (sum of NumOfHeight). */
 long long smx_job_num_hypotheses(const smx_job *job);
 /* Download the cube in CSR order: words[total], rho[total] (rho may be NULL,

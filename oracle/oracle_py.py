@@ -1,5 +1,5 @@
 """ctypes binding of the test oracle (libsmx_oracle.so) and of the reference
-harness (_ref/libsmx_ref.so).
+harnesses (_ref/libsmx_ref.so, _ref/libref_tin.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
 bench.py's cpu_baseline leg, never by the product package.
@@ -11,11 +11,13 @@ import os
 
 import numpy as np
 
-from setsm_amd.records import NCCRESULT_DTYPE, UGRID_DTYPE, SmxLevel
+from setsm_amd.records import (NCCRESULT_DTYPE, POINT3_DTYPE, TRI_DTYPE, UGRID_DTYPE, SmxHrArgs, SmxLevel,
+                               SmxMpsArgs)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 _lib = None
 _ref = None
+_tin = None
 
 
 def build():
@@ -59,6 +61,10 @@ def lib():
         L.orc_gauss_pyramid.argtypes = [vp, C.c_uint32, C.c_uint32, vp]
         L.orc_sobel.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp]
         L.orc_orientation.argtypes = [vp, vp, C.c_uint32, C.c_uint32, ip, vp]
+        L.orc_select_mps.restype = C.c_long
+        L.orc_select_mps.argtypes = [vp, vp, vp, vp, vp, dp]
+        L.orc_set_height_range.restype = ip
+        L.orc_set_height_range.argtypes = [vp, vp, vp, C.c_long, vp, C.c_long, vp, dp, dp]
         L.orc_set_threads.argtypes = [ip]
         L.orc_get_threads.restype = ip
         _lib = L
@@ -85,6 +91,70 @@ def ref():
         R.ref_layout.argtypes = [C.c_int]
         _ref = R
     return _ref
+
+
+def tin():
+    """The reference's TIN (grid_triangulation.cpp via ref_tin.cpp), or None
+    where it was not built."""
+    global _tin
+    if _tin is None:
+        path = os.path.join(HERE, "_ref", "libref_tin.so")
+        if not os.path.exists(path):
+            return None
+        T = C.CDLL(path)
+        T.ref_tin_create.restype = C.c_long
+        T.ref_tin_create.argtypes = [C.c_void_p, C.c_long, C.POINTER(C.c_double), C.c_double, C.c_void_p, C.c_long]
+        _tin = T
+    return _tin
+
+
+def tin_create(pts, boundary, resolution):
+    """TINCreate_list (SubFunctions.cpp:2768-2846) with the reference's own
+    triangulation: triangles (TRI_DTYPE) over the matched points."""
+    T = tin()
+    if T is None:
+        raise RuntimeError("oracle/_ref/libref_tin.so is not built (needs the reference sources)")
+    pts = np.ascontiguousarray(pts, POINT3_DTYPE)
+    out = np.zeros(2 * len(pts) + 16, TRI_DTYPE)
+    mm = (C.c_double * 4)(*[float(b) for b in boundary])
+    n = T.ref_tin_create(_p(pts) if len(pts) else None, len(pts), mm, float(resolution), _p(out), len(out))
+    if n < 0:
+        raise RuntimeError(f"reference TIN failed ({n})")
+    return out[:n].copy()
+
+
+def select_mps(level_inputs, ncc, ugrid, iteration, th_roh, th_roh_min, th_roh_start, th_roh_next, mpp,
+               mpp_stereo_angle, final_level_iteration):
+    """SelectMPs (setsm_code.cpp:8535-8889): returns (points, minimum_th);
+    ugrid is updated in place."""
+    a = SmxMpsArgs(th_roh, th_roh_min, th_roh_start, th_roh_next, mpp, mpp_stereo_angle, int(iteration),
+                   int(final_level_iteration))
+    ncc = np.ascontiguousarray(ncc, NCCRESULT_DTYPE)
+    assert ugrid.dtype == UGRID_DTYPE and ugrid.flags.c_contiguous
+    pts = np.zeros(max(len(ncc), 1), POINT3_DTYPE)
+    th = C.c_double(0)
+    n = lib().orc_select_mps(C.addressof(level_inputs.level), _p(ncc), _p(ugrid), C.addressof(a), _p(pts),
+                             C.byref(th))
+    if n < 0:
+        raise RuntimeError("SelectMPs has no threshold at this level")
+    return pts[:n].copy(), th.value
+
+
+def set_height_range(level_inputs, ugrid, pts, tris, iteration, mpp, check_matching_rate=False,
+                     seed_dem_sigma=0.0):
+    """SetHeightRange (setsm_code.cpp:9851-10261): ugrid becomes the returned
+    grid; returns (minH_grid, maxH_grid)."""
+    a = SmxHrArgs(mpp, seed_dem_sigma, int(iteration), int(bool(check_matching_rate)))
+    pts = np.ascontiguousarray(pts, POINT3_DTYPE)
+    tris = np.ascontiguousarray(tris, TRI_DTYPE)
+    assert ugrid.dtype == UGRID_DTYPE and ugrid.flags.c_contiguous
+    lo, hi = C.c_double(0), C.c_double(0)
+    r = lib().orc_set_height_range(C.addressof(level_inputs.level), C.addressof(a),
+                                   _p(pts) if len(pts) else None, len(pts), _p(tris) if len(tris) else None,
+                                   len(tris), _p(ugrid), C.byref(lo), C.byref(hi))
+    if r < 0:
+        raise MemoryError("oracle SetHeightRange ran out of memory")
+    return lo.value, hi.value
 
 
 def _p(a):

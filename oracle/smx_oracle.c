@@ -1376,3 +1376,375 @@ void orc_orientation(const uint16_t *Gmag, const int16_t *Gdir, uint32_t w, uint
         }
     }
 }
+
+/* ------------------------------------------------------------------------
+ * Between iterations: SelectMPs and SetHeightRange
+ * ---------------------------------------------------------------------- */
+
+/* setsm_code.cpp:8546-8614: the level's minimum peak-rho threshold from the
+ * histogram of first-peak rho (cells with more than two heights). Returns a
+ * negative value where the reference leaves the threshold undefined (a level
+ * above 4 that is not an SGM level). */
+static double mps_minimum_th(const smx_level *L, const smx_nccresult *ncc, int iteration)
+{
+    const int level = L->level;
+    if (L->is_ra || level <= 0) return 0.2;
+    long hist[100] = {0};
+    long total = 0;
+    const long n = (long)L->grid_w * (long)L->grid_h;
+    for (long c = 0; c < n; c++) {
+        if (ncc[c].NumOfHeight <= 2) continue;
+        int r = (int)ceil(dq(ncc[c].result0) * 100);
+        if (r > 99) r = 99;
+        if (r >= 0) hist[r]++;
+        total++;
+    }
+    if (total == 0) return 0.2;
+    double frac;
+    int defined = 1;
+    if (level == 4) frac = 0.05 + (iteration - 1) * 0.01;
+    else if (level == 3) frac = 0.15 + (iteration - 1) * 0.01;
+    else if (level == 2) frac = 0.60;
+    else if (level == 1) frac = 0.80;
+    else { frac = 0; defined = 0; }
+    if (level <= L->sgm_py) {
+        frac = 0.95 - level * 0.1;
+        if (frac < 0.5) frac = 0.5;
+        defined = 1;
+    }
+    if (!defined) return -1.0;
+    double th = 0.2;
+    int acc = 0;
+    for (int r = 99; r > 0; r--) {
+        acc += (int)hist[r];
+        if (acc / (double)total > frac) {
+            th = (double)r / 100.0;
+            break;
+        }
+    }
+    return th > 0.95 ? 0.95 : th;
+}
+
+long orc_select_mps(const smx_level *L, const smx_nccresult *ncc, smx_ugrid *g, const smx_mps_args *a,
+                    smx_point3 *pts, double *minimum_th)
+{
+    const int level = L->level;
+    const int it = a->iteration;
+    const double th = mps_minimum_th(L, ncc, it);
+    if (minimum_th) *minimum_th = th;
+    if (th < 0) return -1;
+    /* setsm_code.cpp:8618-8643 */
+    const int iter_end = (level <= 3 || L->is_ra) ? (a->th_roh - 0.10 < a->th_roh_min)
+                                                   : (a->th_roh - 0.06 < a->th_roh_min);
+    const double roh_th = 0.05;
+    const long n = (long)L->grid_w * (long)L->grid_h;
+    long count = 0;
+    for (long c = 0; c < n; c++) {
+        const smx_nccresult *r = &ncc[c];
+        if (r->NumOfHeight <= 2) continue;
+        smx_ugrid *G = &g[c];
+        if (level == 4 && it == 1) G->Height = -1000.0f;
+        const double r0 = dq(r->result0), r1 = dq(r->result1);
+        /* setsm_code.cpp:8665-8702: ratio of the two peaks, peak tests */
+        double ror;
+        if ((it <= 2 && level >= 3) || (it <= 1 && level == 2)) ror = 1.0;
+        else ror = fabs(r0) > 0 ? (r0 - r1) / r0 : 0;
+        const double ror_min = level == 0 ? 0.1 - 0.03 * (3 - level) : 0.1;
+        const int peak_ok = ror >= ror_min && r0 > th;
+        const int above_prev = r0 > dq(G->roh) - roh_th;
+        const int has_peak = r->result4 > 0;
+        /* setsm_code.cpp:8710-8747 */
+        int take;
+        if (level >= 2 || (level >= 1 && !(a->mpp_stereo_angle > 5)) || (level == 0 && a->final_level_iteration < 3))
+            take = has_peak && peak_ok && above_prev;
+        else if (level >= 1)
+            take = (above_prev && peak_ok) || (above_prev && r0 > th);
+        else
+            take = peak_ok && above_prev;
+        /* setsm_code.cpp:8749-8781: a matched cell whose second peak is strong
+         * keeps both peaks in its range */
+        if (G->Matched_flag != 0) {
+            const double ror_lim = level <= 0 ? 0.1 - 0.03 * (3 - level) : 0.1;
+            const int both = ror < ror_lim && r1 > dq(G->roh) - roh_th;
+            if (both) {
+                if (r->result2 > r->result3) {
+                    G->minHeight = (int16_t)floor(r->result3 - 0.5);
+                    G->maxHeight = (int16_t)ceil(r->result2 + 0.5);
+                } else {
+                    G->minHeight = (int16_t)floor(r->result2 - 0.5);
+                    G->maxHeight = (int16_t)ceil(r->result3 + 0.5);
+                }
+                G->Matched_flag = 4;
+            }
+        }
+        /* setsm_code.cpp:8783-8884 */
+        const double z = (double)r->result2;
+        int emit = 0;
+        if (take) {
+            if (level < 2) {
+                const double pre_h = (double)G->Height;
+                const double pre_range = (double)(G->maxHeight - G->minHeight);
+                const int m = G->Matched_flag;
+                const double tol2 = a->mpp * 2 * (1 << level), tol3 = a->mpp * 3 * (1 << level);
+                const int near_prev = fabs(pre_h - z) <= tol2;
+                const int in_range = fabs(pre_h - z) <= pre_range / 2.0;
+                const int near_min = fabs(z - G->minHeight) <= tol3;
+                const int near_max = fabs(z - G->maxHeight) <= tol3;
+                const int keep = (m == 0) || (m == 2 && near_prev) ||
+                                 ((m == 1 || m == 4) && (near_prev || in_range || near_min || near_max));
+                if (keep && z > -100 && z < 10000) {
+                    emit = 1;
+                    G->roh = r->result0;
+                    if (G->roh < q_result(th)) G->roh = q_result(th);
+                } else {
+                    G->roh = q_result(a->th_roh);
+                    if (iter_end && a->th_roh_start == dq(G->roh)) G->roh = q_result(a->th_roh_next);
+                }
+            } else {
+                emit = z > -100 && z < 10000;
+                G->roh = r->result0;
+                if (G->roh < q_result(th)) G->roh = q_result(th);
+            }
+        } else {
+            G->roh = q_result(a->th_roh);
+            if (iter_end && a->th_roh_start == dq(G->roh)) G->roh = q_result(a->th_roh_next);
+        }
+        if (emit) {
+            smx_point3 *p = &pts[count++];
+            memset(p, 0, sizeof *p);
+            p->x = L->grid_pts[2 * c];
+            p->y = L->grid_pts[2 * c + 1];
+            p->z = z;
+        }
+    }
+    return count;
+}
+
+/* std::min / std::max as the reference calls them: the first argument on a tie */
+static double smin(double a, double b) { return b < a ? b : a; }
+static double smax(double a, double b) { return a < b ? b : a; }
+
+/* SubFunctions.cpp:2946-2995 IsTinInside: inside test by the three edge
+ * cross products, then the height on the triangle's plane through its unit
+ * normal. */
+static int tin_inside(double px, double py, const smx_point3 *p1, const smx_point3 *p2, const smx_point3 *p3,
+                      float *z)
+{
+    int sum = 3;
+    if ((p2->x - p1->x) * (py - p1->y) - (p2->y - p1->y) * (px - p1->x) <= 0) sum--;
+    if ((p3->x - p2->x) * (py - p2->y) - (p3->y - p2->y) * (px - p2->x) <= 0) sum--;
+    if ((p1->x - p3->x) * (py - p3->y) - (p1->y - p3->y) * (px - p3->x) <= 0) sum--;
+    if (sum != 0 && sum != 3) return 0;
+    const double ux = p2->x - p1->x, uy = p2->y - p1->y, uz = p2->z - p1->z;
+    const double vx = p3->x - p1->x, vy = p3->y - p1->y, vz = p3->z - p1->z;
+    double nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
+    const double len = sqrt(nx * nx + ny * ny + nz * nz);
+    if (!(len > 0)) return 0;
+    nx /= len;
+    ny /= len;
+    nz /= len;
+    const double d = -(nx * p1->x + ny * p1->y + nz * p1->z);
+    if (nz == 0) return 0;
+    *z = (float)(-1.0 * ((nx * px) + (ny * py) + d) / nz);
+    return 1;
+}
+
+/* SubFunctions.cpp:2997-3024 SetNormalAngle + SetAngle: slope of the
+ * triangle in degrees. */
+static double tin_slope_deg(const smx_point3 *p0, const smx_point3 *p1, const smx_point3 *p2)
+{
+    const double ux = p1->x - p0->x, uy = p1->y - p0->y, uz = p1->z - p0->z;
+    const double vx = p2->x - p0->x, vy = p2->y - p0->y, vz = p2->z - p0->z;
+    const double nx = uy * vz - vy * uz, ny = -(ux * vz - vx * uz), nz = ux * vy - vx * uy;
+    const double norm = sqrt(nx * nx + ny * ny + nz * nz);
+    double angle = 0;
+    /* RadToDeg expands to 180/PI after the product: (acos * 180) / PI */
+    if (norm != 0) angle = acos(fabs(nz) / norm) * 180 / 3.141592653589793;
+    if (angle <= 0 && angle >= -90) angle = fabs(angle);
+    else if (angle <= -270 && angle >= -360) angle = 360 + angle;
+    else if (angle >= 270 && angle <= 360) angle = 360 - angle;
+    return angle;
+}
+
+int orc_set_height_range(const smx_level *L, const smx_hr_args *a, const smx_point3 *pts, long npts,
+                         const smx_tri *tris, long ntris, smx_ugrid *g, double *minH_grid, double *maxH_grid)
+{
+    const int level = L->level, it = a->iteration;
+    const long W = (long)L->grid_w, H = (long)L->grid_h, n = W * H;
+    const double gres = L->grid_resolution;
+    /* setsm_code.cpp:9864-9923: height buffer and gap limit */
+    double buf = a->mpp * 4.0 * (1 << level);
+    if (level == 1) buf = it >= 2 ? a->mpp * 2 : a->mpp * 3;
+    else if (level == 0) {
+        buf = it == 1 ? a->mpp * 2 : a->mpp;
+        if (buf < 0.5) buf = 0.5;
+    }
+    if (L->pre_dem && buf > a->seed_dem_sigma) buf = a->seed_dem_sigma;
+    buf = ceil(buf);
+    if (buf > 100) buf = 100;
+    double th_hg = 100000;
+    if (L->dem_resolution >= 4) {
+        if (level <= 1) th_hg = 1000;
+    } else {
+        if (!a->check_matching_rate) {
+            if (level == 2 && it >= 5) th_hg = 1000;
+            else if (level == 1) th_hg = 500;
+            else if (level == 0) th_hg = 100;
+        } else if (level <= 2) th_hg = 100;
+        if (level == 0 && L->dem_resolution < 2 && it > 1) th_hg = 50;
+    }
+    double ncc_th;
+    if (level == 4) ncc_th = 0.7;
+    else if (level == 3) ncc_th = 0.6 - (it - 1) * 0.02;
+    else if (level == 2) ncc_th = 0.5 - (it - 1) * 0.02;
+    else if (level == 1) ncc_th = 0.4;
+    else ncc_th = 0.3;
+    if (ncc_th < 0.3) ncc_th = 0.3;
+
+    uint8_t *mb = (uint8_t *)calloc((size_t)n, 1);
+    float *newh = (float *)malloc((size_t)n * sizeof(float));
+    smx_ugrid *snap = (smx_ugrid *)malloc((size_t)n * sizeof(smx_ugrid));
+    if (!mb || !newh || !snap) {
+        free(mb);
+        free(newh);
+        free(snap);
+        return -1;
+    }
+    for (long c = 0; c < n; c++) newh[c] = -1000.0f;
+    double tmin = 100000, tmax = -100000;
+    /* setsm_code.cpp:9928-10091: each triangle, in list order, claims the
+     * cells of its bounding box that lie inside it and are still unclaimed */
+    for (long t = 0; t < ntris; t++) {
+        if (tris[t].a >= (uint32_t)npts || tris[t].b >= (uint32_t)npts || tris[t].c >= (uint32_t)npts) continue;
+        const smx_point3 *p1 = &pts[tris[t].a], *p2 = &pts[tris[t].b], *p3 = &pts[tris[t].c];
+        /* SubFunctions.cpp:2915-2944 SetTinBoundary */
+        const double zmin = smin(smin(p1->z, p2->z), p3->z), zmax = smax(smax(p1->z, p2->z), p3->z);
+        if (zmin < tmin) tmin = zmin;
+        if (zmax > tmax) tmax = zmax;
+        const double bx0 = smin(smin(p1->x, p2->x), p3->x), by0 = smin(smin(p1->y, p2->y), p3->y);
+        const double bx1 = smax(smax(p1->x, p2->x), p3->x), by1 = smax(smax(p1->y, p2->y), p3->y);
+        int c0 = (int)((bx0 - L->boundary[0]) / gres + 0.5) - 1, r0 = (int)((by0 - L->boundary[1]) / gres + 0.5) - 1;
+        int c1 = (int)((bx1 - L->boundary[0]) / gres + 0.5) + 1, r1 = (int)((by1 - L->boundary[1]) / gres + 0.5) + 1;
+        if (c1 >= (int)W) c1 = (int)W - 1;
+        if (r1 >= (int)H) r1 = (int)H - 1;
+        if (c0 < 0) c0 = 0;
+        if (r0 < 0) r0 = 0;
+        const double slope = tin_slope_deg(p1, p2, p3);
+        const double wgt = slope <= 30 ? 1.0 + 0.05 * slope / 5.0 : 1.3;
+        const double bf = level >= 3 ? ceil(buf * wgt) : ceil(buf);
+        for (long row = r0; row <= r1; row++)
+            for (long col = c0; col <= c1; col++) {
+                const long c = W * row + col;
+                if (mb[c]) continue;
+                const double px = col * gres + L->boundary[0], py = row * gres + L->boundary[1];
+                float z = -1000.0f;
+                if (!tin_inside(px, py, p1, p2, p3, &z)) continue;
+                smx_ugrid *G = &g[c];
+                if (level < 2) {
+                    const double d1 = sqrt((px - p1->x) * (px - p1->x) + (py - p1->y) * (py - p1->y));
+                    const double d2 = sqrt((px - p2->x) * (px - p2->x) + (py - p2->y) * (py - p2->y));
+                    const double d3 = sqrt((px - p3->x) * (px - p3->x) + (py - p3->y) * (py - p3->y));
+                    if (d1 == 0) { z = (float)p1->z; G->Matched_flag = 2; }
+                    else if (d2 == 0) { z = (float)p2->z; G->Matched_flag = 2; }
+                    else if (d3 == 0) { z = (float)p3->z; G->Matched_flag = 2; }
+                }
+                mb[c] = 1;
+                G->Height = z;
+                newh[c] = z;
+                const double t1 = smin(zmin, (double)z), t2 = smax(zmax, (double)z);
+                if (G->Matched_flag == 4) {
+                    if (t1 - bf <= G->minHeight) G->minHeight = (int16_t)floor(t1 - bf);
+                    if (t2 + bf >= G->maxHeight) G->maxHeight = (int16_t)ceil(t2 + bf);
+                } else {
+                    G->minHeight = (int16_t)floor(t1 - bf);
+                    G->maxHeight = (int16_t)ceil(t2 + bf);
+                }
+                if (G->minHeight > G->maxHeight) {
+                    const int16_t s = G->minHeight;
+                    G->minHeight = G->maxHeight;
+                    G->maxHeight = s;
+                }
+                if (G->Matched_flag == 2) {
+                    G->minHeight = (int16_t)floor(z - bf);
+                    G->maxHeight = (int16_t)ceil(z + bf);
+                } else {
+                    G->Matched_flag = 1;
+                    if (G->Mean_ortho_ncc > q_result(ncc_th)) {
+                        G->minHeight = (int16_t)floor(z - bf);
+                        G->maxHeight = (int16_t)ceil(z + bf);
+                    } else if (level >= 2) {
+                        G->minHeight = (int16_t)floor(t1 - bf);
+                        G->maxHeight = (int16_t)ceil(t2 + bf);
+                    }
+                }
+                if (G->maxHeight - G->minHeight > th_hg) G->minHeight = G->maxHeight = -100;
+            }
+    }
+    /* setsm_code.cpp:10093-10190: unclaimed cells take the range of their
+     * claimed neighbours (a snapshot), or the TIN's height span at levels 0-1 */
+    for (long c = 0; c < n; c++) {
+        g[c].Height = newh[c];
+        snap[c] = g[c];
+    }
+    double hmin = 100000.0, hmax = -100000.0;
+    for (long row = 0; row < H; row++)
+        for (long col = 0; col < W; col++) {
+            const long c = W * row + col;
+            smx_ugrid *G = &g[c];
+            if (G->Matched_flag == 0) {
+                int lo = 10000, hi = -10000, any = 0;
+                for (long dr = -1; dr <= 1; dr++)
+                    for (long dc = -1; dc <= 1; dc++) {
+                        if (col + dc < 0 || col + dc >= W || row + dr < 0 || row + dr >= H) continue;
+                        const smx_ugrid *q = &snap[(col + dc) + (row + dr) * W];
+                        if (q->Matched_flag == 0) continue;
+                        any = 1;
+                        if (lo > q->minHeight) lo = q->minHeight;
+                        if (hi < q->maxHeight) hi = q->maxHeight;
+                    }
+                if (any) {
+                    G->minHeight = (int16_t)lo;
+                    G->maxHeight = (int16_t)hi;
+                    mb[c] = 2;
+                } else if (level <= 1) {
+                    const double span = fabs(tmax - tmin) / 3.0;
+                    double bf = buf > span ? span : buf;
+                    if (bf < (1 << level) * L->resolution) bf = (1 << level) * L->resolution;
+                    if (G->minHeight < tmin - bf) G->minHeight = (int16_t)floor(tmin - bf);
+                    if (G->maxHeight > tmax + bf) G->maxHeight = (int16_t)ceil(tmax + bf);
+                }
+            } else if (mb[c] == 0)
+                G->Matched_flag = 0;
+            if (hmin > G->minHeight) hmin = (double)G->minHeight;
+            if (hmax < G->maxHeight) hmax = (double)G->maxHeight;
+            if (G->maxHeight - G->minHeight > th_hg) G->minHeight = G->maxHeight = -100;
+        }
+    if (minH_grid) *minH_grid = hmin;
+    if (maxH_grid) *maxH_grid = hmax;
+    /* setsm_code.cpp:10192-10252: the returned grid */
+    for (long c = 0; c < n; c++) {
+        const smx_ugrid G = g[c];
+        smx_ugrid *R = &g[c];
+        int16_t lo = G.minHeight, hi = G.maxHeight;
+        if (lo > hi) {
+            const int16_t s = lo;
+            lo = hi;
+            hi = s;
+        }
+        memset(R, 0, sizeof *R);
+        R->Matched_flag = (mb[c] == 2 && G.Matched_flag == 0) ? 1 : G.Matched_flag;
+        R->roh = G.roh;
+        R->Height = mb[c] == 0 ? -1000.0f : G.Height;
+        R->anchor_flag = 0;
+        R->ortho_ncc[1] = G.ortho_ncc[1];
+        R->Mean_ortho_ncc = G.Mean_ortho_ncc;
+        R->minHeight = lo < -100 ? -100 : lo;
+        R->maxHeight = hi;
+        if (R->minHeight == 0) R->minHeight = -1;
+        if (R->maxHeight == 0) R->maxHeight = 1;
+    }
+    free(mb);
+    free(newh);
+    free(snap);
+    return 0;
+}

@@ -90,6 +90,19 @@ void orc_sgm_aggregate(const smx_level *L, const orc_voxels *vox, const smx_ugri
 void orc_awncc(const smx_level *L, const orc_voxels *vox, const smx_ugrid *gridpt3,
                smx_nccresult *ncc, int iteration, float *sumcost_csr);
 
+/* setsm_code.cpp:8535-8889 SelectMPs: updates GridPT3 (roh, ranges, flags)
+ * and writes the matched points in cell order to pts (capacity: the cell
+ * count). Returns their count, or -1 where the reference leaves the level's
+ * threshold undefined. *minimum_th receives the histogram threshold. */
+long orc_select_mps(const smx_level *L, const smx_nccresult *ncc, smx_ugrid *gridpt3, const smx_mps_args *a,
+                    smx_point3 *pts, double *minimum_th);
+/* setsm_code.cpp:9851-10261 SetHeightRange (SetTinBoundary, IsTinInside,
+ * SetNormalAngle: SubFunctions.cpp:2915-3024): gridpt3 is replaced by the
+ * returned grid. Returns 0, or -1 if host memory ran out. */
+int orc_set_height_range(const smx_level *L, const smx_hr_args *a, const smx_point3 *pts, long npts,
+                         const smx_tri *tris, long ntris, smx_ugrid *gridpt3, double *minH_grid,
+                         double *maxH_grid);
+
 /* Pyramid inputs. Template.hpp:45-132 (9x9, sigma 1.5 decimation),
  * SubFunctions.cpp:2418-2470 (Sobel), SubFunctions.cpp:2473-2571 (Orientation). */
 void orc_gauss_pyramid(const uint16_t *in, uint32_t w, uint32_t h, uint16_t *out);

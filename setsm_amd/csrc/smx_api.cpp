@@ -1298,6 +1298,164 @@ int smx_job_get_sumcost(smx_job *j, float *out)
     return 0;
 }
 
+// ------------------------------------------------------------ between iterations
+
+}  // extern "C"
+
+namespace {
+// Temporaries of one call, returned to the job's allocator when it ends.
+struct CallBuffers {
+    smx_job *j;
+    std::vector<void *> held;
+    explicit CallBuffers(smx_job *job) : j(job) {}
+    template <class T>
+    int get(T **p, size_t count)
+    {
+        int r = jalloc(j, p, count ? count : 1);
+        if (!r) held.push_back((void *)*p);
+        return r;
+    }
+    ~CallBuffers()
+    {
+        for (void *p : held) job_free(j, p);
+    }
+};
+}  // namespace
+
+extern "C" {
+
+int smx_select_mps(smx_job *j, const smx_mps_args *a, smx_point3 *pts, long cap, long *count, double *minimum_th)
+{
+    if (!j || !a || !count) return fail(SMX_E_ARG, "smx_select_mps: null argument");
+    if (!j->level_set || !j->searchspace_set) return fail(SMX_E_STATE, "level and search space must be set");
+    if (!j->d_grid_pts) return fail(SMX_E_STATE, "the level was set without grid_pts (GridPts)");
+    if (cap > 0 && !pts) return fail(SMX_E_ARG, "smx_select_mps: null point buffer");
+    HIPCHK(hipSetDevice(j->device));
+    const long n = j->ncells;
+    CallBuffers B(j);
+    MpsDev D{};
+    D.ncc = j->d_ncc;
+    D.g = j->d_gridpt3;
+    D.grid_pts = j->d_grid_pts;
+    D.n = n;
+    D.level = j->lv.level;
+    D.is_ra = j->lv.is_ra;
+    D.sgm_py = j->lv.sgm_py;
+    D.a = *a;
+    double *d_th = nullptr;
+    int64_t *d_off = nullptr;
+    int r = B.get(&D.hist, kRohBins + 1);
+    if (!r) r = B.get(&d_th, 1);
+    if (!r) r = B.get(&D.sel, (size_t)n + 1);
+    if (!r) r = B.get(&d_off, (size_t)n + 1);
+    if (r) return r;
+    D.th = d_th;
+    D.off = d_off;
+    HIPCHK(launch_select_mps(D, j->js, j->stream));
+    double th = 0;
+    HIPCHK(hipMemcpyAsync(&j->h_small[0], d_off + n, sizeof(int64_t), hipMemcpyDeviceToHost, j->stream));
+    HIPCHK(hipMemcpyAsync(&th, d_th, sizeof(double), hipMemcpyDeviceToHost, j->stream));
+    HIPCHK(hipStreamSynchronize(j->stream));
+    if (minimum_th) *minimum_th = th;
+    if (th < 0)
+        return fail(SMX_E_ARG, "SelectMPs has no peak threshold at level %d (above 4 and above SGM_py)", j->lv.level);
+    j->generation++;   // GridPT3 changed
+    const long m = (long)j->h_small[0];
+    *count = m;
+    if (m > cap) return fail(SMX_E_ARG, "%ld matched points exceed the buffer of %ld", m, cap);
+    if (m > 0) {
+        if ((r = B.get(&D.pts, (size_t)m))) return r;
+        HIPCHK(launch_write_mps(D, j->stream));
+        HIPCHK(hipMemcpyAsync(pts, D.pts, (size_t)m * sizeof(smx_point3), hipMemcpyDeviceToHost, j->stream));
+        HIPCHK(hipStreamSynchronize(j->stream));
+    }
+    return 0;
+}
+
+int smx_set_height_range(smx_job *j, const smx_point3 *pts, long npts, const smx_tri *tris, long ntris,
+                         const smx_hr_args *a, double *minH_grid, double *maxH_grid)
+{
+    if (!j || !a || (npts > 0 && !pts) || (ntris > 0 && !tris) || npts < 0 || ntris < 0)
+        return fail(SMX_E_ARG, "smx_set_height_range: bad argument");
+    if (!j->level_set || !j->searchspace_set) return fail(SMX_E_STATE, "level and search space must be set");
+    if (npts > 0xffffffffL || ntris > (1L << 40)) return fail(SMX_E_ARG, "too many points or triangles");
+    HIPCHK(hipSetDevice(j->device));
+    const smx_level &L = j->lv;
+    const int level = L.level, it = a->iteration;
+    const long n = j->ncells;
+    // setsm_code.cpp:9864-9923 and :10002-10015: the height buffer, the gap
+    // limit and the ortho-NCC threshold of this level and iteration
+    double buf = a->mpp * 4.0 * (1 << level);
+    if (level == 1) buf = it >= 2 ? a->mpp * 2 : a->mpp * 3;
+    else if (level == 0) {
+        buf = it == 1 ? a->mpp * 2 : a->mpp;
+        if (buf < 0.5) buf = 0.5;
+    }
+    if (L.pre_dem && buf > a->seed_dem_sigma) buf = a->seed_dem_sigma;
+    buf = std::ceil(buf);
+    if (buf > 100) buf = 100;
+    double th_hg = 100000;
+    if (L.dem_resolution >= 4) {
+        if (level <= 1) th_hg = 1000;
+    } else {
+        if (!a->check_matching_rate) {
+            if (level == 2 && it >= 5) th_hg = 1000;
+            else if (level == 1) th_hg = 500;
+            else if (level == 0) th_hg = 100;
+        } else if (level <= 2) th_hg = 100;
+        if (level == 0 && L.dem_resolution < 2 && it > 1) th_hg = 50;
+    }
+    double ncc_th;
+    if (level == 4) ncc_th = 0.7;
+    else if (level == 3) ncc_th = 0.6 - (it - 1) * 0.02;
+    else if (level == 2) ncc_th = 0.5 - (it - 1) * 0.02;
+    else if (level == 1) ncc_th = 0.4;
+    else ncc_th = 0.3;
+    if (ncc_th < 0.3) ncc_th = 0.3;
+
+    CallBuffers B(j);
+    HrDev D{};
+    smx_point3 *d_pts = nullptr;
+    smx_tri *d_tris = nullptr;
+    int r = B.get(&D.out, (size_t)n);
+    if (!r) r = B.get(&D.mb, (size_t)n);
+    if (!r) r = B.get(&D.owner, (size_t)n);
+    if (!r) r = B.get(&d_pts, (size_t)npts);
+    if (!r) r = B.get(&d_tris, (size_t)ntris);
+    if (!r) r = B.get(&D.zkeys, 2);
+    if (!r) r = B.get(&D.hbounds, 2);
+    if (r) return r;
+    if (npts) HIPCHK(hipMemcpyAsync(d_pts, pts, (size_t)npts * sizeof(smx_point3), hipMemcpyHostToDevice, j->stream));
+    if (ntris) HIPCHK(hipMemcpyAsync(d_tris, tris, (size_t)ntris * sizeof(smx_tri), hipMemcpyHostToDevice, j->stream));
+    D.g = j->d_gridpt3;
+    D.pts = d_pts;
+    D.tris = d_tris;
+    D.npts = npts;
+    D.ntris = ntris;
+    D.W = (long)L.grid_w;
+    D.H = (long)L.grid_h;
+    D.gres = L.grid_resolution;
+    D.b0 = L.boundary[0];
+    D.b1 = L.boundary[1];
+    D.resolution = L.resolution;
+    D.level = level;
+    D.buf = buf;
+    D.th_hg = th_hg;
+    D.ncc_th = ncc_th;
+    HIPCHK(launch_set_height_range(D, j->stream));
+    HIPCHK(hipMemcpyAsync(j->d_gridpt3, D.out, (size_t)n * sizeof(smx_ugrid), hipMemcpyDeviceToDevice, j->stream));
+    int hb[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(&j->h_small[0], D.hbounds, 2 * sizeof(int), hipMemcpyDeviceToHost, j->stream));
+    HIPCHK(hipStreamSynchronize(j->stream));
+    std::memcpy(hb, &j->h_small[0], sizeof hb);
+    if (minH_grid) *minH_grid = (double)hb[0];
+    if (maxH_grid) *maxH_grid = (double)hb[1];
+    // the returned grid keeps ortho_ncc[1] and zeroes ortho_ncc[0]
+    if (n > 0) j->max_ortho_ncc[0] = 0;
+    j->generation++;
+    return 0;
+}
+
 // ------------------------------------------------------------ graphs
 
 int smx_graph_destroy(smx_graph *g)

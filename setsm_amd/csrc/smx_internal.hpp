@@ -186,6 +186,40 @@ int fast_batch_descriptors(smx_job *const *jobs, int n, const CellArgs *As, cons
                            hipStream_t s);
 int launch_vll_fast_batch(smx_job *const *jobs, int n, const CellArgs *As, const VllScratch *Ss, int half,
                           void *desc, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1);
+// SelectMPs and SetHeightRange (mps.hip)
+constexpr int kRohBins = 100;   // SelectMPs' rohconvert
+struct MpsDev {
+    const smx_nccresult *ncc;
+    smx_ugrid *g;
+    const double *grid_pts;
+    long n;
+    int level, is_ra, sgm_py;
+    smx_mps_args a;
+    unsigned long long *hist;   // kRohBins bins, then the cell total
+    double *th;                 // minimum_Th; negative: undefined for this level
+    int64_t *sel;               // n + 1 selection flags
+    const int64_t *off;         // their exclusive scan
+    smx_point3 *pts;
+};
+hipError_t launch_select_mps(const MpsDev &D, const VllScratch &S, hipStream_t s);   // through the scan
+hipError_t launch_write_mps(const MpsDev &D, hipStream_t s);
+struct HrDev {
+    smx_ugrid *g;            // in: the grid SelectMPs left; after the apply pass: claimed cells updated
+    smx_ugrid *out;          // the returned grid
+    uint8_t *mb;             // m_bHeight: 0 unclaimed, 1 claimed (2 is decided in the last pass)
+    uint32_t *owner;         // first claiming triangle, ~0u if none
+    const smx_point3 *pts;
+    const smx_tri *tris;
+    long npts, ntris;
+    long W, H;
+    double gres, b0, b1;
+    double resolution;
+    int level;
+    double buf, th_hg, ncc_th;
+    unsigned long long *zkeys;   // [0] min, [1] max of the valid triangles' heights (ordered keys)
+    int *hbounds;                // [0] minH_grid, [1] maxH_grid
+};
+hipError_t launch_set_height_range(const HrDev &D, hipStream_t s);
 int fast_num_configs();
 int fast_config_desc(int i, char *buf, int n);
 int fast_config_index(const smx_job *j);

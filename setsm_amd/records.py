@@ -24,6 +24,26 @@ NCCRESULT_DTYPE = np.dtype([
 assert NCCRESULT_DTYPE.itemsize == 24
 
 
+# D3DPOINT (Typedefine.hpp:140-194) and UI3DPOINT (Typedefine.hpp:74-97)
+POINT3_DTYPE = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("flag", "u1"), ("_pad", "V7")])
+assert POINT3_DTYPE.itemsize == 32
+TRI_DTYPE = np.dtype([("a", "<u4"), ("b", "<u4"), ("c", "<u4")])
+assert TRI_DTYPE.itemsize == 12
+
+
+class SmxMpsArgs(C.Structure):
+    """smx_mps_args: SelectMPs' per-call arguments (setsm_code.cpp:8535)."""
+    _fields_ = [("th_roh", C.c_double), ("th_roh_min", C.c_double), ("th_roh_start", C.c_double),
+                ("th_roh_next", C.c_double), ("mpp", C.c_double), ("mpp_stereo_angle", C.c_double),
+                ("iteration", C.c_int32), ("final_level_iteration", C.c_int32)]
+
+
+class SmxHrArgs(C.Structure):
+    """smx_hr_args: SetHeightRange's per-call arguments (setsm_code.cpp:9851)."""
+    _fields_ = [("mpp", C.c_double), ("seed_dem_sigma", C.c_double), ("iteration", C.c_int32),
+                ("check_matching_rate", C.c_int32)]
+
+
 def voxel_flag(words):
     return (np.asarray(words, dtype=np.uint32) & 0xFF).astype(np.uint8)
 

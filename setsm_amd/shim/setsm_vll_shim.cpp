@@ -1,12 +1,14 @@
 // setsm_vll_shim.cpp — SETSM's InitializeVoxel / VerticalLineLocus / AWNCC
-// (setsm_code.hpp:75, 81, 91) with their exact C++ signatures and SETSM's own
-// types, running on the MI355X through libsmx (include/smx_vll.h).
+// (setsm_code.hpp:75, 81, 91) and the SelectMPs / SetHeightRange either side
+// of its TIN (setsm_code.hpp:99, 122) with their exact C++ signatures and
+// SETSM's own types, running on the MI355X through libsmx (include/smx_vll.h).
 //
 // Drop-in use inside SETSM's build: add this file, link -lsmx, and either
-//   * compile it with -DSMX_SHIM_GLOBAL_NAMES and remove the three original
+//   * compile it with -DSMX_SHIM_GLOBAL_NAMES and remove the original
 //     definitions from setsm_code.cpp, or
 //   * keep the originals and call smx_setsm::InitializeVoxel / ... at the
-//     three call sites in Matching_SETSM (setsm_code.cpp:3827, 3829, 3844).
+//     call sites in Matching_SETSM (setsm_code.cpp:3827, 3829, 3844, 3862,
+//     4192-4222).
 //
 // State: one smx job per thread mirrors SETSM's per-level state. A new job is
 // made when the level's image buffers, sizes or origins change; a changed
@@ -27,6 +29,7 @@
 // SMX_PRECISION=fp64 (read at every VerticalLineLocus call) selects the FP64
 // kernel, bit-identical to the CPU path; the default is the FAST kernel
 // (per-hypothesis mean NCC within 1e-5 of FP64).
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -39,6 +42,9 @@ static_assert(sizeof(UGRID) == sizeof(smx_ugrid), "UGRID layout");
 static_assert(sizeof(NCCresult) == sizeof(smx_nccresult), "NCCresult layout");
 static_assert(sizeof(VOXEL) == 4, "VOXEL layout");
 static_assert(sizeof(D2DPOINT) == 2 * sizeof(double), "D2DPOINT layout");
+static_assert(sizeof(D3DPOINT) == sizeof(smx_point3) && offsetof(D3DPOINT, flag) == offsetof(smx_point3, flag),
+              "D3DPOINT layout");
+static_assert(sizeof(UI3DPOINT) == sizeof(smx_tri), "UI3DPOINT layout");
 
 namespace smx_setsm {
 
@@ -53,6 +59,7 @@ struct State {
     smx_level lv;                   // the level record last passed to smx_job_set_level
     bool lv_set = false;
     bool ortho_set = false;         // ortho grid lon/lat uploaded for this level record
+    double minmax[2] = {0, 0};      // minmaxHeight of the level's last InitializeVoxel / VerticalLineLocus
 };
 
 thread_local State g_state;
@@ -260,6 +267,8 @@ void InitializeVoxel(const ProInfo *proinfo, VOXEL **grid_voxel, LevelInfo &plev
     const bool fresh = bind(proinfo, plevelinfo);
     State &s = g_state;
     const long n = *plevelinfo.Grid_length;
+    s.minmax[0] = minmaxHeight[0];
+    s.minmax[1] = minmaxHeight[1];
     std::vector<unsigned short> old_n((size_t)n);
     for (long c = 0; c < n; c++) old_n[c] = nccresult[c].NumOfHeight;
     check(smx_job_set_searchspace(s.job, (const smx_ugrid *)GridPT3, minmaxHeight), "smx_job_set_searchspace");
@@ -292,6 +301,8 @@ int VerticalLineLocus(VOXEL **grid_voxel, const ProInfo *proinfo, NCCresult *ncc
     const bool fresh = bind(proinfo, plevelinfo);
     State &s = g_state;
     const long n = *plevelinfo.Grid_length;
+    s.minmax[0] = minmaxHeight[0];
+    s.minmax[1] = minmaxHeight[1];
     check(smx_job_set_searchspace(s.job, (const smx_ugrid *)GridPT3, minmaxHeight), "smx_job_set_searchspace");
     check(smx_job_set_nccresult(s.job, (const smx_nccresult *)nccresult), "smx_job_set_nccresult");
     if (fresh) upload_rows(s, grid_voxel, nccresult, n);
@@ -324,6 +335,60 @@ void AWNCC(ProInfo *proinfo, VOXEL **grid_voxel, CSize Size_Grid2D, UGRID *GridP
     (void)proinfo;
 }
 
+long SelectMPs(const ProInfo *proinfo, LevelInfo &rlevelinfo, const NCCresult *roh_height, UGRID *GridPT3,
+               const double Th_roh, const double Th_roh_min, const double Th_roh_start, const double Th_roh_next,
+               const int iteration, const double MPP, const int final_level_iteration, const double MPP_stereo_angle,
+               std::vector<D3DPOINT> *linkedlist)
+{
+    bind(proinfo, rlevelinfo);
+    State &s = g_state;
+    const long n = *rlevelinfo.Grid_length;
+    check(smx_job_set_searchspace(s.job, (const smx_ugrid *)GridPT3, s.minmax), "smx_job_set_searchspace");
+    check(smx_job_set_nccresult(s.job, (const smx_nccresult *)roh_height), "smx_job_set_nccresult");
+    smx_mps_args a;
+    a.th_roh = Th_roh;
+    a.th_roh_min = Th_roh_min;
+    a.th_roh_start = Th_roh_start;
+    a.th_roh_next = Th_roh_next;
+    a.mpp = MPP;
+    a.mpp_stereo_angle = MPP_stereo_angle;
+    a.iteration = iteration;
+    a.final_level_iteration = final_level_iteration;
+    std::vector<smx_point3> pts((size_t)(n > 0 ? n : 1));
+    long count = 0;
+    double th = 0;
+    check(smx_select_mps(s.job, &a, pts.data(), (long)pts.size(), &count, &th), "smx_select_mps");
+    printf("minimum TH %f\n", th);
+    check(smx_job_get_searchspace(s.job, (smx_ugrid *)GridPT3), "smx_job_get_searchspace");
+    for (long i = 0; i < count; i++) linkedlist->push_back(D3DPOINT(pts[i].x, pts[i].y, pts[i].z, pts[i].flag));
+    return count;
+}
+
+// Like the reference, frees GridPT3 and returns the new grid (calloc'd).
+UGRID *SetHeightRange(ProInfo *proinfo, LevelInfo &rlevelinfo, NCCresult *nccresult, const int numOfPts,
+                      const int num_triangles, UGRID *GridPT3, const int iteration, double *minH_grid,
+                      double *maxH_grid, D3DPOINT *pts, const UI3DPOINT *tris, const double MPP,
+                      const bool level_check_matching_rate)
+{
+    (void)nccresult;
+    bind(proinfo, rlevelinfo);
+    State &s = g_state;
+    const long n = *rlevelinfo.Grid_length;
+    check(smx_job_set_searchspace(s.job, (const smx_ugrid *)GridPT3, s.minmax), "smx_job_set_searchspace");
+    smx_hr_args a;
+    a.mpp = MPP;
+    a.seed_dem_sigma = proinfo->seedDEMsigma;
+    a.iteration = iteration;
+    a.check_matching_rate = level_check_matching_rate ? 1 : 0;
+    check(smx_set_height_range(s.job, (const smx_point3 *)pts, numOfPts, (const smx_tri *)tris, num_triangles, &a,
+                               minH_grid, maxH_grid),
+          "smx_set_height_range");
+    UGRID *result = (UGRID *)calloc(sizeof(UGRID), (size_t)n);
+    check(smx_job_get_searchspace(s.job, (smx_ugrid *)result), "smx_job_get_searchspace");
+    free(GridPT3);
+    return result;
+}
+
 // Release the thread's job (optional; it is reused across calls).
 void Release()
 {
@@ -349,5 +414,21 @@ void AWNCC(ProInfo *proinfo, VOXEL **grid_voxel, CSize Size_Grid2D, UGRID *GridP
 {
     smx_setsm::AWNCC(proinfo, grid_voxel, Size_Grid2D, GridPT3, nccresult, step_height, Pyramid_step, iteration,
                      MaxNumberofHeightVoxel);
+}
+long SelectMPs(const ProInfo *proinfo, LevelInfo &rlevelinfo, const NCCresult *roh_height, UGRID *GridPT3,
+               const double Th_roh, const double Th_roh_min, const double Th_roh_start, const double Th_roh_next,
+               const int iteration, const double MPP, const int final_level_iteration, const double MPP_stereo_angle,
+               std::vector<D3DPOINT> *linkedlist)
+{
+    return smx_setsm::SelectMPs(proinfo, rlevelinfo, roh_height, GridPT3, Th_roh, Th_roh_min, Th_roh_start,
+                                Th_roh_next, iteration, MPP, final_level_iteration, MPP_stereo_angle, linkedlist);
+}
+UGRID *SetHeightRange(ProInfo *proinfo, LevelInfo &rlevelinfo, NCCresult *nccresult, const int numOfPts,
+                      const int num_triangles, UGRID *GridPT3, const int iteration, double *minH_grid,
+                      double *maxH_grid, D3DPOINT *pts, const UI3DPOINT *tris, const double MPP,
+                      const bool level_check_matching_rate)
+{
+    return smx_setsm::SetHeightRange(proinfo, rlevelinfo, nccresult, numOfPts, num_triangles, GridPT3, iteration,
+                                     minH_grid, maxH_grid, pts, tris, MPP, level_check_matching_rate);
 }
 #endif

@@ -18,7 +18,8 @@ import os
 
 import numpy as np
 
-from .records import NCCRESULT_DTYPE, UGRID_DTYPE, SmxLevel
+from .records import (NCCRESULT_DTYPE, POINT3_DTYPE, TRI_DTYPE, UGRID_DTYPE, SmxHrArgs, SmxLevel,
+                      SmxMpsArgs)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 # SMX_LIBRARY names another build of the same library (e.g. a kernel variant)
@@ -33,7 +34,7 @@ EXPORTS = [
     "smx_job_get_view", "smx_job_set_level", "smx_job_set_searchspace", "smx_job_get_searchspace",
     "smx_job_get_nccresult", "smx_job_set_nccresult", "smx_initialize_voxel", "smx_vertical_line_locus",
     "smx_awncc", "smx_job_num_hypotheses", "smx_job_get_voxels", "smx_job_get_voxel_rows", "smx_job_set_voxels",
-    "smx_job_get_sumcost", "smx_job_set_ortho_wgs",
+    "smx_job_get_sumcost", "smx_job_set_ortho_wgs", "smx_select_mps", "smx_set_height_range",
     "smx_job_last_kernel_ms", "smx_job_last_evaluated", "smx_job_last_redo", "smx_job_sync", "smx_fast_num_configs",
     "smx_job_set_fast_config", "smx_fast_config_desc", "smx_job_last_stats", "smx_vertical_line_locus_batch",
     "smx_vertical_line_locus_noalloc", "smx_release_cached", "smx_job_trim", "smx_job_half", "smx_graph_create", "smx_graph_launch", "smx_graph_last_ms", "smx_graph_destroy",
@@ -84,6 +85,9 @@ def lib():
             "smx_job_set_voxels": (i, [vp, vp]),
             "smx_job_get_sumcost": (i, [vp, vp]),
             "smx_job_set_ortho_wgs": (i, [vp, vp, C.c_uint32, C.c_uint32]),
+            "smx_select_mps": (i, [vp, vp, vp, C.c_long, C.POINTER(C.c_long), C.POINTER(C.c_double)]),
+            "smx_set_height_range": (i, [vp, vp, C.c_long, vp, C.c_long, vp, C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double)]),
             "smx_job_last_kernel_ms": (C.c_double, [vp]),
             "smx_job_last_evaluated": (ll, [vp]),
             "smx_job_last_redo": (i, [vp, vp, vp, vp]),
@@ -229,6 +233,34 @@ class Job:
 
     def awncc(self, iteration: int):
         _check(lib().smx_awncc(self.h, int(iteration)))
+
+    # ------------------------------------------------- between iterations
+    def select_mps(self, iteration: int, th_roh: float, th_roh_min: float, th_roh_start: float,
+                   th_roh_next: float, mpp: float, mpp_stereo_angle: float, final_level_iteration: int):
+        """SelectMPs (setsm_code.cpp:8535-8889) on the resident NCCresult and
+        GridPT3: returns (matched points as POINT3_DTYPE in cell order, the
+        level's minimum peak threshold). GridPT3 is updated in place."""
+        a = SmxMpsArgs(th_roh, th_roh_min, th_roh_start, th_roh_next, mpp, mpp_stereo_angle, int(iteration),
+                       int(final_level_iteration))
+        pts = np.zeros(max(self.ncells, 1), POINT3_DTYPE)
+        count, th = C.c_long(0), C.c_double(0)
+        _check(lib().smx_select_mps(self.h, C.byref(a), _p(pts), C.c_long(len(pts)), C.byref(count),
+                                    C.byref(th)))
+        return pts[: count.value].copy(), th.value
+
+    def set_height_range(self, pts, tris, iteration: int, mpp: float, check_matching_rate: bool = False,
+                         seed_dem_sigma: float = 0.0):
+        """SetHeightRange (setsm_code.cpp:9851-10261): the TIN's triangles over
+        the matched points set the next search space (GridPT3 is replaced).
+        Returns (minH_grid, maxH_grid)."""
+        pts = np.ascontiguousarray(pts, POINT3_DTYPE)
+        tris = np.ascontiguousarray(tris, TRI_DTYPE)
+        a = SmxHrArgs(mpp, seed_dem_sigma, int(iteration), int(bool(check_matching_rate)))
+        lo, hi = C.c_double(0), C.c_double(0)
+        _check(lib().smx_set_height_range(self.h, _p(pts) if len(pts) else None, C.c_long(len(pts)),
+                                          _p(tris) if len(tris) else None, C.c_long(len(tris)), C.byref(a),
+                                          C.byref(lo), C.byref(hi)))
+        return lo.value, hi.value
 
     # ------------------------------------------------------------- cube
     @property

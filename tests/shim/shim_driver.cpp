@@ -2,9 +2,10 @@
 // ProInfo and LevelInfo (Typedefine.hpp:351-419, 604-643) from an smx_level
 // and drives one pyramid level the way Matching_SETSM does
 // (setsm_code.cpp:3827-3844): InitializeVoxel, VerticalLineLocus, AWNCC per
-// iteration, on host arrays (VOXEL **grid_voxel, NCCresult *, UGRID *).
-// The three functions are called through their reference signatures
-// (setsm_code.hpp:75, 81, 91), which libsmx_setsm.so exports.
+// iteration, on host arrays (VOXEL **grid_voxel, NCCresult *, UGRID *), and
+// SelectMPs / SetHeightRange between iterations (setsm_code.cpp:3862,
+// 4192-4222). The functions are called through their reference signatures
+// (setsm_code.hpp:75, 81, 91, 99, 122), which libsmx_setsm.so exports.
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -36,6 +37,14 @@ int VerticalLineLocus(VOXEL **grid_voxel, const ProInfo *proinfo, NCCresult *ncc
                       const UGRID *GridPT3, const uint8 iteration, const double *minmaxHeight);
 void AWNCC(ProInfo *proinfo, VOXEL **grid_voxel, CSize Size_Grid2D, UGRID *GridPT3, NCCresult *nccresult,
            double step_height, uint8 Pyramid_step, uint8 iteration, int MaxNumberofHeightVoxel);
+long SelectMPs(const ProInfo *proinfo, LevelInfo &rlevelinfo, const NCCresult *roh_height, UGRID *GridPT3,
+               const double Th_roh, const double Th_roh_min, const double Th_roh_start, const double Th_roh_next,
+               const int iteration, const double MPP, const int final_level_iteration, const double MPP_stereo_angle,
+               std::vector<D3DPOINT> *linkedlist);
+UGRID *SetHeightRange(ProInfo *proinfo, LevelInfo &rlevelinfo, NCCresult *nccresult, const int numOfPts,
+                      const int num_triangles, UGRID *GridPT3, const int iteration, double *minH_grid,
+                      double *maxH_grid, D3DPOINT *pts, const UI3DPOINT *tris, const double MPP,
+                      const bool level_check_matching_rate);
 
 namespace {
 
@@ -201,6 +210,40 @@ void drv_get(void *h, uint32_t *words, smx_nccresult *ncc_vll, smx_nccresult *nc
         for (int i = 0; i < d->ncc[c].NumOfHeight; i++) memcpy(&words[k++], &d->vox[c][i], 4);
     memcpy(ncc_vll, d->ncc_vll.data(), d->ncc.size() * sizeof(NCCresult));
     memcpy(ncc, d->ncc.data(), d->ncc.size() * sizeof(NCCresult));
+}
+
+// SelectMPs after the last AWNCC: ugrid updated in place, points (in the
+// order SETSM's vector receives them) into out; returns their count.
+long drv_select_mps(void *h, smx_ugrid *ugrid, double th_roh, double th_roh_min, double th_roh_start,
+                    double th_roh_next, int iteration, double mpp, int final_level_iteration,
+                    double mpp_stereo_angle, smx_point3 *out)
+{
+    Driver *d = (Driver *)h;
+    std::vector<D3DPOINT> pts;
+    const long n = SelectMPs(&d->pro, d->L, d->ncc.data(), (UGRID *)ugrid, th_roh, th_roh_min, th_roh_start,
+                             th_roh_next, iteration, mpp, final_level_iteration, mpp_stereo_angle, &pts);
+    for (size_t i = 0; i < pts.size(); i++) {
+        memset(&out[i], 0, sizeof out[i]);
+        out[i].x = pts[i].m_X;
+        out[i].y = pts[i].m_Y;
+        out[i].z = pts[i].m_Z;
+        out[i].flag = pts[i].flag;
+    }
+    return n == (long)pts.size() ? n : -1;
+}
+
+// SetHeightRange on a malloc'd copy of ugrid (the function frees its input,
+// as SETSM's does); the returned grid is copied back into ugrid.
+void drv_set_height_range(void *h, smx_ugrid *ugrid, smx_point3 *pts, long npts, const smx_tri *tris, long ntris,
+                          int iteration, double mpp, int check_matching_rate, double *lo, double *hi)
+{
+    Driver *d = (Driver *)h;
+    UGRID *g = (UGRID *)malloc(sizeof(UGRID) * (size_t)d->grid_length);
+    memcpy(g, ugrid, sizeof(UGRID) * (size_t)d->grid_length);
+    UGRID *r = SetHeightRange(&d->pro, d->L, d->ncc.data(), (int)npts, (int)ntris, g, iteration, lo, hi,
+                              (D3DPOINT *)pts, (const UI3DPOINT *)tris, mpp, check_matching_rate != 0);
+    memcpy(ugrid, r, sizeof(UGRID) * (size_t)d->grid_length);
+    free(r);
 }
 
 void drv_destroy(void *h)

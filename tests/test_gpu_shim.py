@@ -3,7 +3,8 @@ oracle: a C++ driver fills SETSM's own ProInfo / LevelInfo and host arrays
 (VOXEL **grid_voxel, NCCresult *, UGRID *) and calls InitializeVoxel,
 VerticalLineLocus and AWNCC through the reference signatures
 (setsm_code.hpp:75, 81, 91), iteration by iteration as Matching_SETSM does
-(setsm_code.cpp:3827-3844).
+(setsm_code.cpp:3827-3844), and SelectMPs / SetHeightRange
+(setsm_code.hpp:99, 122) between iterations.
 
 The shim and its driver compile against SETSM's Typedefine.hpp, so they exist
 only where SETSM's sources were present at build time; without them these
@@ -41,6 +42,11 @@ def _driver():
     d.drv_destroy.argtypes = [vp]
     d.drv_set_projection.restype = None
     d.drv_set_projection.argtypes = [C.c_double] * 4
+    d.drv_select_mps.restype = C.c_long
+    d.drv_select_mps.argtypes = [vp, vp] + [C.c_double] * 4 + [C.c_int, C.c_double, C.c_int, C.c_double, vp]
+    d.drv_set_height_range.restype = None
+    d.drv_set_height_range.argtypes = [vp, vp, vp, C.c_long, vp, C.c_long, C.c_int, C.c_double, C.c_int,
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
     return d
 
 
@@ -166,3 +172,52 @@ def test_shim_gncc_fp64(require_gpu, monkeypatch):
         _check(ro, rs, f"shim GNCC it{it}", exact=True)
         ug = ro["ug"]
     s.close()
+
+
+def _same_grid(a, b):
+    for f in ("Height", "minHeight", "maxHeight", "roh", "ortho_ncc", "Mean_ortho_ncc", "Matched_flag",
+              "anchor_flag"):
+        x, y = (a[f].view(np.uint32), b[f].view(np.uint32)) if f == "Height" else (a[f], b[f])
+        assert np.array_equal(x, y), f
+
+
+def test_shim_select_mps_and_set_height_range(require_gpu, monkeypatch):
+    """SelectMPs and SetHeightRange through the reference signatures, on the
+    host arrays AWNCC left: the point vector, GridPT3 and the returned grid
+    (the input freed, as SETSM's does) equal the oracle's; the TIN between
+    them is the reference's own."""
+    from oracle import oracle_py as O
+    from setsm_amd.records import POINT3_DTYPE, LevelInputs
+    if O.tin() is None:
+        pytest.skip("oracle/_ref/libref_tin.so not built")
+    monkeypatch.setenv("SMX_PRECISION", "fp64")
+    drv = _driver()
+    sc, pyr = scene(image_px=1024, relief=120.0, seed=5, levels=5)
+    level = 2
+    g = level_grid(sc, level, 48)
+    li = LevelInputs(pyr, level, sc.rpcs, g, grid_resolution=g["grid_res"])
+    s = ShimRun(drv, li, 0)
+    try:
+        ug = g["ugrid"]
+        for it in (1, 2):
+            r = s.step(ug, it, g["minmax"])
+            ug_s = np.ascontiguousarray(r["ug"].copy())
+            ug_o = r["ug"].copy()
+            out = np.zeros(len(ug_s), POINT3_DTYPE)
+            args = (0.6, 0.35, 0.6, 0.5)
+            n = drv.drv_select_mps(s.h, ug_s.ctypes.data, *args, it, 1.0, 1, 10.0, out.ctypes.data)
+            pts_o, _ = O.select_mps(li, r["ncc"], ug_o, it, *args, 1.0, 10.0, 1)
+            assert n == len(pts_o) and n > 10
+            for f in ("x", "y", "z", "flag"):
+                assert np.array_equal(out[:n][f], pts_o[f]), f
+            _same_grid(ug_s, ug_o)
+            tris = O.tin_create(pts_o, g["boundary"], g["grid_res"])
+            lo, hi = C.c_double(0), C.c_double(0)
+            drv.drv_set_height_range(s.h, ug_s.ctypes.data, out.ctypes.data, n, tris.ctypes.data, len(tris), it,
+                                     1.0, 0, C.byref(lo), C.byref(hi))
+            assert (lo.value, hi.value) == O.set_height_range(li, ug_o, pts_o, tris, it, 1.0)
+            _same_grid(ug_s, ug_o)
+            print(f"shim level {level} iteration {it}: {n} points, {len(tris)} triangles", flush=True)
+            ug = ug_s
+    finally:
+        s.close()
