@@ -218,6 +218,24 @@ int  smx_select_mps(smx_job *job, const smx_mps_args *args, smx_point3 *pts, lon
 int  smx_set_height_range(smx_job *job, const smx_point3 *pts, long npts, const smx_tri *tris, long ntris,
                           const smx_hr_args *args, double *minH_grid, double *maxH_grid);
 
+/* DecisionMPs' blunder loop (setsm_code.cpp:8891-9093) calls these two
+ * between its TIN updates; the loop and blunder_detection_TIN stay host code. */
+/* SetHeightRange_blunder (setsm_code.cpp:10393-10491, setsm_code.hpp:120):
+ * each cell's GridPT3.Height becomes the highest TIN height of the triangles
+ * containing it; cells no triangle contains keep theirs. */
+int  smx_set_height_range_blunder(smx_job *job, const smx_point3 *pts, long npts, const smx_tri *tris, long ntris);
+/* The ortho grid of VerticalLineLocus_blunder: blunder_selected_level (the
+ * job must hold both images' views at that level) and the (lon, lat) of
+ * every ortho pixel at resolution * 2^level over the level's Boundary
+ * (w = (Boundary[2]-Boundary[0])/res + 1 columns), as ps2wgs_single gives
+ * them. Kept until the next smx_job_set_level. */
+int  smx_job_set_blunder_level(smx_job *job, int blunder_level, const double *lonlat, uint32_t w, uint32_t h);
+/* VerticalLineLocus_blunder (setsm_code.cpp:7882-8013): every cell's ortho
+ * NCC at its GridPT3 Height on the blunder level's images, FP64 with the
+ * reference's operation order. Writes GridPT3.ortho_ncc[1] and
+ * Mean_ortho_ncc, and (if not NULL) the per-cell value to ortho_ncc. */
+int  smx_vertical_line_locus_blunder(smx_job *job, int iteration, int bblunder, float *ortho_ncc);
+
 /* Number of hypotheses in the cube (sum of NumOfHeight). */
 long long smx_job_num_hypotheses(const smx_job *job);
 /* Download the cube in CSR order: words[total], rho[total] (rho may be NULL,

@@ -65,6 +65,12 @@ def lib():
         L.orc_select_mps.argtypes = [vp, vp, vp, vp, vp, dp]
         L.orc_set_height_range.restype = ip
         L.orc_set_height_range.argtypes = [vp, vp, vp, C.c_long, vp, C.c_long, vp, dp, dp]
+        L.orc_set_height_range_blunder.restype = None
+        L.orc_set_height_range_blunder.argtypes = [vp, vp, C.c_long, vp, C.c_long, vp]
+        L.orc_blunder_half_template.restype = ip
+        L.orc_blunder_half_template.argtypes = [vp, ip, ip, ip]
+        L.orc_vll_blunder.restype = ip
+        L.orc_vll_blunder.argtypes = [vp, vp, ip, vp, ip, ip, vp, vp]
         L.orc_set_threads.argtypes = [ip]
         L.orc_get_threads.restype = ip
         _lib = L
@@ -155,6 +161,45 @@ def set_height_range(level_inputs, ugrid, pts, tris, iteration, mpp, check_match
     if r < 0:
         raise MemoryError("oracle SetHeightRange ran out of memory")
     return lo.value, hi.value
+
+
+def set_height_range_blunder(level_inputs, ugrid, pts, tris):
+    """SetHeightRange_blunder (setsm_code.cpp:10393-10491): ugrid's Height in place."""
+    pts = np.ascontiguousarray(pts, POINT3_DTYPE)
+    tris = np.ascontiguousarray(tris, TRI_DTYPE)
+    assert ugrid.dtype == UGRID_DTYPE and ugrid.flags.c_contiguous
+    lib().orc_set_height_range_blunder(C.addressof(level_inputs.level), _p(pts) if len(pts) else None, len(pts),
+                                       _p(tris) if len(tris) else None, len(tris), _p(ugrid))
+
+
+def blunder_half_template(level_inputs, blunder_level, iteration, bblunder):
+    return lib().orc_blunder_half_template(C.addressof(level_inputs.level), int(blunder_level), int(iteration),
+                                           int(bool(bblunder)))
+
+
+def vll_blunder(level_inputs, pyr, blunder_level, ortho_wgs, ugrid, iteration, bblunder, start=((0.0, 0.0), (0.0, 0.0))):
+    """VerticalLineLocus_blunder (setsm_code.cpp:7882-8013) on the pyramid
+    level blunder_level of pyr (per image (img, mag, ori) levels): returns the
+    per-cell ortho NCC; ugrid's ortho_ncc[1] / Mean_ortho_ncc in place."""
+    from setsm_amd.records import SmxView
+    views = (SmxView * 2)()
+    keep = []
+    for k in range(2):
+        img, mag, ori = (np.ascontiguousarray(a) for a in pyr[k][blunder_level])
+        keep += [img, mag, ori]
+        v = views[k]
+        v.img, v.mag, v.ori = img.ctypes.data, mag.ctypes.data, ori.ctypes.data
+        v.height, v.width = img.shape
+        v.start_x = start[k][0] / (1 << blunder_level)
+        v.start_y = start[k][1] / (1 << blunder_level)
+    ow = np.ascontiguousarray(ortho_wgs, np.float64)
+    assert ugrid.dtype == UGRID_DTYPE and ugrid.flags.c_contiguous
+    out = np.zeros(len(ugrid), np.float32)
+    r = lib().orc_vll_blunder(C.addressof(level_inputs.level), views, int(blunder_level), _p(ow), int(iteration),
+                              int(bool(bblunder)), _p(ugrid), _p(out))
+    if r < 0:
+        raise MemoryError("oracle VerticalLineLocus_blunder ran out of memory")
+    return out
 
 
 def _p(a):

@@ -1748,3 +1748,123 @@ int orc_set_height_range(const smx_level *L, const smx_hr_args *a, const smx_poi
     free(snap);
     return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * DecisionMPs' helpers: SetHeightRange_blunder, VerticalLineLocus_blunder
+ * ---------------------------------------------------------------------- */
+
+void orc_set_height_range_blunder(const smx_level *L, const smx_point3 *pts, long npts, const smx_tri *tris,
+                                  long ntris, smx_ugrid *g)
+{
+    /* setsm_code.cpp:10393-10491: every cell takes the highest TIN height of
+     * the triangles containing it (a float max; untouched cells keep theirs) */
+    const long W = (long)L->grid_w, H = (long)L->grid_h, n = W * H;
+    const double gres = L->grid_resolution;
+    float *hts = (float *)malloc((size_t)(n > 0 ? n : 1) * sizeof(float));
+    if (!hts) return;
+    for (long c = 0; c < n; c++) hts[c] = (float)-1000.0;
+    for (long t = 0; t < ntris; t++) {
+        if (tris[t].a >= (uint32_t)npts || tris[t].b >= (uint32_t)npts || tris[t].c >= (uint32_t)npts) continue;
+        const smx_point3 *p1 = &pts[tris[t].a], *p2 = &pts[tris[t].b], *p3 = &pts[tris[t].c];
+        const double bx0 = smin(smin(p1->x, p2->x), p3->x), by0 = smin(smin(p1->y, p2->y), p3->y);
+        const double bx1 = smax(smax(p1->x, p2->x), p3->x), by1 = smax(smax(p1->y, p2->y), p3->y);
+        int c0 = (int)((bx0 - L->boundary[0]) / gres + 0.5) - 1, r0 = (int)((by0 - L->boundary[1]) / gres + 0.5) - 1;
+        int c1 = (int)((bx1 - L->boundary[0]) / gres + 0.5) + 1, r1 = (int)((by1 - L->boundary[1]) / gres + 0.5) + 1;
+        if (c1 >= (int)W) c1 = (int)W - 1;
+        if (r1 >= (int)H) r1 = (int)H - 1;
+        if (c0 < 0) c0 = 0;
+        if (r0 < 0) r0 = 0;
+        for (long row = r0; row <= r1; row++)
+            for (long col = c0; col <= c1; col++) {
+                float z = -1000.0f;
+                if (tin_inside(col * gres + L->boundary[0], row * gres + L->boundary[1], p1, p2, p3, &z) &&
+                    z > hts[W * row + col])
+                    hts[W * row + col] = z;
+            }
+    }
+    for (long c = 0; c < n; c++)
+        if (hts[c] != -1000.0) g[c].Height = hts[c];
+    free(hts);
+}
+
+int orc_blunder_half_template(const smx_level *L, int blunder_level, int iteration, int bblunder)
+{
+    /* setsm_code.cpp:7884-7911 */
+    int T = (uint8_t)L->template_size;
+    const int tT = (int)((5.0 / (L->resolution * (1 << blunder_level))) / 2.0) * 2 + 1;
+    if (T < tT) T = (uint8_t)tT;
+    int half = (int)(T / 2.0);
+    if (bblunder) {
+        if (L->level == 4) half = T - iteration * 2;
+        else if (L->level == 3) half = T - 2 - iteration * 2;
+        if (half < (int)(T / 2.0)) half = (int)(T / 2.0);
+    }
+    return half;
+}
+
+int orc_vll_blunder(const smx_level *L, const smx_view bview[2], int blunder_level, const double *ortho_wgs,
+                    int iteration, int bblunder, smx_ugrid *g, float *out)
+{
+    /* setsm_code.cpp:7882-8013 VerticalLineLocus_blunder, two images */
+    const int half = orc_blunder_half_template(L, blunder_level, iteration, bblunder);
+    const double im_res = L->resolution * (1 << blunder_level);
+    const long sw = (long)((L->boundary[2] - L->boundary[0]) / im_res) + 1;
+    const long sh = (long)((L->boundary[3] - L->boundary[1]) / im_res) + 1;
+    const long W = (long)L->grid_w, Hg = (long)L->grid_h, n = W * Hg;
+    double *cd[2];
+    for (int k = 0; k < 2; k++) {
+        cd[k] = (double *)calloc((size_t)(sw * sh) * 2, sizeof(double));
+        if (!cd[k]) return -1;
+    }
+    /* SetOrthoImageCoord (setsm_code.cpp:6550-6670) at the blunder level,
+     * check_pyimage = BD: positions in the level's pyramid from BStartpos */
+    smx_level Lb = *L;
+    Lb.view[0] = bview[0];
+    Lb.view[1] = bview[1];
+    Lb.level = blunder_level;
+    Lb.ortho_wgs = ortho_wgs;
+    set_ortho_image_coord(&Lb, g, 0, im_res, 0, sw, sh, 0, 0, cd, NULL);
+    /* the target image of SETSM's loop (ti = 1) is the reference image when
+     * reference_id is 1 */
+    const int right = L->reference_id == 1 ? 0 : 1;
+#pragma omp parallel num_threads(orc_get_threads())
+    {
+        patchvec pv;
+        pv_init(&pv, half);
+#pragma omp for schedule(guided)
+        for (long c = 0; c < n; c++) {
+            out[c] = -1.0f;
+            double max_ncc = -100;
+            int Count_N[3] = {0, 0, 0};
+            double count_GNCC = 0, t = 0;
+            for (int row = -half; row <= half; row++)
+                for (int col = -half; col <= half; col++) {
+                    const int radius2 = row * row + col * col;
+                    if (radius2 > (half - 1) * (half - 1)) continue;
+                    const double t_X = L->grid_pts[2 * c] + col * im_res;
+                    const double t_Y = L->grid_pts[2 * c + 1] + row * im_res;
+                    const long t_col = (long)((t_X - L->boundary[0]) / im_res);
+                    const long t_row = (long)((t_Y - L->boundary[1]) / im_res);
+                    const long pit = t_row * sw + t_col;
+                    const long tt_col = (long)((t_X - L->boundary[0]) / L->grid_resolution);
+                    const long tt_row = (long)((t_Y - L->boundary[1]) / L->grid_resolution);
+                    const long pdem = tt_row * W + tt_col;
+                    if (pit >= 0 && pit < sw * sh && t_col >= 0 && t_col < sw && t_row >= 0 && t_row < sh &&
+                        pdem >= 0 && pdem < n && tt_col >= 0 && tt_col < W && tt_row >= 0 && tt_row < Hg &&
+                        g[pdem].Height != -1000)
+                        set_vec_kernel_value(&pv, half, &bview[0], &bview[right], row, col, cd[0][2 * pit],
+                                             cd[0][2 * pit + 1], cd[right][2 * pit], cd[right][2 * pit + 1], radius2,
+                                             Count_N);
+                }
+            compute_multi_ncc(&pv, Count_N, &count_GNCC, &t);
+            if (max_ncc < t) max_ncc = t;
+            g[c].ortho_ncc[1] = q_result(t);
+            g[c].Mean_ortho_ncc = q_result(max_ncc);
+            out[c] = (float)max_ncc;
+        }
+        pv_free(&pv);
+    }
+    free(cd[0]);
+    free(cd[1]);
+    return 0;
+}

@@ -103,6 +103,20 @@ int orc_set_height_range(const smx_level *L, const smx_hr_args *a, const smx_poi
                          const smx_tri *tris, long ntris, smx_ugrid *gridpt3, double *minH_grid,
                          double *maxH_grid);
 
+/* setsm_code.cpp:10393-10491 SetHeightRange_blunder: each cell's Height
+ * becomes the highest TIN height of the triangles containing it. */
+void orc_set_height_range_blunder(const smx_level *L, const smx_point3 *pts, long npts, const smx_tri *tris,
+                                  long ntris, smx_ugrid *gridpt3);
+/* setsm_code.cpp:7884-7911: VerticalLineLocus_blunder's Half_template_size */
+int  orc_blunder_half_template(const smx_level *L, int blunder_level, int iteration, int bblunder);
+/* setsm_code.cpp:7882-8013 VerticalLineLocus_blunder: the ortho NCC of every
+ * cell at its GridPT3 Height on the blunder level's images bview[0] / [1]
+ * (reference / target, starts at that level), the ortho grid at
+ * resolution * 2^blunder_level with its pixels' lon/lat in ortho_wgs. Writes
+ * GridPT3.ortho_ncc[1], Mean_ortho_ncc and out[cell]. Returns 0 or -1. */
+int  orc_vll_blunder(const smx_level *L, const smx_view bview[2], int blunder_level, const double *ortho_wgs,
+                     int iteration, int bblunder, smx_ugrid *gridpt3, float *out);
+
 /* Pyramid inputs. Template.hpp:45-132 (9x9, sigma 1.5 decimation),
  * SubFunctions.cpp:2418-2470 (Sobel), SubFunctions.cpp:2473-2571 (Orientation). */
 void orc_gauss_pyramid(const uint16_t *in, uint32_t w, uint32_t h, uint16_t *out);

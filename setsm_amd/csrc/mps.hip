@@ -14,6 +14,9 @@
 // decision and the grid update, and an exclusive scan of the selection flags
 // so the points come out in cell order, as the reference appends them.
 //
+// SetHeightRange_blunder, inside DecisionMPs' blunder loop, gives each cell
+// the highest TIN height of the triangles containing it (a CAS float max).
+//
 // SetHeightRange: the reference walks the triangles in list order and lets
 // the first triangle that contains a cell claim it. Here the claim is an
 // atomicMin of the triangle index per cell (16 lanes per triangle over its
@@ -22,6 +25,7 @@
 // snapshot of their neighbours and writes the returned grid.
 
 #include <algorithm>
+#include <cstring>
 
 #include "smx_internal.hpp"
 
@@ -450,6 +454,63 @@ hipError_t launch_set_height_range(const HrDev &D, hipStream_t s)
     const unsigned cb = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_hr_apply, dim3(cb), dim3(256), 0, s, D);
     hipLaunchKernelGGL(k_hr_fill, dim3(cb), dim3(256), 0, s, D);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ SetHeightRange_blunder
+
+// std::atomic<float> max as the reference's update_max does it: replace only
+// while the new value is larger (a NaN never is).
+__device__ __forceinline__ void float_max(float *a, float v)
+{
+    unsigned int *ua = (unsigned int *)a;
+    unsigned int cur = *ua;
+    while (v > __uint_as_float(cur)) {
+        const unsigned int seen = atomicCAS(ua, cur, __float_as_uint(v));
+        if (seen == cur) break;
+        cur = seen;
+    }
+}
+
+// setsm_code.cpp:10404-10483: 16 lanes per triangle over its bounding box,
+// each inside cell keeps the highest TIN height.
+__global__ void __launch_bounds__(256) k_hrb_claim(HrDev D, float *heights)
+{
+    const long t = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int sub = threadIdx.x & 15;
+    if (t >= D.ntris) return;
+    TriGeom T;
+    if (!tri_geom(D, t, T)) return;
+    const long bw = (long)(T.c1 - T.c0 + 1), bh = (long)(T.r1 - T.r0 + 1);
+    if (bw <= 0 || bh <= 0) return;
+    for (long k = sub; k < bw * bh; k += 16) {
+        const long row = T.r0 + k / bw, col = T.c0 + k % bw;
+        float z = -1000.0f;
+        if (tin_inside(col * D.gres + D.b0, row * D.gres + D.b1, *T.p1, *T.p2, *T.p3, z))
+            float_max(&heights[D.W * row + col], z);
+    }
+}
+
+// setsm_code.cpp:10484-10489
+__global__ void __launch_bounds__(256) k_hrb_apply(HrDev D, const float *heights)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= D.W * D.H) return;
+    const float h = heights[c];
+    if (h != -1000.0) D.g[c].Height = h;
+}
+
+hipError_t launch_set_height_range_blunder(const HrDev &D, float *heights, hipStream_t s)
+{
+    const long n = D.W * D.H;
+    const float unset = -1000.0f;
+    uint32_t bits;
+    memcpy(&bits, &unset, 4);
+    hipError_t e = launch_fill_u32((uint32_t *)heights, bits, n, s);
+    if (e != hipSuccess) return e;
+    if (D.ntris > 0)
+        hipLaunchKernelGGL(k_hrb_claim, dim3((unsigned)((D.ntris * 16 + 255) / 256)), dim3(256), 0, s, D, heights);
+    hipLaunchKernelGGL(k_hrb_apply, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, D, (const float *)heights);
     return hipGetLastError();
 }
 

@@ -186,6 +186,9 @@ static void free_level_state(smx_job *j)
     jfree(j, j->d_cd);
     jfree(j, j->d_cdn);
     jfree(j, j->d_gval);
+    jfree(j, j->d_bortho_wgs);
+    j->blunder_level = -1;
+    j->bortho_w = j->bortho_h = 0;
     jfree(j, j->d_peak_rho);
     j->peak_cap = 0;
     j->cd_bytes = j->cdn_bytes = 0;
@@ -1453,6 +1456,123 @@ int smx_set_height_range(smx_job *j, const smx_point3 *pts, long npts, const smx
     // the returned grid keeps ortho_ncc[1] and zeroes ortho_ncc[0]
     if (n > 0) j->max_ortho_ncc[0] = 0;
     j->generation++;
+    return 0;
+}
+
+int smx_set_height_range_blunder(smx_job *j, const smx_point3 *pts, long npts, const smx_tri *tris, long ntris)
+{
+    if (!j || (npts > 0 && !pts) || (ntris > 0 && !tris) || npts < 0 || ntris < 0)
+        return fail(SMX_E_ARG, "smx_set_height_range_blunder: bad argument");
+    if (!j->level_set || !j->searchspace_set) return fail(SMX_E_STATE, "level and search space must be set");
+    if (npts > 0xffffffffL) return fail(SMX_E_ARG, "too many points");
+    HIPCHK(hipSetDevice(j->device));
+    const long n = j->ncells;
+    CallBuffers B(j);
+    HrDev D{};
+    smx_point3 *d_pts = nullptr;
+    smx_tri *d_tris = nullptr;
+    float *d_h = nullptr;
+    int r = B.get(&d_h, (size_t)n);
+    if (!r) r = B.get(&d_pts, (size_t)npts);
+    if (!r) r = B.get(&d_tris, (size_t)ntris);
+    if (r) return r;
+    if (npts) HIPCHK(hipMemcpyAsync(d_pts, pts, (size_t)npts * sizeof(smx_point3), hipMemcpyHostToDevice, j->stream));
+    if (ntris) HIPCHK(hipMemcpyAsync(d_tris, tris, (size_t)ntris * sizeof(smx_tri), hipMemcpyHostToDevice, j->stream));
+    D.g = j->d_gridpt3;
+    D.pts = d_pts;
+    D.tris = d_tris;
+    D.npts = npts;
+    D.ntris = ntris;
+    D.W = (long)j->lv.grid_w;
+    D.H = (long)j->lv.grid_h;
+    D.gres = j->lv.grid_resolution;
+    D.b0 = j->lv.boundary[0];
+    D.b1 = j->lv.boundary[1];
+    HIPCHK(launch_set_height_range_blunder(D, d_h, j->stream));
+    HIPCHK(hipStreamSynchronize(j->stream));
+    j->generation++;
+    return 0;
+}
+
+int smx_job_set_blunder_level(smx_job *j, int blunder_level, const double *lonlat, uint32_t w, uint32_t h)
+{
+    if (!j || !lonlat) return fail(SMX_E_ARG, "smx_job_set_blunder_level: null argument");
+    if (!j->level_set) return fail(SMX_E_STATE, "no level set");
+    if (blunder_level < 0 || blunder_level >= j->num_levels) return fail(SMX_E_ARG, "blunder level out of range");
+    for (int ti = 0; ti < 2; ti++)
+        if (!j->views[ti][blunder_level].texel)
+            return fail(SMX_E_STATE, "image %d level %d not uploaded", ti, blunder_level);
+    const smx_level &p = j->lv;
+    // setsm_code.cpp:7916 and 6552-6553: the ortho grid at resolution * 2^level
+    const double im_res = p.resolution * (double)(1 << blunder_level);
+    const long sw = (long)((p.boundary[2] - p.boundary[0]) / im_res) + 1;
+    const long sh = (long)((p.boundary[3] - p.boundary[1]) / im_res) + 1;
+    if (sw != (long)w || sh != (long)h)
+        return fail(SMX_E_ARG, "blunder ortho grid is %ldx%ld at level %d, got %ux%u", sw, sh, blunder_level, w, h);
+    HIPCHK(hipSetDevice(j->device));
+    jfree(j, j->d_bortho_wgs);
+    j->blunder_level = -1;
+    int r = jalloc(j, &j->d_bortho_wgs, (size_t)sw * sh * 2);
+    if (r) return r;
+    HIPCHK(hipMemcpyAsync(j->d_bortho_wgs, lonlat, (size_t)sw * sh * 16, hipMemcpyHostToDevice, j->stream));
+    HIPCHK(hipStreamSynchronize(j->stream));
+    j->blunder_level = blunder_level;
+    j->bortho_w = w;
+    j->bortho_h = h;
+    return 0;
+}
+
+int smx_vertical_line_locus_blunder(smx_job *j, int iteration, int bblunder, float *ortho_ncc)
+{
+    if (!j) return fail(SMX_E_ARG, "smx_vertical_line_locus_blunder: null job");
+    if (!j->level_set || !j->searchspace_set) return fail(SMX_E_STATE, "level and search space must be set");
+    if (j->blunder_level < 0) return fail(SMX_E_STATE, "no blunder level set (smx_job_set_blunder_level)");
+    if (!j->d_grid_pts) return fail(SMX_E_STATE, "the level was set without grid_pts (GridPts)");
+    HIPCHK(hipSetDevice(j->device));
+    const smx_level &L = j->lv;
+    const int bl = j->blunder_level;
+    // setsm_code.cpp:7884-7911: the window half-size
+    int T = (uint8_t)L.template_size;
+    const int tT = (int)((5.0 / (L.resolution * (1 << bl))) / 2.0) * 2 + 1;
+    if (T < tT) T = (uint8_t)tT;
+    int half = (int)(T / 2.0);
+    if (bblunder) {
+        if (L.level == 4) half = T - iteration * 2;
+        else if (L.level == 3) half = T - 2 - iteration * 2;
+        if (half < (int)(T / 2.0)) half = (int)(T / 2.0);
+    }
+    const long n = j->ncells;
+    CallBuffers CB(j);
+    BlunderArgs B{};
+    B.W = (long)L.grid_w;
+    B.H = (long)L.grid_h;
+    B.sw = (long)j->bortho_w;
+    B.sh = (long)j->bortho_h;
+    B.b0 = L.boundary[0];
+    B.b1 = L.boundary[1];
+    B.im_res = L.resolution * (double)(1 << bl);
+    B.grid_res = L.grid_resolution;
+    B.half = half;
+    B.level_b = bl;
+    B.right = L.reference_id == 1 ? 0 : 1;
+    for (int ti = 0; ti < 2; ti++) {
+        const LevelView &v = j->views[ti][bl];
+        B.view[ti] = DevView{v.texel, v.ori, v.w, v.h, v.sx, v.sy};
+    }
+    B.grid_pts = j->d_grid_pts;
+    B.ortho_wgs = j->d_bortho_wgs;
+    B.gridpt3 = j->d_gridpt3;
+    int r = CB.get(&B.cd[0], (size_t)(B.sw * B.sh) * 2);
+    if (!r) r = CB.get(&B.cd[1], (size_t)(B.sw * B.sh) * 2);
+    if (!r) r = CB.get(&B.out, (size_t)n);
+    if (!r && !j->d_params) r = jalloc(j, &j->d_params, 1);
+    if (r) return r;
+    fill_params(j, iteration);
+    if ((r = put_params(j, j->stream))) return r;
+    HIPCHK(launch_vll_blunder(j->d_params, B, j->stream));
+    if (ortho_ncc) HIPCHK(hipMemcpyAsync(ortho_ncc, B.out, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, j->stream));
+    HIPCHK(hipStreamSynchronize(j->stream));
+    j->generation++;   // GridPT3's ortho NCC changed
     return 0;
 }
 

@@ -77,6 +77,9 @@ struct smx_job {
     double *d_cdn = nullptr;                      // the same at level - 1
     size_t cd_bytes = 0, cdn_bytes = 0;
     double *d_gval = nullptr;                     // per-cell GNCC value (NaN: none)
+    double *d_bortho_wgs = nullptr;               // VerticalLineLocus_blunder: lon/lat of its ortho grid
+    int blunder_level = -1;                       // blunder_selected_level of that grid, -1 = none
+    uint32_t bortho_w = 0, bortho_h = 0;
     double *d_peak_rho = nullptr;                 // RA / matching-rate pass: temp_rho per run lane
     long peak_cap = 0;
     smx_ugrid *d_gridpt3 = nullptr;
@@ -148,6 +151,7 @@ hipError_t launch_unpack(const uint32_t *texel, uint16_t *img, uint16_t *mag, lo
 hipError_t launch_dir_probe(const int *gx, const int *gy, int16_t *out, long n, hipStream_t s);
 hipError_t launch_vll_gate(const VllParams *dP, const CellArgs &A, hipStream_t s);
 hipError_t launch_find_peak(const VllParams *dP, const CellArgs &A, hipStream_t s);
+hipError_t launch_vll_blunder(const VllParams *dP, const BlunderArgs &B, hipStream_t s);
 hipError_t launch_ortho_coords(const VllParams *dP, const OrthoArgs &O, hipStream_t s);
 hipError_t launch_gncc(int half, const VllParams *dP, const OrthoArgs &O, const CellArgs &A, double *gval,
                        hipStream_t s);
@@ -220,6 +224,7 @@ struct HrDev {
     int *hbounds;                // [0] minH_grid, [1] maxH_grid
 };
 hipError_t launch_set_height_range(const HrDev &D, hipStream_t s);
+hipError_t launch_set_height_range_blunder(const HrDev &D, float *heights, hipStream_t s);
 int fast_num_configs();
 int fast_config_desc(int i, char *buf, int n);
 int fast_config_index(const smx_job *j);

@@ -75,6 +75,23 @@ struct OrthoArgs {
     double *cdn[2];                // the same at level - 1
 };
 
+// VerticalLineLocus_blunder (setsm_code.cpp:7882-8013): the ortho grid and
+// pyramid views of the blunder level, and the cell grid it scores.
+struct BlunderArgs {
+    long W, H;                     // Size_Grid2D
+    long sw, sh;                   // ortho grid at im_res = resolution * 2^level_b
+    double b0, b1, im_res, grid_res;
+    int half;                      // Half_template_size
+    int level_b;                   // blunder_selected_level
+    int right;                     // image scored against image 0 (1; 0 when SETSM's reference is image 1)
+    DevView view[2];               // the blunder level's views
+    const double *grid_pts;
+    const double *ortho_wgs;       // (lon, lat) per ortho pixel
+    smx_ugrid *gridpt3;
+    double *cd[2];                 // pyramid image (x, y) per ortho pixel, image 0 / 1
+    float *out;                    // per-cell ortho NCC
+};
+
 struct InitArgs {
     long ncells;
     smx_nccresult *ncc;

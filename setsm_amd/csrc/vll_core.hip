@@ -541,6 +541,118 @@ __global__ void __launch_bounds__(256) k_gncc(const VllParams *__restrict__ P, O
     gval[c] = g;
 }
 
+// ------------------------------------------------------ VerticalLineLocus_blunder
+
+// SetOrthoImageCoord (setsm_code.cpp:6550-6670) at the blunder level
+// (check_pyimage = BD): the ortho pixels' positions in that level's pyramid.
+__global__ void __launch_bounds__(256) k_bd_coords(const VllParams *__restrict__ P, BlunderArgs B)
+{
+    const long it = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= B.sw * B.sh) return;
+    const long pts_row = it / B.sw, pts_col = it % B.sw;
+    const double t_X = B.b0 + pts_col * B.im_res;
+    const double t_Y = B.b1 + pts_row * B.im_res;
+    const long t_col = (long)((t_X - B.b0) / B.grid_res);
+    const long t_row = (long)((t_Y - B.b1) / B.grid_res);
+    const long pt_index = t_row * B.W + t_col;
+    if (!(pt_index < B.W * B.H && t_col < B.W && t_row < B.H)) return;
+    const float Hgt = B.gridpt3[pt_index].Height;
+    if (Hgt == -1000) return;
+    const double lon = B.ortho_wgs[2 * it], lat = B.ortho_wgs[2 * it + 1];
+#pragma unroll 1
+    for (int k = 0; k < 2; k++) {
+        double ix, iy, px, py;
+        rpc_ground_to_image(P->rpc[k], P->adjust[k], lon, lat, (double)Hgt, ix, iy);
+        to_pyramid(ix, iy, B.view[k], B.level_b, px, py);
+        B.cd[k][2 * it] = px;
+        B.cd[k][2 * it + 1] = py;
+    }
+}
+
+// setsm_code.cpp:7929-8000, one thread per cell: the multi-window NCC over
+// the disc of radius Half_template_size - 1 on the ortho grid, sampled on the
+// blunder level's images at the cells' TIN heights, two passes like
+// Correlate (the window size is a run-time value here).
+__global__ void __launch_bounds__(256) k_vll_blunder(BlunderArgs B)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= B.W * B.H) return;
+    const int H = B.half, r2max = (H - 1) * (H - 1);
+    const double gx = B.grid_pts[2 * c], gy = B.grid_pts[2 * c + 1];
+    auto pos = [&](int row, int col, double &lx, double &ly, double &rx, double &ry) -> bool {
+        const double t_X = gx + col * B.im_res;
+        const double t_Y = gy + row * B.im_res;
+        const long t_col = (long)((t_X - B.b0) / B.im_res);
+        const long t_row = (long)((t_Y - B.b1) / B.im_res);
+        const long pit = t_row * B.sw + t_col;
+        const long tt_col = (long)((t_X - B.b0) / B.grid_res);
+        const long tt_row = (long)((t_Y - B.b1) / B.grid_res);
+        const long pdem = tt_row * B.W + tt_col;
+        if (!(pit >= 0 && pit < B.sw * B.sh && t_col >= 0 && t_col < B.sw && t_row >= 0 && t_row < B.sh &&
+              pdem >= 0 && pdem < B.W * B.H && tt_col >= 0 && tt_col < B.W && tt_row >= 0 && tt_row < B.H))
+            return false;
+        if (B.gridpt3[pdem].Height == -1000) return false;
+        lx = B.cd[0][2 * pit];
+        ly = B.cd[0][2 * pit + 1];
+        rx = B.cd[B.right][2 * pit];
+        ry = B.cd[B.right][2 * pit + 1];
+        return true;
+    };
+    const DevView &Lv = B.view[0], &Rv = B.view[B.right];
+    double sL[3] = {0, 0, 0}, sLm[3] = {0, 0, 0}, sR[3] = {0, 0, 0}, sRm[3] = {0, 0, 0};
+    int cnt[3] = {0, 0, 0};
+    for (int row = -H; row <= H; row++)
+        for (int col = -H; col <= H; col++) {
+            if (row * row + col * col > r2max) continue;
+            const int bits = window_bits_ce(H, row, col);
+            double lx, ly, rx, ry, l, lm, r, rm;
+            if (!pos(row, col, lx, ly, rx, ry) || !sample_fp64(Lv, Rv, lx, ly, rx, ry, l, lm, r, rm)) continue;
+            for (int k = 0; k < 3; k++)
+                if (bits & (1 << k)) {
+                    sL[k] += l; sLm[k] += lm; sR[k] += r; sRm[k] += rm; cnt[k]++;
+                }
+        }
+    double t = 0;
+    if (cnt[0] > 0 && cnt[1] > 0 && cnt[2] > 0) {
+        double mL[3], mLm[3], mR[3], mRm[3];
+        for (int k = 0; k < 3; k++) {
+            mL[k] = sL[k] / cnt[k]; mR[k] = sR[k] / cnt[k];
+            mLm[k] = sLm[k] / cnt[k]; mRm[k] = sRm[k] / cnt[k];
+        }
+        double LR[3] = {0, 0, 0}, L2[3] = {0, 0, 0}, R2[3] = {0, 0, 0};
+        double mLR[3] = {0, 0, 0}, mL2[3] = {0, 0, 0}, mR2[3] = {0, 0, 0};
+        for (int row = -H; row <= H; row++)
+            for (int col = -H; col <= H; col++) {
+                if (row * row + col * col > r2max) continue;
+                const int bits = window_bits_ce(H, row, col);
+                double lx, ly, rx, ry, l, lm, r, rm;
+                if (!pos(row, col, lx, ly, rx, ry) || !sample_fp64(Lv, Rv, lx, ly, rx, ry, l, lm, r, rm)) continue;
+                for (int k = 0; k < 3; k++)
+                    if (bits & (1 << k)) {
+                        LR[k] += (l - mL[k]) * (r - mR[k]);
+                        L2[k] += (l - mL[k]) * (l - mL[k]);
+                        R2[k] += (r - mR[k]) * (r - mR[k]);
+                        mLR[k] += (lm - mLm[k]) * (rm - mRm[k]);
+                        mL2[k] += (lm - mLm[k]) * (lm - mLm[k]);
+                        mR2[k] += (rm - mRm[k]) * (rm - mRm[k]);
+                    }
+            }
+        double temp_roh = 0, count_roh = 0;
+        for (int k = 0; k < 3; k++) {
+            const double ncc = corr_from(LR[k], L2[k], R2[k]);
+            if (ncc != -99) { count_roh++; temp_roh += ncc; }
+            const double ncm = corr_from(mLR[k], mL2[k], mR2[k]);
+            if (ncm != -99) { count_roh++; temp_roh += ncm; }
+        }
+        if (count_roh > 0) t += temp_roh / count_roh;
+    }
+    double max_ncc = -100;
+    if (max_ncc < t) max_ncc = t;
+    B.gridpt3[c].ortho_ncc[1] = q_result(t);
+    B.gridpt3[c].Mean_ortho_ncc = q_result(max_ncc);
+    B.out[c] = (float)max_ncc;
+}
+
 // ------------------------------------------------------ InitializeVoxel
 
 // setsm_code.cpp:5759-6012, one thread per cell. Writes the new NCCresult
@@ -848,6 +960,18 @@ hipError_t launch_gncc(int half, const VllParams *dP, const OrthoArgs &O, const 
     case 7: hipLaunchKernelGGL(k_gncc<7>, g, b, 0, s, dP, O, A, gval); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, long n, hipStream_t s);
+
+hipError_t launch_vll_blunder(const VllParams *dP, const BlunderArgs &B, hipStream_t s)
+{
+    hipError_t e = launch_fill_u64((unsigned long long *)B.cd[0], 0, B.sw * B.sh * 2, s);
+    if (e == hipSuccess) e = launch_fill_u64((unsigned long long *)B.cd[1], 0, B.sw * B.sh * 2, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_bd_coords, dim3(nblk(B.sw * B.sh)), dim3(256), 0, s, dP, B);
+    hipLaunchKernelGGL(k_vll_blunder, dim3(nblk(B.W * B.H)), dim3(256), 0, s, B);
     return hipGetLastError();
 }
 

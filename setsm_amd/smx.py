@@ -35,6 +35,7 @@ EXPORTS = [
     "smx_job_get_nccresult", "smx_job_set_nccresult", "smx_initialize_voxel", "smx_vertical_line_locus",
     "smx_awncc", "smx_job_num_hypotheses", "smx_job_get_voxels", "smx_job_get_voxel_rows", "smx_job_set_voxels",
     "smx_job_get_sumcost", "smx_job_set_ortho_wgs", "smx_select_mps", "smx_set_height_range",
+    "smx_set_height_range_blunder", "smx_job_set_blunder_level", "smx_vertical_line_locus_blunder",
     "smx_job_last_kernel_ms", "smx_job_last_evaluated", "smx_job_last_redo", "smx_job_sync", "smx_fast_num_configs",
     "smx_job_set_fast_config", "smx_fast_config_desc", "smx_job_last_stats", "smx_vertical_line_locus_batch",
     "smx_vertical_line_locus_noalloc", "smx_release_cached", "smx_job_trim", "smx_job_half", "smx_graph_create", "smx_graph_launch", "smx_graph_last_ms", "smx_graph_destroy",
@@ -88,6 +89,9 @@ def lib():
             "smx_select_mps": (i, [vp, vp, vp, C.c_long, C.POINTER(C.c_long), C.POINTER(C.c_double)]),
             "smx_set_height_range": (i, [vp, vp, C.c_long, vp, C.c_long, vp, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double)]),
+            "smx_set_height_range_blunder": (i, [vp, vp, C.c_long, vp, C.c_long]),
+            "smx_job_set_blunder_level": (i, [vp, i, vp, C.c_uint32, C.c_uint32]),
+            "smx_vertical_line_locus_blunder": (i, [vp, i, i, vp]),
             "smx_job_last_kernel_ms": (C.c_double, [vp]),
             "smx_job_last_evaluated": (ll, [vp]),
             "smx_job_last_redo": (i, [vp, vp, vp, vp]),
@@ -261,6 +265,27 @@ class Job:
                                           _p(tris) if len(tris) else None, C.c_long(len(tris)), C.byref(a),
                                           C.byref(lo), C.byref(hi)))
         return lo.value, hi.value
+
+    def set_height_range_blunder(self, pts, tris):
+        """SetHeightRange_blunder (setsm_code.cpp:10393-10491) on GridPT3."""
+        pts = np.ascontiguousarray(pts, POINT3_DTYPE)
+        tris = np.ascontiguousarray(tris, TRI_DTYPE)
+        _check(lib().smx_set_height_range_blunder(self.h, _p(pts) if len(pts) else None, C.c_long(len(pts)),
+                                                  _p(tris) if len(tris) else None, C.c_long(len(tris))))
+
+    def set_blunder_level(self, blunder_level: int, ortho_lonlat):
+        """The blunder level's ortho grid: (h, w, 2) lon/lat per ortho pixel."""
+        ow = np.ascontiguousarray(ortho_lonlat, np.float64)
+        assert ow.ndim == 3 and ow.shape[2] == 2
+        self._bortho = ow
+        _check(lib().smx_job_set_blunder_level(self.h, int(blunder_level), _p(ow), ow.shape[1], ow.shape[0]))
+
+    def vertical_line_locus_blunder(self, iteration: int, bblunder: bool):
+        """VerticalLineLocus_blunder (setsm_code.cpp:7882-8013): per-cell ortho
+        NCC (float); GridPT3's ortho_ncc[1] / Mean_ortho_ncc updated."""
+        out = np.zeros(max(self.ncells, 1), np.float32)
+        _check(lib().smx_vertical_line_locus_blunder(self.h, int(iteration), int(bool(bblunder)), _p(out)))
+        return out[: self.ncells]
 
     # ------------------------------------------------------------- cube
     @property
