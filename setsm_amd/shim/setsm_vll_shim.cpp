@@ -1,14 +1,16 @@
 // setsm_vll_shim.cpp — SETSM's InitializeVoxel / VerticalLineLocus / AWNCC
-// (setsm_code.hpp:75, 81, 91) and the SelectMPs / SetHeightRange either side
-// of its TIN (setsm_code.hpp:99, 122) with their exact C++ signatures and
-// SETSM's own types, running on the MI355X through libsmx (include/smx_vll.h).
+// (setsm_code.hpp:75, 81, 91), the SelectMPs / SetHeightRange either side of
+// its TIN (setsm_code.hpp:99, 122) and DecisionMPs' VerticalLineLocus_blunder /
+// SetHeightRange_blunder (setsm_code.hpp:95, 120) with their exact C++
+// signatures and SETSM's own types, running on the MI355X through libsmx
+// (include/smx_vll.h).
 //
 // Drop-in use inside SETSM's build: add this file, link -lsmx, and either
 //   * compile it with -DSMX_SHIM_GLOBAL_NAMES and remove the original
 //     definitions from setsm_code.cpp, or
 //   * keep the originals and call smx_setsm::InitializeVoxel / ... at the
 //     call sites in Matching_SETSM (setsm_code.cpp:3827, 3829, 3844, 3862,
-//     4192-4222).
+//     4192-4222) and DecisionMPs (:8961-8962, 9081-9087, 9113-9115).
 //
 // State: one smx job per thread mirrors SETSM's per-level state. A new job is
 // made when the level's image buffers, sizes or origins change; a changed
@@ -60,6 +62,9 @@ struct State {
     bool lv_set = false;
     bool ortho_set = false;         // ortho grid lon/lat uploaded for this level record
     double minmax[2] = {0, 0};      // minmaxHeight of the level's last InitializeVoxel / VerticalLineLocus
+    int blunder_level = -1;         // blunder level whose views and ortho grid the job holds
+    const void *bviews[2][2] = {};
+    double bstarts[2][2] = {};
 };
 
 thread_local State g_state;
@@ -148,7 +153,8 @@ bool bind(const ProInfo *proinfo, LevelInfo &L)
     if (!same_views) {
         if (s.job) smx_job_destroy(s.job);
         s = State{};
-        check(smx_job_create(&s.job, level + 1), "smx_job_create");
+        // one level above for VerticalLineLocus_blunder's blunder_selected_level
+        check(smx_job_create(&s.job, level + 2 > 8 ? 8 : level + 2), "smx_job_create");
         for (int k = 0; k < 2; k++) {
             check(smx_job_set_view(s.job, k, level, L.py_Images[ids[k]], L.py_MagImages[ids[k]],
                                    L.py_OriImages[ids[k]], sizes[k][0], sizes[k][1], starts[k][0], starts[k][1]),
@@ -196,7 +202,56 @@ bool bind(const ProInfo *proinfo, LevelInfo &L)
     s.lv = lv;
     s.lv_set = true;
     s.ortho_set = false;
+    s.blunder_level = -1;
     return true;
+}
+
+// VerticalLineLocus_blunder's inputs: the blunder level's images
+// (py_BImages / py_BMagImages at py_BStartpos; the NCC reads no orientation,
+// so a zero one is uploaded unless the level is the matching level itself)
+// and its ortho grid's lon/lat from SETSM's ps2wgs_single.
+void ensure_blunder(State &s, const ProInfo *proinfo, LevelInfo &L)
+{
+    const int B = *L.blunder_selected_level;
+    const int ref = L.reference_id, ids[2] = {ref, ref == 0 ? 1 : 0};
+    const void *bv[2][2];
+    double bs[2][2];
+    for (int k = 0; k < 2; k++) {
+        bv[k][0] = L.py_BImages[ids[k]];
+        bv[k][1] = L.py_BMagImages[ids[k]];
+        bs[k][0] = L.py_BStartpos[ids[k]].m_X;
+        bs[k][1] = L.py_BStartpos[ids[k]].m_Y;
+    }
+    if (s.blunder_level == B && !memcmp(bv, s.bviews, sizeof bv) && !memcmp(bs, s.bstarts, sizeof bs)) return;
+    if (B < 0 || B >= 8) {
+        printf("ERROR: blunder_selected_level %d out of range\n", B);
+        exit(1);
+    }
+    for (int k = 0; k < 2; k++) {
+        if (B == s.level && bv[k][0] == (const void *)L.py_Images[ids[k]]) continue;   // the level's own views
+        const int w = L.py_Sizes[ids[k]][B].width, h = L.py_Sizes[ids[k]][B].height;
+        std::vector<uint8> zero_ori((size_t)w * h, 0);
+        check(smx_job_set_view(s.job, k, B, L.py_BImages[ids[k]], L.py_BMagImages[ids[k]], zero_ori.data(), w, h,
+                               bs[k][0], bs[k][1]),
+              "smx_job_set_view(blunder level)");
+    }
+    const double im_res = proinfo->resolution * (double)(1 << B);
+    const long sw = (long)((L.Boundary[2] - L.Boundary[0]) / im_res) + 1;
+    const long sh = (long)((L.Boundary[3] - L.Boundary[1]) / im_res) + 1;
+    std::vector<double> lonlat((size_t)sw * sh * 2);
+#pragma omp parallel for schedule(guided)
+    for (long it = 0; it < sw * sh; it++) {
+        D2DPOINT p;
+        p.m_X = L.Boundary[0] + (it % sw) * im_res;
+        p.m_Y = L.Boundary[1] + (it / sw) * im_res;
+        const D2DPOINT w = ps2wgs_single(*L.param, p);
+        lonlat[2 * it] = w.m_X;
+        lonlat[2 * it + 1] = w.m_Y;
+    }
+    check(smx_job_set_blunder_level(s.job, B, lonlat.data(), (uint32_t)sw, (uint32_t)sh), "smx_job_set_blunder_level");
+    s.blunder_level = B;
+    memcpy(s.bviews, bv, sizeof bv);
+    memcpy(s.bstarts, bs, sizeof bs);
 }
 
 // setsm_code.cpp:6100-6129, 6240: when some cell takes the GNCC term, hand
@@ -389,6 +444,38 @@ UGRID *SetHeightRange(ProInfo *proinfo, LevelInfo &rlevelinfo, NCCresult *nccres
     return result;
 }
 
+// SetHeightRange_blunder takes no ProInfo: it runs on the job of the level
+// the thread's last call bound, which must be this LevelInfo's.
+bool SetHeightRange_blunder(LevelInfo &rlevelinfo, const D3DPOINT *pts, const int numPts, UI3DPOINT *tris,
+                            const long num_triangles, UGRID *GridPT3)
+{
+    State &s = g_state;
+    if (!s.job || !s.lv_set || s.level != *rlevelinfo.Pyramid_step ||
+        s.lv.grid_w != (unsigned)rlevelinfo.Size_Grid2D->width ||
+        s.lv.grid_h != (unsigned)rlevelinfo.Size_Grid2D->height) {
+        printf("ERROR: SetHeightRange_blunder called without the level's VerticalLineLocus on the MI355X\n");
+        exit(1);
+    }
+    check(smx_job_set_searchspace(s.job, (const smx_ugrid *)GridPT3, s.minmax), "smx_job_set_searchspace");
+    check(smx_set_height_range_blunder(s.job, (const smx_point3 *)pts, numPts, (const smx_tri *)tris, num_triangles),
+          "smx_set_height_range_blunder");
+    check(smx_job_get_searchspace(s.job, (smx_ugrid *)GridPT3), "smx_job_get_searchspace");
+    return true;
+}
+
+bool VerticalLineLocus_blunder(const ProInfo *proinfo, LevelInfo &rlevelinfo, float *nccresult, UGRID *GridPT3,
+                               uint8 iteration, bool bblunder)
+{
+    bind(proinfo, rlevelinfo);
+    State &s = g_state;
+    ensure_blunder(s, proinfo, rlevelinfo);
+    check(smx_job_set_searchspace(s.job, (const smx_ugrid *)GridPT3, s.minmax), "smx_job_set_searchspace");
+    check(smx_vertical_line_locus_blunder(s.job, iteration, bblunder ? 1 : 0, nccresult),
+          "smx_vertical_line_locus_blunder");
+    check(smx_job_get_searchspace(s.job, (smx_ugrid *)GridPT3), "smx_job_get_searchspace");
+    return true;
+}
+
 // Release the thread's job (optional; it is reused across calls).
 void Release()
 {
@@ -430,5 +517,15 @@ UGRID *SetHeightRange(ProInfo *proinfo, LevelInfo &rlevelinfo, NCCresult *nccres
 {
     return smx_setsm::SetHeightRange(proinfo, rlevelinfo, nccresult, numOfPts, num_triangles, GridPT3, iteration,
                                      minH_grid, maxH_grid, pts, tris, MPP, level_check_matching_rate);
+}
+bool SetHeightRange_blunder(LevelInfo &rlevelinfo, const D3DPOINT *pts, const int numPts, UI3DPOINT *tris,
+                            const long num_triangles, UGRID *GridPT3)
+{
+    return smx_setsm::SetHeightRange_blunder(rlevelinfo, pts, numPts, tris, num_triangles, GridPT3);
+}
+bool VerticalLineLocus_blunder(const ProInfo *proinfo, LevelInfo &rlevelinfo, float *nccresult, UGRID *GridPT3,
+                               uint8 iteration, bool bblunder)
+{
+    return smx_setsm::VerticalLineLocus_blunder(proinfo, rlevelinfo, nccresult, GridPT3, iteration, bblunder);
 }
 #endif

@@ -4,8 +4,10 @@
 // (setsm_code.cpp:3827-3844): InitializeVoxel, VerticalLineLocus, AWNCC per
 // iteration, on host arrays (VOXEL **grid_voxel, NCCresult *, UGRID *), and
 // SelectMPs / SetHeightRange between iterations (setsm_code.cpp:3862,
-// 4192-4222). The functions are called through their reference signatures
-// (setsm_code.hpp:75, 81, 91, 99, 122), which libsmx_setsm.so exports.
+// 4192-4222) and DecisionMPs' VerticalLineLocus_blunder /
+// SetHeightRange_blunder. The functions are called through their reference
+// signatures (setsm_code.hpp:75, 81, 91, 95, 99, 120, 122), which
+// libsmx_setsm.so exports.
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -41,6 +43,10 @@ long SelectMPs(const ProInfo *proinfo, LevelInfo &rlevelinfo, const NCCresult *r
                const double Th_roh, const double Th_roh_min, const double Th_roh_start, const double Th_roh_next,
                const int iteration, const double MPP, const int final_level_iteration, const double MPP_stereo_angle,
                std::vector<D3DPOINT> *linkedlist);
+bool VerticalLineLocus_blunder(const ProInfo *proinfo, LevelInfo &rlevelinfo, float *nccresult, UGRID *GridPT3,
+                               uint8 iteration, bool bblunder);
+bool SetHeightRange_blunder(LevelInfo &rlevelinfo, const D3DPOINT *pts, const int numPts, UI3DPOINT *tris,
+                            const long num_triangles, UGRID *GridPT3);
 UGRID *SetHeightRange(ProInfo *proinfo, LevelInfo &rlevelinfo, NCCresult *nccresult, const int numOfPts,
                       const int num_triangles, UGRID *GridPT3, const int iteration, double *minH_grid,
                       double *maxH_grid, D3DPOINT *pts, const UI3DPOINT *tris, const double MPP,
@@ -73,6 +79,10 @@ struct Driver {
     std::vector<VOXEL *> vox;
     std::vector<NCCresult> ncc_vll;
     long long acc = 0;
+    int blunder_level = 0;
+    const uint16 *bimg[2] = {nullptr, nullptr}, *bmag[2] = {nullptr, nullptr};
+    D2DPOINT bstart[2];
+    bool swap = false;
 };
 
 }  // namespace
@@ -107,6 +117,7 @@ void *drv_create(const smx_level *lv, int swap_ids)
     d->level = lv->level;
     d->combined = lv->py_combined_level;
     d->reference_id = swap_ids ? 1 : 0;
+    d->swap = swap_ids != 0;
     d->template_size = (unsigned char)lv->template_size;
     d->bin_angle = lv->bin_angle;
     d->height_step = lv->height_step;
@@ -244,6 +255,46 @@ void drv_set_height_range(void *h, smx_ugrid *ugrid, smx_point3 *pts, long npts,
                               (D3DPOINT *)pts, (const UI3DPOINT *)tris, mpp, check_matching_rate != 0);
     memcpy(ugrid, r, sizeof(UGRID) * (size_t)d->grid_length);
     free(r);
+}
+
+// The blunder level's images (smx image order k = 0 reference, 1 target),
+// as Matching_SETSM sets py_BImages / py_BMagImages / py_BStartpos
+// (setsm_code.cpp:3560-3600).
+void drv_set_blunder(void *h, int B, const uint16 *img0, const uint16 *mag0, const uint16 *img1, const uint16 *mag1,
+                     int w0, int h0, int w1, int h1, double sx0, double sy0, double sx1, double sy1)
+{
+    Driver *d = (Driver *)h;
+    const uint16 *im[2] = {img0, img1}, *mg[2] = {mag0, mag1};
+    const int ws[2] = {w0, w1}, hs[2] = {h0, h1};
+    const double sx[2] = {sx0, sx1}, sy[2] = {sy0, sy1};
+    for (int k = 0; k < 2; k++) {
+        const int ti = d->swap ? 1 - k : k;
+        d->bimg[ti] = im[k];
+        d->bmag[ti] = mg[k];
+        d->bstart[ti].m_X = sx[k];
+        d->bstart[ti].m_Y = sy[k];
+        if ((int)d->sizes[ti].size() < B + 1) d->sizes[ti].resize((size_t)B + 1, CSize(0, 0));
+        d->sizes[ti][B] = CSize(ws[k], hs[k]);
+        d->size_rows[ti] = d->sizes[ti].data();
+    }
+    d->blunder_level = B;
+    d->L.py_BImages = d->bimg;
+    d->L.py_BMagImages = d->bmag;
+    d->L.py_BStartpos = d->bstart;
+    d->L.blunder_selected_level = &d->blunder_level;
+    d->L.py_Sizes = d->size_rows;
+}
+
+void drv_set_height_range_blunder(void *h, smx_ugrid *ugrid, smx_point3 *pts, long npts, smx_tri *tris, long ntris)
+{
+    Driver *d = (Driver *)h;
+    SetHeightRange_blunder(d->L, (const D3DPOINT *)pts, (int)npts, (UI3DPOINT *)tris, ntris, (UGRID *)ugrid);
+}
+
+void drv_vll_blunder(void *h, smx_ugrid *ugrid, int iteration, int bblunder, float *out)
+{
+    Driver *d = (Driver *)h;
+    VerticalLineLocus_blunder(&d->pro, d->L, out, (UGRID *)ugrid, (uint8)iteration, bblunder != 0);
 }
 
 void drv_destroy(void *h)

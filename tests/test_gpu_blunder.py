@@ -15,11 +15,16 @@ from tests.test_gpu_mps import SEL, grids_equal
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(O.tin() is None, reason="oracle/_ref/libref_tin.so not built")]
 
 
-@pytest.mark.parametrize("level,B,bblunder,it", [(3, 3, 1, 1), (2, 3, 1, 2), (1, 2, 0, 1), (0, 1, 0, 1)])
-def test_blunder_pass_matches_oracle(require_gpu, level, B, bblunder, it):
+@pytest.mark.parametrize("level,B,bblunder,it,cells", [
+    (4, 4, 1, 1, 40),     # the blunder loop's window at level 4: half 15 - 2 = 13
+    (3, 3, 1, 1, 80),     # level 3: half 15 - 2 - 2 = 11
+    (2, 3, 1, 1, 80),
+    (1, 2, 0, 1, 80),
+    (0, 1, 0, 1, 80)])
+def test_blunder_pass_matches_oracle(require_gpu, level, B, bblunder, it, cells):
     from setsm_amd import smx
     sc, pyr = scene(image_px=1024, relief=90.0, seed=17, levels=5)
-    g = level_grid(sc, level, 80, band=48.0 * (1 << level), seed=3)
+    g = level_grid(sc, level, cells, band=min(48.0 * (1 << level), 400.0), seed=3)
     job = smx.Job(5)
     try:
         upload_pyramid(job, pyr)
@@ -43,7 +48,7 @@ def test_blunder_pass_matches_oracle(require_gpu, level, B, bblunder, it):
         grids_equal(job.get_searchspace(), ug_o)
         print(f"level {level} blunder level {B}: {len(pts)} points, {len(tris)} triangles, half "
               f"{O.blunder_half_template(li, B, it, bblunder)}, mean ortho NCC {out_o.mean():.3f}", flush=True)
-        assert out_o.mean() > 0.3
+        assert out_o.mean() > 0.3 and len(tris) > 0
     finally:
         job.close()
 

@@ -44,6 +44,12 @@ def _driver():
     d.drv_set_projection.argtypes = [C.c_double] * 4
     d.drv_select_mps.restype = C.c_long
     d.drv_select_mps.argtypes = [vp, vp] + [C.c_double] * 4 + [C.c_int, C.c_double, C.c_int, C.c_double, vp]
+    d.drv_set_blunder.restype = None
+    d.drv_set_blunder.argtypes = [vp, C.c_int, vp, vp, vp, vp] + [C.c_int] * 4 + [C.c_double] * 4
+    d.drv_set_height_range_blunder.restype = None
+    d.drv_set_height_range_blunder.argtypes = [vp, vp, vp, C.c_long, vp, C.c_long]
+    d.drv_vll_blunder.restype = None
+    d.drv_vll_blunder.argtypes = [vp, vp, C.c_int, C.c_int, vp]
     d.drv_set_height_range.restype = None
     d.drv_set_height_range.argtypes = [vp, vp, vp, C.c_long, vp, C.c_long, C.c_int, C.c_double, C.c_int,
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -219,5 +225,51 @@ def test_shim_select_mps_and_set_height_range(require_gpu, monkeypatch):
             _same_grid(ug_s, ug_o)
             print(f"shim level {level} iteration {it}: {n} points, {len(tris)} triangles", flush=True)
             ug = ug_s
+    finally:
+        s.close()
+
+
+def test_shim_blunder_helpers(require_gpu, monkeypatch):
+    """DecisionMPs' SetHeightRange_blunder and VerticalLineLocus_blunder
+    through the reference signatures: the blunder level one above the
+    matching level (py_BImages at level + 1, as Matching_SETSM selects it
+    below the top level), the ortho lon/lat from the test double of
+    ps2wgs_single; GridPT3 and the ortho NCC equal the oracle's."""
+    import math
+    from oracle import oracle_py as O
+    from setsm_amd.records import POINT3_DTYPE, LevelInputs
+    from setsm_amd.synth import EARTH_R, ortho_wgs
+    if O.tin() is None:
+        pytest.skip("oracle/_ref/libref_tin.so not built")
+    monkeypatch.setenv("SMX_PRECISION", "fp64")
+    drv = _driver()
+    sc, pyr = scene(image_px=1024, relief=120.0, seed=5, levels=5)
+    drv.drv_set_projection(sc.spec.lat0, sc.spec.lon0, EARTH_R, EARTH_R * math.cos(math.radians(sc.spec.lat0)))
+    level, B = 2, 3
+    g = level_grid(sc, level, 48)
+    li = LevelInputs(pyr, level, sc.rpcs, g, grid_resolution=g["grid_res"])
+    s = ShimRun(drv, li, 0)
+    try:
+        r = s.step(g["ugrid"], 1, g["minmax"])
+        ug_s = np.ascontiguousarray(r["ug"].copy())
+        ug_o = r["ug"].copy()
+        out = np.zeros(len(ug_s), POINT3_DTYPE)
+        n = drv.drv_select_mps(s.h, ug_s.ctypes.data, 0.6, 0.35, 0.6, 0.5, 1, 1.0, 1, 10.0, out.ctypes.data)
+        pts_o, _ = O.select_mps(li, r["ncc"], ug_o, 1, 0.6, 0.35, 0.6, 0.5, 1.0, 10.0, 1)
+        tris = O.tin_create(pts_o, g["boundary"], g["grid_res"])
+        drv.drv_set_height_range_blunder(s.h, ug_s.ctypes.data, out.ctypes.data, n, tris.ctypes.data, len(tris))
+        O.set_height_range_blunder(li, ug_o, pts_o, tris)
+        _same_grid(ug_s, ug_o)
+        b = [tuple(np.ascontiguousarray(a) for a in pyr[k][B][:2]) for k in range(2)]
+        drv.drv_set_blunder(s.h, B, b[0][0].ctypes.data, b[0][1].ctypes.data, b[1][0].ctypes.data,
+                            b[1][1].ctypes.data, b[0][0].shape[1], b[0][0].shape[0], b[1][0].shape[1],
+                            b[1][0].shape[0], 0.0, 0.0, 0.0, 0.0)
+        ncc_s = np.zeros(len(ug_s), np.float32)
+        drv.drv_vll_blunder(s.h, ug_s.ctypes.data, 1, 1, ncc_s.ctypes.data)
+        ncc_o = O.vll_blunder(li, pyr, B, ortho_wgs(sc, g["boundary"], B), ug_o, 1, True)
+        assert np.array_equal(ncc_s.view(np.uint32), ncc_o.view(np.uint32))
+        _same_grid(ug_s, ug_o)
+        print(f"shim blunder level {B}: {n} points, {len(tris)} triangles, mean ortho NCC {ncc_o.mean():.3f}",
+              flush=True)
     finally:
         s.close()
