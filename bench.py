@@ -198,7 +198,7 @@ def headline(args, out):
     g["boundary"] = np.array([inset, inset, inset + cells * 2.0, inset + cells * 2.0])
     g["shape"] = (cells, cells)
     g["grid_res"] = 2.0
-    job = level_job(sc, g)
+    job = level_job(sc, g, levels=2)   # level 1 feeds VerticalLineLocus_blunder
     log(f"job resident: {job.device_bytes / 2**30:.1f} GiB, {job.num_hypotheses} hypotheses")
     fph = flops_per_hypothesis(5)
 
@@ -250,7 +250,7 @@ def headline(args, out):
     log(f"fast: {fast['wall_s'] * 1e3:.1f} ms/step, kernel {fast['kernel_ms']:.1f} ms, err {fast_err}")
 
     # SGM aggregation + peak scan (AWNCC at a level <= SGM_py) on this cube
-    sgm = None
+    sgm = between = None
     if args.sgm:
         stub = [[(sc.images[ti], sc.images[ti], np.zeros((1, 1), np.uint8))] for ti in range(2)]
         li = LevelInputs(stub, 0, sc.rpcs, g, grid_resolution=2.0, sgm_py=1)
@@ -268,6 +268,9 @@ def headline(args, out):
         sgm = dict(ms=1e3 * min(ts), hypotheses=job.num_hypotheses, paths=12,
                    hyp_per_s=job.num_hypotheses / min(ts), cells=cells * cells)
         log(f"SGM + peak scan: {sgm}")
+        if args.between:
+            between = between_run(args, job, sc, g, li, r0)
+            log(f"between iterations: {between}")
 
     cpu = parity = None
     if not args.no_cpu:
@@ -337,7 +340,104 @@ def headline(args, out):
         "fast_launch": dict(tuned or {}, **fast["stats"]),
         "kernel_ms": fast["kernel_ms"],
         "sgm": sgm,
+        "between_iterations": between,
     })
+
+
+# ------------------------------------------------------------ between iterations
+MPS_ARGS = dict(th_roh=0.4, th_roh_min=0.1, th_roh_start=0.4, th_roh_next=0.3, mpp=1.0, mpp_stereo_angle=10.0,
+                final_level_iteration=1)
+
+
+def lattice_tris(w, h):
+    """The triangles of the full cell lattice, each grid square split along
+    one diagonal, in row order and clockwise, as the reference's TIN lists
+    its triangles (so vertex and edge cells are inside): a TIN over every
+    cell of the grid."""
+    c, r = np.meshgrid(np.arange(w - 1, dtype=np.uint32), np.arange(h - 1, dtype=np.uint32))
+    i = (r * w + c).ravel()
+    t = np.empty((2 * len(i), 3), np.uint32)
+    t[0::2] = np.stack([i, i + w + 1, i + 1], 1)
+    t[1::2] = np.stack([i, i + w, i + w + 1], 1)
+    from setsm_amd.records import TRI_DTYPE
+    return t.view(TRI_DTYPE).ravel()
+
+
+def between_run(args, job, sc, g, li, r0):
+    """The data-parallel steps between two iterations of SETSM's loop on the
+    headline level after its AWNCC: SelectMPs, SetHeightRange and DecisionMPs'
+    SetHeightRange_blunder / VerticalLineLocus_blunder (blunder level 1). The
+    TIN is the full cell lattice (a point per cell at its peak height, two
+    triangles per grid square). Per call: the kernel time on the job stream
+    (HIP events; cells_per_s from it) and the wall clock with the host copies
+    (the points come back for the caller's TIN, the TIN goes up from pageable
+    memory). The oracle is timed beside it on a stripe of the same level."""
+    from setsm_amd.records import POINT3_DTYPE
+    from setsm_amd.synth import ortho_wgs
+    W, H = g["shape"][1], g["shape"][0]
+    out = {}
+    ncc = job.get_nccresult()
+    ug0 = job.get_searchspace()
+    pts_all = np.zeros(W * H, POINT3_DTYPE)
+    pts_all["x"], pts_all["y"] = g["grid_pts"][:, 0], g["grid_pts"][:, 1]
+    pts_all["z"] = ncc["result2"]
+    tris = lattice_tris(W, H)
+    t0 = time.time()
+    ow = ortho_wgs(sc, g["boundary"], 1)
+    log(f"blunder ortho grid {ow.shape[1]}x{ow.shape[0]} ready in {time.time() - t0:.1f}s")
+    job.set_blunder_level(1, ow)
+    steps = {"select_mps": lambda: job.select_mps(1, **MPS_ARGS),
+             "set_height_range": lambda: job.set_height_range(pts_all, tris, 1, 1.0),
+             "set_height_range_blunder": lambda: job.set_height_range_blunder(pts_all, tris),
+             "vertical_line_locus_blunder": lambda: job.vertical_line_locus_blunder(1, False)}
+    for name, fn in steps.items():
+        ts, ks = [], []
+        for _ in range(3):
+            job.set_searchspace(ug0, g["minmax"])
+            job.sync()
+            t = time.perf_counter()
+            r = fn()
+            ts.append(time.perf_counter() - t)
+            ks.append(job.last_kernel_ms)
+        out[name] = dict(kernel_ms=min(ks), cells_per_s=W * H / (min(ks) / 1e3), wall_ms=1e3 * min(ts))
+        if name == "select_mps":
+            out[name]["points"] = int(len(r[0]))
+            out[name]["minimum_th"] = r[1]
+    out["cells"] = W * H
+    out["triangles"] = int(len(tris))
+    out["blunder_ortho_px"] = int(ow.shape[0] * ow.shape[1])
+    if not args.no_cpu:
+        from oracle import oracle_py as O
+        from setsm_amd.records import LevelInputs
+        rows = max(1, args.between_cpu_rows)
+        # the stripe as a level of its own: its boundary starts at its first row
+        sub = sub_grid(g, r0, rows)
+        b = g["boundary"]
+        sub["boundary"] = np.array([b[0], b[1] + r0 * g["grid_res"], b[2], b[1] + (r0 + rows) * g["grid_res"]])
+        stub = [[(sc.images[ti], sc.images[ti], np.zeros((1, 1), np.uint8))] for ti in range(2)]
+        lio = LevelInputs(stub, 0, sc.rpcs, sub, grid_resolution=g["grid_res"], sgm_py=1)
+        c0, c1 = r0 * W, (r0 + rows) * W
+        ncc_s, ug_s, pts_s = ncc[c0:c1].copy(), ug0[c0:c1].copy(), pts_all[c0:c1].copy()
+        tris_s = lattice_tris(W, rows)
+        ow_s = ortho_wgs(sc, sub["boundary"], 1)
+        lv1 = [job.get_view(ti, 1, (sc.images[ti].shape[0] // 2, sc.images[ti].shape[1] // 2)) for ti in range(2)]
+        pyr1 = [[None, lv1[ti]] for ti in range(2)]
+        cpu = {}
+        for name, fn in (("select_mps", lambda u: O.select_mps(lio, ncc_s, u, 1, **MPS_ARGS)),
+                         ("set_height_range", lambda u: O.set_height_range(lio, u, pts_s, tris_s, 1, 1.0)),
+                         ("set_height_range_blunder", lambda u: O.set_height_range_blunder(lio, u, pts_s, tris_s)),
+                         ("vertical_line_locus_blunder", lambda u: O.vll_blunder(lio, pyr1, 1, ow_s, u, 1, False))):
+            u = ug_s.copy()
+            t = time.perf_counter()
+            fn(u)
+            dt = time.perf_counter() - t
+            cpu[name] = dict(ms=1e3 * dt, cells_per_s=(c1 - c0) / dt)
+        out["cpu_baseline"] = dict(cpu, cores=O.threads(), kind="port",
+                                   sample=f"{rows} x {W} cell stripe (rows {r0}..{r0 + rows - 1}) of the same level "
+                                          f"as a level of its own, its lattice and ortho grid; oracle restatement "
+                                          f"(SelectMPs / SetHeightRange single-threaded as the reference runs "
+                                          f"them, the blunder NCC and ortho grid on OpenMP)")
+    return out
 
 
 # ------------------------------------------------------------ configs[0]
@@ -696,6 +796,9 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=64, help="cell rows of the CPU baseline stripe")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sgm", dest="sgm", action="store_false")
+    ap.add_argument("--no-between", dest="between", action="store_false",
+                    help="skip the between-iterations leg (SelectMPs, SetHeightRange, blunder helpers)")
+    ap.add_argument("--between-cpu-rows", type=int, default=200, help="cell rows of its CPU baseline stripe")
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--scene-cache", default="", help="directory to load / save the headline scene")
     ap.add_argument("--autotune", action="store_true", help="time every FAST launch shape and store the best")

@@ -235,6 +235,8 @@ int  smx_job_set_blunder_level(smx_job *job, int blunder_level, const double *lo
  * reference's operation order. Writes GridPT3.ortho_ncc[1] and
  * Mean_ortho_ncc, and (if not NULL) the per-cell value to ortho_ncc. */
 int  smx_vertical_line_locus_blunder(smx_job *job, int iteration, int bblunder, float *ortho_ncc);
+/* The four calls above leave their kernel time on the job stream (host
+ * copies excluded) in smx_job_last_kernel_ms. */
 
 /* Number of hypotheses in the cube (sum of NumOfHeight). */
 long long smx_job_num_hypotheses(const smx_job *job);

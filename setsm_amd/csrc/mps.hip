@@ -287,27 +287,82 @@ __device__ __forceinline__ double tin_slope_deg(const smx_point3 &p0, const smx_
     return angle;
 }
 
+// Block-wide min / max into one global atomic each: a shuffle reduction per
+// wavefront, then one LDS atomic per wavefront. Every thread of the block
+// must call these (neutral values for threads without one).
+__device__ __forceinline__ void block_minmax_u64(unsigned long long lo, unsigned long long hi,
+                                                 unsigned long long *gmin, unsigned long long *gmax)
+{
+    __shared__ unsigned long long s_lo, s_hi;
+    if (threadIdx.x == 0) {
+        s_lo = ~0ull;
+        s_hi = 0ull;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long a = __shfl_xor(lo, o), b = __shfl_xor(hi, o);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&s_lo, lo);
+        atomicMax(&s_hi, hi);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicMin(gmin, s_lo);
+        atomicMax(gmax, s_hi);
+    }
+}
+
+__device__ __forceinline__ void block_minmax_i32(int lo, int hi, int *gmin, int *gmax)
+{
+    __shared__ int s_lo, s_hi;
+    if (threadIdx.x == 0) {
+        s_lo = 0x7fffffff;
+        s_hi = -0x7fffffff - 1;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o));
+        hi = max(hi, __shfl_xor(hi, o));
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&s_lo, lo);
+        atomicMax(&s_hi, hi);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicMin(gmin, s_lo);
+        atomicMax(gmax, s_hi);
+    }
+}
+
 // 16 lanes per triangle walk its bounding box; a cell inside the triangle
-// takes the smallest claiming index.
+// takes the smallest claiming index. Groups stride over the triangles (the
+// grid is bounded), so the valid triangles' height span reaches zkeys in one
+// atomic per block.
 __global__ void __launch_bounds__(256) k_hr_claim(HrDev D)
 {
-    const long t = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const long groups = ((long)gridDim.x * blockDim.x) >> 4;
     const int sub = threadIdx.x & 15;
-    if (t >= D.ntris) return;
-    TriGeom T;
-    if (!tri_geom(D, t, T)) return;
-    if (sub == 0) {
-        atomicMin(&D.zkeys[0], okey(T.zmin));
-        atomicMax(&D.zkeys[1], okey(T.zmax));
+    unsigned long long kmin = ~0ull, kmax = 0ull;
+    for (long t = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 4; t < D.ntris; t += groups) {
+        TriGeom T;
+        if (!tri_geom(D, t, T)) continue;
+        const unsigned long long a = okey(T.zmin), b = okey(T.zmax);
+        kmin = a < kmin ? a : kmin;
+        kmax = b > kmax ? b : kmax;
+        const long bw = (long)(T.c1 - T.c0 + 1), bh = (long)(T.r1 - T.r0 + 1);
+        if (bw <= 0 || bh <= 0) continue;
+        for (long k = sub; k < bw * bh; k += 16) {
+            const long row = T.r0 + k / bw, col = T.c0 + k % bw;
+            const double px = col * D.gres + D.b0, py = row * D.gres + D.b1;
+            float z;
+            if (tin_inside(px, py, *T.p1, *T.p2, *T.p3, z)) atomicMin(&D.owner[D.W * row + col], (uint32_t)t);
+        }
     }
-    const long bw = (long)(T.c1 - T.c0 + 1), bh = (long)(T.r1 - T.r0 + 1);
-    if (bw <= 0 || bh <= 0) return;
-    for (long k = sub; k < bw * bh; k += 16) {
-        const long row = T.r0 + k / bw, col = T.c0 + k % bw;
-        const double px = col * D.gres + D.b0, py = row * D.gres + D.b1;
-        float z;
-        if (tin_inside(px, py, *T.p1, *T.p2, *T.p3, z)) atomicMin(&D.owner[D.W * row + col], (uint32_t)t);
-    }
+    block_minmax_u64(kmin, kmax, &D.zkeys[0], &D.zkeys[1]);
 }
 
 // setsm_code.cpp:9959-10087: the owner triangle's update of a claimed cell.
@@ -378,8 +433,9 @@ __global__ void __launch_bounds__(256) k_hr_apply(HrDev D)
 // returned grid.
 __global__ void __launch_bounds__(256) k_hr_fill(HrDev D)
 {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= D.W * D.H) return;
+    const long c0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = c0 < D.W * D.H;
+    const long c = in ? c0 : 0;
     const long row = c / D.W, col = c % D.W;
     smx_ugrid G = D.g[c];
     int mb = D.mb[c];
@@ -409,8 +465,9 @@ __global__ void __launch_bounds__(256) k_hr_fill(HrDev D)
         }
     } else if (mb == 0)
         G.Matched_flag = 0;
-    atomicMin(&D.hbounds[0], (int)G.minHeight);
-    atomicMax(&D.hbounds[1], (int)G.maxHeight);
+    block_minmax_i32(in ? (int)G.minHeight : 0x7fffffff, in ? (int)G.maxHeight : -0x7fffffff - 1, &D.hbounds[0],
+                     &D.hbounds[1]);
+    if (!in) return;
     if (G.maxHeight - G.minHeight > D.th_hg) G.minHeight = G.maxHeight = -100;
     int16_t lo = G.minHeight, hi = G.maxHeight;
     if (lo > hi) {
@@ -450,7 +507,8 @@ hipError_t launch_set_height_range(const HrDev &D, hipStream_t s)
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_hr_init, dim3(1), dim3(64), 0, s, D);
     if (D.ntris > 0)
-        hipLaunchKernelGGL(k_hr_claim, dim3((unsigned)((D.ntris * 16 + 255) / 256)), dim3(256), 0, s, D);
+        hipLaunchKernelGGL(k_hr_claim, dim3((unsigned)std::min<long>((D.ntris * 16 + 255) / 256, 8192)), dim3(256),
+                           0, s, D);
     const unsigned cb = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_hr_apply, dim3(cb), dim3(256), 0, s, D);
     hipLaunchKernelGGL(k_hr_fill, dim3(cb), dim3(256), 0, s, D);

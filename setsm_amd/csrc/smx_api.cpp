@@ -1354,12 +1354,17 @@ int smx_select_mps(smx_job *j, const smx_mps_args *a, smx_point3 *pts, long cap,
     if (r) return r;
     D.th = d_th;
     D.off = d_off;
+    HIPCHK(hipEventRecord(j->ev0, j->stream));
     HIPCHK(launch_select_mps(D, j->js, j->stream));
+    HIPCHK(hipEventRecord(j->ev1, j->stream));
     double th = 0;
     HIPCHK(hipMemcpyAsync(&j->h_small[0], d_off + n, sizeof(int64_t), hipMemcpyDeviceToHost, j->stream));
     HIPCHK(hipMemcpyAsync(&th, d_th, sizeof(double), hipMemcpyDeviceToHost, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
     if (minimum_th) *minimum_th = th;
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, j->ev0, j->ev1));
+    j->last_ms = ms;   // kernel time on the job stream, host copies excluded
     if (th < 0)
         return fail(SMX_E_ARG, "SelectMPs has no peak threshold at level %d (above 4 and above SGM_py)", j->lv.level);
     j->generation++;   // GridPT3 changed
@@ -1368,9 +1373,13 @@ int smx_select_mps(smx_job *j, const smx_mps_args *a, smx_point3 *pts, long cap,
     if (m > cap) return fail(SMX_E_ARG, "%ld matched points exceed the buffer of %ld", m, cap);
     if (m > 0) {
         if ((r = B.get(&D.pts, (size_t)m))) return r;
+        HIPCHK(hipEventRecord(j->ev0, j->stream));
         HIPCHK(launch_write_mps(D, j->stream));
+        HIPCHK(hipEventRecord(j->ev1, j->stream));
         HIPCHK(hipMemcpyAsync(pts, D.pts, (size_t)m * sizeof(smx_point3), hipMemcpyDeviceToHost, j->stream));
         HIPCHK(hipStreamSynchronize(j->stream));
+        HIPCHK(hipEventElapsedTime(&ms, j->ev0, j->ev1));
+        j->last_ms += ms;
     }
     return 0;
 }
@@ -1445,11 +1454,16 @@ int smx_set_height_range(smx_job *j, const smx_point3 *pts, long npts, const smx
     D.buf = buf;
     D.th_hg = th_hg;
     D.ncc_th = ncc_th;
+    HIPCHK(hipEventRecord(j->ev0, j->stream));
     HIPCHK(launch_set_height_range(D, j->stream));
     HIPCHK(hipMemcpyAsync(j->d_gridpt3, D.out, (size_t)n * sizeof(smx_ugrid), hipMemcpyDeviceToDevice, j->stream));
+    HIPCHK(hipEventRecord(j->ev1, j->stream));
     int hb[2] = {0, 0};
     HIPCHK(hipMemcpyAsync(&j->h_small[0], D.hbounds, 2 * sizeof(int), hipMemcpyDeviceToHost, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, j->ev0, j->ev1));
+    j->last_ms = ms;   // kernels and the grid copy on the job stream; the uploads excluded
     std::memcpy(hb, &j->h_small[0], sizeof hb);
     if (minH_grid) *minH_grid = (double)hb[0];
     if (maxH_grid) *maxH_grid = (double)hb[1];
@@ -1488,8 +1502,13 @@ int smx_set_height_range_blunder(smx_job *j, const smx_point3 *pts, long npts, c
     D.gres = j->lv.grid_resolution;
     D.b0 = j->lv.boundary[0];
     D.b1 = j->lv.boundary[1];
+    HIPCHK(hipEventRecord(j->ev0, j->stream));
     HIPCHK(launch_set_height_range_blunder(D, d_h, j->stream));
+    HIPCHK(hipEventRecord(j->ev1, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, j->ev0, j->ev1));
+    j->last_ms = ms;
     j->generation++;
     return 0;
 }
@@ -1569,9 +1588,14 @@ int smx_vertical_line_locus_blunder(smx_job *j, int iteration, int bblunder, flo
     if (r) return r;
     fill_params(j, iteration);
     if ((r = put_params(j, j->stream))) return r;
+    HIPCHK(hipEventRecord(j->ev0, j->stream));
     HIPCHK(launch_vll_blunder(j->d_params, B, j->stream));
+    HIPCHK(hipEventRecord(j->ev1, j->stream));
     if (ortho_ncc) HIPCHK(hipMemcpyAsync(ortho_ncc, B.out, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, j->ev0, j->ev1));
+    j->last_ms = ms;
     j->generation++;   // GridPT3's ortho NCC changed
     return 0;
 }

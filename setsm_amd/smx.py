@@ -246,11 +246,11 @@ class Job:
         level's minimum peak threshold). GridPT3 is updated in place."""
         a = SmxMpsArgs(th_roh, th_roh_min, th_roh_start, th_roh_next, mpp, mpp_stereo_angle, int(iteration),
                        int(final_level_iteration))
-        pts = np.zeros(max(self.ncells, 1), POINT3_DTYPE)
+        pts = np.empty(max(self.ncells, 1), POINT3_DTYPE)
         count, th = C.c_long(0), C.c_double(0)
         _check(lib().smx_select_mps(self.h, C.byref(a), _p(pts), C.c_long(len(pts)), C.byref(count),
                                     C.byref(th)))
-        return pts[: count.value].copy(), th.value
+        return pts[: count.value], th.value
 
     def set_height_range(self, pts, tris, iteration: int, mpp: float, check_matching_rate: bool = False,
                          seed_dem_sigma: float = 0.0):
