@@ -127,6 +127,49 @@ def test_million_cell_grid_matches_oracle(require_gpu, level):
         job.close()
 
 
+@pytest.mark.parametrize("level,kw,hr", [
+    (2, dict(is_ra=1), {}),                                   # RA: fixed 0.2 threshold
+    (1, dict(sgm_py=1), {}),                                  # SGM level: fraction 0.85
+    (3, dict(pre_dem=1), dict(seed_dem_sigma=6.0)),           # seed DEM caps the buffer
+    (1, dict(dem_resolution=8.0), {}),                        # DEM >= 4 m: gap limit 1000
+    (2, dict(check_matching_rate=1), dict(check_matching_rate=True)),
+    (0, dict(dem_resolution=1.0), {}),                        # level 0, DEM < 2 m: gap limit 50
+])
+def test_branches_match_oracle(require_gpu, level, kw, hr):
+    """The level-record and argument branches of both functions on a
+    300 x 300 synthetic grid, two iterations apart (it 1 and it 5)."""
+    from setsm_amd import smx
+    sc, pyr = scene(image_px=512, relief=60.0, seed=3, levels=4)
+    W = 300
+    gres = 2.0 * (1 << level)
+    c, r = np.meshgrid(np.arange(W), np.arange(W))
+    g = dict(grid_pts=np.stack([50 + c.ravel() * gres, 80 + r.ravel() * gres], 1), grid_wgs=np.zeros((W * W, 2)),
+             boundary=np.array([50, 80, 50 + (W - 1) * gres, 80 + (W - 1) * gres]), shape=(W, W), grid_res=gres)
+    args = dict(dem_resolution=2.0)
+    args.update(kw)
+    li = LevelInputs(pyr, level, sc.rpcs, g, grid_resolution=gres, **args)
+    for it, seed in ((1, 7), (5, 8)):
+        ncc, ug = synthetic(W * W, np.random.default_rng(seed + 10 * level), level)
+        job = smx.Job(4)
+        try:
+            upload_pyramid(job, pyr)
+            job.set_level(li.level)
+            job.set_searchspace(ug, np.array([0.0, 100.0]))
+            job.set_nccresult(ncc)
+            ug_o = ug.copy()
+            pts_g, th_g = job.select_mps(it, mpp=0.7, **SEL)
+            pts_o, th_o = O.select_mps(li, ncc, ug_o, it, mpp=0.7, **SEL)
+            assert th_g == th_o
+            points_equal(pts_g, pts_o)
+            grids_equal(job.get_searchspace(), ug_o)
+            tris = O.tin_create(pts_o, g["boundary"], gres)
+            assert job.set_height_range(pts_g, tris, it, 0.7, **hr) == \
+                O.set_height_range(li, ug_o, pts_o, tris, it, 0.7, **hr)
+            grids_equal(job.get_searchspace(), ug_o)
+        finally:
+            job.close()
+
+
 def test_edge_cases(require_gpu):
     """Triangles indexing past the points are skipped, no triangles at all,
     a point buffer too small, and a level with no threshold."""
