@@ -825,10 +825,22 @@ __global__ void __launch_bounds__(256) k_relayout(InitArgs A)
 // (or blended) value at the chosen peak.
 __global__ void __launch_bounds__(256) k_awncc(const VllParams *__restrict__ P, PeakArgs A)
 {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= A.ncells) return;
-    smx_nccresult nr = A.ncc[c];
-    const smx_ugrid g = A.gridpt3[c];
+    // The block's rows are staged through LDS in chunks of kChunk heights:
+    // consecutive threads load consecutive words of one row (the rows of
+    // neighbouring cells are neighbours in CSR), then each lane scans its own
+    // row from LDS. Rows are padded to kChunk + 1 words so the lanes' reads
+    // of one step fall in different banks.
+    constexpr int kChunk = 16, kPitch = kChunk + 1;
+    __shared__ uint32_t s_word[256 * kPitch];
+    __shared__ float s_sum[256 * kPitch];
+    __shared__ long s_base[256];
+    __shared__ int s_nh[256];
+    __shared__ int s_maxnh;
+    const long c0 = (long)blockIdx.x * blockDim.x;
+    const long c = c0 + threadIdx.x;
+    const bool valid = c < A.ncells;
+    smx_nccresult nr = A.ncc[valid ? c : c0];
+    const smx_ugrid g = A.gridpt3[valid ? c : c0];
     const int Pyramid_step = P->level;
     const int iteration = P->iteration;
     const double ncc_alpha = P->ncc_alpha;
@@ -848,27 +860,35 @@ __global__ void __launch_bounds__(256) k_awncc(const VllParams *__restrict__ P, 
     nr.result4 = 0;
     const bool max_mode = (iteration <= 3 && Pyramid_step == 4) || (iteration <= 2 && Pyramid_step == 3) ||
                           (iteration <= 1 && Pyramid_step == 2);
-    const long base = A.vox_off[c];
+    const long base = valid ? A.vox_off[c] : 0;
     // an RA pass may leave NumOfHeight above the row InitializeVoxel laid
     // out; never read past the row (the reference would read past its heap
     // row there)
-    const long nh = min((long)nr.NumOfHeight, (long)(A.vox_off[c + 1] - base));
+    const long nh = valid ? min((long)nr.NumOfHeight, (long)(A.vox_off[c + 1] - base)) : 0;
     const double gncc = dq(nr.GNCC);
-    // The row is walked in blocks of kAhead heights whose words (and SGM
-    // sums) are loaded before any of them is used: each lane walks its own
-    // row, so without that every step waits out one memory latency.
-    constexpr int kAhead = 8;
-    for (long h0 = 0; h0 < nh; h0 += kAhead) {
-        uint32_t wbuf[kAhead];
-        float sbuf[kAhead];
-#pragma unroll
-        for (int q = 0; q < kAhead; q++) {
-            const bool in = h0 + q < nh;
-            wbuf[q] = in ? A.vox[base + h0 + q] : 0u;
-            sbuf[q] = (in && A.sum) ? A.sum[base + h0 + q] : 0.f;
+    if (threadIdx.x == 0) s_maxnh = 0;
+    s_base[threadIdx.x] = base;
+    s_nh[threadIdx.x] = (int)nh;
+    __syncthreads();
+    atomicMax(&s_maxnh, (int)nh);
+    __syncthreads();
+    const int maxnh = s_maxnh;
+    for (long h0 = 0; h0 < maxnh; h0 += kChunk) {
+        __syncthreads();   // the previous chunk has been scanned
+#pragma unroll 4
+        for (int i = 0; i < kChunk; i++) {
+            const int idx = i * 256 + (int)threadIdx.x;
+            const int cl = idx / kChunk, q = idx % kChunk;
+            if (h0 + q < s_nh[cl]) {
+                const long at = s_base[cl] + h0 + q;
+                s_word[cl * kPitch + q] = A.vox[at];
+                if (A.sum) s_sum[cl * kPitch + q] = A.sum[at];
+            }
         }
-#pragma unroll
-        for (int k = 0; k < kAhead; k++) {
+        __syncthreads();
+        const uint32_t *wbuf = &s_word[threadIdx.x * kPitch];
+        const float *sbuf = &s_sum[threadIdx.x * kPitch];
+        for (int k = 0; k < kChunk; k++) {
             const long hs = h0 + k;
             if (hs >= nh) break;
             const float iter_height = (float)(nr.minHeight + hs * P->height_step);
@@ -926,6 +946,7 @@ __global__ void __launch_bounds__(256) k_awncc(const VllParams *__restrict__ P, 
             direction = t_direction;
         }
     }
+    if (!valid) return;
     if (A.sum && Pyramid_step == 0 && iteration >= 2) {
         nr.result0 = q_result(max_roh);
         nr.result1 = q_result(-1.0);
