@@ -105,25 +105,55 @@ __global__ void __launch_bounds__(64) k_sgm_path(SgmArgs A)
         return m;
     };
     auto inside = [&](int r, int q) { return r >= 0 && r < A.H && q >= 0 && q < A.W; };
+    // The first kPre * 64 heights of the next cell's cube words and sums are
+    // loaded one step ahead into registers (the cell's record two steps
+    // ahead), so a step waits for LDS and its arithmetic only; a cell's sums
+    // are touched by no other path of the launch in between.
+    constexpr int kPre = 2;
+    uint32_t wn[kPre];
+    float sn[kPre];
+    auto prefetch = [&](const Meta &mm) {
+        const int n2 = (int)min((long)mm.nr.NumOfHeight, (long)(mm.end - mm.off));
+#pragma unroll
+        for (int j = 0; j < kPre; j++) {
+            const int h = lane + 64 * j;
+            wn[j] = h < n2 ? A.vox[mm.off + h] : 0u;
+            sn[j] = h < n2 ? A.sum[mm.off + h] : 0.0f;
+        }
+    };
     row += A.dr;
     col += A.dc;
-    Meta next{};
-    if (inside(row, col)) next = load_meta((long)row * A.W + col);
+    Meta next{}, next2{};
+    if (inside(row, col)) {
+        next = load_meta((long)row * A.W + col);
+        prefetch(next);
+    }
+    if (inside(row + A.dr, col + A.dc)) next2 = load_meta((long)(row + A.dr) * A.W + col + A.dc);
     for (; inside(row, col); row += A.dr, col += A.dc) {
         // SGM_con_pos for cell (row, col); its predecessor on the path,
         // (row + v_row[d], col + u_col[d]) in the reference's tables
         // (setsm_code.cpp:7036-7037), is the previous cell.
         c = (long)row * A.W + col;
         const Meta m = next;
-        if (inside(row + A.dr, col + A.dc)) next = load_meta((long)(row + A.dr) * A.W + col + A.dc);
+        uint32_t wc[kPre];
+        float sc[kPre];
+#pragma unroll
+        for (int j = 0; j < kPre; j++) {
+            wc[j] = wn[j];
+            sc[j] = sn[j];
+        }
+        next = next2;
+        if (inside(row + A.dr, col + A.dc)) prefetch(next);
+        if (inside(row + 2 * A.dr, col + 2 * A.dc))
+            next2 = load_meta((long)(row + 2 * A.dr) * A.W + col + 2 * A.dc);
         nh = (int)min((long)m.nr.NumOfHeight, (long)(m.end - m.off));
         minH = m.nr.minHeight;
         gmin = m.gmin;
         gmax = m.gmax;
         off = m.off;
         const double V4 = (double)t_maxW - P2;
-        for (int h = lane; h < nh; h += 64) {
-            const uint32_t w = A.vox[off + h];
+        // one height of the cell: its cube word w and its sum before this path
+        auto visit = [&](int h, uint32_t w, float sum_before) {
             const float ih = (float)(minH + h * step);
             float out = 0.0f;
             if (ih >= gmin && ih <= gmax) {
@@ -151,11 +181,15 @@ __global__ void __launch_bounds__(64) k_sgm_path(SgmArgs A)
                 if (zero3) m = max2(0.0, m);
                 const double t_WNCC_sum = m - (double)t_maxW;
                 out = (float)(WNCC_sum + t_WNCC_sum);
-                A.sum[off + h] += out;
+                A.sum[off + h] = sum_before + out;
             }
             cur[h] = out;
             fcur[h] = (unsigned char)SMX_VOXEL_FLAG(w);
-        }
+        };
+#pragma unroll
+        for (int j = 0; j < kPre; j++)
+            if (lane + 64 * j < nh) visit(lane + 64 * j, wc[j], sc[j]);
+        for (int h = lane + 64 * kPre; h < nh; h += 64) visit(h, A.vox[off + h], A.sum[off + h]);
         if (!kLds) __threadfence_block();
         __syncthreads();
         float *tf = pre;
