@@ -23,11 +23,18 @@ __device__ __forceinline__ bool gate_cell(const VllParams *__restrict__ P, CellA
 // setsm_code.cpp:6157-6204 + 6259-6279: which cells enter the INCC loop.
 __global__ void __launch_bounds__(256) k_vll_gate(const VllParams *__restrict__ P, CellArgs A)
 {
+    // the accessible count: a ballot per wavefront, an LDS sum per block, one
+    // global atomic per block
+    __shared__ unsigned int s_acc;
+    if (threadIdx.x == 0) s_acc = 0;
+    __syncthreads();
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     bool accessible = false;
     if (c < A.ncells) accessible = gate_cell(P, A, c);
     const unsigned long long any = __ballot(accessible);
-    if ((threadIdx.x & 63) == 0 && any) atomicAdd(A.accessible, (unsigned long long)__popcll(any));
+    if ((threadIdx.x & 63) == 0 && any) atomicAdd(&s_acc, (unsigned int)__popcll(any));
+    __syncthreads();
+    if (threadIdx.x == 0 && s_acc) atomicAdd(A.accessible, (unsigned long long)s_acc);
 }
 
 __device__ __forceinline__ bool gate_cell(const VllParams *__restrict__ P, CellArgs &A, long c)
@@ -97,21 +104,40 @@ __global__ void __launch_bounds__(256) k_vll_fill_runs(CellArgs A)
 __global__ void __launch_bounds__(256) k_nh_stats(const smx_nccresult *__restrict__ ncc, long n,
                                                   unsigned long long *stats)
 {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned nh = c < n ? ncc[c].NumOfHeight : 0u;
-    unsigned long long runs[3];
+    // grid-stride sums in registers, a shuffle reduction per wavefront, LDS
+    // atomics per block, then one global atomic per block and counter (the
+    // four counters are shared by the whole launch)
+    __shared__ unsigned long long s_runs[3];
+    __shared__ unsigned int s_max;
+    if (threadIdx.x == 0) {
+        s_runs[0] = s_runs[1] = s_runs[2] = 0;
+        s_max = 0;
+    }
+    unsigned long long runs[3] = {0, 0, 0};
+    unsigned nh = 0;
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
+        const unsigned v = ncc[c].NumOfHeight;
 #pragma unroll
-    for (int k = 0; k < 3; k++) runs[k] = (nh + kRunLens[k] - 1) / kRunLens[k];
+        for (int k = 0; k < 3; k++) runs[k] += (v + kRunLens[k] - 1) / kRunLens[k];
+        nh = max(nh, v);
+    }
     for (int o = 32; o > 0; o >>= 1) {
 #pragma unroll
         for (int k = 0; k < 3; k++) runs[k] += (unsigned long long)__shfl_xor((long long)runs[k], o);
         nh = max(nh, (unsigned)__shfl_xor((int)nh, o));
     }
+    __syncthreads();
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
+        for (int k = 0; k < 3; k++) atomicAdd(&s_runs[k], runs[k]);
+        atomicMax(&s_max, nh);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
         for (int k = 0; k < 3; k++)
-            if (runs[k]) atomicAdd(&stats[k], runs[k]);
-        if (nh) atomicMax(&stats[3], (unsigned long long)nh);
+            if (s_runs[k]) atomicAdd(&stats[k], s_runs[k]);
+        if (s_max) atomicMax(&stats[3], (unsigned long long)s_max);
     }
 }
 
@@ -1047,7 +1073,7 @@ hipError_t launch_vll_redo(int half, const VllParams *dP, const CellArgs &A, hip
 
 hipError_t launch_nh_stats(const smx_nccresult *ncc, long n, unsigned long long *stats, hipStream_t s)
 {
-    if (n > 0) hipLaunchKernelGGL(k_nh_stats, dim3(nblk(n)), dim3(256), 0, s, ncc, n, stats);
+    if (n > 0) hipLaunchKernelGGL(k_nh_stats, dim3((unsigned)std::min<long>(nblk(n), 2048)), dim3(256), 0, s, ncc, n, stats);
     return hipGetLastError();
 }
 
