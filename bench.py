@@ -19,7 +19,7 @@ The same run reports beside it:
     stripe of the level (median of 3 runs, the reported CPU baseline),
     checked word for word against the GPU cube;
   * SGM aggregation + peak scan (AWNCC at a level <= SGM_py) on the level;
-  * `configs0`: the reference's CPU-runnable case (2048^2 pair, levels
+  * `reference_cpu_case`: the reference's CPU-runnable case (2048^2 pair, levels
     L4..L0, two iterations each) on the CPU restatement, timed beside the
     GPU FP64 and FAST paths;
   * `batched`: the mixed-size tile workload — 64 tiles with sides log-uniform
@@ -440,9 +440,9 @@ def between_run(args, job, sc, g, li, r0):
     return out
 
 
-# ------------------------------------------------------------ configs[0]
+# ------------------------------------------------------------ the reference's CPU-runnable case
 def c0_run(args):
-    """configs[0], the reference's CPU-runnable case: a 2048^2 pair over 1 km
+    """The reference's CPU-runnable case: a 2048^2 pair over 1 km
     (300 m relief, views +-12 degrees), matching levels L4..L0 at SETSM's grid
     spacings (8/8/6/3/2 m), 64 heights per cell, two iterations per level
     (the second adds the level-below term above level 0; its search space
@@ -558,7 +558,7 @@ def c0_run(args):
     dq = max(int(np.abs((a >> 16).astype(np.uint16).view(np.int16).astype(int)
                         - (b >> 16).astype(np.uint16).view(np.int16).astype(int)).max(initial=0))
              for a, b in zip(wf, wc))
-    out = dict(config=dict(workload="configs[0]: 2048^2 pair over 1 km, L4..L0 at 8/8/6/3/2 m, 64 heights, "
+    out = dict(config=dict(workload="reference CPU case: 2048^2 pair over 1 km, L4..L0 at 8/8/6/3/2 m, 64 heights, "
                                     "2 iterations per level", cells=[int(s[1]["shape"][0]) for s in steps]),
                hypotheses_evaluated=evf, cpu_s=tc, cpu_cores=O.threads(), cpu_runs_s=[round(c[0], 3) for c in cpu],
                cpu_hyp_per_s=ev64 / tc, gpu_fp64_s=t64, gpu_fast_s=tf, speedup_fp64_vs_cpu=tc / t64,
@@ -567,7 +567,7 @@ def c0_run(args):
                files=dict(files, note="pair read back from GeoTIFF + DigitalGlobe XML; DEMs written as GeoTIFF"))
     import shutil
     shutil.rmtree(fdir, ignore_errors=True)
-    log(f"configs[0]: {out}")
+    log(f"reference CPU case: {out}")
     return out
 
 
@@ -588,7 +588,7 @@ def render_pool(draws):
 
 
 def batched_run(args, draws, scenes):
-    """all tiles in one persistent FAST launch (captured with the
+    """Mixed-size tiles: all tiles in one persistent FAST launch (captured with the
     peak scan as a hipGraph and replayed) vs one FAST launch per tile vs the
     plain FP64 kernel per tile."""
     import torch
@@ -690,7 +690,7 @@ def batched_run(args, draws, scenes):
     achieved = (evaluated - redo_hyps) * fph / (kernel_ms / 1e3) / 1e12
     plain = ev64 / fp64_s
     return dict(
-        config=dict(workload="configs[3]: mixed-size tiles, one persistent launch", tiles=len(draws),
+        config=dict(workload="mixed-size tiles, one persistent launch", tiles=len(draws),
                     sides_cells=[min(d.side for d in draws), max(d.side for d in draws)],
                     heights=[min(d.nheights for d in draws), max(d.nheights for d in draws)],
                     relief_m=[round(min(d.relief for d in draws)), round(max(d.relief for d in draws))],
@@ -713,7 +713,7 @@ def batched_run(args, draws, scenes):
 
 
 def served_run(args, draws, scenes, cfg):
-    """Poisson requests (the tiles above, drawn without
+    """Served: Poisson requests (the tiles above, drawn without
     replacement) at `rate`/s, 4 levels x 2 iterations each, 200 GB budget."""
     from setsm_amd import smx
     from setsm_amd.serve import Request, Server
@@ -775,7 +775,7 @@ def served_run(args, draws, scenes, cfg):
                             note="wall time of the batched VerticalLineLocus calls (launch + hand-over + "
                                  "max_WNCC included)")
     plain["roofline_frac_fp64_peak"] = plain["achieved_tflops_in_vll_calls"] / PEAK_FP64_TFLOPS
-    return dict(config=dict(workload="configs[4]: Poisson tile requests, continuous batching", rate_per_s=args.serve_rate,
+    return dict(config=dict(workload="Poisson tile requests, continuous batching", rate_per_s=args.serve_rate,
                             requests=n, arrival_window_s=round(float(t[-1]), 3), levels=4, iterations_per_level=2,
                             budget_gib=200, sizes_cells=[int(min(draws[k].side for k in pick)),
                                                          int(max(draws[k].side for k in pick))],
@@ -804,7 +804,7 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="time every FAST launch shape and store the best")
     ap.add_argument("--tuning-out", default=None,
                     help="file the --autotune table is written to (default: the package's tuning.json)")
-    ap.add_argument("--no-c0", dest="c0", action="store_false", help="skip the configs[0] levels L4..L0 leg")
+    ap.add_argument("--no-c0", dest="c0", action="store_false", help="skip the reference CPU case (2048^2 pair, levels L4..L0)")
     ap.add_argument("--tiles", type=int, default=64, help="tiles of the mixed-size workload (0 = skip)")
     ap.add_argument("--tile-seed", type=int, default=2026)
     ap.add_argument("--batch-reps", type=int, default=2)
@@ -816,7 +816,7 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the VerticalLineLocus path shards by whole tiles with no "
-              "collective, and multi-GPU runs are not part of this build (use --gpus 1).", file=sys.stderr)
+              "collective, and runs over several GPUs are not implemented (use --gpus 1).", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -835,7 +835,7 @@ def main():
     out = {}
     headline(args, out)
     if args.c0:
-        out["configs0"] = c0_run(args)
+        out["reference_cpu_case"] = c0_run(args)
     if args.tiles > 0 or args.serve_requests > 0:
         from setsm_amd import autotune
         from setsm_amd.synth import draw_tiles

@@ -1,7 +1,7 @@
 /*
  * smx_oracle.h — CPU FP64 restatement of SETSM's VerticalLineLocus hot path.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker that the MI355X
+ * TEST INFRASTRUCTURE ONLY. This library is the reference the MI355X
  * kernels are compared against and the CPU baseline that bench.py times. It is
  * never linked into, loaded by or called from the product library; only
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.

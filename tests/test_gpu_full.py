@@ -1,14 +1,14 @@
 """Full-size workloads on the GPU, checked against the oracle on stripes.
 
-  * a 16384^2 level-0 pair over 8 km, 4000^2 cells at 2 m, 128
+  * the headline level: a 16384^2 level-0 pair over 8 km, 4000^2 cells at 2 m, 128
     heights per cell (2.05e9 hypotheses) in FP64 — a 64-row stripe (32.8 M
     hypotheses) bit for bit against the CPU restatement;
-  * configs[2]: the same level in the precision form — mean NCC within 1e-5
+  * the same level in the precision form — mean NCC within 1e-5
     of FP64 on the stripe, INCC within one quantum, the per-cell peak
     (first argmax of the mean NCC) the same except near-ties (FP64 top two
     closer than 1e-5), which are counted;
-  * configs[3]'s distribution: a 512^2-cell and a 4096^2-cell tile drawn from
-    it (relief, convergence and heights seeded), both precisions against the
+  * the mixed-tile distribution: a 512^2-cell and a 4096^2-cell tile drawn
+    from it (relief, convergence and heights seeded), both precisions against the
     oracle on a stripe.
 Over the whole level, FP64 and FAST must evaluate the same hypotheses and
 gate the same cells."""
@@ -125,7 +125,7 @@ def test_configs1_and_2_full_level(require_gpu):
     g["boundary"] = np.array([inset, inset, inset + cells * 2.0, inset + cells * 2.0])
     g["shape"] = (cells, cells)
     g["grid_res"] = 2.0
-    _check_level(sc, g, cells // 2, 64, "configs[1]")
+    _check_level(sc, g, cells // 2, 64, "headline level")
 
 
 @pytest.mark.parametrize("side,seed", [(512, 301), (4096, 302)])
@@ -137,4 +137,4 @@ def test_configs3_tiles(require_gpu, side, seed):
     print(f"tile: {d}", flush=True)
     sc, g = render_tile(d, device="cuda")
     torch.cuda.empty_cache()
-    _check_level(sc, g, side // 2, 16 if side > 1024 else 32, f"configs[3] {side}^2")
+    _check_level(sc, g, side // 2, 16 if side > 1024 else 32, f"mixed tile {side}^2")

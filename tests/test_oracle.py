@@ -1,5 +1,5 @@
-"""Pins of the CPU oracle (oracle/smx_oracle.c), the checker every GPU result
-is compared against. Each test checks the restatement against something it
+"""Pins of the CPU oracle (oracle/smx_oracle.c), the reference every GPU
+result is compared against. Each test checks the restatement against something it
 does not share code with: the reference's own header code (golden vectors
 from oracle/_ref, and the live library where it is built), closed forms,
 numpy, brute force on tiny inputs, or hand-made sequences.
