@@ -899,19 +899,33 @@ __global__ void __launch_bounds__(256) k_awncc(const VllParams *__restrict__ P, 
     atomicMax(&s_maxnh, (int)nh);
     __syncthreads();
     const int maxnh = s_maxnh;
-    for (long h0 = 0; h0 < maxnh; h0 += kChunk) {
-        __syncthreads();   // the previous chunk has been scanned
-#pragma unroll 4
+    // this thread's share of a chunk: item i is word q of cell cl; the next
+    // chunk's items are loaded into registers while this one is scanned
+    uint32_t rw[kChunk];
+    float rs[kChunk];
+    auto fetch = [&](long h0) {
+#pragma unroll
         for (int i = 0; i < kChunk; i++) {
             const int idx = i * 256 + (int)threadIdx.x;
             const int cl = idx / kChunk, q = idx % kChunk;
-            if (h0 + q < s_nh[cl]) {
-                const long at = s_base[cl] + h0 + q;
-                s_word[cl * kPitch + q] = A.vox[at];
-                if (A.sum) s_sum[cl * kPitch + q] = A.sum[at];
-            }
+            const bool in = h0 + q < s_nh[cl];
+            const long at = s_base[cl] + h0 + q;
+            rw[i] = in ? A.vox[at] : 0u;
+            rs[i] = (in && A.sum) ? A.sum[at] : 0.0f;
+        }
+    };
+    if (maxnh > 0) fetch(0);
+    for (long h0 = 0; h0 < maxnh; h0 += kChunk) {
+        __syncthreads();   // the previous chunk has been scanned
+#pragma unroll
+        for (int i = 0; i < kChunk; i++) {
+            const int idx = i * 256 + (int)threadIdx.x;
+            const int cl = idx / kChunk, q = idx % kChunk;
+            s_word[cl * kPitch + q] = rw[i];
+            s_sum[cl * kPitch + q] = rs[i];
         }
         __syncthreads();
+        if (h0 + kChunk < maxnh) fetch(h0 + kChunk);
         const uint32_t *wbuf = &s_word[threadIdx.x * kPitch];
         const float *sbuf = &s_sum[threadIdx.x * kPitch];
         for (int k = 0; k < kChunk; k++) {
