@@ -1,8 +1,9 @@
-// mps.hip — the two data-parallel steps of SETSM's per-iteration host loop
-// that sit either side of the TIN: SelectMPs (setsm_code.cpp:8535-8889),
-// which turns AWNCC's peaks into matched points and updates the search-space
-// grid, and SetHeightRange (setsm_code.cpp:9851-10261), which rasterises the
-// TIN's triangles into the next iteration's per-cell height ranges. The TIN
+// mps.hip — the data-parallel steps of SETSM's per-iteration host loop that
+// sit either side of the TIN: SelectMPs (setsm_code.cpp:8535-8889), which
+// turns AWNCC's peaks into matched points and updates the search-space grid,
+// SetHeightRange (setsm_code.cpp:9851-10261), which rasterises the TIN's
+// triangles into the next iteration's per-cell height ranges, and
+// DecisionMPs' SetHeightRange_blunder (setsm_code.cpp:10393-10491). The TIN
 // itself stays with the caller's host code.
 //
 // Built with -ffp-contract=off: every expression keeps the reference's
@@ -14,15 +15,15 @@
 // decision and the grid update, and an exclusive scan of the selection flags
 // so the points come out in cell order, as the reference appends them.
 //
-// SetHeightRange_blunder, inside DecisionMPs' blunder loop, gives each cell
-// the highest TIN height of the triangles containing it (a CAS float max).
-//
 // SetHeightRange: the reference walks the triangles in list order and lets
 // the first triangle that contains a cell claim it. Here the claim is an
 // atomicMin of the triangle index per cell (16 lanes per triangle over its
 // bounding box), which picks the same triangle; a per-cell pass then applies
 // the owner's update, and a last per-cell pass fills unclaimed cells from a
 // snapshot of their neighbours and writes the returned grid.
+//
+// SetHeightRange_blunder, inside DecisionMPs' blunder loop, gives each cell
+// the highest TIN height of the triangles containing it (a CAS float max).
 
 #include <algorithm>
 #include <cstring>
@@ -32,7 +33,6 @@
 namespace smx {
 
 // ------------------------------------------------------------ SelectMPs
-
 
 __global__ void __launch_bounds__(256) k_mps_hist(MpsDev D)
 {
@@ -52,8 +52,8 @@ __global__ void __launch_bounds__(256) k_mps_hist(MpsDev D)
         if (h[i]) atomicAdd(&D.hist[i], (unsigned long long)h[i]);
 }
 
-// setsm_code.cpp:8546-8614: the level's minimum peak rho, the smallest bin
-// whose upper tail holds more than the level's fraction of the cells.
+// setsm_code.cpp:8546-8614: the level's minimum peak rho, the highest bin at
+// which the cells in it and above hold more than the level's fraction.
 __global__ void k_mps_threshold(MpsDev D)
 {
     if (threadIdx.x != 0) return;
@@ -201,7 +201,6 @@ hipError_t launch_write_mps(const MpsDev &D, hipStream_t s)
 }
 
 // ------------------------------------------------------------ SetHeightRange
-
 
 // doubles as unsigned keys whose integer order is the numeric order
 __device__ __forceinline__ unsigned long long okey(double v)
