@@ -171,6 +171,16 @@ typedef struct smx_hr_args {
     int32_t check_matching_rate;   /* level_check_matching_rate                */
 } smx_hr_args;
 
+/* blunder_detection_TIN's per-call arguments (setsm_code.cpp:9158); the
+ * level, IsRA, grid and Boundary come from smx_level. */
+typedef struct smx_bd_args {
+    double  hinterval;             /* *LevelInfo.Hinterval                     */
+    int32_t iteration;
+    int32_t flag_blunder;          /* flag_blunder: 1 blunder pass, 0 anchors  */
+    int32_t count_bl;              /* DecisionMPs' loop count (a uint16)       */
+    int32_t pad_;
+} smx_bd_args;
+
 #if defined(__cplusplus)
 static_assert(sizeof(smx_point3) == 32 && offsetof(smx_point3, flag) == 24, "D3DPOINT layout");
 static_assert(sizeof(smx_tri) == 12, "UI3DPOINT layout");

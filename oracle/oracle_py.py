@@ -11,8 +11,8 @@ import os
 
 import numpy as np
 
-from setsm_amd.records import (NCCRESULT_DTYPE, POINT3_DTYPE, TRI_DTYPE, UGRID_DTYPE, SmxHrArgs, SmxLevel,
-                               SmxMpsArgs)
+from setsm_amd.records import (NCCRESULT_DTYPE, POINT3_DTYPE, TRI_DTYPE, UGRID_DTYPE, SmxBdArgs, SmxHrArgs,
+                               SmxLevel, SmxMpsArgs)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 _lib = None
@@ -71,6 +71,8 @@ def lib():
         L.orc_blunder_half_template.argtypes = [vp, ip, ip, ip]
         L.orc_vll_blunder.restype = ip
         L.orc_vll_blunder.argtypes = [vp, vp, ip, vp, ip, ip, vp, vp]
+        L.orc_blunder_detection_tin.restype = ip
+        L.orc_blunder_detection_tin.argtypes = [vp, vp, vp, vp, C.c_long, vp, C.c_long, vp, vp, vp, dp, dp, dp]
         L.orc_set_threads.argtypes = [ip]
         L.orc_get_threads.restype = ip
         _lib = L
@@ -200,6 +202,29 @@ def vll_blunder(level_inputs, pyr, blunder_level, ortho_wgs, ugrid, iteration, b
     if r < 0:
         raise MemoryError("oracle VerticalLineLocus_blunder ran out of memory")
     return out
+
+
+def blunder_detection_tin(level_inputs, ugrid, ortho_ncc, pts, tris, iteration, flag_blunder, hinterval,
+                          count_bl=0, blunder_count=(0, 0), minz=100000.0, maxz=-100000.0):
+    """blunder_detection_TIN (setsm_code.cpp:9158-9782): pts' flags and
+    ugrid's anchor_flag in place. Returns (detected, blunder_count, minz,
+    maxz, height_th); the counts and heights continue from the values given,
+    as DecisionMPs' running totals do."""
+    a = SmxBdArgs(float(hinterval), int(iteration), int(bool(flag_blunder)), int(count_bl), 0)
+    assert pts.dtype == POINT3_DTYPE and pts.flags.c_contiguous
+    tris = np.ascontiguousarray(tris, TRI_DTYPE)
+    assert ugrid.dtype == UGRID_DTYPE and ugrid.flags.c_contiguous
+    on = np.ascontiguousarray(ortho_ncc, np.float32)
+    assert len(on) == len(ugrid)
+    det = np.zeros(max(len(pts), 1), np.uint8)
+    bc = (C.c_long * 2)(*blunder_count)
+    lo, hi, th = C.c_double(minz), C.c_double(maxz), C.c_double(float("nan"))
+    r = lib().orc_blunder_detection_tin(C.addressof(level_inputs.level), C.addressof(a), _p(on),
+                                        _p(pts) if len(pts) else None, len(pts), _p(tris) if len(tris) else None,
+                                        len(tris), _p(ugrid), _p(det), bc, C.byref(lo), C.byref(hi), C.byref(th))
+    if r < 0:
+        raise MemoryError("oracle blunder_detection_TIN ran out of memory")
+    return det[:len(pts)].astype(bool), (bc[0], bc[1]), lo.value, hi.value, th.value
 
 
 def _p(a):

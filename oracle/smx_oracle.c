@@ -13,8 +13,11 @@
  * (test_oracle.py), window counts, numpy corrcoef, brute-force bilinear,
  * hand-derived tables for SetNCC_alpha, check_image_boundary, the
  * VerticalLineLocus gate, InitializeVoxel and FindPeakNcc
- * (test_oracle_pins.py), a brute-force GNCC (test_oracle_gncc.py) and
- * closed-form SGM cases (test_oracle_sgm.py).
+ * (test_oracle_pins.py), a brute-force GNCC (test_oracle_gncc.py),
+ * closed-form SGM cases (test_oracle_sgm.py), and hand-derived cases of the
+ * steps between iterations: SelectMPs and SetHeightRange
+ * (test_oracle_mps.py), DecisionMPs' blunder helpers (test_oracle_blunder.py)
+ * and blunder_detection_TIN (test_oracle_bdtin.py).
  *
  * Scope: two images (reference_id 0 or 1, ti the other) and the SB (RPC)
  * sensor. The ortho GNCC term (setsm_code.cpp:6240-6305) is computed, from
@@ -1866,5 +1869,287 @@ int orc_vll_blunder(const smx_level *L, const smx_view bview[2], int blunder_lev
     }
     free(cd[0]);
     free(cd[1]);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * blunder_detection_TIN (setsm_code.cpp:9158-9782), the OpenMP loop run in
+ * index order. Every schedule of that loop ends the same: a point's own test
+ * only ever sets flag 1 on itself, a neighbour test only turns a flag-0 point
+ * into 3, and both count as blunders at the end.
+ */
+
+/* setsm_code.cpp:9121-9156 set_blunder: only a point still at flag 0 changes,
+ * and only then is it recorded as newly detected. */
+static void bd_set_blunder(long index, uint8_t val, smx_point3 *pts, uint8_t *detected)
+{
+    if (pts[index].flag != 0) return;
+    pts[index].flag = val;
+    detected[index] = 1;
+}
+
+static long bd_cell(const smx_level *L, double x, double y)
+{
+    const long col = (long)((x - L->boundary[0]) / L->grid_resolution + 0.5);
+    const long row = (long)((y - L->boundary[1]) / L->grid_resolution + 0.5);
+    return (long)L->grid_w * row + col;
+}
+
+int orc_blunder_detection_tin(const smx_level *L, const smx_bd_args *a, const float *ortho_ncc, smx_point3 *pts,
+                              long npts, const smx_tri *tris, long ntris, smx_ugrid *g, uint8_t *detected,
+                              long blunder_count[2], double *minz, double *maxz, double *height_th_out)
+{
+    const int step = L->level, iteration = a->iteration, IsRA = L->is_ra;
+    const int flag_blunder = a->flag_blunder != 0;
+    const unsigned count_bl = (uint16_t)a->count_bl;
+    const double gridspace = L->grid_resolution;
+    memset(detected, 0, (size_t)(npts > 0 ? npts : 0));
+    if (step == 0 && iteration == 3) return 0;
+    enum { max_nodes = 30 };
+    const uint32_t hdiffcount = (uint32_t)a->hinterval;
+    uint32_t *savenode = (uint32_t *)malloc(sizeof(uint32_t) * max_nodes * (size_t)(npts > 0 ? npts : 1));
+    uint32_t *nodecount = (uint32_t *)calloc((size_t)(npts > 0 ? npts : 1), sizeof(uint32_t));
+    uint32_t *hdiffbin = (uint32_t *)calloc((size_t)hdiffcount + 1, sizeof(uint32_t));
+    if (!savenode || !nodecount || !hdiffbin) {
+        free(savenode);
+        free(nodecount);
+        free(hdiffbin);
+        return -1;
+    }
+    /* setsm_code.cpp:9183-9287: each point's first 30 triangles, and the
+     * height differences along the edges of the triangles whose vertices all
+     * exist. The ortho-NCC mean and deviation gathered there feed a threshold
+     * that :9304-9313 overwrites, so they are not computed. */
+    double sum_data2 = 0.0, sum_data = 0.0;
+    long dh_count = 0;
+    for (long t = 0; t < ntris; t++) {
+        const uint32_t v[3] = {tris[t].a, tris[t].b, tris[t].c};
+        smx_point3 p[3] = {{0}};
+        int check_pt_index = 1;
+        for (int k = 0; k < 3; k++) {
+            if ((long)v[k] < npts) {
+                if (nodecount[v[k]] < max_nodes) {
+                    savenode[(long)v[k] * max_nodes + nodecount[v[k]]] = (uint32_t)t;
+                    nodecount[v[k]]++;
+                }
+                p[k] = pts[v[k]];
+            } else
+                check_pt_index = 0;
+        }
+        if (check_pt_index) {
+            double dh1 = (double)fabs(p[0].z - p[1].z);
+            sum_data += dh1;
+            sum_data2 += dh1 * dh1;
+            dh_count++;
+            double dh2 = (double)fabs(p[0].z - p[2].z);
+            sum_data += dh2;
+            sum_data2 += dh2 * dh2;
+            dh_count++;
+            double dh3 = (double)fabs(p[1].z - p[2].z);
+            sum_data += dh3;
+            sum_data2 += dh3 * dh3;
+            dh_count++;
+            dh1 = (uint32_t)dh1;
+            if (dh1 < hdiffcount) hdiffbin[(uint32_t)dh1]++;
+            dh2 = (uint32_t)dh2;
+            if (dh2 < hdiffcount) hdiffbin[(uint32_t)dh2]++;
+            dh3 = (uint32_t)dh3;
+            if (dh3 < hdiffcount) hdiffbin[(uint32_t)dh3]++;
+        }
+    }
+    /* setsm_code.cpp:9289-9318: the ortho-NCC thresholds of the level */
+    double ortho_ncc_th;
+    if (step == 4) ortho_ncc_th = 0.6 - (iteration - 1) * 0.01;
+    else if (step >= 3) ortho_ncc_th = 0.5 - (iteration - 1) * 0.01;
+    else if (step == 2) ortho_ncc_th = 0.4 - (iteration - 1) * 0.01;
+    else if (step == 1) ortho_ncc_th = 0.3;
+    else ortho_ncc_th = 0.2;
+    const double ortho_ancc_th = 100.;
+    double th_ref_ncc = 0.1 + (iteration - 1) * 0.05;
+    if (th_ref_ncc > ortho_ncc_th) th_ref_ncc = ortho_ncc_th;
+    /* setsm_code.cpp:9320-9418: the height-difference threshold */
+    int check_dh = 0;
+    if (step <= 4 && step >= 3) check_dh = 1;
+    else if (step == 2) {
+        if (iteration <= 4) check_dh = 1;
+    } else if (step <= 1) {
+        if (iteration <= 2) check_dh = 1;
+    }
+    /* GSD is read only where step >= 3; at step 0 the reference's
+     * pwrtwo(step - 1) is a negative shift, and the branch below for
+     * step < 3 replaces the threshold it would feed. */
+    double GSD = gridspace;
+    if (step >= 3 && gridspace < ORC_PWRTWO(step - 1)) GSD = ORC_PWRTWO(step - 1);
+    const double height_th_1 = a->hinterval;
+    double height_th_2 = GSD * 10;
+    if (IsRA == 1) height_th_2 = GSD * 30;
+    double height_th = height_th_1 < height_th_2 ? height_th_1 : height_th_2;
+    if (step >= 3) {
+        double up_3sigma;
+        if (check_dh) {
+            const double mean = sum_data / dh_count;
+            const double std = sqrt((sum_data2 - (sum_data) * (sum_data) / dh_count) / dh_count);
+            up_3sigma = mean + 3 * std;
+        } else
+            up_3sigma = height_th;
+        if (up_3sigma < height_th) {
+            if (up_3sigma < height_th / 2.0) height_th = height_th / 2.0;
+            else height_th = up_3sigma;
+        }
+    } else {
+        uint32_t total_dh = 0, th80 = 1000;
+        long tcnt = 0;
+        int check_total_dh = 0;
+        while (tcnt < (long)hdiffcount && !check_total_dh) {
+            total_dh += hdiffbin[tcnt];
+            const double per = (double)total_dh / (double)dh_count * 100.0;
+            if (per > 99) {
+                th80 = (uint32_t)tcnt;
+                check_total_dh = 1;
+            }
+            tcnt++;
+        }
+        if (!check_total_dh) th80 = (uint32_t)(tcnt - 1);
+        const double mean = sum_data / dh_count;
+        const double std = sqrt((sum_data2 - (sum_data) * (sum_data) / dh_count) / dh_count);
+        const double up_3sigma = mean + 3 * std;
+        if (up_3sigma < th80) height_th = up_3sigma;
+        else height_th = th80;
+        if (height_th > 50) height_th = 50;
+    }
+    free(hdiffbin);
+    if (height_th_out) *height_th_out = height_th;
+
+    /* setsm_code.cpp:9441-9756: each point against its triangles */
+    for (long index = 0; index < npts; index++) {
+        if (pts[index].flag == 1) continue;
+        int pt_is_blunder = 0, count_th_positive = 0, count_th_negative = 0, count = 0, check_neigh = 0;
+        const int max_iter = nodecount[index] < max_nodes ? (int)nodecount[index] : max_nodes;
+        const long ref_index = bd_cell(L, pts[index].x, pts[index].y);
+        g[ref_index].anchor_flag = 0;
+        if (!IsRA) {
+            if (ortho_ncc[ref_index] < th_ref_ncc && step >= 2) pt_is_blunder = 1;
+        }
+        if (step >= 3 && iteration >= 4) {
+            if (ortho_ncc[ref_index] >= 0.95 && flag_blunder) g[ref_index].anchor_flag = 1;
+        }
+        for (int it = 0; it < max_iter; it++) {
+            const uint32_t tri = savenode[index * max_nodes + it];
+            if (!((long)tri < ntris && tri > 0)) continue;
+            long reference_index = 0, target_index_0 = 0, target_index_1 = 0;
+            const uint32_t temp_tri[3] = {tris[tri].a, tris[tri].b, tris[tri].c};
+            int check_index = 0;
+            for (int kk = 0; kk < 3; kk++) {
+                if ((long)temp_tri[kk] == index) reference_index = index;
+                else if (!check_index) {
+                    target_index_0 = temp_tri[kk];
+                    check_index = 1;
+                } else
+                    target_index_1 = temp_tri[kk];
+            }
+            if (!(reference_index < npts && target_index_0 < npts && target_index_1 < npts)) continue;
+            const smx_point3 pt0 = pts[reference_index], pt1 = pts[target_index_0], pt2 = pts[target_index_1];
+            const double angle = tin_slope_deg(&pt0, &pt1, &pt2);
+            const double dh1 = pt0.z - pt1.z, dh2 = pt0.z - pt2.z, dh3 = pt1.z - pt2.z;
+            if (dh1 >= 0 && dh2 >= 0 && angle > 30) count_th_positive++;
+            if (dh1 < 0 && dh2 < 0 && angle > 30) count_th_negative++;
+            int check_match = 0;
+            if (step > 1) check_match = 1;
+            else if (step == 1) {
+                if (iteration == 1 && count_bl <= 5) check_match = 1;
+            } else if (step == 0) {
+                if (iteration == 1 && count_bl <= 5) check_match = 1;
+            }
+            if (check_match) {
+                const double ddh_1 = fabs(dh1) - fabs(dh2), ddh_2 = fabs(dh1) - fabs(dh3),
+                             ddh_3 = fabs(dh2) - fabs(dh3);
+                if (fabs(ddh_1) > height_th || fabs(ddh_2) > height_th || fabs(ddh_3) > height_th) {
+                    const long order[3] = {reference_index, target_index_0, target_index_1};
+                    const double height[3] = {pt0.z, pt1.z, pt2.z};
+                    const long Index[3] = {bd_cell(L, pt0.x, pt0.y), bd_cell(L, pt1.x, pt1.y),
+                                           bd_cell(L, pt2.x, pt2.y)};
+                    int t_o_min, t_o_max, t_o_mid;
+                    check_neigh = 1;
+                    if (height[0] > height[1]) {
+                        if (height[0] > height[2]) {
+                            t_o_max = 0;
+                            if (height[1] > height[2]) { t_o_min = 2; t_o_mid = 1; }
+                            else { t_o_min = 1; t_o_mid = 2; }
+                        } else { t_o_max = 2; t_o_min = 1; t_o_mid = 0; }
+                    } else {
+                        if (height[1] > height[2]) {
+                            t_o_max = 1;
+                            if (height[0] > height[2]) { t_o_min = 2; t_o_mid = 0; }
+                            else { t_o_min = 0; t_o_mid = 2; }
+                        } else { t_o_max = 2; t_o_mid = 1; t_o_min = 0; }
+                    }
+                    const double h1 = height[t_o_mid] - height[t_o_min], h2 = height[t_o_max] - height[t_o_mid];
+                    const double dh = h1 - h2;
+                    long nb = -1;
+                    if (dh > 0 && dh > height_th) {
+                        if (IsRA == 1) nb = order[t_o_min];
+                        else if (flag_blunder) {
+                            if (ortho_ncc[Index[t_o_min]] < ortho_ncc_th) nb = order[t_o_min];
+                        } else if (ortho_ncc[Index[t_o_min]] < ortho_ancc_th)
+                            nb = order[t_o_min];
+                    } else if (dh < 0 && fabs(dh) > height_th) {
+                        if (IsRA == 1) nb = order[t_o_max];
+                        else if (flag_blunder) {
+                            if (ortho_ncc[Index[t_o_max]] < ortho_ncc_th) nb = order[t_o_max];
+                        } else if (ortho_ncc[Index[t_o_max]] < ortho_ancc_th)
+                            nb = order[t_o_max];
+                    }
+                    if (nb >= 0) bd_set_blunder(nb, 3, pts, detected);
+                }
+            }
+            count++;
+        }
+        if (IsRA == 1) {
+            if (check_neigh && !flag_blunder) pt_is_blunder = 1;
+            if ((count_th_positive >= (int)(count * 0.7 + 0.5) || count_th_negative >= (int)(count * 0.7 + 0.5)) &&
+                count > 1)
+                pt_is_blunder = 1;
+        } else {
+            if (check_neigh && !flag_blunder) {
+                if (ortho_ncc[ref_index] < ortho_ancc_th) pt_is_blunder = 1;
+            }
+            double count_threshold = count * 0.5;
+            if (step >= 2) count_threshold = count * 0.5;
+            else if (step == 1) count_threshold = (int)(count * 0.6 + 0.5);
+            else count_threshold = (int)(count * 0.7 + 0.5);
+            if ((count_th_positive >= count_threshold || count_th_negative >= count_threshold) && count > 1) {
+                if (flag_blunder) {
+                    double tmp_th = 0.6 - (4 - step) * 0.1;
+                    if (step == 1) tmp_th = 0.6;
+                    if (step >= 1) {
+                        if (ortho_ncc[ref_index] < tmp_th) pt_is_blunder = 1;
+                    } else if (step == 0) {
+                        if (iteration <= 1) {
+                            if (ortho_ncc[ref_index] < 0.9) pt_is_blunder = 1;
+                        } else if (iteration == 2)
+                            pt_is_blunder = 1;
+                        else if (ortho_ncc[ref_index] < ortho_ncc_th)
+                            pt_is_blunder = 1;
+                    }
+                } else if (ortho_ncc[ref_index] < ortho_ancc_th)
+                    pt_is_blunder = 1;
+            }
+        }
+        if (pt_is_blunder) bd_set_blunder(index, 1, pts, detected);
+    }
+    /* setsm_code.cpp:9758-9776: counts, flag 3 -> 1, the kept points' heights */
+    for (long t = 0; t < npts; t++) {
+        if (pts[t].flag == 1) blunder_count[1]++;
+        else if (pts[t].flag == 3) {
+            blunder_count[1]++;
+            pts[t].flag = 1;
+        } else {
+            blunder_count[0]++;
+            if (*minz > pts[t].z) *minz = pts[t].z;
+            if (*maxz < pts[t].z) *maxz = pts[t].z;
+        }
+    }
+    free(savenode);
+    free(nodecount);
     return 0;
 }

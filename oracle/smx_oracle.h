@@ -117,6 +117,17 @@ int  orc_blunder_half_template(const smx_level *L, int blunder_level, int iterat
 int  orc_vll_blunder(const smx_level *L, const smx_view bview[2], int blunder_level, const double *ortho_wgs,
                      int iteration, int bblunder, smx_ugrid *gridpt3, float *out);
 
+/* setsm_code.cpp:9158-9782 blunder_detection_TIN: every point that is not
+ * an old blunder (flag 1) is tested against its first 30 triangles; new
+ * blunders end at flag 1 and are marked in detected (npts bytes, cleared
+ * first), GridPT3.anchor_flag is reset or set at each tested point's cell,
+ * blunder_count[0] / [1] and *minz / *maxz accumulate as in the reference.
+ * *height_th (may be NULL) receives the height-difference threshold. Does
+ * nothing at level 0 iteration 3. Returns 0, or -1 if host memory ran out. */
+int orc_blunder_detection_tin(const smx_level *L, const smx_bd_args *a, const float *ortho_ncc, smx_point3 *pts,
+                              long npts, const smx_tri *tris, long ntris, smx_ugrid *gridpt3, uint8_t *detected,
+                              long blunder_count[2], double *minz, double *maxz, double *height_th);
+
 /* Pyramid inputs. Template.hpp:45-132 (9x9, sigma 1.5 decimation),
  * SubFunctions.cpp:2418-2470 (Sobel), SubFunctions.cpp:2473-2571 (Orientation). */
 void orc_gauss_pyramid(const uint16_t *in, uint32_t w, uint32_t h, uint16_t *out);

@@ -44,6 +44,12 @@ class SmxHrArgs(C.Structure):
                 ("check_matching_rate", C.c_int32)]
 
 
+class SmxBdArgs(C.Structure):
+    """smx_bd_args: blunder_detection_TIN's per-call arguments (setsm_code.cpp:9158)."""
+    _fields_ = [("hinterval", C.c_double), ("iteration", C.c_int32), ("flag_blunder", C.c_int32),
+                ("count_bl", C.c_int32), ("pad_", C.c_int32)]
+
+
 def voxel_flag(words):
     return (np.asarray(words, dtype=np.uint32) & 0xFF).astype(np.uint8)
 
