@@ -235,8 +235,25 @@ int  smx_job_set_blunder_level(smx_job *job, int blunder_level, const double *lo
  * reference's operation order. Writes GridPT3.ortho_ncc[1] and
  * Mean_ortho_ncc, and (if not NULL) the per-cell value to ortho_ncc. */
 int  smx_vertical_line_locus_blunder(smx_job *job, int iteration, int bblunder, float *ortho_ncc);
-/* The four calls above leave their kernel time on the job stream (host
- * copies excluded) in smx_job_last_kernel_ms. */
+/* blunder_detection_TIN (setsm_code.cpp:9158-9782, setsm_code.hpp:116): each
+ * matched point that is not an old blunder (flag 1) is tested against its
+ * first 30 triangles of the TIN (host arrays, as DecisionMPs holds them) and
+ * the per-cell ortho NCC: ortho_ncc (host, one float per cell) or, if NULL,
+ * the last smx_vertical_line_locus_blunder output on this level, which stays
+ * on the device. New blunders end at flag 1 in pts and, if detected is not
+ * NULL, at 1 in detected (npts bytes, all written); GridPT3.anchor_flag is
+ * reset or set at each tested point's cell. blunder_count[0] (kept points)
+ * and [1] (blunders) and *minz_mp / *maxz_mp (may be NULL: the kept points'
+ * heights) continue from the values given, as DecisionMPs' running totals
+ * do. A no-op at level 0 iteration 3, as in the reference. SMX_E_ARG if a
+ * point lies outside the grid, before anything changes. The points are
+ * tested on the GPU; the edge statistics the threshold comes from are
+ * summed on the host in triangle order, as the reference sums them. */
+int  smx_blunder_detection_tin(smx_job *job, const smx_bd_args *args, const float *ortho_ncc, smx_point3 *pts,
+                               long npts, const smx_tri *tris, long ntris, uint8_t *detected, long blunder_count[2],
+                               double *minz_mp, double *maxz_mp);
+/* The five calls above leave their kernel time on the job stream (host
+ * copies and host passes excluded) in smx_job_last_kernel_ms. */
 
 /* Number of hypotheses in the cube (sum of NumOfHeight). */
 long long smx_job_num_hypotheses(const smx_job *job);

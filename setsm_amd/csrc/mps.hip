@@ -3,8 +3,9 @@
 // turns AWNCC's peaks into matched points and updates the search-space grid,
 // SetHeightRange (setsm_code.cpp:9851-10261), which rasterises the TIN's
 // triangles into the next iteration's per-cell height ranges, and
-// DecisionMPs' SetHeightRange_blunder (setsm_code.cpp:10393-10491). The TIN
-// itself stays with the caller's host code.
+// DecisionMPs' SetHeightRange_blunder (setsm_code.cpp:10393-10491) and
+// blunder_detection_TIN (setsm_code.cpp:9158-9782). The TIN itself stays with
+// the caller's host code.
 //
 // Built with -ffp-contract=off: every expression keeps the reference's
 // operand order and its double rounding, so the grid, the point list and the
@@ -23,7 +24,8 @@
 // snapshot of their neighbours and writes the returned grid.
 //
 // SetHeightRange_blunder, inside DecisionMPs' blunder loop, gives each cell
-// the highest TIN height of the triangles containing it (a CAS float max).
+// the highest TIN height of the triangles containing it (a CAS float max);
+// blunder_detection_TIN tests each point against its triangles (below).
 
 #include <algorithm>
 #include <cstring>
@@ -568,6 +570,241 @@ hipError_t launch_set_height_range_blunder(const HrDev &D, float *heights, hipSt
     if (D.ntris > 0)
         hipLaunchKernelGGL(k_hrb_claim, dim3((unsigned)((D.ntris * 16 + 255) / 256)), dim3(256), 0, s, D, heights);
     hipLaunchKernelGGL(k_hrb_apply, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, D, (const float *)heights);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ blunder_detection_TIN
+
+// setsm_code.cpp:9158-9782. The reference's serial pre-pass over the triangles
+// builds each point's list of its first 30 triangles and the statistics of
+// the edge height differences; the statistics are summed on the host in
+// triangle order (bit-exact), the lists are built here: counts, an exclusive
+// scan, then the lists. A list holds a point's triangles in no set order;
+// only a point with more than 30 reads them in index order, through a
+// selection of its 30 smallest. The per-point pass is the reference's OpenMP
+// loop with the race taken out: a point's own test and a neighbour's blame
+// each set a bit, and the host turns the bits into flags.
+
+__device__ __forceinline__ long bd_cell(const BdDev &D, double x, double y)
+{
+    const long col = (long)((x - D.b0) / D.gres + 0.5);
+    const long row = (long)((y - D.b1) / D.gres + 0.5);
+    return D.W * row + col;
+}
+
+__device__ __forceinline__ void bd_mark(const BdDev &D, long i, uint32_t bit)
+{
+    atomicOr(&D.state[i >> 2], bit << (8 * (i & 3)));
+}
+
+// every point's cell inside the grid (the reference indexes the ortho NCC
+// with it unchecked)
+__global__ void __launch_bounds__(256) k_bd_check(BdDev D)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D.npts) return;
+    const smx_point3 P = D.pts[i];
+    const double fc = (P.x - D.b0) / D.gres + 0.5, fr = (P.y - D.b1) / D.gres + 0.5;
+    if (!(fc > -1.0 && fc < (double)D.W && fr > -1.0 && fr < (double)D.H)) atomicOr(D.err, 1);
+}
+
+__global__ void __launch_bounds__(256) k_bd_degree(BdDev D)
+{
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < D.ntris; t += stride) {
+        const smx_tri tr = D.tris[t];
+        const uint32_t v[3] = {tr.a, tr.b, tr.c};
+        for (int k = 0; k < 3; k++)
+            if ((long)v[k] < D.npts) atomicAdd((unsigned long long *)&D.deg[v[k]], 1ull);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_bd_fill(BdDev D)
+{
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < D.ntris; t += stride) {
+        const smx_tri tr = D.tris[t];
+        const uint32_t v[3] = {tr.a, tr.b, tr.c};
+        for (int k = 0; k < 3; k++)
+            if ((long)v[k] < D.npts) {
+                const unsigned long long left = atomicAdd((unsigned long long *)&D.deg[v[k]], ~0ull);
+                D.adj[D.off[v[k]] + (long)left - 1] = (uint32_t)t;
+            }
+    }
+}
+
+// setsm_code.cpp:9441-9756, one lane per point
+__global__ void __launch_bounds__(256) k_bd_points(BdDev D)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D.npts) return;
+    const smx_point3 P = D.pts[i];
+    if (P.flag == 1) return;
+    const int level = D.level;
+    const long ref = bd_cell(D, P.x, P.y);
+    const double on = (double)D.oncc[ref];
+    bool blunder = false;
+    if (!D.is_ra && on < D.th_ref_ncc && level >= 2) blunder = true;
+    const unsigned anchor = (level >= 3 && D.iteration >= 4 && on >= 0.95 && D.flag_blunder) ? 1u : 0u;
+    atomicMax(&D.akey[ref], ((unsigned long long)i << 1) | anchor);
+
+    int cpos = 0, cneg = 0, count = 0;
+    bool neigh = false;
+    auto visit = [&](uint32_t t) {
+        if (t == 0 || (long)t >= D.ntris) return;
+        const smx_tri tr = D.tris[t];
+        const uint32_t tt[3] = {tr.a, tr.b, tr.c};
+        long refi = 0, t0 = 0, t1 = 0;
+        bool ci = false;
+        for (int kk = 0; kk < 3; kk++) {
+            if ((long)tt[kk] == i) refi = i;
+            else if (!ci) {
+                t0 = tt[kk];
+                ci = true;
+            } else
+                t1 = tt[kk];
+        }
+        if (!(refi < D.npts && t0 < D.npts && t1 < D.npts)) return;
+        const smx_point3 p0 = D.pts[refi], p1 = D.pts[t0], p2 = D.pts[t1];
+        const double angle = tin_slope_deg(p0, p1, p2);
+        const double dh1 = p0.z - p1.z, dh2 = p0.z - p2.z, dh3 = p1.z - p2.z;
+        if (dh1 >= 0 && dh2 >= 0 && angle > 30) cpos++;
+        if (dh1 < 0 && dh2 < 0 && angle > 30) cneg++;
+        if (D.check_match) {
+            const double ddh_1 = fabs(dh1) - fabs(dh2), ddh_2 = fabs(dh1) - fabs(dh3), ddh_3 = fabs(dh2) - fabs(dh3);
+            if (fabs(ddh_1) > D.height_th || fabs(ddh_2) > D.height_th || fabs(ddh_3) > D.height_th) {
+                neigh = true;
+                // setsm_code.cpp:9577-9620: the same comparisons, so ties
+                // resolve to the same vertices (selects, not a local array,
+                // keep the vertices in registers)
+                const double h0 = p0.z, h1 = p1.z, h2 = p2.z;
+                int mx, mn, md;
+                if (h0 > h1) {
+                    if (h0 > h2) {
+                        mx = 0;
+                        if (h1 > h2) { mn = 2; md = 1; }
+                        else { mn = 1; md = 2; }
+                    } else { mx = 2; mn = 1; md = 0; }
+                } else {
+                    if (h1 > h2) {
+                        mx = 1;
+                        if (h0 > h2) { mn = 2; md = 0; }
+                        else { mn = 0; md = 2; }
+                    } else { mx = 2; md = 1; mn = 0; }
+                }
+                auto pick = [](int k, double a, double b, double c) { return k == 0 ? a : (k == 1 ? b : c); };
+                const double hmd = pick(md, h0, h1, h2);
+                const double dh = (hmd - pick(mn, h0, h1, h2)) - (pick(mx, h0, h1, h2) - hmd);
+                int k = -1;
+                if (dh > 0 && dh > D.height_th) k = mn;
+                else if (dh < 0 && fabs(dh) > D.height_th) k = mx;
+                if (k >= 0) {
+                    bool take = D.is_ra == 1;
+                    if (!take) {
+                        const double ok =
+                            (double)D.oncc[bd_cell(D, pick(k, p0.x, p1.x, p2.x), pick(k, p0.y, p1.y, p2.y))];
+                        take = D.flag_blunder ? ok < D.ortho_ncc_th : ok < 100.;
+                    }
+                    if (take) bd_mark(D, k == 0 ? refi : (k == 1 ? t0 : t1), 2u);
+                }
+            }
+        }
+        count++;
+    };
+    const long o = D.off[i];
+    const long deg = D.off[i + 1] - o;
+    if (deg <= kBdMaxNodes) {
+        for (long k = 0; k < deg; k++) visit(D.adj[o + k]);
+    } else {
+        // the reference keeps the first 30 in triangle order: the 30
+        // smallest indices, repeats included
+        long last = -1;
+        int taken = 0;
+        while (taken < kBdMaxNodes) {
+            uint32_t best = 0xffffffffu;
+            int mult = 0;
+            for (long k = 0; k < deg; k++) {
+                const uint32_t u = D.adj[o + k];
+                if ((long)u <= last) continue;
+                if (u < best) {
+                    best = u;
+                    mult = 1;
+                } else if (u == best)
+                    mult++;
+            }
+            if (mult == 0) break;
+            for (int r = 0; r < mult && taken < kBdMaxNodes; r++, taken++) visit(best);
+            last = best;
+        }
+    }
+
+    if (D.is_ra == 1) {
+        if (neigh && !D.flag_blunder) blunder = true;
+        if ((cpos >= (int)(count * 0.7 + 0.5) || cneg >= (int)(count * 0.7 + 0.5)) && count > 1) blunder = true;
+    } else {
+        if (neigh && !D.flag_blunder && on < 100.) blunder = true;
+        double cth;
+        if (level >= 2) cth = count * 0.5;
+        else if (level == 1) cth = (int)(count * 0.6 + 0.5);
+        else cth = (int)(count * 0.7 + 0.5);
+        if ((cpos >= cth || cneg >= cth) && count > 1) {
+            if (D.flag_blunder) {
+                double tmp_th = 0.6 - (4 - level) * 0.1;
+                if (level == 1) tmp_th = 0.6;
+                if (level >= 1) {
+                    if (on < tmp_th) blunder = true;
+                } else if (level == 0) {
+                    if (D.iteration <= 1) {
+                        if (on < 0.9) blunder = true;
+                    } else if (D.iteration == 2)
+                        blunder = true;
+                    else if (on < D.ortho_ncc_th)
+                        blunder = true;
+                }
+            } else if (on < 100.)
+                blunder = true;
+        }
+    }
+    if (blunder) bd_mark(D, i, 1u);
+}
+
+// the anchor flag of the highest tested point of each cell, as the
+// reference's last write leaves it
+__global__ void __launch_bounds__(256) k_bd_anchor(BdDev D)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D.npts) return;
+    const smx_point3 P = D.pts[i];
+    if (P.flag == 1) return;
+    const long ref = bd_cell(D, P.x, P.y);
+    const unsigned long long key = D.akey[ref];
+    if ((long)(key >> 1) == i) D.g[ref].anchor_flag = (uint8_t)(key & 1);
+}
+
+hipError_t launch_bd_prepare(const BdDev &D, const VllScratch &S, hipStream_t s)
+{
+    hipError_t e = launch_fill_u64((unsigned long long *)D.deg, 0, D.npts + 1, s);
+    if (e == hipSuccess) e = launch_fill_u32((uint32_t *)D.err, 0, 1, s);
+    if (e == hipSuccess) e = launch_fill_u32(D.state, 0, (D.npts + 3) / 4, s);
+    if (e != hipSuccess) return e;
+    if (D.npts > 0) hipLaunchKernelGGL(k_bd_check, dim3((unsigned)((D.npts + 255) / 256)), dim3(256), 0, s, D);
+    const unsigned tb = (unsigned)std::min<long>((D.ntris + 255) / 256, 8192);
+    if (D.ntris > 0) hipLaunchKernelGGL(k_bd_degree, dim3(tb), dim3(256), 0, s, D);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = job_scan(S, D.deg, (int64_t *)D.off, D.npts + 1, s)) != hipSuccess) return e;
+    if (D.ntris > 0) hipLaunchKernelGGL(k_bd_fill, dim3(tb), dim3(256), 0, s, D);
+    return hipGetLastError();
+}
+
+hipError_t launch_bd_points(const BdDev &D, hipStream_t s)
+{
+    if (D.npts == 0) return hipSuccess;
+    const long n = D.W * D.H;
+    hipError_t e = launch_fill_u64(D.akey, 0, n, s);
+    if (e != hipSuccess) return e;
+    const unsigned pb = (unsigned)((D.npts + 255) / 256);
+    hipLaunchKernelGGL(k_bd_points, dim3(pb), dim3(256), 0, s, D);
+    hipLaunchKernelGGL(k_bd_anchor, dim3(pb), dim3(256), 0, s, D);
     return hipGetLastError();
 }
 

@@ -189,6 +189,8 @@ static void free_level_state(smx_job *j)
     jfree(j, j->d_bortho_wgs);
     j->blunder_level = -1;
     j->bortho_w = j->bortho_h = 0;
+    jfree(j, j->d_bncc);
+    j->bncc_valid = false;
     jfree(j, j->d_peak_rho);
     j->peak_cap = 0;
     j->cd_bytes = j->cdn_bytes = 0;
@@ -1583,9 +1585,12 @@ int smx_vertical_line_locus_blunder(smx_job *j, int iteration, int bblunder, flo
     B.gridpt3 = j->d_gridpt3;
     int r = CB.get(&B.cd[0], (size_t)(B.sw * B.sh) * 2);
     if (!r) r = CB.get(&B.cd[1], (size_t)(B.sw * B.sh) * 2);
-    if (!r) r = CB.get(&B.out, (size_t)n);
+    // the output stays on the device for smx_blunder_detection_tin
+    if (!r && !j->d_bncc) r = jalloc(j, &j->d_bncc, (size_t)std::max(n, 1L));
     if (!r && !j->d_params) r = jalloc(j, &j->d_params, 1);
     if (r) return r;
+    j->bncc_valid = false;
+    B.out = j->d_bncc;
     fill_params(j, iteration);
     if ((r = put_params(j, j->stream))) return r;
     HIPCHK(hipEventRecord(j->ev0, j->stream));
@@ -1596,7 +1601,194 @@ int smx_vertical_line_locus_blunder(smx_job *j, int iteration, int bblunder, flo
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, j->ev0, j->ev1));
     j->last_ms = ms;
+    j->bncc_valid = true;
     j->generation++;   // GridPT3's ortho NCC changed
+    return 0;
+}
+
+}  // extern "C"
+
+namespace {
+// setsm_code.cpp:9183-9418, the serial part of blunder_detection_TIN: the
+// edge height differences of the triangles whose vertices all exist, summed
+// in triangle order as the reference sums them, then the height-difference
+// threshold, the level's ortho-NCC thresholds and whether the neighbour rule
+// runs (:9489-9507).
+void bd_thresholds(const smx_level &L, const smx_bd_args &a, const smx_point3 *pts, long npts, const smx_tri *tris,
+                   long ntris, BdDev &D)
+{
+    const int step = L.level, iteration = a.iteration;
+    const uint32_t hdiffcount = (uint32_t)a.hinterval;
+    std::vector<uint32_t> hdiffbin((size_t)hdiffcount + 1, 0);
+    double sum_data = 0.0, sum_data2 = 0.0;
+    long dh_count = 0;
+    for (long t = 0; t < ntris; t++) {
+        const smx_tri &tr = tris[t];
+        if ((long)tr.a >= npts || (long)tr.b >= npts || (long)tr.c >= npts) continue;
+        const double z0 = pts[tr.a].z, z1 = pts[tr.b].z, z2 = pts[tr.c].z;
+        const double d[3] = {std::fabs(z0 - z1), std::fabs(z0 - z2), std::fabs(z1 - z2)};
+        for (double v : d) {
+            sum_data += v;
+            sum_data2 += v * v;
+            dh_count++;
+        }
+        for (double v : d) {
+            const double b = (uint32_t)v;
+            if (b < hdiffcount) hdiffbin[(uint32_t)b]++;
+        }
+    }
+    double ortho_ncc_th;
+    if (step == 4) ortho_ncc_th = 0.6 - (iteration - 1) * 0.01;
+    else if (step >= 3) ortho_ncc_th = 0.5 - (iteration - 1) * 0.01;
+    else if (step == 2) ortho_ncc_th = 0.4 - (iteration - 1) * 0.01;
+    else if (step == 1) ortho_ncc_th = 0.3;
+    else ortho_ncc_th = 0.2;
+    double th_ref_ncc = 0.1 + (iteration - 1) * 0.05;
+    if (th_ref_ncc > ortho_ncc_th) th_ref_ncc = ortho_ncc_th;
+    bool check_dh = false;
+    if (step <= 4 && step >= 3) check_dh = true;
+    else if (step == 2) check_dh = iteration <= 4;
+    else if (step <= 1) check_dh = iteration <= 2;
+    // GSD only reaches the step >= 3 branch (the reference's pwrtwo(step - 1)
+    // at step 0 is never read: the other branch sets the threshold anew)
+    double GSD = L.grid_resolution;
+    if (step >= 3 && L.grid_resolution < (double)(1 << (step - 1))) GSD = (double)(1 << (step - 1));
+    const double height_th_1 = a.hinterval;
+    double height_th_2 = GSD * 10;
+    if (L.is_ra == 1) height_th_2 = GSD * 30;
+    double height_th = height_th_1 < height_th_2 ? height_th_1 : height_th_2;
+    if (step >= 3) {
+        double up_3sigma = height_th;
+        if (check_dh) {
+            const double mean = sum_data / dh_count;
+            const double sd = std::sqrt((sum_data2 - (sum_data) * (sum_data) / dh_count) / dh_count);
+            up_3sigma = mean + 3 * sd;
+        }
+        if (up_3sigma < height_th) height_th = up_3sigma < height_th / 2.0 ? height_th / 2.0 : up_3sigma;
+    } else {
+        uint32_t total_dh = 0, th80 = 1000;
+        long tcnt = 0;
+        bool found = false;
+        while (tcnt < (long)hdiffcount && !found) {
+            total_dh += hdiffbin[tcnt];
+            const double per = (double)total_dh / (double)dh_count * 100.0;
+            if (per > 99) {
+                th80 = (uint32_t)tcnt;
+                found = true;
+            }
+            tcnt++;
+        }
+        if (!found) th80 = (uint32_t)(tcnt - 1);
+        const double mean = sum_data / dh_count;
+        const double sd = std::sqrt((sum_data2 - (sum_data) * (sum_data) / dh_count) / dh_count);
+        const double up_3sigma = mean + 3 * sd;
+        height_th = up_3sigma < th80 ? up_3sigma : (double)th80;
+        if (height_th > 50) height_th = 50;
+    }
+    const unsigned count_bl = (uint16_t)a.count_bl;
+    int check_match = 0;
+    if (step > 1) check_match = 1;
+    else if (step >= 0) check_match = iteration == 1 && count_bl <= 5;
+    D.height_th = height_th;
+    D.ortho_ncc_th = ortho_ncc_th;
+    D.th_ref_ncc = th_ref_ncc;
+    D.check_match = check_match;
+}
+}  // namespace
+
+extern "C" {
+
+int smx_blunder_detection_tin(smx_job *j, const smx_bd_args *a, const float *ortho_ncc, smx_point3 *pts, long npts,
+                              const smx_tri *tris, long ntris, uint8_t *detected, long blunder_count[2],
+                              double *minz_mp, double *maxz_mp)
+{
+    if (!j || !a || !blunder_count || (npts > 0 && !pts) || (ntris > 0 && !tris) || npts < 0 || ntris < 0)
+        return fail(SMX_E_ARG, "smx_blunder_detection_tin: bad argument");
+    if (!(a->hinterval >= 0 && a->hinterval <= 1e7)) return fail(SMX_E_ARG, "Hinterval %g out of range", a->hinterval);
+    if (ntris >= 0xffffffffL) return fail(SMX_E_ARG, "too many triangles");
+    if (!j->level_set || !j->searchspace_set) return fail(SMX_E_STATE, "level and search space must be set");
+    if (!ortho_ncc && !j->bncc_valid)
+        return fail(SMX_E_STATE, "no ortho NCC given and no smx_vertical_line_locus_blunder output on this level");
+    if (detected && npts > 0) std::memset(detected, 0, (size_t)npts);
+    const smx_level &L = j->lv;
+    if (L.level == 0 && a->iteration == 3) return 0;   // setsm_code.cpp:9164
+    HIPCHK(hipSetDevice(j->device));
+    const long n = j->ncells;
+    CallBuffers B(j);
+    BdDev D{};
+    smx_point3 *d_pts = nullptr;
+    smx_tri *d_tris = nullptr;
+    int64_t *d_off = nullptr;
+    float *d_on = nullptr;
+    int r = B.get(&d_pts, (size_t)npts);
+    if (!r) r = B.get(&d_tris, (size_t)ntris);
+    if (!r) r = B.get(&D.deg, (size_t)npts + 1);
+    if (!r) r = B.get(&d_off, (size_t)npts + 1);
+    if (!r) r = B.get(&D.adj, (size_t)ntris * 3);
+    if (!r) r = B.get(&D.akey, (size_t)n);
+    if (!r) r = B.get(&D.state, (size_t)(npts + 3) / 4);
+    if (!r) r = B.get(&D.err, 1);
+    if (!r && ortho_ncc) r = B.get(&d_on, (size_t)n);
+    if (!r) r = grow(j, j->js.scan_part, j->js.scan_part_cap, scan_partials(npts + 1));   // more points than cells
+    if (r) return r;
+    if (npts + 1 > scan_max_n()) return fail(SMX_E_ARG, "too many points");
+    if (npts) HIPCHK(hipMemcpyAsync(d_pts, pts, (size_t)npts * sizeof(smx_point3), hipMemcpyHostToDevice, j->stream));
+    if (ntris) HIPCHK(hipMemcpyAsync(d_tris, tris, (size_t)ntris * sizeof(smx_tri), hipMemcpyHostToDevice, j->stream));
+    if (ortho_ncc) HIPCHK(hipMemcpyAsync(d_on, ortho_ncc, (size_t)n * sizeof(float), hipMemcpyHostToDevice, j->stream));
+    D.pts = d_pts;
+    D.npts = npts;
+    D.tris = d_tris;
+    D.ntris = ntris;
+    D.off = d_off;
+    D.oncc = ortho_ncc ? d_on : j->d_bncc;
+    D.g = j->d_gridpt3;
+    D.W = (long)L.grid_w;
+    D.H = (long)L.grid_h;
+    D.gres = L.grid_resolution;
+    D.b0 = L.boundary[0];
+    D.b1 = L.boundary[1];
+    D.level = L.level;
+    D.iteration = a->iteration;
+    D.is_ra = L.is_ra;
+    D.flag_blunder = a->flag_blunder != 0;
+    HIPCHK(hipEventRecord(j->ev0, j->stream));
+    HIPCHK(launch_bd_prepare(D, j->js, j->stream));
+    HIPCHK(hipEventRecord(j->ev1, j->stream));
+    HIPCHK(hipMemcpyAsync(&j->h_small[0], D.err, sizeof(int), hipMemcpyDeviceToHost, j->stream));
+    bd_thresholds(L, *a, pts, npts, tris, ntris, D);   // on the host while the lists build
+    HIPCHK(hipStreamSynchronize(j->stream));
+    int err = 0;
+    std::memcpy(&err, &j->h_small[0], sizeof err);
+    float ms = 0, ms2 = 0;
+    HIPCHK(hipEventElapsedTime(&ms, j->ev0, j->ev1));
+    if (err) return fail(SMX_E_ARG, "a matched point lies outside the level's grid");
+    std::vector<uint8_t> st((size_t)((npts + 3) / 4) * 4);
+    HIPCHK(hipEventRecord(j->ev0, j->stream));
+    HIPCHK(launch_bd_points(D, j->stream));
+    HIPCHK(hipEventRecord(j->ev1, j->stream));
+    if (npts) HIPCHK(hipMemcpyAsync(st.data(), D.state, st.size(), hipMemcpyDeviceToHost, j->stream));
+    HIPCHK(hipStreamSynchronize(j->stream));
+    HIPCHK(hipEventElapsedTime(&ms2, j->ev0, j->ev1));
+    j->last_ms = ms + ms2;   // kernels only: the host threshold pass and the copies excluded
+    // setsm_code.cpp:9121-9156 and :9758-9776: a point still at flag 0 that
+    // its own test or a neighbour's blames becomes a new blunder; flags 1 and
+    // 3 count as blunders (3 ends as 1); the rest are kept points
+    for (long i = 0; i < npts; i++) {
+        const uint8_t f0 = pts[i].flag;
+        if (f0 == 0 && st[(size_t)i]) {
+            pts[i].flag = 1;
+            if (detected) detected[i] = 1;
+            blunder_count[1]++;
+        } else if (f0 == 1 || f0 == 3) {
+            pts[i].flag = 1;
+            blunder_count[1]++;
+        } else {
+            blunder_count[0]++;
+            if (minz_mp && *minz_mp > pts[i].z) *minz_mp = pts[i].z;
+            if (maxz_mp && *maxz_mp < pts[i].z) *maxz_mp = pts[i].z;
+        }
+    }
+    j->generation++;   // anchor flags
     return 0;
 }
 

@@ -80,6 +80,8 @@ struct smx_job {
     double *d_bortho_wgs = nullptr;               // VerticalLineLocus_blunder: lon/lat of its ortho grid
     int blunder_level = -1;                       // blunder_selected_level of that grid, -1 = none
     uint32_t bortho_w = 0, bortho_h = 0;
+    float *d_bncc = nullptr;                      // its last output, the per-cell ortho NCC
+    bool bncc_valid = false;
     double *d_peak_rho = nullptr;                 // RA / matching-rate pass: temp_rho per run lane
     long peak_cap = 0;
     smx_ugrid *d_gridpt3 = nullptr;
@@ -225,6 +227,27 @@ struct HrDev {
 };
 hipError_t launch_set_height_range(const HrDev &D, hipStream_t s);
 hipError_t launch_set_height_range_blunder(const HrDev &D, float *heights, hipStream_t s);
+constexpr int kBdMaxNodes = 30;   // triangles per point blunder_detection_TIN reads
+struct BdDev {
+    const smx_point3 *pts;
+    long npts;
+    const smx_tri *tris;
+    long ntris;
+    int64_t *deg;                 // per point (+1): incident triangles, then the fill cursor
+    const int64_t *off;           // their exclusive scan
+    uint32_t *adj;                // incident triangles grouped per point (unordered within a point)
+    const float *oncc;            // per-cell ortho NCC
+    smx_ugrid *g;                 // GridPT3: anchor_flag
+    unsigned long long *akey;     // per cell: (highest tested point << 1) | its anchor flag
+    uint32_t *state;              // a byte per point: bit 0 its own test, bit 1 a neighbour's
+    int *err;                     // a point outside the grid
+    long W, H;
+    double gres, b0, b1;
+    int level, iteration, is_ra, flag_blunder, check_match;
+    double height_th, ortho_ncc_th, th_ref_ncc;
+};
+hipError_t launch_bd_prepare(const BdDev &D, const VllScratch &S, hipStream_t s);   // check + incidence lists
+hipError_t launch_bd_points(const BdDev &D, hipStream_t s);
 int fast_num_configs();
 int fast_config_desc(int i, char *buf, int n);
 int fast_config_index(const smx_job *j);

@@ -1,16 +1,16 @@
 // setsm_vll_shim.cpp — SETSM's InitializeVoxel / VerticalLineLocus / AWNCC
 // (setsm_code.hpp:75, 81, 91), the SelectMPs / SetHeightRange either side of
 // its TIN (setsm_code.hpp:99, 122) and DecisionMPs' VerticalLineLocus_blunder /
-// SetHeightRange_blunder (setsm_code.hpp:95, 120) with their exact C++
-// signatures and SETSM's own types, running on the MI355X through libsmx
-// (include/smx_vll.h).
+// SetHeightRange_blunder / blunder_detection_TIN (setsm_code.hpp:95, 120,
+// 116) with their exact C++ signatures and SETSM's own types, running on the
+// MI355X through libsmx (include/smx_vll.h).
 //
 // Drop-in use inside SETSM's build: add this file, link -lsmx, and either
 //   * compile it with -DSMX_SHIM_GLOBAL_NAMES and remove the original
 //     definitions from setsm_code.cpp, or
 //   * keep the originals and call smx_setsm::InitializeVoxel / ... at the
 //     call sites in Matching_SETSM (setsm_code.cpp:3827, 3829, 3844, 3862,
-//     4192-4222) and DecisionMPs (:8961-8962, 9081-9087, 9113-9115).
+//     4192-4222) and DecisionMPs (:8961-8967, 9081-9087, 9113-9115).
 //
 // State: one smx job per thread mirrors SETSM's per-level state. A new job is
 // made when the level's image buffers, sizes or origins change; a changed
@@ -476,6 +476,32 @@ bool VerticalLineLocus_blunder(const ProInfo *proinfo, LevelInfo &rlevelinfo, fl
     return true;
 }
 
+// blunder_detection_TIN: on the level bound as for VerticalLineLocus_blunder,
+// which DecisionMPs calls just before it (setsm_code.cpp:8962-8967).
+bool blunder_detection_TIN(const ProInfo *proinfo, LevelInfo &rlevelinfo, const int iteration, float *ortho_ncc,
+                           bool flag_blunder, uint16 count_bl, D3DPOINT *pts, bool *detectedBlunders,
+                           long int num_points, UI3DPOINT *tris, long int num_triangles, UGRID *Gridpts,
+                           long *blunder_count, double *minz_mp, double *maxz_mp)
+{
+    bind(proinfo, rlevelinfo);
+    State &s = g_state;
+    smx_bd_args a{};
+    a.hinterval = *rlevelinfo.Hinterval;
+    a.iteration = iteration;
+    a.flag_blunder = flag_blunder ? 1 : 0;
+    a.count_bl = count_bl;
+    std::vector<uint8_t> det((size_t)(num_points > 0 ? num_points : 1), 0);
+    check(smx_job_set_searchspace(s.job, (const smx_ugrid *)Gridpts, s.minmax), "smx_job_set_searchspace");
+    check(smx_blunder_detection_tin(s.job, &a, ortho_ncc, (smx_point3 *)pts, num_points, (const smx_tri *)tris,
+                                    num_triangles, det.data(), blunder_count, minz_mp, maxz_mp),
+          "smx_blunder_detection_tin");
+    // the reference only ever sets entries of detectedBlunders
+    for (long i = 0; i < num_points; i++)
+        if (det[(size_t)i]) detectedBlunders[i] = true;
+    check(smx_job_get_searchspace(s.job, (smx_ugrid *)Gridpts), "smx_job_get_searchspace");
+    return true;
+}
+
 // Release the thread's job (optional; it is reused across calls).
 void Release()
 {
@@ -527,5 +553,14 @@ bool VerticalLineLocus_blunder(const ProInfo *proinfo, LevelInfo &rlevelinfo, fl
                                uint8 iteration, bool bblunder)
 {
     return smx_setsm::VerticalLineLocus_blunder(proinfo, rlevelinfo, nccresult, GridPT3, iteration, bblunder);
+}
+bool blunder_detection_TIN(const ProInfo *proinfo, LevelInfo &rlevelinfo, const int iteration, float *ortho_ncc,
+                           bool flag_blunder, uint16 count_bl, D3DPOINT *pts, bool *detectedBlunders,
+                           long int num_points, UI3DPOINT *tris, long int num_triangles, UGRID *Gridpts,
+                           long *blunder_count, double *minz_mp, double *maxz_mp)
+{
+    return smx_setsm::blunder_detection_TIN(proinfo, rlevelinfo, iteration, ortho_ncc, flag_blunder, count_bl, pts,
+                                            detectedBlunders, num_points, tris, num_triangles, Gridpts,
+                                            blunder_count, minz_mp, maxz_mp);
 }
 #endif

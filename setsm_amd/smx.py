@@ -18,7 +18,7 @@ import os
 
 import numpy as np
 
-from .records import (NCCRESULT_DTYPE, POINT3_DTYPE, TRI_DTYPE, UGRID_DTYPE, SmxHrArgs, SmxLevel,
+from .records import (NCCRESULT_DTYPE, POINT3_DTYPE, TRI_DTYPE, UGRID_DTYPE, SmxBdArgs, SmxHrArgs, SmxLevel,
                       SmxMpsArgs)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -36,6 +36,7 @@ EXPORTS = [
     "smx_awncc", "smx_job_num_hypotheses", "smx_job_get_voxels", "smx_job_get_voxel_rows", "smx_job_set_voxels",
     "smx_job_get_sumcost", "smx_job_set_ortho_wgs", "smx_select_mps", "smx_set_height_range",
     "smx_set_height_range_blunder", "smx_job_set_blunder_level", "smx_vertical_line_locus_blunder",
+    "smx_blunder_detection_tin",
     "smx_job_last_kernel_ms", "smx_job_last_evaluated", "smx_job_last_redo", "smx_job_sync", "smx_fast_num_configs",
     "smx_job_set_fast_config", "smx_fast_config_desc", "smx_job_last_stats", "smx_vertical_line_locus_batch",
     "smx_vertical_line_locus_noalloc", "smx_release_cached", "smx_job_trim", "smx_job_half", "smx_graph_create", "smx_graph_launch", "smx_graph_last_ms", "smx_graph_destroy",
@@ -92,6 +93,8 @@ def lib():
             "smx_set_height_range_blunder": (i, [vp, vp, C.c_long, vp, C.c_long]),
             "smx_job_set_blunder_level": (i, [vp, i, vp, C.c_uint32, C.c_uint32]),
             "smx_vertical_line_locus_blunder": (i, [vp, i, i, vp]),
+            "smx_blunder_detection_tin": (i, [vp, vp, vp, vp, C.c_long, vp, C.c_long, vp, C.POINTER(C.c_long),
+                                              C.POINTER(C.c_double), C.POINTER(C.c_double)]),
             "smx_job_last_kernel_ms": (C.c_double, [vp]),
             "smx_job_last_evaluated": (ll, [vp]),
             "smx_job_last_redo": (i, [vp, vp, vp, vp]),
@@ -286,6 +289,30 @@ class Job:
         out = np.zeros(max(self.ncells, 1), np.float32)
         _check(lib().smx_vertical_line_locus_blunder(self.h, int(iteration), int(bool(bblunder)), _p(out)))
         return out[: self.ncells]
+
+    def blunder_detection_tin(self, pts, tris, iteration: int, flag_blunder: bool, hinterval: float,
+                              count_bl: int = 0, ortho_ncc=None, blunder_count=(0, 0), minz=100000.0,
+                              maxz=-100000.0):
+        """blunder_detection_TIN (setsm_code.cpp:9158-9782): pts (POINT3
+        records) get their new flags in place; GridPT3's anchor flags are
+        updated. ortho_ncc None reads the last vertical_line_locus_blunder
+        output kept on the device. Returns (detected, blunder_count, minz,
+        maxz), the counts and heights continued from the values given."""
+        assert pts.dtype == POINT3_DTYPE and pts.flags.c_contiguous
+        tris = np.ascontiguousarray(tris, TRI_DTYPE)
+        on = None
+        if ortho_ncc is not None:
+            on = np.ascontiguousarray(ortho_ncc, np.float32)
+            assert len(on) == self.ncells
+        a = SmxBdArgs(float(hinterval), int(iteration), int(bool(flag_blunder)), int(count_bl), 0)
+        det = np.zeros(max(len(pts), 1), np.uint8)
+        bc = (C.c_long * 2)(*blunder_count)
+        lo, hi = C.c_double(minz), C.c_double(maxz)
+        _check(lib().smx_blunder_detection_tin(self.h, C.byref(a), _p(on) if on is not None else None,
+                                               _p(pts) if len(pts) else None, C.c_long(len(pts)),
+                                               _p(tris) if len(tris) else None, C.c_long(len(tris)), _p(det), bc,
+                                               C.byref(lo), C.byref(hi)))
+        return det[: len(pts)].astype(bool), (bc[0], bc[1]), lo.value, hi.value
 
     # ------------------------------------------------------------- cube
     @property

@@ -5,9 +5,9 @@
 // iteration, on host arrays (VOXEL **grid_voxel, NCCresult *, UGRID *), and
 // SelectMPs / SetHeightRange between iterations (setsm_code.cpp:3862,
 // 4192-4222) and DecisionMPs' VerticalLineLocus_blunder /
-// SetHeightRange_blunder. The functions are called through their reference
-// signatures (setsm_code.hpp:75, 81, 91, 95, 99, 120, 122), which
-// libsmx_setsm.so exports.
+// SetHeightRange_blunder / blunder_detection_TIN. The functions are called
+// through their reference signatures (setsm_code.hpp:75, 81, 91, 95, 99, 116,
+// 120, 122), which libsmx_setsm.so exports.
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -47,6 +47,10 @@ bool VerticalLineLocus_blunder(const ProInfo *proinfo, LevelInfo &rlevelinfo, fl
                                uint8 iteration, bool bblunder);
 bool SetHeightRange_blunder(LevelInfo &rlevelinfo, const D3DPOINT *pts, const int numPts, UI3DPOINT *tris,
                             const long num_triangles, UGRID *GridPT3);
+bool blunder_detection_TIN(const ProInfo *proinfo, LevelInfo &rlevelinfo, const int iteration, float *ortho_ncc,
+                           bool flag_blunder, uint16 count_bl, D3DPOINT *pts, bool *detectedBlunders,
+                           long int num_points, UI3DPOINT *tris, long int num_triangles, UGRID *Gridpts,
+                           long *blunder_count, double *minz_mp, double *maxz_mp);
 UGRID *SetHeightRange(ProInfo *proinfo, LevelInfo &rlevelinfo, NCCresult *nccresult, const int numOfPts,
                       const int num_triangles, UGRID *GridPT3, const int iteration, double *minH_grid,
                       double *maxH_grid, D3DPOINT *pts, const UI3DPOINT *tris, const double MPP,
@@ -82,6 +86,7 @@ struct Driver {
     int blunder_level = 0;
     const uint16 *bimg[2] = {nullptr, nullptr}, *bmag[2] = {nullptr, nullptr};
     D2DPOINT bstart[2];
+    double hinterval = 0;
     bool swap = false;
 };
 
@@ -295,6 +300,17 @@ void drv_vll_blunder(void *h, smx_ugrid *ugrid, int iteration, int bblunder, flo
 {
     Driver *d = (Driver *)h;
     VerticalLineLocus_blunder(&d->pro, d->L, out, (UGRID *)ugrid, (uint8)iteration, bblunder != 0);
+}
+
+void drv_blunder_detection(void *h, smx_ugrid *ugrid, int iteration, float *ortho, int flag_blunder, int count_bl,
+                           smx_point3 *pts, long npts, smx_tri *tris, long ntris, double hinterval, uint8_t *det,
+                           long *blunder_count, double *minz, double *maxz)
+{
+    Driver *d = (Driver *)h;
+    d->hinterval = hinterval;
+    d->L.Hinterval = &d->hinterval;
+    blunder_detection_TIN(&d->pro, d->L, iteration, ortho, flag_blunder != 0, (uint16)count_bl, (D3DPOINT *)pts,
+                          (bool *)det, npts, (UI3DPOINT *)tris, ntris, (UGRID *)ugrid, blunder_count, minz, maxz);
 }
 
 void drv_destroy(void *h)

@@ -64,7 +64,7 @@ def test_setsm_typed_shim_exports_the_reference_symbols():
     defined = syms(shim, "--defined-only")
     wanted = {s for s in syms(drv, "--undefined-only") if s.startswith("_Z")
               and any(k in s for k in ("InitializeVoxel", "VerticalLineLocus", "AWNCC", "SelectMPs",
-                                       "SetHeightRange"))}
-    assert len(wanted) == 7, wanted
+                                       "SetHeightRange", "blunder_detection_TIN"))}
+    assert len(wanted) == 8, wanted
     assert wanted <= defined, f"missing: {wanted - defined}"
     assert "_Z15InitializeVoxelPK11ProjectInfoPP12tagVoxelinfoR12taglevelinfoP10UpdateGridP12tagNCCresultiPKd" in wanted

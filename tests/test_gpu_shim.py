@@ -50,6 +50,9 @@ def _driver():
     d.drv_set_height_range_blunder.argtypes = [vp, vp, vp, C.c_long, vp, C.c_long]
     d.drv_vll_blunder.restype = None
     d.drv_vll_blunder.argtypes = [vp, vp, C.c_int, C.c_int, vp]
+    d.drv_blunder_detection.restype = None
+    d.drv_blunder_detection.argtypes = [vp, vp, C.c_int, vp, C.c_int, C.c_int, vp, C.c_long, vp, C.c_long,
+                                        C.c_double, vp, vp, vp, vp]
     d.drv_set_height_range.restype = None
     d.drv_set_height_range.argtypes = [vp, vp, vp, C.c_long, vp, C.c_long, C.c_int, C.c_double, C.c_int,
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -230,8 +233,8 @@ def test_shim_select_mps_and_set_height_range(require_gpu, monkeypatch):
 
 
 def test_shim_blunder_helpers(require_gpu, monkeypatch):
-    """DecisionMPs' SetHeightRange_blunder and VerticalLineLocus_blunder
-    through the reference signatures: the blunder level one above the
+    """DecisionMPs' SetHeightRange_blunder, VerticalLineLocus_blunder and
+    blunder_detection_TIN through the reference signatures: the blunder level one above the
     matching level (py_BImages at level + 1, as Matching_SETSM selects it
     below the top level), the ortho lon/lat from the test double of
     ps2wgs_single; GridPT3 and the ortho NCC equal the oracle's."""
@@ -269,7 +272,22 @@ def test_shim_blunder_helpers(require_gpu, monkeypatch):
         ncc_o = O.vll_blunder(li, pyr, B, ortho_wgs(sc, g["boundary"], B), ug_o, 1, True)
         assert np.array_equal(ncc_s.view(np.uint32), ncc_o.view(np.uint32))
         _same_grid(ug_s, ug_o)
-        print(f"shim blunder level {B}: {n} points, {len(tris)} triangles, mean ortho NCC {ncc_o.mean():.3f}",
-              flush=True)
+        # blunder_detection_TIN on the same pass; the detected flags the
+        # caller holds are only ever set, as in the reference
+        pts_s = np.ascontiguousarray(out[:n].copy())
+        det_s = np.zeros(n, np.uint8)
+        det_s[0] = 1
+        bc = (C.c_long * 2)(0, 0)
+        lo, hi = C.c_double(100000.0), C.c_double(-100000.0)
+        drv.drv_blunder_detection(s.h, ug_s.ctypes.data, 1, ncc_s.ctypes.data, 1, 0, pts_s.ctypes.data, n,
+                                  tris.ctypes.data, len(tris), 400.0, det_s.ctypes.data, bc, C.byref(lo), C.byref(hi))
+        det_o, bc_o, lo_o, hi_o, _ = O.blunder_detection_tin(li, ug_o, ncc_o, pts_o, tris, 1, True, 400.0)
+        det_o[0] = True
+        assert np.array_equal(pts_s["flag"], pts_o["flag"])
+        assert np.array_equal(det_s.astype(bool), det_o)
+        assert (bc[0], bc[1]) == bc_o and (lo.value, hi.value) == (lo_o, hi_o)
+        _same_grid(ug_s, ug_o)
+        print(f"shim blunder level {B}: {n} points, {len(tris)} triangles, mean ortho NCC {ncc_o.mean():.3f}, "
+              f"blunder counts {bc_o}", flush=True)
     finally:
         s.close()
