@@ -347,6 +347,8 @@ def headline(args, out):
 # ------------------------------------------------------------ between iterations
 MPS_ARGS = dict(th_roh=0.4, th_roh_min=0.1, th_roh_start=0.4, th_roh_next=0.3, mpp=1.0, mpp_stereo_angle=10.0,
                 final_level_iteration=1)
+# Hinterval of blunder_detection_TIN: the span of the level's height bands
+BD_HINTERVAL = 500.0
 
 
 def lattice_tris(w, h):
@@ -366,12 +368,15 @@ def lattice_tris(w, h):
 def between_run(args, job, sc, g, li, r0):
     """The data-parallel steps between two iterations of SETSM's loop on the
     headline level after its AWNCC: SelectMPs, SetHeightRange and DecisionMPs'
-    SetHeightRange_blunder / VerticalLineLocus_blunder (blunder level 1). The
-    TIN is the full cell lattice (a point per cell at its peak height, two
-    triangles per grid square). Per call: the kernel time on the job stream
-    (HIP events; cells_per_s from it) and the wall clock with the host copies
-    (the points come back for the caller's TIN, the TIN goes up from pageable
-    memory). The oracle is timed beside it on a stripe of the same level."""
+    SetHeightRange_blunder / VerticalLineLocus_blunder (blunder level 1) /
+    blunder_detection_TIN. The TIN is the full cell lattice (a point per cell
+    at its peak height, two triangles per grid square). Per call: the kernel
+    time on the job stream (HIP events; cells_per_s from it) and the wall
+    clock with the host copies and host passes (the points come back for the
+    caller's TIN, the TIN goes up from pageable memory; blunder_detection_TIN
+    sums its edge statistics on the host in triangle order and writes the
+    flags back into the caller's point records). The oracle is timed beside
+    it on a stripe of the same level."""
     from setsm_amd.records import POINT3_DTYPE
     from setsm_amd.synth import ortho_wgs
     W, H = g["shape"][1], g["shape"][0]
@@ -386,14 +391,18 @@ def between_run(args, job, sc, g, li, r0):
     ow = ortho_wgs(sc, g["boundary"], 1)
     log(f"blunder ortho grid {ow.shape[1]}x{ow.shape[0]} ready in {time.time() - t0:.1f}s")
     job.set_blunder_level(1, ow)
+    pts_bd = pts_all.copy()
     steps = {"select_mps": lambda: job.select_mps(1, **MPS_ARGS),
              "set_height_range": lambda: job.set_height_range(pts_all, tris, 1, 1.0),
              "set_height_range_blunder": lambda: job.set_height_range_blunder(pts_all, tris),
-             "vertical_line_locus_blunder": lambda: job.vertical_line_locus_blunder(1, False)}
+             "vertical_line_locus_blunder": lambda: job.vertical_line_locus_blunder(1, False),
+             # on the ortho NCC the previous step left on the device
+             "blunder_detection_tin": lambda: job.blunder_detection_tin(pts_bd, tris, 1, True, BD_HINTERVAL)}
     for name, fn in steps.items():
         ts, ks = [], []
         for _ in range(3):
             job.set_searchspace(ug0, g["minmax"])
+            pts_bd["flag"] = 0
             job.sync()
             t = time.perf_counter()
             r = fn()
@@ -403,6 +412,9 @@ def between_run(args, job, sc, g, li, r0):
         if name == "select_mps":
             out[name]["points"] = int(len(r[0]))
             out[name]["minimum_th"] = r[1]
+        if name == "blunder_detection_tin":
+            out[name]["points"] = int(len(pts_bd))
+            out[name]["new_blunders"] = int(r[0].sum())
     out["cells"] = W * H
     out["triangles"] = int(len(tris))
     out["blunder_ortho_px"] = int(ow.shape[0] * ow.shape[1])
@@ -426,17 +438,22 @@ def between_run(args, job, sc, g, li, r0):
         for name, fn in (("select_mps", lambda u: O.select_mps(lio, ncc_s, u, 1, **MPS_ARGS)),
                          ("set_height_range", lambda u: O.set_height_range(lio, u, pts_s, tris_s, 1, 1.0)),
                          ("set_height_range_blunder", lambda u: O.set_height_range_blunder(lio, u, pts_s, tris_s)),
-                         ("vertical_line_locus_blunder", lambda u: O.vll_blunder(lio, pyr1, 1, ow_s, u, 1, False))):
+                         ("vertical_line_locus_blunder", lambda u: O.vll_blunder(lio, pyr1, 1, ow_s, u, 1, False)),
+                         ("blunder_detection_tin",
+                          lambda u: O.blunder_detection_tin(lio, u, on_s, pts_s.copy(), tris_s, 1, True,
+                                                            BD_HINTERVAL))):
             u = ug_s.copy()
             t = time.perf_counter()
-            fn(u)
+            r = fn(u)
             dt = time.perf_counter() - t
             cpu[name] = dict(ms=1e3 * dt, cells_per_s=(c1 - c0) / dt)
+            if name == "vertical_line_locus_blunder":
+                on_s = r
         out["cpu_baseline"] = dict(cpu, cores=O.threads(), kind="port",
                                    sample=f"{rows} x {W} cell stripe (rows {r0}..{r0 + rows - 1}) of the same level "
                                           f"as a level of its own, its lattice and ortho grid; oracle restatement "
-                                          f"(SelectMPs / SetHeightRange single-threaded as the reference runs "
-                                          f"them, the blunder NCC and ortho grid on OpenMP)")
+                                          f"(SelectMPs / SetHeightRange / blunder_detection_TIN single-threaded, "
+                                          f"the blunder NCC and ortho grid on OpenMP)")
     return out
 
 

@@ -11,6 +11,7 @@
 #include <vector>
 #include <mutex>
 #include <map>
+#include <thread>
 
 #include "smx_vll.h"
 #include "smx_kernels.hpp"
@@ -1614,8 +1615,13 @@ namespace {
 // in triangle order as the reference sums them, then the height-difference
 // threshold, the level's ortho-NCC thresholds and whether the neighbour rule
 // runs (:9489-9507).
+struct BdThresholds {
+    double height_th, ortho_ncc_th, th_ref_ncc;
+    int check_match;
+};
+
 void bd_thresholds(const smx_level &L, const smx_bd_args &a, const smx_point3 *pts, long npts, const smx_tri *tris,
-                   long ntris, BdDev &D)
+                   long ntris, BdThresholds &D)
 {
     const int step = L.level, iteration = a.iteration;
     const uint32_t hdiffcount = (uint32_t)a.hinterval;
@@ -1712,6 +1718,17 @@ int smx_blunder_detection_tin(smx_job *j, const smx_bd_args *a, const float *ort
     if (detected && npts > 0) std::memset(detected, 0, (size_t)npts);
     const smx_level &L = j->lv;
     if (L.level == 0 && a->iteration == 3) return 0;   // setsm_code.cpp:9164
+    // the serial pass over the triangles runs on its own thread while the
+    // inputs go up and the incidence lists build
+    BdThresholds th{};
+    struct Joiner {
+        std::thread t;
+        ~Joiner()
+        {
+            if (t.joinable()) t.join();
+        }
+    } stats;
+    stats.t = std::thread([&] { bd_thresholds(L, *a, pts, npts, tris, ntris, th); });
     HIPCHK(hipSetDevice(j->device));
     const long n = j->ncells;
     CallBuffers B(j);
@@ -1755,8 +1772,12 @@ int smx_blunder_detection_tin(smx_job *j, const smx_bd_args *a, const float *ort
     HIPCHK(launch_bd_prepare(D, j->js, j->stream));
     HIPCHK(hipEventRecord(j->ev1, j->stream));
     HIPCHK(hipMemcpyAsync(&j->h_small[0], D.err, sizeof(int), hipMemcpyDeviceToHost, j->stream));
-    bd_thresholds(L, *a, pts, npts, tris, ntris, D);   // on the host while the lists build
     HIPCHK(hipStreamSynchronize(j->stream));
+    stats.t.join();
+    D.height_th = th.height_th;
+    D.ortho_ncc_th = th.ortho_ncc_th;
+    D.th_ref_ncc = th.th_ref_ncc;
+    D.check_match = th.check_match;
     int err = 0;
     std::memcpy(&err, &j->h_small[0], sizeof err);
     float ms = 0, ms2 = 0;
@@ -1772,21 +1793,45 @@ int smx_blunder_detection_tin(smx_job *j, const smx_bd_args *a, const float *ort
     j->last_ms = ms + ms2;   // kernels only: the host threshold pass and the copies excluded
     // setsm_code.cpp:9121-9156 and :9758-9776: a point still at flag 0 that
     // its own test or a neighbour's blames becomes a new blunder; flags 1 and
-    // 3 count as blunders (3 ends as 1); the rest are kept points
-    for (long i = 0; i < npts; i++) {
-        const uint8_t f0 = pts[i].flag;
-        if (f0 == 0 && st[(size_t)i]) {
-            pts[i].flag = 1;
-            if (detected) detected[i] = 1;
-            blunder_count[1]++;
-        } else if (f0 == 1 || f0 == 3) {
-            pts[i].flag = 1;
-            blunder_count[1]++;
-        } else {
-            blunder_count[0]++;
-            if (minz_mp && *minz_mp > pts[i].z) *minz_mp = pts[i].z;
-            if (maxz_mp && *maxz_mp < pts[i].z) *maxz_mp = pts[i].z;
+    // 3 count as blunders (3 ends as 1); the rest are kept points. In chunks
+    // on host threads: each chunk's first lowest / highest kept height, then
+    // the chunks in order, which is what the reference's one pass keeps.
+    struct Part {
+        long kept = 0, blunders = 0;
+        double lo = INFINITY, hi = -INFINITY;
+    };
+    const long nth = std::max(1L, std::min<long>({16L, (long)std::thread::hardware_concurrency(), npts >> 16}));
+    std::vector<Part> part((size_t)nth);
+    auto walk = [&](long k) {
+        Part &q = part[(size_t)k];
+        const long i0 = npts * k / nth, i1 = npts * (k + 1) / nth;
+        for (long i = i0; i < i1; i++) {
+            const uint8_t f0 = pts[i].flag;
+            if (f0 == 0 && st[(size_t)i]) {
+                pts[i].flag = 1;
+                if (detected) detected[i] = 1;
+                q.blunders++;
+            } else if (f0 == 1 || f0 == 3) {
+                pts[i].flag = 1;
+                q.blunders++;
+            } else {
+                q.kept++;
+                if (q.lo > pts[i].z) q.lo = pts[i].z;
+                if (q.hi < pts[i].z) q.hi = pts[i].z;
+            }
         }
+    };
+    {
+        std::vector<std::thread> pool;
+        for (long k = 1; k < nth; k++) pool.emplace_back(walk, k);
+        walk(0);
+        for (std::thread &t : pool) t.join();
+    }
+    for (const Part &q : part) {
+        blunder_count[0] += q.kept;
+        blunder_count[1] += q.blunders;
+        if (minz_mp && *minz_mp > q.lo) *minz_mp = q.lo;
+        if (maxz_mp && *maxz_mp < q.hi) *maxz_mp = q.hi;
     }
     j->generation++;   // anchor flags
     return 0;
