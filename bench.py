@@ -349,6 +349,8 @@ MPS_ARGS = dict(th_roh=0.4, th_roh_min=0.1, th_roh_start=0.4, th_roh_next=0.3, m
                 final_level_iteration=1)
 # Hinterval of blunder_detection_TIN: the span of the level's height bands
 BD_HINTERVAL = 500.0
+# SelectMPs' thresholds in the reference CPU case's chain (MPP passed apart)
+C0_MPS = {k: v for k, v in MPS_ARGS.items() if k != "mpp"}
 
 
 def lattice_tris(w, h):
@@ -458,14 +460,25 @@ def between_run(args, job, sc, g, li, r0):
 
 
 # ------------------------------------------------------------ the reference's CPU-runnable case
+def same_records(a, b):
+    """Field-wise equality of two record arrays (padding bytes ignored)."""
+    return len(a) == len(b) and all(np.array_equal(a[f], b[f]) for f in a.dtype.names if not f.startswith("_"))
+
+
 def c0_run(args):
     """The reference's CPU-runnable case: a 2048^2 pair over 1 km
     (300 m relief, views +-12 degrees), matching levels L4..L0 at SETSM's grid
     spacings (8/8/6/3/2 m), 64 heights per cell, two iterations per level
-    (the second adds the level-below term above level 0; its search space
-    from the seeded host-step stand-in). CPU FP64 restatement (median of 3
-    passes) against the GPU FP64 and FAST paths, InitializeVoxel +
-    VerticalLineLocus + AWNCC per step, with word parity per step.
+    (the second adds the level-below term above level 0). The first
+    iteration of a level searches a seeded band about the terrain; the
+    second searches the grid SETSM's loop makes from the first: SelectMPs,
+    DecisionMPs (anchor points, then all points, with the ortho-NCC blunder
+    tests at the blunder level), SetHeightRange over a TIN of the surviving
+    points (Qhull's, as the caller's TIN), run on the GPU FP64 path and
+    checked grid for grid against the same chain on the oracle. CPU FP64
+    restatement (median of 3 passes) against the GPU FP64 and FAST paths,
+    InitializeVoxel + VerticalLineLocus + AWNCC per step, with word parity
+    per step.
 
     The pair goes through the reference's formats first: each image is
     written as a GeoTIFF (WriteGeotiff, uint16) with its RPCs as a
@@ -477,8 +490,9 @@ def c0_run(args):
     from oracle import oracle_py as O
     from setsm_amd import smx
     from setsm_amd.records import LevelInputs, height_step
-    from setsm_amd.serve import next_searchspace, setgrids_spacing
-    from setsm_amd.synth import GridSpec, SceneSpec, make_grid, make_scene
+    from setsm_amd import host_loop
+    from setsm_amd.serve import setgrids_spacing
+    from setsm_amd.synth import GridSpec, SceneSpec, make_grid, make_scene, ortho_wgs
     import tempfile
     from setsm_amd import io as sio
     sc = make_scene(SceneSpec(image_px=2048, relief=300.0, seed=21, view_deg=(12.0, -12.0)), device="cuda")
@@ -495,7 +509,11 @@ def c0_run(args):
         files[f"img{t}_tif_bytes"] = os.path.getsize(tif)
     pyr = [O.build_pyramid(im, 5) for im in sc.images]   # the same views on both sides
     side = sc.spec.image_px * sc.spec.gsd
-    steps = []
+    steps, chains = [], []
+    setup = smx.Job(5)
+    for ti in range(2):
+        for lv, (img, mag, ori) in enumerate(pyr[ti]):
+            setup.set_view(ti, lv, img, mag, ori, start=(0.0, 0.0))
     for lv in (4, 3, 2, 1, 0):
         res = setgrids_spacing(lv)
         cells = int(math.ceil(1000.0 / res))
@@ -506,9 +524,59 @@ def c0_run(args):
         g["boundary"] = np.array([inset, inset, inset + cells * res, inset + cells * res])
         g["shape"] = (cells, cells)
         g["grid_res"] = res
-        li = LevelInputs(pyr, lv, sc.rpcs, g, grid_resolution=res)
-        ug2 = next_searchspace(g["ugrid"], 500 + lv, lv)
+        # the level's ortho grid: the chain leaves ortho NCC in GridPT3, which
+        # turns the GNCC term on in the second iteration
+        li = LevelInputs(pyr, lv, sc.rpcs, g, grid_resolution=res, ortho_wgs=ortho_wgs(sc, g["boundary"], lv))
+        # iteration 1 on the GPU FP64 path, then the chain to iteration 2's grid
+        B = lv if lv >= 4 else lv + 1
+        ow = ortho_wgs(sc, g["boundary"], B)
+        setup.set_level(li.level)
+        setup.set_searchspace(g["ugrid"], g["minmax"])
+        setup.initialize_voxel(1)
+        setup.vertical_line_locus(1, precision=smx.PREC_FP64)
+        setup.awncc(1)
+        ncc1, ug1 = setup.get_nccresult(), setup.get_searchspace()
+        setup.set_blunder_level(B, ow)
+        kw = dict(mps_args=C0_MPS, hinterval=float(g["minmax"][1] - g["minmax"][0]), mpp=1.0)
+        setup.sync()
+        t0 = time.perf_counter()
+        ch = host_loop.iteration_chain(setup, lv, 1, g["boundary"], res, **kw)
+        gpu_chain_s = time.perf_counter() - t0
+        ug2 = setup.get_searchspace()
+        chains.append(dict(level=lv, ncc1=ncc1, ug1=ug1, ug2=ug2, B=B, ow=ow, kw=kw, gpu_s=gpu_chain_s,
+                           gpu_tin_s=ch.tin_s, tins=ch.tins,
+                           points=ch.points, kept=ch.kept, triangles=ch.triangles,
+                           blunder_passes=ch.blunders.passes if ch.blunders else 0,
+                           blunders=ch.blunders.blunders if ch.blunders else 0))
         steps.append((lv, g, li, [g["ugrid"], ug2]))
+    setup.close()
+
+    def cpu_chains():
+        """The same chains on the oracle, from the oracle's own first
+        iteration (its NCCresult and GridPT3 equal the GPU FP64 ones word for
+        word): the grids must come out equal."""
+        equal, t, t_tin, diff = True, 0.0, 0.0, []
+        for (lv, g, li, ugs), c in zip(steps, chains):
+            st = O.OracleState(li, keep_rho=False)
+            ug = ugs[0].copy()
+            st.initialize_voxel(ug, 1, g["minmax"])
+            st.vertical_line_locus(ug, 1, g["minmax"])
+            st.awncc(ug, 1)
+            first = same_records(st.ncc, c["ncc1"]) and same_records(ug, c["ug1"])
+            t0 = time.perf_counter()
+            ch = host_loop.iteration_chain(O.ChainJob(li, st.ncc, ug), lv, 1, g["boundary"], g["grid_res"],
+                                           ops=O.ChainOps(li, ug, pyr, c["B"], c["ow"]), **c["kw"])
+            t += time.perf_counter() - t0
+            t_tin += ch.tin_s
+            ok = first and same_records(ug, c["ug2"])
+            if not ok:
+                diff.append(dict(level=lv, first_iteration_equal=bool(first),
+                                 oracle=(ch.points, ch.kept, ch.triangles, ch.blunders.passes if ch.blunders else 0,
+                                         ch.blunders.blunders if ch.blunders else 0),
+                                 gpu=(c["points"], c["kept"], c["triangles"], c["blunder_passes"], c["blunders"]),
+                                 cells={f: int(np.count_nonzero(ug[f] != c["ug2"][f])) for f in ug.dtype.names}))
+            equal = equal and ok
+        return equal, t, t_tin, diff
 
     def cpu_pass():
         words, t, hyps = [], 0.0, 0
@@ -569,6 +637,7 @@ def c0_run(args):
                    fast_vs_terrain_m=float(np.sqrt(np.mean((demf[both] - truth[both]) ** 2))),
                    note="level-0 AWNCC peak heights of iteration 2 (before blunder filtering and TIN)")
     cpu = [cpu_pass() for _ in range(3)]
+    chain_equal, chain_cpu_s, chain_cpu_tin_s, chain_diff = cpu_chains()
     tc = float(np.median([c[0] for c in cpu]))
     wc = cpu[0][1]
     parity = all(np.array_equal(a, b) for a, b in zip(w64, wc))
@@ -581,6 +650,18 @@ def c0_run(args):
                cpu_hyp_per_s=ev64 / tc, gpu_fp64_s=t64, gpu_fast_s=tf, speedup_fp64_vs_cpu=tc / t64,
                speedup_fast_vs_cpu=tc / tf, fp64_words_equal_cpu=bool(parity), fast_incc_max_quanta=dq,
                evaluated_equal=bool(ev64 == evf), dem=dem_rms,
+               chain=dict(levels=[{k: c[k] for k in ("level", "points", "kept", "triangles", "blunder_passes",
+                                                     "blunders", "tins")} for c in chains],
+                          gpu_s=sum(c["gpu_s"] for c in chains), cpu_s=chain_cpu_s,
+                          tin_s=sum(c["gpu_tin_s"] for c in chains),
+                          gpu_steps_s=sum(c["gpu_s"] - c["gpu_tin_s"] for c in chains),
+                          cpu_steps_s=chain_cpu_s - chain_cpu_tin_s,
+                          grids_equal_cpu=bool(chain_equal), differences=chain_diff,
+                          note="iteration 1 -> 2 of every level: SelectMPs, DecisionMPs (anchor points, then "
+                               "all points), SetHeightRange over a Qhull TIN of the kept points; wall clock, "
+                               "and without the host TINs (Qhull, the same on both sides: tin_s is the GPU "
+                               "side's); the CPU side on the oracle (its blunder NCC on OpenMP, the rest "
+                               "single-threaded)"),
                files=dict(files, note="pair read back from GeoTIFF + DigitalGlobe XML; DEMs written as GeoTIFF"))
     import shutil
     shutil.rmtree(fdir, ignore_errors=True)

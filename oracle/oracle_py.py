@@ -227,6 +227,43 @@ def blunder_detection_tin(level_inputs, ugrid, ortho_ncc, pts, tris, iteration, 
     return det[:len(pts)].astype(bool), (bc[0], bc[1]), lo.value, hi.value, th.value
 
 
+class ChainJob:
+    """select_mps / set_height_range with the signatures of `smx.Job`, on the
+    oracle's NCCresult and GridPT3 (ugrid updated in place), for driving
+    `setsm_amd.host_loop.iteration_chain` on the CPU."""
+
+    def __init__(self, level_inputs, ncc, ugrid):
+        self.li, self.ncc, self.ug = level_inputs, ncc, ugrid
+        self.ncells = len(ugrid)
+
+    def select_mps(self, iteration, mpp, **kw):
+        return select_mps(self.li, self.ncc, self.ug, iteration, mpp=mpp, **kw)
+
+    def set_height_range(self, pts, tris, iteration, mpp, check_matching_rate=False):
+        return set_height_range(self.li, self.ug, pts, tris, iteration, mpp, check_matching_rate)
+
+
+class ChainOps:
+    """DecisionMPs' three steps (the `ops` of `setsm_amd.host_loop`) on the
+    oracle, the blunder level's ortho NCC kept between them."""
+
+    def __init__(self, level_inputs, ugrid, pyr, blunder_level, ortho_wgs):
+        self.li, self.ug, self.pyr, self.B, self.ow = level_inputs, ugrid, pyr, blunder_level, ortho_wgs
+        self.on = None
+
+    def set_height_range_blunder(self, pts, tris):
+        set_height_range_blunder(self.li, self.ug, pts, tris)
+
+    def vll_blunder(self, iteration, bblunder):
+        self.on = vll_blunder(self.li, self.pyr, self.B, self.ow, self.ug, iteration, bblunder)
+
+    def blunder_detection(self, pts, tris, iteration, flag_blunder, hinterval, count_bl, zero_ortho, minz, maxz):
+        on = np.zeros(len(self.ug), np.float32) if zero_ortho else self.on
+        det, bc, lo, hi, _ = blunder_detection_tin(self.li, self.ug, on, pts, tris, iteration, flag_blunder,
+                                                   hinterval, count_bl=count_bl, minz=minz, maxz=maxz)
+        return det, bc, lo, hi
+
+
 def _p(a):
     return a.ctypes.data
 

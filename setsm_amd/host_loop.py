@@ -18,6 +18,7 @@ drives the GPU (`JobOps`) and, in the tests, the oracle.
 """
 from __future__ import annotations
 
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -157,6 +158,8 @@ class ChainResult:
     kept: int
     triangles: int
     min_max_height: tuple
+    tin_s: float = 0.0      # host time in the TIN calls
+    tins: int = 0
 
 
 def iteration_chain(job, level, iteration, boundary, grid_res, mps_args, hinterval, mpp, dem_resolution=2.0,
@@ -168,6 +171,15 @@ def iteration_chain(job, level, iteration, boundary, grid_res, mps_args, hinterv
     unless is_ra. The job's GridPT3 becomes the next iteration's search
     space."""
     ops = ops or JobOps(job)
+    spent = [0.0, 0]
+
+    def timed_tin(p, b, r):
+        t0 = time.perf_counter()
+        out = tin(p, b, r)
+        spent[0] += time.perf_counter() - t0
+        spent[1] += 1
+        return out
+
     pts, th = job.select_mps(iteration, mpp=mpp, **mps_args)
     pts = np.ascontiguousarray(pts.copy(), POINT3_DTYPE)
     anchors = blunders = None
@@ -176,13 +188,14 @@ def iteration_chain(job, level, iteration, boundary, grid_res, mps_args, hinterv
         ortho_level = 3 if dem_resolution >= 8 else 2
         check_ortho = level >= ortho_level and not (level == 2 and iteration > 5)
         if check_ortho and not is_ra:
-            anchors = decision_mps(ops, pts, level, iteration, hinterval, False, boundary, grid_res, is_ra, tin)
+            anchors = decision_mps(ops, pts, level, iteration, hinterval, False, boundary, grid_res, is_ra,
+                                   timed_tin)
             pts["flag"] = 0
-        blunders = decision_mps(ops, pts, level, iteration, hinterval, True, boundary, grid_res, is_ra, tin)
+        blunders = decision_mps(ops, pts, level, iteration, hinterval, True, boundary, grid_res, is_ra, timed_tin)
         kept = pts[pts["flag"] != 1]
     else:
         kept = pts
-    tris = tin(kept, boundary, grid_res)
+    tris = timed_tin(kept, boundary, grid_res)
     lohi = job.set_height_range(kept, tris, iteration, mpp, check_matching_rate)
     return ChainResult(points=n, minimum_th=th, anchors=anchors, blunders=blunders, kept=len(kept),
-                       triangles=len(tris), min_max_height=lohi)
+                       triangles=len(tris), min_max_height=lohi, tin_s=spent[0], tins=spent[1])

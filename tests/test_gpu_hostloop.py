@@ -19,40 +19,6 @@ from tests.test_gpu_mps import SEL, grids_equal
 pytestmark = pytest.mark.gpu
 
 
-class OracleJob:
-    """select_mps / set_height_range of `smx.Job`, on the oracle."""
-
-    def __init__(self, li, ncc, ug):
-        self.li, self.ncc, self.ug = li, ncc, ug
-        self.ncells = len(ug)
-
-    def select_mps(self, iteration, mpp, **kw):
-        return O.select_mps(self.li, self.ncc, self.ug, iteration, mpp=mpp, **kw)
-
-    def set_height_range(self, pts, tris, iteration, mpp, check_matching_rate=False):
-        return O.set_height_range(self.li, self.ug, pts, tris, iteration, mpp, check_matching_rate)
-
-
-class OracleOps:
-    """DecisionMPs' steps on the oracle, the ortho NCC kept between them."""
-
-    def __init__(self, li, ug, pyr, B, ow):
-        self.li, self.ug, self.pyr, self.B, self.ow = li, ug, pyr, B, ow
-        self.on = None
-
-    def set_height_range_blunder(self, pts, tris):
-        O.set_height_range_blunder(self.li, self.ug, pts, tris)
-
-    def vll_blunder(self, iteration, bblunder):
-        self.on = O.vll_blunder(self.li, self.pyr, self.B, self.ow, self.ug, iteration, bblunder)
-
-    def blunder_detection(self, pts, tris, iteration, flag_blunder, hinterval, count_bl, zero_ortho, minz, maxz):
-        on = np.zeros(len(self.ug), np.float32) if zero_ortho else self.on
-        det, bc, lo, hi, _ = O.blunder_detection_tin(self.li, self.ug, on, pts, tris, iteration, flag_blunder,
-                                                     hinterval, count_bl=count_bl, minz=minz, maxz=maxz)
-        return det, bc, lo, hi
-
-
 def dec_equal(a, b):
     if a is None or b is None:
         assert a is None and b is None
@@ -90,8 +56,8 @@ def test_iteration_chain_matches_oracle(require_gpu, level, its, cells):
             ncc, ug_o = job.get_nccresult(), job.get_searchspace()
             kw = dict(mps_args=SEL, hinterval=hint, mpp=1.0, dem_resolution=2.0)
             rg = host_loop.iteration_chain(job, level, it, g["boundary"], g["grid_res"], **kw)
-            ro = host_loop.iteration_chain(OracleJob(gr.li, ncc, ug_o), level, it, g["boundary"], g["grid_res"],
-                                           ops=OracleOps(gr.li, ug_o, pyr, B, ow), **kw)
+            ro = host_loop.iteration_chain(O.ChainJob(gr.li, ncc, ug_o), level, it, g["boundary"], g["grid_res"],
+                                           ops=O.ChainOps(gr.li, ug_o, pyr, B, ow), **kw)
             assert (rg.points, rg.kept, rg.triangles, rg.minimum_th) == \
                 (ro.points, ro.kept, ro.triangles, ro.minimum_th)
             assert rg.min_max_height == ro.min_max_height
