@@ -255,6 +255,18 @@ int  smx_blunder_detection_tin(smx_job *job, const smx_bd_args *args, const floa
 /* The five calls above leave their kernel time on the job stream (host
  * copies and host passes excluded) in smx_job_last_kernel_ms. */
 
+/* A Delaunay TIN of matched points, host code, for callers without a TIN of
+ * their own (the role of SETSM's TINCreate_list, SubFunctions.cpp:2768-2846;
+ * this build's own triangulation, not the reference's). Points snap to the
+ * lattice of min_max / resolution as the reference snaps them, the last
+ * point of a lattice cell standing for it; triangles (indices into pts)
+ * come out clockwise. Among cocircular points the split is its own.
+ * Returns the triangle count (at most 2 * npts), 0 for fewer than 3 points
+ * or a single line, -1 for a lattice beyond 32767 per side or a point
+ * outside it, -2 if cap is too small, -3 for a bad argument. */
+long smx_tin_create(const smx_point3 *pts, long npts, const double min_max[4], double resolution, smx_tri *out,
+                    long cap);
+
 /* Number of hypotheses in the cube (sum of NumOfHeight). */
 long long smx_job_num_hypotheses(const smx_job *job);
 /* Download the cube in CSR order: words[total], rho[total] (rho may be NULL,

@@ -7,11 +7,12 @@ Every data-parallel step of that chain runs on the GPU through `smx.Job`.
 This module holds the control flow around them: DecisionMPs' loop
 (setsm_code.cpp:8891-9093), restated step for step, and the chain. The
 TIN is the caller's. SETSM uses its own grid triangulation. A caller without
-one can use `qhull_tin`, a Delaunay triangulation by scipy's Qhull of the
-points' lattice positions, listed clockwise as SETSM's TIN lists its
-triangles. Among cocircular lattice points it splits quads its own way, so
-its triangles are not the reference's, but they are a TIN of the same
-points.
+one can use `native_tin` (libsmx's `smx_tin_create`, a radial-sweep
+Delaunay on the lattice with exact integer predicates) or `qhull_tin`
+(scipy's Qhull, slower on lattices), both listed clockwise as SETSM's TIN
+lists its triangles. Among cocircular lattice points each splits quads its
+own way, so their triangles are not the reference's, but they are Delaunay
+TINs of the same points.
 
 The loop takes its steps from an `ops` object, so the same control flow
 drives the GPU (`JobOps`) and, in the tests, the oracle.
@@ -54,6 +55,14 @@ def qhull_tin(pts, boundary, grid_res):
     return out
 
 
+def native_tin(pts, boundary, grid_res):
+    """Delaunay TIN of the matched points by libsmx's `smx_tin_create`
+    (TINCreate_list's role): boundary = (min_x, min_y, max_x, max_y) of the
+    level's lattice."""
+    from . import smx
+    return smx.tin_create(pts, boundary, grid_res)
+
+
 class JobOps:
     """DecisionMPs' three steps on a job's resident GridPT3, on the GPU; the
     ortho NCC between them stays on the device."""
@@ -85,7 +94,7 @@ class DecisionResult:
 
 
 def decision_mps(ops, pts, level, iteration, hinterval, flag_blunder, boundary, grid_res, is_ra=False,
-                 tin=qhull_tin):
+                 tin=native_tin):
     """DecisionMPs (setsm_code.cpp:8891-9093) on the points pts
     (POINT3_DTYPE, flags updated in place). Each pass: SetHeightRange_blunder
     and VerticalLineLocus_blunder (not in RA mode, whose ortho NCC stays
@@ -163,7 +172,7 @@ class ChainResult:
 
 
 def iteration_chain(job, level, iteration, boundary, grid_res, mps_args, hinterval, mpp, dem_resolution=2.0,
-                    is_ra=False, check_matching_rate=False, tin=qhull_tin, ops=None):
+                    is_ra=False, check_matching_rate=False, tin=native_tin, ops=None):
     """SelectMPs -> DecisionMPs (anchor points, then all points) ->
     SetHeightRange over the TIN of the points that are not blunders
     (setsm_code.cpp:3862-3945, 4192-4222), on the job's level after its

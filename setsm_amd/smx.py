@@ -36,7 +36,7 @@ EXPORTS = [
     "smx_awncc", "smx_job_num_hypotheses", "smx_job_get_voxels", "smx_job_get_voxel_rows", "smx_job_set_voxels",
     "smx_job_get_sumcost", "smx_job_set_ortho_wgs", "smx_select_mps", "smx_set_height_range",
     "smx_set_height_range_blunder", "smx_job_set_blunder_level", "smx_vertical_line_locus_blunder",
-    "smx_blunder_detection_tin",
+    "smx_blunder_detection_tin", "smx_tin_create",
     "smx_job_last_kernel_ms", "smx_job_last_evaluated", "smx_job_last_redo", "smx_job_sync", "smx_fast_num_configs",
     "smx_job_set_fast_config", "smx_fast_config_desc", "smx_job_last_stats", "smx_vertical_line_locus_batch",
     "smx_vertical_line_locus_noalloc", "smx_release_cached", "smx_job_trim", "smx_job_half", "smx_graph_create", "smx_graph_launch", "smx_graph_last_ms", "smx_graph_destroy",
@@ -95,6 +95,7 @@ def lib():
             "smx_vertical_line_locus_blunder": (i, [vp, i, i, vp]),
             "smx_blunder_detection_tin": (i, [vp, vp, vp, vp, C.c_long, vp, C.c_long, vp, C.POINTER(C.c_long),
                                               C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "smx_tin_create": (C.c_long, [vp, C.c_long, vp, C.c_double, vp, C.c_long]),
             "smx_job_last_kernel_ms": (C.c_double, [vp]),
             "smx_job_last_evaluated": (ll, [vp]),
             "smx_job_last_redo": (i, [vp, vp, vp, vp]),
@@ -126,6 +127,20 @@ def _check(rc):
         msg = lib().smx_last_error().decode(errors="replace")
         raise SmxError(f"libsmx error {rc}: {msg}")
     return rc
+
+
+def tin_create(pts, min_max, resolution):
+    """Delaunay TIN of matched points on the level's lattice (host code; the
+    role of SETSM's TINCreate_list): triangles (TRI_DTYPE, clockwise,
+    indices into pts)."""
+    pts = np.ascontiguousarray(pts, POINT3_DTYPE)
+    mm = (C.c_double * 4)(*[float(v) for v in min_max[:4]])
+    out = np.zeros(2 * len(pts) + 1, TRI_DTYPE)
+    n = lib().smx_tin_create(_p(pts) if len(pts) else None, len(pts), mm, float(resolution), _p(out), len(out))
+    if n < 0:
+        raise SmxError(f"smx_tin_create failed ({n}): " + {-1: "lattice beyond 32767 cells or a point outside it",
+                                                            -2: "triangle buffer too small"}.get(n, "bad argument"))
+    return out[:n].copy()
 
 
 def _p(a):
