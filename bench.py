@@ -474,7 +474,7 @@ def c0_run(args):
     second searches the grid SETSM's loop makes from the first: SelectMPs,
     DecisionMPs (anchor points, then all points, with the ortho-NCC blunder
     tests at the blunder level), SetHeightRange over a TIN of the surviving
-    points (Qhull's, as the caller's TIN), run on the GPU FP64 path and
+    points (the library's host TIN, as the caller's), run on the GPU FP64 path and
     checked grid for grid against the same chain on the oracle. CPU FP64
     restatement (median of 3 passes) against the GPU FP64 and FAST paths,
     InitializeVoxel + VerticalLineLocus + AWNCC per step, with word parity
@@ -658,10 +658,10 @@ def c0_run(args):
                           cpu_steps_s=chain_cpu_s - chain_cpu_tin_s,
                           grids_equal_cpu=bool(chain_equal), differences=chain_diff,
                           note="iteration 1 -> 2 of every level: SelectMPs, DecisionMPs (anchor points, then "
-                               "all points), SetHeightRange over a Qhull TIN of the kept points; wall clock, "
-                               "and without the host TINs (Qhull, the same on both sides: tin_s is the GPU "
-                               "side's); the CPU side on the oracle (its blunder NCC on OpenMP, the rest "
-                               "single-threaded)"),
+                               "all points), SetHeightRange over a TIN of the kept points (smx_tin_create, "
+                               "host); wall clock, and without the host TINs (the same on both sides: tin_s "
+                               "is the GPU side's); the CPU side on the oracle (its blunder NCC on OpenMP, "
+                               "the rest single-threaded)"),
                files=dict(files, note="pair read back from GeoTIFF + DigitalGlobe XML; DEMs written as GeoTIFF"))
     import shutil
     shutil.rmtree(fdir, ignore_errors=True)
