@@ -263,7 +263,8 @@ int  smx_blunder_detection_tin(smx_job *job, const smx_bd_args *args, const floa
  * come out clockwise. Among cocircular points the split is its own.
  * Returns the triangle count (at most 2 * npts), 0 for fewer than 3 points
  * or a single line, -1 for a lattice beyond 32767 per side or a point
- * outside it, -2 if cap is too small, -3 for a bad argument. */
+ * outside it, -2 if cap is too small, -3 for a bad argument, -5 if host
+ * memory ran out. */
 long smx_tin_create(const smx_point3 *pts, long npts, const double min_max[4], double resolution, smx_tri *out,
                     long cap);
 

@@ -1728,7 +1728,11 @@ int smx_blunder_detection_tin(smx_job *j, const smx_bd_args *a, const float *ort
             if (t.joinable()) t.join();
         }
     } stats;
-    stats.t = std::thread([&] { bd_thresholds(L, *a, pts, npts, tris, ntris, th); });
+    try {
+        stats.t = std::thread([&] { bd_thresholds(L, *a, pts, npts, tris, ntris, th); });
+    } catch (...) {
+        bd_thresholds(L, *a, pts, npts, tris, ntris, th);   // no thread to be had: the same pass in line
+    }
     HIPCHK(hipSetDevice(j->device));
     const long n = j->ncells;
     CallBuffers B(j);
@@ -1773,7 +1777,7 @@ int smx_blunder_detection_tin(smx_job *j, const smx_bd_args *a, const float *ort
     HIPCHK(hipEventRecord(j->ev1, j->stream));
     HIPCHK(hipMemcpyAsync(&j->h_small[0], D.err, sizeof(int), hipMemcpyDeviceToHost, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
-    stats.t.join();
+    if (stats.t.joinable()) stats.t.join();
     D.height_th = th.height_th;
     D.ortho_ncc_th = th.ortho_ncc_th;
     D.th_ref_ncc = th.th_ref_ncc;
@@ -1823,7 +1827,12 @@ int smx_blunder_detection_tin(smx_job *j, const smx_bd_args *a, const float *ort
     };
     {
         std::vector<std::thread> pool;
-        for (long k = 1; k < nth; k++) pool.emplace_back(walk, k);
+        long k = 1;
+        try {
+            for (; k < nth; k++) pool.emplace_back(walk, k);
+        } catch (...) {
+            for (; k < nth; k++) walk(k);   // chunks no thread took, in line
+        }
         walk(0);
         for (std::thread &t : pool) t.join();
     }

@@ -120,8 +120,8 @@ struct Tin {
 
 }  // namespace
 
-extern "C" long smx_tin_create(const smx_point3 *pts, long npts, const double min_max[4], double resolution,
-                               smx_tri *out, long cap)
+namespace {
+long tin_create(const smx_point3 *pts, long npts, const double min_max[4], double resolution, smx_tri *out, long cap)
 {
     if (npts < 0 || (npts > 0 && (!pts || !min_max)) || !(resolution > 0)) return -3;
     if (npts <= 2) return 0;
@@ -270,4 +270,15 @@ extern "C" long smx_tin_create(const smx_point3 *pts, long npts, const double mi
         out[t].c = (uint32_t)orig[(size_t)T.tv[(size_t)(3 * t + 1)]];
     }
     return nt;
+}
+}  // namespace
+
+extern "C" long smx_tin_create(const smx_point3 *pts, long npts, const double min_max[4], double resolution,
+                               smx_tri *out, long cap)
+{
+    try {
+        return tin_create(pts, npts, min_max, resolution, out, cap);
+    } catch (...) {   // host memory: no exception crosses the C ABI
+        return -5;
+    }
 }
