@@ -73,6 +73,13 @@ def test_native_tin_special_inputs():
     bad["x"][0] = 0.0
     with pytest.raises(smx.SmxError):
         host_loop.native_tin(bad, BOX, 2.0)
+    # through the C entry point: a buffer too small, a bad resolution
+    import ctypes as C
+    mm = (C.c_double * 4)(*BOX)
+    small = np.zeros(3, host_loop.TRI_DTYPE)
+    lib = smx.lib()
+    assert lib.smx_tin_create(dup.ctypes.data, len(dup), mm, 2.0, small.ctypes.data, len(small)) == -2
+    assert lib.smx_tin_create(dup.ctypes.data, len(dup), mm, 0.0, small.ctypes.data, len(small)) == -3
 
 
 def test_qhull_tin_degenerate_inputs():
