@@ -19,8 +19,8 @@
 // calls of a level, and every call hands the host arrays back exactly as the
 // CPU path leaves them (grid_voxel rows re-allocated like InitializeVoxel
 // does, VOXEL bytes identical to the device cube). Call
-// smx_setsm::Release() when a tile is done, so that a later tile whose
-// buffers land at the same addresses is not taken for the same one.
+// smx_setsm::Release() when a tile is done; a later tile whose buffers land
+// at the same addresses is also told apart by a sample of its pixels.
 //
 // Errors follow SETSM's convention (setsm_code.cpp:6571-6575): a message on
 // stdout and exit(1). Inputs that need a mode libsmx lacks (frame sensors,
@@ -58,6 +58,7 @@ struct State {
     const void *views[2][6] = {};   // the pointers and geometry the job's views came from
     double starts[2][4] = {};
     int sizes[2][4] = {};
+    uint64_t pixels = 0;            // a sample of the views' pixels (see view_sample)
     smx_level lv;                   // the level record last passed to smx_job_set_level
     bool lv_set = false;
     bool ortho_set = false;         // ortho grid lon/lat uploaded for this level record
@@ -113,6 +114,28 @@ int precision_from_env()
 // (level, grid, RPCs, ImageAdjust, ...) is passed to smx_job_set_level, which
 // starts the level's device state afresh. Returns true when the device holds
 // no cube for this level yet, so that the caller uploads the host's.
+// FNV-1a over 256 evenly spaced elements of each view buffer: a later tile
+// whose buffers land at the same addresses, sizes and origins (the caller
+// skipped Release) still differs here, so its pixels are uploaded.
+uint64_t view_sample(const void *const views[2][6], const int sizes[2][4])
+{
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](uint64_t v) {
+        h ^= v;
+        h *= 1099511628211ull;
+    };
+    for (int k = 0; k < 2; k++)
+        for (int v = 0; v < 6; v++) {
+            if (!views[k][v]) continue;
+            const long n = (long)sizes[k][v < 3 ? 0 : 2] * (long)sizes[k][v < 3 ? 1 : 3];
+            for (long i = 0; i < 256 && n > 0; i++) {
+                const long at = i * n / 256;
+                mix(v % 3 == 2 ? ((const uint8 *)views[k][v])[at] : ((const uint16 *)views[k][v])[at]);
+            }
+        }
+    return h;
+}
+
 bool bind(const ProInfo *proinfo, LevelInfo &L)
 {
     State &s = g_state;
@@ -148,8 +171,10 @@ bool bind(const ProInfo *proinfo, LevelInfo &L)
             sizes[k][3] = L.py_Sizes[ti][level - 1].height;
         }
     }
+    const uint64_t pixels = view_sample(views, sizes);
     const bool same_views = s.job && s.level == level && !memcmp(views, s.views, sizeof views) &&
-                            !memcmp(starts, s.starts, sizeof starts) && !memcmp(sizes, s.sizes, sizeof sizes);
+                            !memcmp(starts, s.starts, sizeof starts) && !memcmp(sizes, s.sizes, sizeof sizes) &&
+                            pixels == s.pixels;
     if (!same_views) {
         if (s.job) smx_job_destroy(s.job);
         s = State{};
@@ -169,6 +194,7 @@ bool bind(const ProInfo *proinfo, LevelInfo &L)
         memcpy(s.views, views, sizeof views);
         memcpy(s.starts, starts, sizeof starts);
         memcpy(s.sizes, sizes, sizeof sizes);
+        s.pixels = pixels;
     }
 
     smx_level lv;

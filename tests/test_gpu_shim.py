@@ -104,6 +104,30 @@ def _check(o, s, tag, exact):
             assert np.array_equal(o["ncc_vll"][f], s["ncc_vll"][f]), f"{tag}: {f}"
 
 
+def test_shim_reused_buffers_with_new_pixels(require_gpu, monkeypatch):
+    """A caller that skips Release() and lands the next tile's pyramid in the
+    same buffers (same addresses, sizes and origins): the shim tells the
+    tiles apart by their pixels and matches the oracle on both."""
+    from setsm_amd.records import LevelInputs
+    monkeypatch.setenv("SMX_PRECISION", "fp64")
+    drv = _driver()
+    sc, pyr = scene(image_px=512, levels=1)
+    g = level_grid(sc, 0, 64)
+    li = LevelInputs(pyr, 0, sc.rpcs, g, grid_resolution=g["grid_res"])
+    s = ShimRun(drv, li, 0)
+    try:
+        ro = OracleRun(pyr, sc.rpcs, 0, g).step(g["ugrid"], 1, g["minmax"])
+        _check(ro, s.step(g["ugrid"], 1, g["minmax"]), "tile A", exact=True)
+        pyr_b = [[tuple(np.ascontiguousarray(np.roll(a, 7, axis=1)) for a in pyr[k][0])] for k in range(2)]
+        for k in range(2):
+            for j in range(3):
+                li.keep[3 * k + j][...] = pyr_b[k][0][j]   # the buffers the shim was given
+        ro = OracleRun(pyr_b, sc.rpcs, 0, g).step(g["ugrid"], 1, g["minmax"])
+        _check(ro, s.step(g["ugrid"], 1, g["minmax"]), "tile B in tile A's buffers", exact=True)
+    finally:
+        s.close()
+
+
 @pytest.mark.parametrize("level,swap,iters", [(0, 0, (1, 2, 3)), (1, 1, (1, 2)), (3, 0, (1, 2))])
 def test_shim_fp64_bit_exact(require_gpu, monkeypatch, level, swap, iters):
     """SMX_PRECISION=fp64: host VOXEL rows, NCCresult (after VerticalLineLocus
